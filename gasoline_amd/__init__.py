@@ -1,0 +1,15 @@
+"""gasoline_amd: MI355X force hot path for gasoline (tree gravity + SPH).
+
+The compute lives in libgasoline_amd.so (HIP kernels for gfx950 behind the C
+ABI of include/gasoline_amd.h).  This package only binds that ABI.  There is
+no CPU fallback: importing the binding without the built library raises.
+"""
+from .pkd import (GGError, Pkd, Smf, lib, FIELDS, PREC_FP32, PREC_FP64,
+                  SMX_DENSITY, SMX_DENDVDX, SMX_SPHPRESSURETERMS,
+                  TYPE_ACTIVE, TYPE_TREEACTIVE, TYPE_SMOOTHACTIVE, TYPE_SMOOTHDONE,
+                  TYPE_NbrOfACTIVE, TYPE_GAS)
+
+__all__ = ["GGError", "Pkd", "Smf", "lib", "FIELDS", "PREC_FP32", "PREC_FP64",
+           "SMX_DENSITY", "SMX_DENDVDX", "SMX_SPHPRESSURETERMS", "TYPE_ACTIVE",
+           "TYPE_TREEACTIVE", "TYPE_SMOOTHACTIVE", "TYPE_SMOOTHDONE",
+           "TYPE_NbrOfACTIVE", "TYPE_GAS"]

@@ -1,0 +1,839 @@
+// Tree gravity on the GPU: per-bucket interaction-list walk and bucket
+// interaction kernels (FP64 plain and FP32 cell-relative with compensated
+// accumulation), behind gg_build_binary / gg_grav_all.
+//
+// Reference path: pkdGravAll (pkd.c:2869-3060) -> pkdBucketWalk
+// (walk.c:306-439) -> pkdBucketInteract (grav.c:23-249).
+#include "gg_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace gg {
+
+struct WalkNode {        // 64 bytes: what the walk reads per visited cell
+    double cx, cy, cz, fOpen2, fSoft;
+    int pLower, pUpper, iLower, iNext;
+    int pad0, pad1;
+};
+
+// Per node FP64 record read when a cell is on an interaction list.
+constexpr int CR_M = 0, CR_X = 1, CR_H = 4, CR_QR = 5, CR_QT = 11, CR_N = 20;
+constexpr int OH_N = 25;  // octupole (10) + hexadecapole (15)
+
+// ------------------------------------------------------------------ walk
+// INTERSECTNP (walk.h:12-30): squared distance from point to box <= fBall2.
+__device__ __forceinline__ bool box_within(const double *bb, double fBall2,
+                                           double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    double dx = bb[0] - x, dx1 = x - bb[3];
+    double dy = bb[1] - y, dy1 = y - bb[4];
+    double dz = bb[2] - z, dz1 = z - bb[5];
+    double d2;
+    if (dx > 0.0) d2 = dx * dx;
+    else if (dx1 > 0.0) d2 = dx1 * dx1;
+    else d2 = 0.0;
+    if (dy > 0.0) d2 += dy * dy;
+    else if (dy1 > 0.0) d2 += dy1 * dy1;
+    if (dz > 0.0) d2 += dz * dz;
+    else if (dz1 > 0.0) d2 += dz1 * dz1;
+    return d2 <= fBall2;
+}
+
+// One thread per active bucket walks the threaded tree (pkdLocalWalk,
+// walk.c:58-178).  fill == 0: count the PP particles, softened cells and
+// Newtonian cells.  fill == 1: write them at the scanned offsets.
+__global__ void k_walk(int nb, const int *__restrict__ bucket,
+                       const double *__restrict__ bbox,
+                       const WalkNode *__restrict__ wn, int iRoot, int fill,
+                       int *__restrict__ cnt, const int64_t *__restrict__ off,
+                       int *__restrict__ ent)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int ib = bucket[b];
+    double bb[6];
+    for (int k = 0; k < 6; ++k) bb[k] = bbox[8 * (size_t)b + k];
+    const double fSoftMax = bbox[8 * (size_t)b + 6];
+    int nP = 0, nS = 0, nN = 0;
+    int64_t oP = 0, oS = 0, oN = 0;
+    if (fill) {
+        oP = off[3 * (size_t)b];
+        oS = off[3 * (size_t)b + 1];
+        oN = off[3 * (size_t)b + 2];
+    }
+    int ic = iRoot;
+    while (ic != -1) {
+        const WalkNode c = wn[ic];
+        bool open = box_within(bb, c.fOpen2, c.cx, c.cy, c.cz);
+        if (c.pUpper - c.pLower < 3) open = true;   // < 4 particles: open
+        if (open) {
+            if (c.iLower != -1) {
+                ic = c.iLower;
+                continue;
+            }
+            if (ic != ib) {
+                if (fill)
+                    for (int pj = c.pLower; pj <= c.pUpper; ++pj) ent[oP + nP++] = pj;
+                else
+                    nP += c.pUpper - c.pLower + 1;
+            }
+            ic = c.iNext;
+        } else {
+            double twoh2 = c.fSoft + fSoftMax;
+            twoh2 *= twoh2;
+            bool soft = (twoh2 < c.fOpen2) ? false : box_within(bb, twoh2, c.cx, c.cy, c.cz);
+            if (soft) {
+                if (fill) ent[oS + nS] = ic;
+                ++nS;
+            } else {
+                if (fill) ent[oN + nN] = ic;
+                ++nN;
+            }
+            ic = c.iNext;
+        }
+    }
+    if (!fill) {
+        cnt[3 * (size_t)b] = nP;
+        cnt[3 * (size_t)b + 1] = nS;
+        cnt[3 * (size_t)b + 2] = nN;
+    }
+}
+
+// Interaction counters of pkd.c:2946-2949 and the flop model of
+// grav.c:246-247 (QEVAL_FLOP, qeval.h:66), summed exactly in integers.
+__global__ void k_grav_stats(int nb, const int *__restrict__ bucket,
+                             const double *__restrict__ bbox,
+                             const WalkNode *__restrict__ wn,
+                             const int *__restrict__ cnt, int iOrder,
+                             unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long red[5][256];
+    const int qf[5] = {10, 10, 41, 120, 277};
+    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+         b += gridDim.x * blockDim.x) {
+        const WalkNode c = wn[bucket[b]];
+        const long long na = (long long)bbox[8 * (size_t)b + 7];
+        const long long n = c.pUpper - c.pLower + 1;
+        const long long nP = cnt[3 * (size_t)b], nS = cnt[3 * (size_t)b + 1],
+                        nN = cnt[3 * (size_t)b + 2];
+        v[0] += na;
+        v[1] += na * nP + na * (2 * (long long)(c.pUpper - c.pLower) - na + 1) / 2;
+        v[2] += na * nN;
+        v[3] += na * nS;
+        v[4] += na * ((nP + n) * 38 + nS * 82 + nN * (35 + qf[iOrder]));
+    }
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) atomicAdd(&out[k], red[k][0]);
+}
+
+// --------------------------------------------------------------- interact
+template <typename T> __device__ __forceinline__ T rsq(T x);
+template <> __device__ __forceinline__ float rsq<float>(float x)
+{
+    return __builtin_amdgcn_rsqf(x);
+}
+template <> __device__ __forceinline__ double rsq<double>(double x)
+{
+    return 1.0 / sqrt(x);
+}
+
+// Running sum of per-chunk partials: Kahan-compensated in FP32, plain in FP64.
+template <typename T> struct Sum {
+    T s = 0, c = 0;
+    __device__ __forceinline__ void add(T v)
+    {
+        if constexpr (sizeof(T) == 4) {
+            T y = v - c;
+            T t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+    __device__ __forceinline__ double value() const
+    {
+        return (double)s - (double)c;
+    }
+};
+
+template <typename T> struct Part {   // per-chunk partial sums of one target
+    T ax = 0, ay = 0, az = 0, pot = 0;
+};
+
+// Spline-softened monopole (SPLINEM, grav.h:50-71) with
+// twoh = h_i + h_j; Newtonian beyond twoh.
+template <typename T>
+__device__ __forceinline__ void spline_ab(T r2, T dir, T twoh, T &a, T &b)
+{
+    if (r2 < twoh * twoh) {
+        const T dih = T(2) / twoh;
+        T u;
+        if constexpr (sizeof(T) == 4) u = dih * (r2 * dir);
+        else u = dih / dir;
+        if (u < T(1)) {
+            a = dih * (T(7.0 / 5.0) - T(2.0 / 3.0) * u * u + T(3.0 / 10.0) * u * u * u * u
+                       - T(1.0 / 10.0) * u * u * u * u * u);
+            b = dih * dih * dih * (T(4.0 / 3.0) - T(6.0 / 5.0) * u * u + T(1.0 / 2.0) * u * u * u);
+        } else {
+            a = T(-1.0 / 15.0) * dir + dih * (T(8.0 / 5.0) - T(4.0 / 3.0) * u * u + u * u * u
+                       - T(3.0 / 10.0) * u * u * u * u + T(1.0 / 30.0) * u * u * u * u * u);
+            b = T(-1.0 / 15.0) * dir * dir * dir + dih * dih * dih * (T(8.0 / 3.0) - T(3) * u
+                       + T(6.0 / 5.0) * u * u - T(1.0 / 6.0) * u * u * u);
+        }
+    } else {
+        a = dir;
+        b = a * a * a;
+    }
+}
+
+// Spline-softened quadrupole (SPLINEQ, grav.h:17-47).
+template <typename T>
+__device__ __forceinline__ void spline_abcd(T r2, T dir, T twoh, T &a, T &b, T &c, T &d)
+{
+    if (r2 < twoh * twoh) {
+        const T dih = T(2) / twoh;
+        T u;
+        if constexpr (sizeof(T) == 4) u = dih * (r2 * dir);
+        else u = dih / dir;
+        const T dih2 = dih * dih, dih3 = dih2 * dih, dih5 = dih3 * dih2;
+        if (u < T(1)) {
+            a = dih * (T(7.0 / 5.0) - T(2.0 / 3.0) * u * u + T(3.0 / 10.0) * u * u * u * u
+                       - T(1.0 / 10.0) * u * u * u * u * u);
+            b = dih3 * (T(4.0 / 3.0) - T(6.0 / 5.0) * u * u + T(1.0 / 2.0) * u * u * u);
+            c = dih5 * (T(12.0 / 5.0) - T(3.0 / 2.0) * u);
+            d = T(3.0 / 2.0) * dih5 * dih * dir;
+        } else {
+            a = T(-1.0 / 15.0) * dir + dih * (T(8.0 / 5.0) - T(4.0 / 3.0) * u * u + u * u * u
+                       - T(3.0 / 10.0) * u * u * u * u + T(1.0 / 30.0) * u * u * u * u * u);
+            b = T(-1.0 / 15.0) * dir * dir * dir + dih3 * (T(8.0 / 3.0) - T(3) * u
+                       + T(6.0 / 5.0) * u * u - T(1.0 / 6.0) * u * u * u);
+            c = T(-1.0 / 5.0) * dir * dir * dir * dir * dir + T(3) * dih2 * dih2 * dir
+                + dih5 * (T(-12.0 / 5.0) + T(1.0 / 2.0) * u);
+            d = -dir * dir * dir * dir * dir * dir * dir + T(3) * dih2 * dih2 * dir * dir * dir
+                - T(1.0 / 2.0) * dih5 * dih * dir;
+        }
+    } else {
+        a = dir;
+        b = a * a * a;
+        c = T(3) * b * a * a;
+        d = T(5) * c * a * a;
+    }
+}
+
+// LDS staging of one chunk of 64 list entries, structure of arrays.
+// PP: x y z m h.  Softened cell: x y z m h Q(6, raw).  Newtonian cell:
+// x y z m Q(6, traceless) [O(10) H(15)].
+template <typename T, int ORDER> struct Stage {
+    static constexpr int NF = ORDER >= 4 ? 35 : (ORDER == 3 ? 20 : 11);
+    T f[NF][64];
+};
+
+struct IArgs {
+    int nb, iOrder;
+    const int *bucket;
+    const WalkNode *wn;
+    const double *cell, *momOH;
+    const double *px, *py, *pz, *pm, *ph;
+    const uint32_t *pact;
+    const int *cnt;
+    const int64_t *off;
+    const int *ent;
+    double *acc;   // per slot: ax ay az pot dtgrav
+};
+
+// One 64-lane workgroup per active bucket.  Lane = (target t, slice s):
+// TPW = next power of two >= bucket size targets, S = 64/TPW slices that
+// split each staged chunk; slice partials are reduced at the end.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void k_interact(IArgs A)
+{
+    __shared__ Stage<T, ORDER> L;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const int ib = A.bucket[b];
+    const WalkNode bn = A.wn[ib];
+    const int pL = bn.pLower, nT = bn.pUpper - bn.pLower + 1;
+    int TPW = 1;
+    while (TPW < nT) TPW <<= 1;
+    const int S = 64 / TPW;
+    const int t = lane & (TPW - 1), s = lane / TPW;
+    const bool valid = t < nT;
+    const int it = pL + (valid ? t : 0);
+    const bool act = valid && (A.pact[it] & 1u);
+    const double cx = bn.cx, cy = bn.cy, cz = bn.cz;     // bucket centre of mass
+    const T xi = (T)(A.px[it] - cx), yi = (T)(A.py[it] - cy), zi = (T)(A.pz[it] - cz);
+    const T hi = (T)A.ph[it], mi = (T)A.pm[it];
+    Sum<T> SX, SY, SZ, SP;
+    T dtg = 0;
+
+    // (0) the bucket's own particles, every pair i != j (grav.c:211-242)
+    if (lane < nT) {
+        L.f[0][lane] = (T)(A.px[pL + lane] - cx);
+        L.f[1][lane] = (T)(A.py[pL + lane] - cy);
+        L.f[2][lane] = (T)(A.pz[pL + lane] - cz);
+        L.f[3][lane] = (T)A.pm[pL + lane];
+        L.f[4][lane] = (T)A.ph[pL + lane];
+    }
+    __syncthreads();
+    if (act) {
+        Part<T> P;
+        for (int e = s; e < nT; e += S) {
+            if (e == t) continue;
+            const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+            const T mj = L.f[3][e];
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            T a, bb;
+            spline_ab<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb);
+            const T idt2 = (mi + mj) * bb;
+            dtg = idt2 > dtg ? idt2 : dtg;
+            a *= mj;
+            bb *= mj;
+            P.pot -= a;
+            P.ax -= dx * bb;
+            P.ay -= dy * bb;
+            P.az -= dz * bb;
+        }
+        SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
+    }
+    __syncthreads();
+
+    const size_t b3 = 3 * (size_t)b;
+    // (1) particle-particle list (grav.c:81-108)
+    {
+        const int n = A.cnt[b3];
+        const int *E = A.ent + A.off[b3];
+        for (int base = 0; base < n; base += 64) {
+            const int m = min(64, n - base);
+            if (lane < m) {
+                const int j = E[base + lane];
+                L.f[0][lane] = (T)(A.px[j] - cx);
+                L.f[1][lane] = (T)(A.py[j] - cy);
+                L.f[2][lane] = (T)(A.pz[j] - cz);
+                L.f[3][lane] = (T)A.pm[j];
+                L.f[4][lane] = (T)A.ph[j];
+            }
+            __syncthreads();
+            if (act) {
+                Part<T> P;
+                for (int e = s; e < m; e += S) {
+                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+                    const T mj = L.f[3][e];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    T a, bb;
+                    spline_ab<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb);
+                    const T idt2 = (mi + mj) * bb;
+                    dtg = idt2 > dtg ? idt2 : dtg;
+                    a *= mj;
+                    bb *= mj;
+                    P.pot -= a;
+                    P.ax -= dx * bb;
+                    P.ay -= dy * bb;
+                    P.az -= dz * bb;
+                }
+                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
+            }
+            __syncthreads();
+        }
+    }
+    // (2) softened cells: monopole + raw quadrupole (grav.c:117-150)
+    {
+        const int n = A.cnt[b3 + 1];
+        const int *E = A.ent + A.off[b3 + 1];
+        for (int base = 0; base < n; base += 64) {
+            const int m = min(64, n - base);
+            if (lane < m) {
+                const double *c = A.cell + (size_t)CR_N * E[base + lane];
+                L.f[0][lane] = (T)(c[CR_X] - cx);
+                L.f[1][lane] = (T)(c[CR_X + 1] - cy);
+                L.f[2][lane] = (T)(c[CR_X + 2] - cz);
+                L.f[3][lane] = (T)c[CR_M];
+                L.f[4][lane] = (T)c[CR_H];
+                for (int k = 0; k < 6; ++k) L.f[5 + k][lane] = (T)c[CR_QR + k];
+            }
+            __syncthreads();
+            if (act) {
+                Part<T> P;
+                for (int e = s; e < m; e += S) {
+                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+                    const T mc = L.f[3][e];
+                    const T xx = L.f[5][e], yy = L.f[6][e], zz = L.f[7][e];
+                    const T xy = L.f[8][e], xz = L.f[9][e], yz = L.f[10][e];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    T a, bb, c, d;
+                    spline_abcd<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb, c, d);
+                    const T qirx = xx * dx + xy * dy + xz * dz;
+                    const T qiry = xy * dx + yy * dy + yz * dz;
+                    const T qirz = xz * dx + yz * dy + zz * dz;
+                    const T qir = T(0.5) * (qirx * dx + qiry * dy + qirz * dz);
+                    const T tr = T(0.5) * (xx + yy + zz);
+                    const T qir3 = bb * mc + d * qir - c * tr;
+                    P.pot -= a * mc + c * qir - bb * tr;
+                    P.ax -= qir3 * dx - c * qirx;
+                    P.ay -= qir3 * dy - c * qiry;
+                    P.az -= qir3 * dz - c * qirz;
+                    const T idt2 = (mi + mc) * bb;
+                    dtg = idt2 > dtg ? idt2 : dtg;
+                }
+                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
+            }
+            __syncthreads();
+        }
+    }
+    // (3) Newtonian cells: reduced multipoles to ORDER (grav.c:164-191,
+    // QEVAL qeval.h:21-64)
+    {
+        const int n = A.cnt[b3 + 2];
+        const int *E = A.ent + A.off[b3 + 2];
+        for (int base = 0; base < n; base += 64) {
+            const int m = min(64, n - base);
+            if (lane < m) {
+                const int ic = E[base + lane];
+                const double *c = A.cell + (size_t)CR_N * ic;
+                L.f[0][lane] = (T)(c[CR_X] - cx);
+                L.f[1][lane] = (T)(c[CR_X + 1] - cy);
+                L.f[2][lane] = (T)(c[CR_X + 2] - cz);
+                L.f[3][lane] = (T)c[CR_M];
+                for (int k = 0; k < 6; ++k) L.f[4 + k][lane] = (T)c[CR_QT + k];
+                if constexpr (ORDER >= 3) {
+                    const double *oh = A.momOH + (size_t)OH_N * ic;
+                    for (int k = 0; k < 10; ++k) L.f[10 + k][lane] = (T)oh[k];
+                    if constexpr (ORDER >= 4)
+                        for (int k = 0; k < 15; ++k) L.f[20 + k][lane] = (T)oh[10 + k];
+                }
+            }
+            __syncthreads();
+            if (act) {
+                Part<T> P;
+                for (int e = s; e < m; e += S) {
+                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+                    const T mc = L.f[3][e];
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    T g[6];
+                    g[0] = rsq<T>(r2);
+                    const T dir2 = g[0] * g[0];
+                    g[1] = g[0] * dir2;
+                    g[2] = T(3) * g[1] * dir2;
+                    g[3] = T(5) * g[2] * dir2;
+                    g[4] = T(7) * g[3] * dir2;
+                    g[5] = T(9) * g[4] * dir2;
+                    T Qx, Qy, Qz, Qm, Qta = 0;
+                    if constexpr (ORDER >= 4) {
+                        const T *H = nullptr;
+                        (void)H;
+                        const T xxxx = L.f[20][e], xyyy = L.f[21][e], xxxy = L.f[22][e],
+                                yyyy = L.f[23][e], xxxz = L.f[24][e], yyyz = L.f[25][e],
+                                xxyy = L.f[26][e], xxyz = L.f[27][e], xyyz = L.f[28][e],
+                                xxzz = L.f[29][e], xyzz = L.f[30][e], xzzz = L.f[31][e],
+                                yyzz = L.f[32][e], yzzz = L.f[33][e], zzzz = L.f[34][e];
+                        Qx = T(1.0 / 6.0) * (xzzz * dz * dz * dz + T(3) * xyzz * dy * dz * dz
+                             + T(3) * xyyz * dy * dy * dz + xyyy * dy * dy * dy
+                             + T(3) * xxzz * dx * dz * dz + T(6) * xxyz * dx * dy * dz
+                             + T(3) * xxyy * dx * dy * dy + T(3) * xxxz * dx * dx * dz
+                             + T(3) * xxxy * dx * dx * dy + xxxx * dx * dx * dx);
+                        Qy = T(1.0 / 6.0) * (yzzz * dz * dz * dz + T(3) * xyzz * dx * dz * dz
+                             + T(3) * xxyz * dx * dx * dz + xxxy * dx * dx * dx
+                             + T(3) * yyzz * dy * dz * dz + T(6) * xyyz * dx * dy * dz
+                             + T(3) * xxyy * dx * dx * dy + T(3) * yyyz * dy * dy * dz
+                             + T(3) * xyyy * dx * dy * dy + yyyy * dy * dy * dy);
+                        Qz = T(1.0 / 6.0) * (yyyz * dy * dy * dy + T(3) * xyyz * dx * dy * dy
+                             + T(3) * xxyz * dx * dx * dy + xxxz * dx * dx * dx
+                             + T(3) * yyzz * dy * dy * dz + T(6) * xyzz * dx * dy * dz
+                             + T(3) * xxzz * dx * dx * dz + T(3) * yzzz * dy * dz * dz
+                             + T(3) * xzzz * dx * dz * dz + zzzz * dz * dz * dz);
+                        Qm = T(1.0 / 4.0) * (Qx * dx + Qy * dy + Qz * dz);
+                        P.pot -= g[4] * Qm;
+                        Qta += g[5] * Qm;
+                        P.ax += g[4] * Qx;
+                        P.ay += g[4] * Qy;
+                        P.az += g[4] * Qz;
+                    }
+                    if constexpr (ORDER >= 3) {
+                        const T xxx = L.f[10][e], xyy = L.f[11][e], xxy = L.f[12][e],
+                                yyy = L.f[13][e], xxz = L.f[14][e], yyz = L.f[15][e],
+                                xyz = L.f[16][e], xzz = L.f[17][e], yzz = L.f[18][e],
+                                zzz = L.f[19][e];
+                        Qx = T(0.5) * (xzz * dz * dz + T(2) * xyz * dy * dz + xyy * dy * dy
+                             + T(2) * xxz * dx * dz + T(2) * xxy * dx * dy + xxx * dx * dx);
+                        Qy = T(0.5) * (yzz * dz * dz + T(2) * xyz * dx * dz + xxy * dx * dx
+                             + T(2) * yyz * dy * dz + T(2) * xyy * dx * dy + yyy * dy * dy);
+                        Qz = T(0.5) * (yyz * dy * dy + T(2) * xyz * dx * dy + xxz * dx * dx
+                             + T(2) * yzz * dy * dz + T(2) * xzz * dx * dz + zzz * dz * dz);
+                        Qm = T(1.0 / 3.0) * (Qx * dx + Qy * dy + Qz * dz);
+                        P.pot -= g[3] * Qm;
+                        Qta += g[4] * Qm;
+                        P.ax += g[3] * Qx;
+                        P.ay += g[3] * Qy;
+                        P.az += g[3] * Qz;
+                    }
+                    if constexpr (ORDER >= 2) {
+                        const T xx = L.f[4][e], yy = L.f[5][e], zz = L.f[6][e];
+                        const T xy = L.f[7][e], xz = L.f[8][e], yz = L.f[9][e];
+                        Qx = xz * dz + xy * dy + xx * dx;
+                        Qy = yz * dz + xy * dx + yy * dy;
+                        Qz = yz * dy + xz * dx + zz * dz;
+                        Qm = T(0.5) * (Qx * dx + Qy * dy + Qz * dz);
+                        P.pot -= g[2] * Qm;
+                        Qta += g[3] * Qm;
+                        P.ax += g[2] * Qx;
+                        P.ay += g[2] * Qy;
+                        P.az += g[2] * Qz;
+                    }
+                    P.pot -= g[0] * mc;
+                    Qta += g[1] * mc;
+                    P.ax -= dx * Qta;
+                    P.ay -= dy * Qta;
+                    P.az -= dz * Qta;
+                    const T idt2 = (mi + mc) * g[1];
+                    dtg = idt2 > dtg ? idt2 : dtg;
+                }
+                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
+            }
+            __syncthreads();
+        }
+    }
+    // reduce the S slices of each target (lanes t, t+TPW, ...)
+    double ax = SX.value(), ay = SY.value(), az = SZ.value(), pot = SP.value();
+    double dg = (double)dtg;
+    for (int o = TPW; o < 64; o <<= 1) {
+        ax += __shfl_xor(ax, o);
+        ay += __shfl_xor(ay, o);
+        az += __shfl_xor(az, o);
+        pot += __shfl_xor(pot, o);
+        const double od = __shfl_xor(dg, o);
+        dg = od > dg ? od : dg;
+    }
+    if (act && s == 0) {
+        double *o = A.acc + 5 * (size_t)it;
+        o[0] = ax;
+        o[1] = ay;
+        o[2] = az;
+        o[3] = pot;
+        o[4] = dg;
+    }
+}
+
+// Add the per-slot results to the caller-order fields of active particles.
+__global__ void k_grav_scatter(int n, const int *__restrict__ perm,
+                               const uint32_t *__restrict__ pact,
+                               const double *__restrict__ acc,
+                               double *__restrict__ a, double *__restrict__ pot,
+                               double *__restrict__ dtg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(pact[i] & 1u)) return;
+    const int c = perm[i];
+    const double *o = acc + 5 * (size_t)i;
+    a[3 * (size_t)c] += o[0];
+    a[3 * (size_t)c + 1] += o[1];
+    a[3 * (size_t)c + 2] += o[2];
+    pot[c] += o[3];
+    if (o[4] > dtg[c]) dtg[c] = o[4];
+}
+
+// ------------------------------------------------------------- host side
+int grav_upload(GGPKD pkd)
+{
+    GravDev &G = pkd->grav;
+    HostGravTree &t = G.host;
+    const int n = t.n, nn = t.nNodes;
+    hipStream_t st = pkd->stream;
+    GG_TRY(G.px.reserve(sizeof(double) * std::max(n, 1)));
+    GG_TRY(G.py.reserve(sizeof(double) * std::max(n, 1)));
+    GG_TRY(G.pz.reserve(sizeof(double) * std::max(n, 1)));
+    GG_TRY(G.pm.reserve(sizeof(double) * std::max(n, 1)));
+    GG_TRY(G.ph.reserve(sizeof(double) * std::max(n, 1)));
+    GG_TRY(G.pact.reserve(sizeof(uint32_t) * std::max(n, 1)));
+    GG_TRY(G.perm.reserve(sizeof(int) * std::max(n, 1)));
+    GG_TRY(G.acc.reserve(sizeof(double) * 5 * std::max(n, 1)));
+    GG_TRY(G.wnode.reserve(sizeof(WalkNode) * nn));
+    GG_TRY(G.cell.reserve(sizeof(double) * CR_N * nn));
+    if (n > 0) {
+        GG_HIP(hipMemcpyAsync(G.px.p, t.x.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.py.p, t.y.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.pz.p, t.z.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.pm.p, t.m.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.ph.p, t.h.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.pact.p, t.act.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.perm.p, t.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    }
+    std::vector<WalkNode> wn(nn);
+    std::vector<double> cell((size_t)CR_N * nn, 0.0);
+    std::vector<double> oh;
+    if (t.iOrder >= 3) oh.assign((size_t)OH_N * nn, 0.0);
+    for (int c = 0; c < nn; ++c) {
+        WalkNode &w = wn[c];
+        w.cx = t.com[3 * c];
+        w.cy = t.com[3 * c + 1];
+        w.cz = t.com[3 * c + 2];
+        w.fOpen2 = t.fOpen2[c];
+        w.fSoft = t.soft[c];
+        w.pLower = t.pLower[c];
+        w.pUpper = t.pUpper[c];
+        w.iLower = t.iLower[c];
+        w.iNext = t.iNext[c];
+        w.pad0 = w.pad1 = 0;
+        double *r = &cell[(size_t)CR_N * c];
+        r[CR_M] = t.mass[c];
+        for (int d = 0; d < 3; ++d) r[CR_X + d] = t.com[3 * c + d];
+        r[CR_H] = t.soft[c];
+        for (int k = 0; k < 6; ++k) {
+            r[CR_QR + k] = t.mom[(size_t)MOM_N * c + MOM_QR + k];
+            r[CR_QT + k] = t.mom[(size_t)MOM_N * c + MOM_QT + k];
+        }
+        if (t.iOrder >= 3)
+            for (int k = 0; k < 25; ++k) oh[(size_t)OH_N * c + k] = t.mom[(size_t)MOM_N * c + MOM_O + k];
+    }
+    GG_HIP(hipMemcpyAsync(G.wnode.p, wn.data(), sizeof(WalkNode) * nn, hipMemcpyHostToDevice, st));
+    GG_HIP(hipMemcpyAsync(G.cell.p, cell.data(), sizeof(double) * CR_N * nn, hipMemcpyHostToDevice, st));
+    if (t.iOrder >= 3) {
+        GG_TRY(G.momOH.reserve(sizeof(double) * OH_N * nn));
+        GG_HIP(hipMemcpyAsync(G.momOH.p, oh.data(), sizeof(double) * OH_N * nn, hipMemcpyHostToDevice, st));
+    }
+    // active buckets, their active-particle bounds and maximum softening
+    // (pkd.c:2916-2937, walk.c:319-324)
+    std::vector<int> bk;
+    std::vector<double> bb;
+    for (int c = 0; c < nn; ++c) {
+        if (t.iDim[c] != -1 || t.pLower[c] > t.pUpper[c]) continue;
+        double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+        double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+        int na = 0;
+        double fsm = 0.0;
+        for (int i = t.pLower[c]; i <= t.pUpper[c]; ++i) {
+            if (t.h[i] > fsm) fsm = t.h[i];
+            if (!(t.act[i] & 1u)) continue;
+            ++na;
+            const double r[3] = {t.x[i], t.y[i], t.z[i]};
+            for (int d = 0; d < 3; ++d) {
+                if (r[d] < lo[d]) lo[d] = r[d];
+                if (r[d] > hi[d]) hi[d] = r[d];
+            }
+        }
+        if (na == 0) continue;
+        bk.push_back(c);
+        for (int d = 0; d < 3; ++d) bb.push_back(lo[d]);
+        for (int d = 0; d < 3; ++d) bb.push_back(hi[d]);
+        bb.push_back(fsm);
+        bb.push_back((double)na);
+    }
+    G.nb = (int)bk.size();
+    // the interaction kernel maps one bucket onto one 64-lane wavefront
+    for (int c : bk)
+        GG_CHECK(t.pUpper[c] - t.pLower[c] + 1 <= 64, GG_EUNSUP,
+                 "bucket %d holds %d particles (> 64): coincident positions?", c,
+                 t.pUpper[c] - t.pLower[c] + 1);
+    GG_TRY(G.bucket.reserve(sizeof(int) * std::max(G.nb, 1)));
+    GG_TRY(G.bbox.reserve(sizeof(double) * 8 * std::max(G.nb, 1)));
+    GG_TRY(G.cnt.reserve(sizeof(int) * 3 * std::max(G.nb, 1)));
+    GG_TRY(G.off.reserve(sizeof(int64_t) * (3 * (size_t)std::max(G.nb, 1) + 1)));
+    if (G.nb) {
+        GG_HIP(hipMemcpyAsync(G.bucket.p, bk.data(), sizeof(int) * G.nb, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.bbox.p, bb.data(), sizeof(double) * 8 * G.nb, hipMemcpyHostToDevice, st));
+    }
+    GG_HIP(hipStreamSynchronize(st));
+    G.n = n;
+    G.nNodes = nn;
+    G.iRoot = t.iRoot;
+    G.iOrder = t.iOrder;
+    G.nBucket = t.nBucket;
+    G.built = 1;
+    return GG_OK;
+}
+
+template <typename T>
+static int launch_interact(int order, const IArgs &A, hipStream_t st)
+{
+    dim3 grid(A.nb), block(64);
+    switch (order) {
+    case 1: hipLaunchKernelGGL((k_interact<T, 1>), grid, block, 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_interact<T, 2>), grid, block, 0, st, A); break;
+    case 3: hipLaunchKernelGGL((k_interact<T, 3>), grid, block, 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_interact<T, 4>), grid, block, 0, st, A); break;
+    default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
+    }
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCrit,
+                               int iOrder, int bTreeActiveOnly, int bGravity)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GG_CHECK(nBucket >= 1 && nBucket <= 64, GG_EINVAL, "nBucket %d outside 1..64", nBucket);
+    GG_CHECK(iOpenType == 0 || iOpenType == 1, GG_EUNSUP,
+             "iOpenType %d: only OPEN_JOSH (1) and Bmax (0) are implemented", iOpenType);
+    GG_CHECK(iOrder >= 1 && iOrder <= 4, GG_EINVAL, "iOrder %d outside 1..4", iOrder);
+    GG_CHECK(bGravity == 1, GG_EUNSUP, "gg_build_binary builds gravity trees only");
+    GG_CHECK(bTreeActiveOnly == 0, GG_EUNSUP, "bTreeActiveOnly is not supported");
+    GG_CHECK(dCrit > 0, GG_EINVAL, "dCrit must be positive");
+    const int n = pkd->nLocal;
+    GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
+             "positions, masses and softenings must be put before gg_build_binary");
+    GG_TRY(ensure_field(pkd, GG_IACTIVE));
+    std::vector<double> r(3 * (size_t)n), m(n), h(n);
+    std::vector<uint32_t> bits(n);
+    hipStream_t st = pkd->stream;
+    if (n > 0) {
+        GG_HIP(hipMemcpyAsync(r.data(), pkd->f[GG_R].p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(m.data(), pkd->f[GG_MASS].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(h.data(), pkd->f[GG_SOFT].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(bits.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+    }
+    GG_TRY(build_binary_host(pkd->grav.host, n, r.data(), m.data(), h.data(), bits.data(),
+                             nBucket, iOpenType, dCrit, iOrder));
+    return grav_upload(pkd);
+}
+
+extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
+                           int iEwOrder, double fEwCut, double fEwhCut, int bComove,
+                           double dRhoFac, int bDoSun, double dSunSoft, double *aSun,
+                           int *nActive, double *pdPartSum, double *pdCellSum,
+                           double *pdSoftSum, double *pdFlop)
+{
+    (void)iEwOrder; (void)fEwCut; (void)fEwhCut; (void)dRhoFac; (void)dSunSoft; (void)aSun;
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.built, GG_ESTATE, "gg_grav_all before gg_build_binary");
+    GG_CHECK(G.n == pkd->nLocal, GG_ESTATE, "particle count changed since the tree was built");
+    GG_CHECK(nReps == 0 && !bPeriodic && !bEwald, GG_EUNSUP,
+             "periodic replicas and Ewald summation are outside this library");
+    GG_CHECK(!bComove && !bDoSun, GG_EUNSUP, "comoving and solar-system terms are not supported");
+    GG_CHECK(iOrder >= 1 && iOrder <= G.iOrder, GG_EINVAL,
+             "iOrder %d exceeds the tree's moments (%d)", iOrder, G.iOrder);
+    hipStream_t st = pkd->stream;
+    GG_TRY(ensure_field(pkd, GG_A));
+    GG_TRY(ensure_field(pkd, GG_POT));
+    GG_TRY(ensure_field(pkd, GG_DTGRAV));
+    const int nb = G.nb;
+    unsigned long long hs[5] = {0, 0, 0, 0, 0};
+    if (nb > 0) {
+        timer_mark(pkd, 0);
+        // walk: count, scan, fill
+        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
+        GG_HIP(hipGetLastError());
+        int64_t total = 0;
+        GG_TRY(exclusive_scan_i64(st, G.cnt.as<int>(), G.off.as<int64_t>(), 3 * nb, G.scan, &total));
+        G.nEnt = total;
+        GG_TRY(G.ent.reserve(sizeof(int) * std::max<int64_t>(total, 1)));
+        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
+        GG_HIP(hipGetLastError());
+        GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
+        GG_HIP(hipMemsetAsync(G.stats.p, 0, sizeof(unsigned long long) * 5, st));
+        hipLaunchKernelGGL(k_grav_stats, dim3(std::min((nb + 255) / 256, 1024)), dim3(256), 0, st,
+                           nb, G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
+        GG_HIP(hipGetLastError());
+        timer_mark(pkd, 1);
+        IArgs A;
+        A.nb = nb;
+        A.iOrder = iOrder;
+        A.bucket = G.bucket.as<int>();
+        A.wn = G.wnode.as<WalkNode>();
+        A.cell = G.cell.as<double>();
+        A.momOH = G.momOH.as<double>();
+        A.px = G.px.as<double>();
+        A.py = G.py.as<double>();
+        A.pz = G.pz.as<double>();
+        A.pm = G.pm.as<double>();
+        A.ph = G.ph.as<double>();
+        A.pact = G.pact.as<uint32_t>();
+        A.cnt = G.cnt.as<int>();
+        A.off = G.off.as<int64_t>();
+        A.ent = G.ent.as<int>();
+        A.acc = G.acc.as<double>();
+        if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_interact<float>(iOrder, A, st));
+        else GG_TRY(launch_interact<double>(iOrder, A, st));
+        timer_mark(pkd, 2);
+        hipLaunchKernelGGL(k_grav_scatter, dim3((G.n + 255) / 256), dim3(256), 0, st, G.n,
+                           G.perm.as<int>(), G.pact.as<uint32_t>(), G.acc.as<double>(),
+                           pkd->f[GG_A].as<double>(), pkd->f[GG_POT].as<double>(),
+                           pkd->f[GG_DTGRAV].as<double>());
+        GG_HIP(hipGetLastError());
+        timer_mark(pkd, 3);
+        GG_HIP(hipMemcpyAsync(hs, G.stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        const int phase[3] = {0, 1, 7};
+        timer_collect(pkd, 0, 3, phase);
+    }
+    if (nActive) *nActive = (int)hs[0];
+    if (pdPartSum) *pdPartSum = (double)hs[1];
+    if (pdCellSum) *pdCellSum = (double)hs[2];
+    if (pdSoftSum) *pdSoftSum = (double)hs[3];
+    if (pdFlop) *pdFlop = (double)hs[4];
+    return GG_OK;
+}
+
+extern "C" int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucket_node,
+                             int64_t *off, int *cnt, int *ent)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.built, GG_ESTATE, "no gravity tree");
+    if (nBuckets) *nBuckets = G.nb;
+    if (nEnt) *nEnt = G.nEnt;
+    if (!ent) return GG_OK;
+    hipStream_t st = pkd->stream;
+    if (G.nb) {
+        GG_HIP(hipMemcpyAsync(bucket_node, G.bucket.p, sizeof(int) * G.nb, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(off, G.off.p, sizeof(int64_t) * 3 * G.nb, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(cnt, G.cnt.p, sizeof(int) * 3 * G.nb, hipMemcpyDeviceToHost, st));
+    }
+    if (G.nEnt) GG_HIP(hipMemcpyAsync(ent, G.ent.p, sizeof(int) * G.nEnt, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    return GG_OK;
+}
+
+extern "C" int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
+                            double *dbls)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.built, GG_ESTATE, "no gravity tree");
+    const HostGravTree &t = G.host;
+    if (nNodes) *nNodes = t.nNodes;
+    if (iRoot) *iRoot = t.iRoot;
+    if (perm) std::memcpy(perm, t.perm.data(), sizeof(int) * t.n);
+    if (ints)
+        for (int c = 0; c < t.nNodes; ++c) {
+            ints[6 * c] = t.iDim[c];
+            ints[6 * c + 1] = t.pLower[c];
+            ints[6 * c + 2] = t.pUpper[c];
+            ints[6 * c + 3] = t.iLower[c];
+            ints[6 * c + 4] = t.iNext[c];
+            ints[6 * c + 5] = 0;
+        }
+    if (dbls)
+        for (int c = 0; c < t.nNodes; ++c) {
+            double *o = dbls + 8 * (size_t)c;
+            o[0] = t.mass[c];
+            o[1] = t.soft[c];
+            o[2] = t.com[3 * c];
+            o[3] = t.com[3 * c + 1];
+            o[4] = t.com[3 * c + 2];
+            o[5] = t.fOpen2[c];
+            o[6] = t.bmaxr[c];
+            o[7] = t.fSplit[c];
+        }
+    return GG_OK;
+}

@@ -1,0 +1,124 @@
+// Internal declarations shared by the translation units of libgasoline_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/gasoline_amd.h"
+#include "gg_tree.h"
+
+namespace gg {
+
+void set_error(const char *fmt, ...);
+
+#define GG_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            ::gg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,      \
+                            hipGetErrorString(e_));                         \
+            return GG_EHIP;                                                 \
+        }                                                                   \
+    } while (0)
+
+#define GG_CHECK(cond, code, ...)                                           \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            ::gg::set_error(__VA_ARGS__);                                   \
+            return (code);                                                  \
+        }                                                                   \
+    } while (0)
+
+#define GG_TRY(expr)                                                        \
+    do {                                                                    \
+        int r_ = (expr);                                                    \
+        if (r_ != GG_OK) return r_;                                         \
+    } while (0)
+
+// Device buffer that only grows; contents are not preserved on growth.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t nbytes);
+    void release();
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Gravity state on the device.
+struct GravDev {
+    int built = 0, n = 0, nNodes = 0, iRoot = -1, iOrder = 2, nBucket = 0;
+    int nb = 0;            // active buckets
+    int64_t nEnt = 0;      // list entries of the last walk
+    DevBuf px, py, pz, pm, ph, pact, perm;    // storage-order particles
+    DevBuf wnode;          // WalkNode per node
+    DevBuf cell;           // CellRec per node
+    DevBuf momOH;          // O,H per node (iOrder > 2)
+    DevBuf bucket;         // active bucket ids
+    DevBuf bbox;           // per active bucket: 6 bounds + fSoftMax + nActive
+    DevBuf cnt, off;       // per active bucket: 3 counts, 3 offsets
+    DevBuf ent;            // list entries
+    DevBuf acc;            // per slot: ax ay az pot dtg (FP64)
+    DevBuf stats;          // reduction scratch
+    DevBuf scan;           // scan scratch
+    HostGravTree host;
+};
+
+// SPH state on the device (search structure, lists and per-call scratch).
+struct SphDev {
+    int built = 0, n = 0, bPeriodic = 0, nLeaf = 0, nL2 = 1, nBucket = 0;
+    double L[3] = {0, 0, 0};
+    int listK = 0, listValid = 0, listQueries = 0;
+    unsigned ties = 0, fails = 0;
+    DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
+    DevBuf sx, sy, sz, nodeBox;                // slot order positions, boxes
+    DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
+    DevBuf gcnt, goff, gent;                   // ball-gather CSR
+    DevBuf rcnt, roff, rcur, ridx;             // reverse lists
+    DevBuf qf, act, nba, err, guess, diffcnt;
+    DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
+    DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
+    void release_all()
+    {
+        DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
+                         &sx, &sy, &sz, &nodeBox, &nbr, &fb2s, &fb2knn, &qfknn, &off,
+                         &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &qf, &act, &nba,
+                         &err, &guess, &diffcnt, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
+                         &bal, &dif, &dt, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
+                         &o_bal, &o_diff, &res};
+        for (DevBuf *b : all) b->release();
+    }
+};
+
+struct Timers {
+    hipEvent_t ev[16];
+    int created = 0;
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+}  // namespace gg
+
+struct gg_pkd_s {
+    int device = 0;
+    int nStore = 0, nLocal = 0;
+    int prec = GG_PREC_FP32;
+    hipStream_t stream = nullptr;
+    gg::DevBuf f[GG_NFIELDS];      // caller-order fields
+    gg::GravDev grav;
+    gg::SphDev sph;
+    gg::Timers tm;
+};
+
+namespace gg {
+int field_width(int field);                 // doubles per particle (0: uint32)
+int ensure_field(GGPKD pkd, int field);     // allocate (zeroed) if missing
+// out[0..n]: exclusive prefix sums, out[n] = total (out holds n+1 entries)
+int exclusive_scan_i64(hipStream_t s, const int *in, int64_t *out, int n,
+                       DevBuf &scratch, int64_t *total_host);
+int grav_upload(GGPKD pkd);
+void sph_release(GGPKD pkd);
+void timer_mark(GGPKD pkd, int slot);
+void timer_collect(GGPKD pkd, int first_slot, int nphases, const int *phase);
+}  // namespace gg

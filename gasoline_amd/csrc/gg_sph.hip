@@ -1,0 +1,1354 @@
+// SPH neighbour loop on the GPU: search structure, exact kNN, DenDVDX,
+// equation of state, ball gather and the symmetric pressure/viscosity force
+// evaluated in gather form.
+//
+// Reference path: msrSph (master.c:9258-9362) -> pkdBuildLocal
+// (pkd.c:2732-2848) -> smSmooth + DenDVDX (smooth.c:1439-1713,
+// smoothfcn.c:3390-3627) -> pkdGasPressure (pkd.c:5750-5785) -> smReSmooth
+// + SphPressureTermsSym (smooth.c:1716-1749, smoothfcn.c:2276-2385,
+// SphPressureTerms.h).
+//
+// Search structure: gas particles sorted by 63-bit Morton key, cut into
+// leaves of 32 consecutive particles, with a complete binary hierarchy of
+// FP64 bounding boxes over the leaves (heap layout, root = 1).  The
+// neighbour SET of a particle is fixed by FP64 distances alone, so it does
+// not depend on this structure.
+#include "gg_internal.h"
+
+#include <cstring>
+#include <algorithm>
+#include <cmath>
+#include <cfloat>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace gg {
+
+constexpr int LEAF = 32;          // particles per leaf
+constexpr int FCAP = 1024;        // BFS frontier capacity (nodes)
+constexpr int LCAP = 512;         // leaves per query capacity
+constexpr int CAP = 256;          // candidate buffer (4 per lane)
+constexpr int MAXTRY = 48;        // radius adjustments per query
+constexpr double GG_FLT_MAX = (double)3.402823466E+38F;   // floattype.h:9
+
+// ------------------------------------------------------------ geometry
+// Separation p - q, minimum periodic image chosen per axis, image applied to
+// the query point first (sx = x +/- lx, INTERSECT smooth.h:291-390).
+__device__ __forceinline__ double img_sep(double p, double q, int per, double L)
+{
+#pragma clang fp contract(off)
+    double s = p;
+    if (per) {
+        const double t = p - q;
+        if (t > 0.5 * L) s = p - L;
+        else if (t < -0.5 * L) s = p + L;
+    }
+    return s - q;
+}
+
+__device__ __forceinline__ double sq_dist(double dx, double dy, double dz)
+{
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Squared distance from a point to a box, minimised over periodic images.
+__device__ __forceinline__ double box_d2(const double *b, double x, double y,
+                                         double z, int per, const double *L)
+{
+    const double p[3] = {x, y, z};
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double lo = b[k], hi = b[3 + k];
+        double d = p[k] < lo ? lo - p[k] : (p[k] > hi ? p[k] - hi : 0.0);
+        if (per) {
+            const double a = p[k] + L[k], c = p[k] - L[k];
+            const double da = a < lo ? lo - a : (a > hi ? a - hi : 0.0);
+            const double dc = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+            d = fmin(d, fmin(da, dc));
+        }
+        d2 += d * d;
+    }
+    return d2;
+}
+
+// ---------------------------------------------------- structure kernels
+__global__ void k_select(int n, const uint32_t *__restrict__ bits, uint32_t mask,
+                         int *__restrict__ sel)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sel[i] = ((bits[i] & mask) == mask) ? 1 : 0;
+}
+
+__global__ void k_compact(int n, const int *__restrict__ sel,
+                          const int64_t *__restrict__ pos, int *__restrict__ gidx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && sel[i]) gidx[pos[i]] = i;
+}
+
+__global__ void k_bbox(int n, const int *__restrict__ gidx, const double *__restrict__ r,
+                       double *__restrict__ part)
+{
+    __shared__ double s[6][256];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int i = gidx[k];
+        for (int d = 0; d < 3; ++d) {
+            const double v = r[3 * (size_t)i + d];
+            lo[d] = fmin(lo[d], v);
+            hi[d] = fmax(hi[d], v);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = lo[d];
+        s[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+                s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; ++d) part[6 * (size_t)blockIdx.x + d] = s[d][0];
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ void k_morton(int n, const int *__restrict__ gidx, const double *__restrict__ r,
+                         double lx, double ly, double lz, double sx, double sy, double sz,
+                         uint64_t *__restrict__ key, int *__restrict__ val)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = gidx[k];
+    const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
+    uint64_t q[3];
+    for (int d = 0; d < 3; ++d) {
+        double t = (r[3 * (size_t)i + d] - lo[d]) * sc[d];
+        t = fmin(fmax(t, 0.0), 2097151.0);
+        q[d] = (uint64_t)t;
+    }
+    key[k] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+    val[k] = i;
+}
+
+__global__ void k_gather_pos(int n, const int *__restrict__ order, const double *__restrict__ r,
+                             double *__restrict__ sx, double *__restrict__ sy,
+                             double *__restrict__ sz)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    sx[s] = r[3 * (size_t)i];
+    sy[s] = r[3 * (size_t)i + 1];
+    sz[s] = r[3 * (size_t)i + 2];
+}
+
+__global__ void k_leaf_box(int n, int nLeaf, int nL2, const double *__restrict__ sx,
+                           const double *__restrict__ sy, const double *__restrict__ sz,
+                           double *__restrict__ box)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nL2) return;
+    double *b = box + 6 * (size_t)(nL2 + k);
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (k < nLeaf) {
+        const int s0 = k * LEAF, s1 = min(s0 + LEAF, n);
+        for (int s = s0; s < s1; ++s) {
+            lo[0] = fmin(lo[0], sx[s]); hi[0] = fmax(hi[0], sx[s]);
+            lo[1] = fmin(lo[1], sy[s]); hi[1] = fmax(hi[1], sy[s]);
+            lo[2] = fmin(lo[2], sz[s]); hi[2] = fmax(hi[2], sz[s]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        b[d] = lo[d];
+        b[3 + d] = hi[d];
+    }
+}
+
+__global__ void k_level_box(int first, int count, double *__restrict__ box)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int i = first + k;
+    double *b = box + 6 * (size_t)i;
+    const double *l = box + 6 * (size_t)(2 * i), *u = box + 6 * (size_t)(2 * i + 1);
+    for (int d = 0; d < 3; ++d) {
+        b[d] = fmin(l[d], u[d]);
+        b[3 + d] = fmax(l[3 + d], u[3 + d]);
+    }
+}
+
+// -------------------------------------------------------- kNN / gather
+struct QArgs {
+    int n, nLeaf, nL2, K, per, mode;   // mode 0 kNN, 1 gather count, 2 gather fill
+    double L[3];
+    const double *sx, *sy, *sz, *box;
+    const int *order;                  // slot -> caller (tie key)
+    const uint32_t *qflag;             // slot order: nonzero = query
+    const double *guess;               // slot order: fBall2 guess (<= 0: none)
+    double *fb2;                       // kNN out / gather in (slot order)
+    int *nbr;                          // kNN: K per slot; fill: CSR entries
+    int *cnt;                          // gather count out
+    const int64_t *off;                // gather fill offsets
+    unsigned int *err;                 // [0] failures, [1] boundary ties
+};
+
+__device__ __forceinline__ bool kv_less(double da, int ca, double db, int cb)
+{
+    return da < db || (da == db && ca < cb);
+}
+
+// Bitonic sort of 256 (d2, caller, slot) triples held 4 per lane
+// (element e = lane + 64 k), ascending in (d2, caller).
+__device__ void bitonic256(double (&d)[4], int (&c)[4], int (&s)[4], int lane)
+{
+    for (int size = 2; size <= 256; size <<= 1) {
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int kk = j >> 6;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int k2 = k ^ kk;
+                    if (k2 <= k) continue;
+                    const int e = lane + 64 * k;
+                    const bool asc = (e & size) == 0;
+                    const bool sw = asc ? kv_less(d[k2], c[k2], d[k], c[k])
+                                        : kv_less(d[k], c[k], d[k2], c[k2]);
+                    if (sw) {
+                        const double td = d[k]; d[k] = d[k2]; d[k2] = td;
+                        const int tc = c[k]; c[k] = c[k2]; c[k2] = tc;
+                        const int ts = s[k]; s[k] = s[k2]; s[k2] = ts;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int e = lane + 64 * k;
+                    const double pd = __shfl_xor(d[k], j);
+                    const int pc = __shfl_xor(c[k], j);
+                    const int ps = __shfl_xor(s[k], j);
+                    const bool lower = (lane & j) == 0;
+                    const bool asc = (e & size) == 0;
+                    const bool take = (lower == asc) ? kv_less(pd, pc, d[k], c[k])
+                                                     : kv_less(d[k], c[k], pd, pc);
+                    if (take) {
+                        d[k] = pd;
+                        c[k] = pc;
+                        s[k] = ps;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// One 64-lane workgroup per query slot.  mode 0: exact kNN by (d2, caller
+// index) with a radius that is grown or shrunk until the ball holds between
+// K and CAP particles; mode 1/2: ball gather at fBall2 (d2 <= fBall2).
+__global__ __launch_bounds__(64) void k_query(QArgs A)
+{
+    __shared__ int fr[2][FCAP];
+    __shared__ int lf[LCAP];
+    __shared__ double cd[CAP];
+    __shared__ int cc[CAP], cs[CAP];
+    const int lane = threadIdx.x;
+    const int q = blockIdx.x;
+    if (!A.qflag[q]) {
+        if (A.mode == 1 && lane == 0) A.cnt[q] = 0;
+        return;
+    }
+    const double x = A.sx[q], y = A.sy[q], z = A.sz[q];
+    const int K = A.K;
+    double R2;
+    if (A.mode != 0) {
+        R2 = A.fb2[q];
+    } else if (A.guess && A.guess[q] > 0) {
+        R2 = A.guess[q] * 1.2;
+    } else {
+        // leaf density estimate, aiming at ~2K particles in the ball
+        const double *b = A.box + 6 * (size_t)(A.nL2 + q / LEAF);
+        const double ex = b[3] - b[0], ey = b[4] - b[1], ez = b[5] - b[2];
+        double vol = ex * ey * ez;
+        const double emax = fmax(ex, fmax(ey, ez));
+        if (!(vol > 0)) vol = emax * emax * emax;
+        const int nl = min(LEAF, A.n - (q / LEAF) * LEAF);
+        double r3 = vol * (2.0 * K) / fmax(nl, 1) * (3.0 / (4.0 * M_PI));
+        if (!(r3 > 0)) r3 = 1e-30;
+        R2 = pow(r3, 2.0 / 3.0);
+    }
+    for (int attempt = 0; attempt < MAXTRY; ++attempt) {
+        // breadth-first search of the box hierarchy for leaves within R
+        const double R2b = R2 * (1.0 + 1e-12);
+        int nf = 1, cur = 0, nl = 0;
+        bool ovf = false;
+        if (lane == 0) fr[0][0] = 1;
+        __syncthreads();
+        while (nf > 0) {
+            int nn = 0;
+            for (int base = 0; base < nf; base += 64) {
+                const int k = base + lane;
+                int node = 0;
+                bool hit = false;
+                if (k < nf) {
+                    node = fr[cur][k];
+                    hit = box_d2(A.box + 6 * (size_t)node, x, y, z, A.per, A.L) <= R2b;
+                }
+                const bool leaf = node >= A.nL2;
+                const unsigned long long ml = __ballot(hit && leaf);
+                const unsigned long long mi = __ballot(hit && !leaf);
+                if (hit && leaf) {
+                    const int p = nl + __popcll(ml & ((1ull << lane) - 1));
+                    if (p < LCAP) lf[p] = node - A.nL2;
+                }
+                if (hit && !leaf) {
+                    const int p = nn + 2 * __popcll(mi & ((1ull << lane) - 1));
+                    if (p + 1 < FCAP) {
+                        fr[cur ^ 1][p] = 2 * node;
+                        fr[cur ^ 1][p + 1] = 2 * node + 1;
+                    }
+                }
+                nl += __popcll(ml);
+                nn += 2 * __popcll(mi);
+            }
+            __syncthreads();
+            if (nn > FCAP || nl > LCAP) {
+                ovf = true;
+                break;
+            }
+            nf = nn;
+            cur ^= 1;
+        }
+        if (ovf) {
+            if (A.mode != 0) break;   // a ball gather cannot change its radius
+            R2 *= 0.25;               // kNN: ball far too large for the buffers
+            continue;
+        }
+        // candidates within R, two leaves per pass
+        int total = 0;
+        for (int li = 0; li < nl; li += 2) {
+            const int leaf = li + (lane >> 5);
+            bool ok = false;
+            double d2 = 0.0;
+            int slot = 0;
+            if (leaf < nl) {
+                const int lk = lf[leaf];
+                slot = lk * LEAF + (lane & 31);
+                if (slot < A.n && (lane & 31) < LEAF) {
+                    const double dx = img_sep(x, A.sx[slot], A.per, A.L[0]);
+                    const double dy = img_sep(y, A.sy[slot], A.per, A.L[1]);
+                    const double dz = img_sep(z, A.sz[slot], A.per, A.L[2]);
+                    d2 = sq_dist(dx, dy, dz);
+                    ok = d2 <= R2;
+                }
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int p = total + __popcll(m & ((1ull << lane) - 1));
+                if (p < CAP) {
+                    cd[p] = d2;
+                    cs[p] = slot;
+                    cc[p] = A.order[slot];
+                }
+            }
+            total += __popcll(m);
+        }
+        __syncthreads();
+        if (A.mode == 0) {
+            if (total < K) {
+                const double f = total > 0 ? pow(2.0 * K / total, 2.0 / 3.0) : 4.0;
+                R2 *= fmin(fmax(f, 1.6), 16.0);
+                continue;
+            }
+            if (total > CAP) {
+                const double f = pow(1.5 * K / total, 2.0 / 3.0);
+                R2 *= fmin(fmax(f, 0.05), 0.9);
+                continue;
+            }
+        } else if (total > CAP) {
+            break;   // ball gather larger than the buffer: reported below
+        }
+        double d[4];
+        int c[4], s[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = lane + 64 * k;
+            if (e < total) {
+                d[k] = cd[e];
+                c[k] = cc[e];
+                s[k] = cs[e];
+            } else {
+                d[k] = DBL_MAX;
+                c[k] = 0x7fffffff;
+                s[k] = -1;
+            }
+        }
+        bitonic256(d, c, s, lane);
+        if (A.mode == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = lane + 64 * k;
+                if (e < K) A.nbr[(size_t)q * K + e] = s[k];
+            }
+            // fBall2 = K-th smallest; a tie with the (K+1)-th is reported
+            const int eK = K - 1;
+            double dK = 0, dK1 = DBL_MAX;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double a = __shfl(d[k], eK & 63);
+                if (k == (eK >> 6)) dK = a;
+                const double b = __shfl(d[k], K & 63);
+                if (k == (K >> 6) && K < CAP) dK1 = b;
+            }
+            if (lane == 0) {
+                A.fb2[q] = dK;
+                if (total > K && dK1 == dK) atomicAdd(&A.err[1], 1u);
+            }
+        } else if (A.mode == 1) {
+            if (lane == 0) A.cnt[q] = total;
+        } else {
+            const int64_t o = A.off[q];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = lane + 64 * k;
+                if (e < total) A.nbr[o + e] = s[k];
+            }
+        }
+        return;
+    }
+    if (lane == 0) {
+        atomicAdd(&A.err[0], 1u);
+        if (A.mode == 0) A.fb2[q] = -1.0;
+        if (A.mode == 1) A.cnt[q] = 0;
+    }
+}
+
+// -------------------------------------------------------------- kernels
+template <typename T> __device__ __forceinline__ T tsqrt(T x);
+template <> __device__ __forceinline__ float tsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ double tsqrt<double>(double x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T trcp(T x);
+template <> __device__ __forceinline__ float trcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double trcp<double>(double x) { return 1.0 / x; }
+
+// M4 kernel W (smoothfcn.c:199-205) and dW/(r/h) (smoothfcn.c:206-213),
+// in units where only the 1/pi normalisation is left to the caller.
+template <typename T> __device__ __forceinline__ T m4_w(T r2)
+{
+    T ak = T(2) - tsqrt<T>(r2);
+    return r2 < T(1) ? (T(1) - T(0.75) * ak * r2) : T(0.25) * ak * ak * ak;
+}
+template <typename T> __device__ __forceinline__ T m4_dw(T r2)
+{
+    const T r = tsqrt<T>(r2);
+    if (r2 < T(1)) return T(-3) + T(2.25) * r;
+    const T t = T(2) - r;
+    return T(-0.75) * t * t * trcp<T>(r);
+}
+
+struct DArgs {
+    int n, op;                  // op: GG_SMX_DENSITY or GG_SMX_DENDVDX
+    int per;
+    double L[3];
+    const double *sx, *sy, *sz, *fb2;        // slot order
+    const int64_t *off;                      // CSR (slot order)
+    const int *idx;
+    const uint32_t *qflag;
+    const double *m, *vx, *vy, *vz, *u;      // slot order, FP64
+    const uint32_t *act;                     // slot order: TYPE_ACTIVE
+    double H, a, gamma;
+    int bCannonical, iViscosityLimiter, bConstantDiffusion;
+    double *rho, *divv, *curl, *dvds, *divvt, *bal, *diff;   // slot order
+    uint32_t *nbract;                        // slot order flags (atomicOr)
+};
+
+// DenDVDX (smoothfcn.c:3390-3627): 16 lanes per particle over its list,
+// pair arithmetic in T, sums in FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
+{
+    const int g = (blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (g >= A.n) return;
+    const int i = g;
+    if (!A.qflag[i]) return;
+    const double fb2 = A.fb2[i];
+    const double ih2d = 4.0 / fb2;
+    const T vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
+    const double x = A.sx[i], y = A.sy[i], z = A.sz[i];
+    const T vxi = (T)A.vx[i], vyi = (T)A.vy[i], vzi = (T)A.vz[i];
+    const bool pact = A.act[i] & 1u;
+    double rho = 0, xx = 0, xy = 0, xz = 0, yx = 0, yy = 0, yz = 0, zx = 0, zy = 0, zz = 0;
+    double dn = 0, gx = 0, gy = 0, gz = 0;
+    unsigned qact = 0;
+    const int64_t o = A.off[i], e1 = A.off[i + 1];
+    for (int64_t e = o + sub; e < e1; e += 16) {
+        const int q = A.idx[e];
+        const double dxd = img_sep(x, A.sx[q], A.per, A.L[0]);
+        const double dyd = img_sep(y, A.sy[q], A.per, A.L[1]);
+        const double dzd = img_sep(z, A.sz[q], A.per, A.L[2]);
+        const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
+        const T r2 = (T)(sq_dist(dxd, dyd, dzd) * ih2d);
+        const T mq = (T)A.m[q];
+        if (pact) atomicOr(&A.nbract[q], 32u);
+        qact |= A.act[q];
+        rho += (double)(m4_w<T>(r2) * mq);
+        if (A.op == GG_SMX_DENDVDX) {
+            const T rs1 = m4_dw<T>(r2) * mq;
+            const T dvx = ((T)A.vx[q] - vxi) * vFac;
+            const T dvy = ((T)A.vy[q] - vyi) * vFac;
+            const T dvz = ((T)A.vz[q] - vzi) * vFac;
+            const T uq = (T)A.u[q];
+            xx += (double)(dvx * dx * rs1); xy += (double)(dvx * dy * rs1); xz += (double)(dvx * dz * rs1);
+            yx += (double)(dvy * dx * rs1); yy += (double)(dvy * dy * rs1); yz += (double)(dvy * dz * rs1);
+            zx += (double)(dvz * dx * rs1); zy += (double)(dvz * dy * rs1); zz += (double)(dvz * dz * rs1);
+            dn += (double)((dx * dx + dy * dy + dz * dz) * rs1);
+            gx += (double)(uq * dx * rs1); gy += (double)(uq * dy * rs1); gz += (double)(uq * dz * rs1);
+        }
+    }
+    for (int w = 1; w < 16; w <<= 1) {
+        rho += __shfl_xor(rho, w);
+        qact |= __shfl_xor(qact, w);
+        if (A.op == GG_SMX_DENDVDX) {
+            xx += __shfl_xor(xx, w); xy += __shfl_xor(xy, w); xz += __shfl_xor(xz, w);
+            yx += __shfl_xor(yx, w); yy += __shfl_xor(yy, w); yz += __shfl_xor(yz, w);
+            zx += __shfl_xor(zx, w); zy += __shfl_xor(zy, w); zz += __shfl_xor(zz, w);
+            dn += __shfl_xor(dn, w);
+            gx += __shfl_xor(gx, w); gy += __shfl_xor(gy, w); gz += __shfl_xor(gz, w);
+        }
+    }
+    if (sub != 0) return;
+    const double ih = sqrt(ih2d);
+    const double fNorm = M_1_PI * ih2d * ih;
+    if (qact & 1u) atomicOr(&A.nbract[i], 32u);
+    A.rho[i] = rho * fNorm;
+    if (A.op != GG_SMX_DENDVDX) return;
+    const double trace = xx + yy + zz;
+    const double fN1 = (dn != 0 ? 3 / fabs(dn) : 0);
+    const double dv = fN1 * trace + 3 * A.H;
+    A.divvt[i] = fN1 * trace;
+    A.divv[i] = dv;
+    const double c0 = fN1 * (zy - yz), c1 = fN1 * (xz - zx), c2 = fN1 * (yx - xy);
+    A.curl[3 * (size_t)i] = c0;
+    A.curl[3 * (size_t)i + 1] = c1;
+    A.curl[3 * (size_t)i + 2] = c2;
+    const double Hc = (fN1 != 0 ? A.H / fN1 : 0);
+    double gn = gx * gx + gy * gy + gz * gz;
+    if (gn > 0) gn = 1 / sqrt(gn);
+    gx *= gn; gy *= gn; gz *= gn;
+    const double dvdr = (((xx + Hc) * gx + xy * gy + xz * gz) * gx
+                         + (yx * gx + (yy + Hc) * gy + yz * gz) * gy
+                         + (zx * gx + zy * gy + (zz + Hc) * gz) * gz) * fN1;
+    const double ds = (dv < 0 ? 1.5 * (dvdr - (1. / 3.) * dv) : dvdr);
+    A.dvds[i] = ds;
+    const double cm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    double bs;
+    if (A.iViscosityLimiter == 0) bs = 1;
+    else if (A.iViscosityLimiter == 1) bs = (dv != 0.0) ? fabs(dv) / (fabs(dv) + cm) : 0;
+    else bs = (ds != 0) ? fabs(ds) / (fabs(ds) + cm) : 0.01;   // ALPHAMIN, pkd.h:41
+    A.bal[i] = bs;
+    const double ot = (1. / 3.) * trace;
+    const double sxx = xx - ot, syy = yy - ot, szz = zz - ot;
+    const double sxy = 0.5 * (xy + yx), sxz = 0.5 * (xz + zx), syz = 0.5 * (yz + zy);
+    const double S2 = fN1 * fN1 * (sxx * sxx + syy * syy + szz * szz
+                                   + 2 * (sxy * sxy + sxz * sxz + syz * syz));
+    A.diff[i] = A.bConstantDiffusion ? 1 : 0.25 * fb2 * sqrt(2 * S2);
+}
+
+// Pressure of the adiabatic gas (pkdGasPressure, pkd.c:5750-5785), caller
+// order, particles with the selection mask only.
+__global__ void k_gas_pressure(int n, const uint32_t *__restrict__ bits, uint32_t mask,
+                               const double *__restrict__ upred, const double *__restrict__ rho,
+                               const double *__restrict__ fb2, const double *__restrict__ udp,
+                               const double *__restrict__ uda, const double *__restrict__ udd,
+                               double gamma, double dtFacCourant, double dResolveJeans,
+                               double *__restrict__ pr2, double *__restrict__ c,
+                               double *__restrict__ dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (bits[i] & mask) != mask) return;
+    const double prg = (gamma - 1) * upred[i];
+    const double cg = sqrt(gamma * prg);
+    const double floorJ = 0.25 * fb2[i] * dResolveJeans * rho[i];
+    double pr = prg;
+    const double prj = (pr < floorJ ? floorJ - pr : 0);
+    pr += prj;
+    pr2[i] = pr / rho[i];
+    const double ci = sqrt(cg * cg + 2.0 * prj);   // GAMMA_JEANS = 2
+    c[i] = ci;
+    const double udot = udp[i] + uda[i] + udd[i];  // UDOT_HYDRO, pkd.h:65
+    double dti;
+    if (udot > 0) dti = dtFacCourant * sqrt(fb2[i] * 0.25 / (4 * (ci * ci + (5. / 3.) * udot * dt[i])));
+    else dti = dtFacCourant * sqrt(fb2[i] * 0.25) / (2 * ci);
+    if (dti < dt[i]) dt[i] = dti;
+}
+
+struct HArgs {
+    int n, per;
+    double L[3];
+    const double *sx, *sy, *sz, *fb2;
+    const int64_t *off, *roff;
+    const int *idx, *ridx;
+    const uint32_t *qflag, *act;
+    const double *m, *vx, *vy, *vz, *u, *rho, *pr2, *c, *bal, *diff, *dt;
+    double a, H, alpha, beta, dtFacCourant, dtFacDiffusion, dThermalDiffusionCoeff;
+    int bCannonical;
+    double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
+};
+
+// One (p, q) pair of SphPressureTermsSym (SphPressureTerms.h:112-152) as it
+// affects particle "self": role 0 = self is p (own list), 1 = self is q
+// (reverse list, evaluated with p's smoothing length).
+template <typename T>
+__device__ __forceinline__ void hydro_pair(const HArgs &A, int self, int other, int role,
+                                           double &ax, double &ay, double &az, double &pdv,
+                                           double &av, double &dif, double &mu, double &dtn)
+{
+    const int p = role ? other : self, q = role ? self : other;
+    const bool pa = A.act[p] & 1u, qa = A.act[q] & 1u;
+    if (!pa && !qa) return;
+    const bool sa = role ? qa : pa;    // is self active
+    const double dxd = img_sep(A.sx[p], A.sx[q], A.per, A.L[0]);
+    const double dyd = img_sep(A.sy[p], A.sy[q], A.per, A.L[1]);
+    const double dzd = img_sep(A.sz[p], A.sz[q], A.per, A.L[2]);
+    const double d2d = sq_dist(dxd, dyd, dzd);
+    const double fbp = A.fb2[p];
+    const T ph = (T)sqrt(0.25 * fbp);
+    const T ih2 = (T)(4.0 / fbp);
+    const T fNorm1 = (T)(0.5 * M_1_PI * (4.0 / fbp) / sqrt(0.25 * fbp) * (4.0 / fbp));
+    const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, d2 = (T)d2d;
+    const T r2 = (T)(d2d * (4.0 / fbp));
+    (void)ih2;
+    const T rs1 = m4_dw<T>(r2) * fNorm1;
+    const T rp = rs1 * (T)A.m[p], rq = rs1 * (T)A.m[q];
+    const T vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
+    const T dvx = (T)A.vx[p] - (T)A.vx[q];
+    const T dvy = (T)A.vy[p] - (T)A.vy[q];
+    const T dvz = (T)A.vz[p] - (T)A.vz[q];
+    const T dvdotdr = vFac * (dvx * dx + dvy * dy + dvz * dz) + d2 * (T)A.H;
+    const T pP = (T)A.pr2[p], qP = (T)A.pr2[q];
+    const T pc = (T)A.c[p], qc = (T)A.c[q];
+    const T prho = (T)A.rho[p], qrho = (T)A.rho[q];
+    // PRES_PDV(a,b) = a, PRES_ACC(a,b) = a+b (smoothfcn.c:20-23); without
+    // an external pressure P/rho^2 "f" equals P/rho^2.
+    const T r_self = role ? rp : rq;          // rq for p, rp for q
+    if (sa) pdv += (double)(r_self * (role ? qP : pP) * dvdotdr);
+    T Acc = pP + qP;
+    T dtv;
+    if (dvdotdr >= T(0)) {
+        dtv = (T)A.dtFacCourant * ph * trcp<T>(T(2) * (pc > qc ? pc : qc));
+    } else {
+        const T absmu = -dvdotdr * (T)A.a * trcp<T>(tsqrt<T>(d2));
+        mu = fmax(mu, (double)absmu);
+        T visc = (T)A.alpha * (pc + qc) + (T)A.beta * T(1.5) * absmu;
+        dtv = (T)A.dtFacCourant * ph * trcp<T>(T(0.625) * (pc + qc) + T(0.375) * visc);
+        visc = (T(0.5) * ((T)A.bal[p] + (T)A.bal[q])) * visc * absmu * trcp<T>(prho + qrho);
+        if (sa) av += (double)(r_self * (T(0.5) * visc) * dvdotdr);
+        Acc += visc;
+    }
+    if (sa) {
+        Acc *= r_self * (T)A.a;
+        const double s = role ? 1.0 : -1.0;      // a_p -= Acc dx, a_q += Acc dx
+        ax += s * (double)(Acc * dx);
+        ay += s * (double)(Acc * dy);
+        az += s * (double)(Acc * dz);
+    }
+    // thermal diffusion (SphPressureTerms.h:32-70, 142-145)
+    const double dsum = A.diff[p] + A.diff[q];
+    if (dsum != 0 && A.dThermalDiffusionCoeff > 0) {
+        const double diffTh = 2 * A.dThermalDiffusionCoeff * dsum / (A.rho[p] + A.rho[q]);
+        const double dtd = A.dtFacDiffusion * (double)ph * (double)ph / (diffTh * A.rho[p]);
+        if (diffTh > 0 && dtd < (double)dtv) dtv = (T)dtd;
+        const double du = diffTh * (A.u[p] - A.u[q]);
+        if (sa) dif += role ? -du * (double)rp : du * (double)rq;
+    }
+    // SETDTNEW_PQ (SphPressureTerms.h:97-100)
+    dtn = fmin(dtn, (double)dtv);
+    dtn = fmin(dtn, 4 * A.dt[other]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_hydro(HArgs A)
+{
+    const int g = (blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (g >= A.n) return;
+    const int i = g;
+    double ax = 0, ay = 0, az = 0, pdv = 0, av = 0, dif = 0, mu = 0, dtn = GG_FLT_MAX;
+    if (A.qflag[i]) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16)
+            hydro_pair<T>(A, i, A.idx[e], 0, ax, ay, az, pdv, av, dif, mu, dtn);
+    }
+    for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 16)
+        hydro_pair<T>(A, i, A.ridx[e], 1, ax, ay, az, pdv, av, dif, mu, dtn);
+    for (int w = 1; w < 16; w <<= 1) {
+        ax += __shfl_xor(ax, w);
+        ay += __shfl_xor(ay, w);
+        az += __shfl_xor(az, w);
+        pdv += __shfl_xor(pdv, w);
+        av += __shfl_xor(av, w);
+        dif += __shfl_xor(dif, w);
+        mu = fmax(mu, __shfl_xor(mu, w));
+        dtn = fmin(dtn, __shfl_xor(dtn, w));
+    }
+    if (sub == 0) {
+        double *o = A.out + 8 * (size_t)i;
+        o[0] = ax; o[1] = ay; o[2] = az; o[3] = pdv; o[4] = av; o[5] = dif;
+        o[6] = mu; o[7] = dtn;
+    }
+}
+
+// Apply the hydro results to the caller-order fields: for smooth-active
+// particles initSphPressureTermsParticle (smoothfcn.c:2204-2221) resets
+// mumax, dtNew and the uDot terms of ACTIVE ones; a is accumulated.
+__global__ void k_hydro_scatter(int n, const int *__restrict__ order,
+                                const uint32_t *__restrict__ qflag, const double *__restrict__ res,
+                                const uint32_t *__restrict__ bits, double *__restrict__ acc,
+                                double *__restrict__ pdv, double *__restrict__ av,
+                                double *__restrict__ dif, double *__restrict__ mumax,
+                                double *__restrict__ dtnew, int touched_only)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int c = order[s];
+    const double *o = res + 8 * (size_t)s;
+    const bool active = bits[c] & 1u;
+    const bool init = qflag[s] && active;
+    double mu = init ? 0.0 : mumax[c];
+    double dn = init ? GG_FLT_MAX : dtnew[c];
+    mumax[c] = fmax(mu, o[6]);
+    dtnew[c] = fmin(dn, o[7]);
+    (void)touched_only;
+    if (active) {
+        acc[3 * (size_t)c] += o[0];
+        acc[3 * (size_t)c + 1] += o[1];
+        acc[3 * (size_t)c + 2] += o[2];
+        pdv[c] = (init ? 0.0 : pdv[c]) + o[3];
+        av[c] = (init ? 0.0 : av[c]) + o[4];
+        dif[c] = (init ? 0.0 : dif[c]) + o[5];
+    }
+}
+
+// ------------------------------------------------------- list plumbing
+__global__ void k_rev_count(int n, const uint32_t *__restrict__ qflag,
+                            const int64_t *__restrict__ off, const int *__restrict__ idx,
+                            int *__restrict__ rcnt)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !qflag[p]) return;
+    for (int64_t e = off[p]; e < off[p + 1]; ++e) atomicAdd(&rcnt[idx[e]], 1);
+}
+
+__global__ void k_rev_fill(int n, const uint32_t *__restrict__ qflag,
+                           const int64_t *__restrict__ off, const int *__restrict__ idx,
+                           const int64_t *__restrict__ roff, int *__restrict__ rcur,
+                           int *__restrict__ ridx)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !qflag[p]) return;
+    for (int64_t e = off[p]; e < off[p + 1]; ++e) {
+        const int q = idx[e];
+        const int k = atomicAdd(&rcur[q], 1);
+        ridx[roff[q] + k] = p;
+    }
+}
+
+// Sort each reverse list so the summation order is deterministic.
+__global__ void k_rev_sort(int n, const int64_t *__restrict__ roff, int *__restrict__ ridx)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int *a = ridx + roff[q];
+    const int m = (int)(roff[q + 1] - roff[q]);
+    for (int i = 1; i < m; ++i) {
+        const int v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) {
+            a[j + 1] = a[j];
+            --j;
+        }
+        a[j + 1] = v;
+    }
+}
+
+__global__ void k_fixed_off(int n, int K, int64_t *__restrict__ off)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) off[i] = (int64_t)K * i;
+}
+
+template <typename S, typename D>
+__global__ void k_gather(int n, const int *__restrict__ order, const S *__restrict__ src,
+                         int width, D *__restrict__ dst)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int c = order[s];
+    for (int k = 0; k < width; ++k) dst[(size_t)width * s + k] = (D)src[(size_t)width * c + k];
+}
+
+__global__ void k_gather_comp(int n, const int *__restrict__ order, const double *__restrict__ src,
+                              int width, int comp, double *__restrict__ dst)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    dst[s] = src[(size_t)width * order[s] + comp];
+}
+
+__global__ void k_count_diff(int n, const double *__restrict__ a, const double *__restrict__ b,
+                             const uint32_t *__restrict__ qa, const uint32_t *__restrict__ qb,
+                             unsigned *__restrict__ out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (qa[s] != qb[s] || (qa[s] && a[s] != b[s])) atomicAdd(out, 1u);
+}
+
+template <typename T>
+__global__ void k_scatter(int n, const int *__restrict__ order, const uint32_t *__restrict__ qflag,
+                          const T *__restrict__ src, int width, T *__restrict__ dst)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || (qflag && !qflag[s])) return;
+    const int c = order[s];
+    for (int k = 0; k < width; ++k) dst[(size_t)width * c + k] = src[(size_t)width * s + k];
+}
+
+__global__ void k_qflag(int n, const int *__restrict__ order, const uint32_t *__restrict__ bits,
+                        uint32_t mask, uint32_t *__restrict__ qf, uint32_t *__restrict__ act,
+                        uint32_t *__restrict__ nba)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t b = bits[order[s]];
+    qf[s] = (b & mask) == mask ? 1u : 0u;
+    act[s] = b & 1u;
+    if (nba) nba[s] = 0;
+}
+
+__global__ void k_flags_out(int n, const int *__restrict__ order, const uint32_t *__restrict__ qf,
+                            const uint32_t *__restrict__ nba, uint32_t *__restrict__ bits)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int c = order[s];
+    uint32_t b = bits[c] & ~(uint32_t)GG_TYPE_NbrOfACTIVE;
+    if (qf[s]) b |= GG_TYPE_SMOOTHDONE;
+    if (nba && (nba[s] & 32u)) b |= GG_TYPE_NbrOfACTIVE;
+    bits[c] = b;
+}
+
+
+// ------------------------------------------------------------ host side
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+static int gather_field(GGPKD pkd, int field, int width, int comp, DevBuf &dst)
+{
+    SphDev &S = pkd->sph;
+    GG_TRY(ensure_field(pkd, field));
+    GG_TRY(dst.reserve(sizeof(double) * std::max(S.n, 1)));
+    hipLaunchKernelGGL(k_gather_comp, dim3(nblk(S.n, 256)), dim3(256), 0, pkd->stream, S.n,
+                       S.order.as<int>(), pkd->f[field].as<double>(), width, comp,
+                       dst.as<double>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+static int scatter_field(GGPKD pkd, const DevBuf &src, int field, int width,
+                         const uint32_t *qf)
+{
+    SphDev &S = pkd->sph;
+    GG_TRY(ensure_field(pkd, field));
+    hipLaunchKernelGGL((k_scatter<double>), dim3(nblk(S.n, 256)), dim3(256), 0, pkd->stream,
+                       S.n, S.order.as<int>(), qf, src.as<double>(), width,
+                       pkd->f[field].as<double>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// The SPH structure holds TYPE_GAS particles, or every particle when no
+// particle carries TYPE_GAS.
+static int gas_mask(GGPKD pkd, uint32_t *mask)
+{
+    const int n = pkd->nLocal;
+    std::vector<uint32_t> b(n);
+    if (n) {
+        GG_HIP(hipMemcpyAsync(b.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n,
+                              hipMemcpyDeviceToHost, pkd->stream));
+        GG_HIP(hipStreamSynchronize(pkd->stream));
+    }
+    *mask = 0;
+    for (int i = 0; i < n; ++i)
+        if (b[i] & GG_TYPE_GAS) {
+            *mask = GG_TYPE_GAS;
+            break;
+        }
+    return GG_OK;
+}
+
+// Query flags (slot order) for particles with all bits of qmask.
+static int make_flags(GGPKD pkd, uint32_t qmask)
+{
+    SphDev &S = pkd->sph;
+    const int n = S.n;
+    GG_TRY(S.qf.reserve(sizeof(uint32_t) * n));
+    GG_TRY(S.act.reserve(sizeof(uint32_t) * n));
+    GG_TRY(S.nba.reserve(sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(k_qflag, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
+                       S.order.as<int>(), pkd->f[GG_IACTIVE].as<uint32_t>(), qmask,
+                       S.qf.as<uint32_t>(), S.act.as<uint32_t>(), S.nba.as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+static QArgs query_args(GGPKD pkd, int mode, int K)
+{
+    SphDev &S = pkd->sph;
+    QArgs Q;
+    Q.n = S.n;
+    Q.nLeaf = S.nLeaf;
+    Q.nL2 = S.nL2;
+    Q.K = K;
+    Q.per = S.bPeriodic;
+    Q.mode = mode;
+    for (int d = 0; d < 3; ++d) Q.L[d] = S.L[d];
+    Q.sx = S.sx.as<double>();
+    Q.sy = S.sy.as<double>();
+    Q.sz = S.sz.as<double>();
+    Q.box = S.nodeBox.as<double>();
+    Q.order = S.order.as<int>();
+    Q.qflag = S.qf.as<uint32_t>();
+    Q.guess = nullptr;
+    Q.fb2 = S.fb2s.as<double>();
+    Q.nbr = nullptr;
+    Q.cnt = nullptr;
+    Q.off = nullptr;
+    Q.err = S.err.as<unsigned>();
+    return Q;
+}
+
+static int read_err(GGPKD pkd, unsigned *e)
+{
+    GG_HIP(hipMemcpyAsync(e, pkd->sph.err.p, sizeof(unsigned) * 2, hipMemcpyDeviceToHost,
+                          pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const double *fPeriod)
+{
+    (void)nBucket;
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GG_CHECK(pkd->f[GG_R].p, GG_ESTATE, "positions must be put before gg_build_local");
+    GG_CHECK(!bPeriodic || fPeriod, GG_EINVAL, "periodic build needs fPeriod");
+    if (bPeriodic)
+        for (int d = 0; d < 3; ++d)
+            GG_CHECK(fPeriod[d] > 0, GG_EINVAL, "fPeriod[%d] must be positive", d);
+    GG_TRY(ensure_field(pkd, GG_IACTIVE));
+    SphDev &S = pkd->sph;
+    hipStream_t st = pkd->stream;
+    const int N = pkd->nLocal;
+    GG_CHECK(N > 0, GG_EINVAL, "empty particle store");
+    timer_mark(pkd, 0);
+    uint32_t mask;
+    GG_TRY(gas_mask(pkd, &mask));
+    GG_TRY(S.sel.reserve(sizeof(int) * N));
+    GG_TRY(S.pos.reserve(sizeof(int64_t) * (N + 1)));
+    GG_TRY(S.gidx.reserve(sizeof(int) * N));
+    hipLaunchKernelGGL(k_select, dim3(nblk(N, 256)), dim3(256), 0, st, N,
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), mask, S.sel.as<int>());
+    int64_t ng = 0;
+    GG_TRY(exclusive_scan_i64(st, S.sel.as<int>(), S.pos.as<int64_t>(), N, S.sortTmp, &ng));
+    const int n = (int)ng;
+    GG_CHECK(n > 0, GG_EINVAL, "no particles for the SPH structure");
+    hipLaunchKernelGGL(k_compact, dim3(nblk(N, 256)), dim3(256), 0, st, N, S.sel.as<int>(),
+                       S.pos.as<int64_t>(), S.gidx.as<int>());
+    const int nbb = std::min(nblk(n, 256), 1024);
+    GG_TRY(S.part.reserve(sizeof(double) * 6 * nbb));
+    hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, st, n, S.gidx.as<int>(),
+                       pkd->f[GG_R].as<double>(), S.part.as<double>());
+    std::vector<double> part(6 * (size_t)nbb);
+    GG_HIP(hipMemcpyAsync(part.data(), S.part.p, sizeof(double) * 6 * nbb, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = 0; b < nbb; ++b)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], part[6 * (size_t)b + d]);
+            hi[d] = std::max(hi[d], part[6 * (size_t)b + 3 + d]);
+        }
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
+    if (!(ext > 0)) ext = 1.0;
+    const double sc = 2097151.0 / ext;
+    GG_TRY(S.key.reserve(sizeof(uint64_t) * n));
+    GG_TRY(S.order.reserve(sizeof(int) * n));
+    GG_TRY(S.k2.reserve(sizeof(uint64_t) * n));
+    GG_TRY(S.v2.reserve(sizeof(int) * n));
+    hipLaunchKernelGGL(k_morton, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.gidx.as<int>(),
+                       pkd->f[GG_R].as<double>(), lo[0], lo[1], lo[2], sc, sc, sc,
+                       S.k2.as<uint64_t>(), S.v2.as<int>());
+    size_t tmpb = 0;
+    GG_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, S.k2.as<uint64_t>(), S.key.as<uint64_t>(),
+                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 63, st));
+    GG_TRY(S.sortTmp.reserve(std::max<size_t>(tmpb, 16)));
+    GG_HIP(rocprim::radix_sort_pairs(S.sortTmp.p, tmpb, S.k2.as<uint64_t>(), S.key.as<uint64_t>(),
+                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 63, st));
+    GG_TRY(S.sx.reserve(sizeof(double) * n));
+    GG_TRY(S.sy.reserve(sizeof(double) * n));
+    GG_TRY(S.sz.reserve(sizeof(double) * n));
+    hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                       pkd->f[GG_R].as<double>(), S.sx.as<double>(), S.sy.as<double>(),
+                       S.sz.as<double>());
+    const int nLeaf = (n + LEAF - 1) / LEAF;
+    int nL2 = 1;
+    while (nL2 < nLeaf) nL2 <<= 1;
+    GG_TRY(S.nodeBox.reserve(sizeof(double) * 6 * 2 * (size_t)nL2));
+    hipLaunchKernelGGL(k_leaf_box, dim3(nblk(nL2, 256)), dim3(256), 0, st, n, nLeaf, nL2,
+                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),
+                       S.nodeBox.as<double>());
+    for (int first = nL2 >> 1; first >= 1; first >>= 1)
+        hipLaunchKernelGGL(k_level_box, dim3(nblk(first, 256)), dim3(256), 0, st, first, first,
+                           S.nodeBox.as<double>());
+    GG_HIP(hipGetLastError());
+    S.n = n;
+    S.nLeaf = nLeaf;
+    S.nL2 = nL2;
+    S.bPeriodic = bPeriodic;
+    for (int d = 0; d < 3; ++d) S.L[d] = bPeriodic ? fPeriod[d] : 0.0;
+    S.nBucket = LEAF;
+    S.built = 1;
+    S.listValid = 0;
+    timer_mark(pkd, 1);
+    GG_HIP(hipStreamSynchronize(st));
+    const int ph[1] = {2};
+    timer_collect(pkd, 0, 1, ph);
+    return GG_OK;
+}
+
+extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+                         int bSymmetric, int iSmoothType, int *pnSmoothed)
+{
+    GG_CHECK(pkd && smf, GG_EINVAL, "null argument");
+    SphDev &S = pkd->sph;
+    GG_CHECK(S.built, GG_ESTATE, "gg_smooth before gg_build_local");
+    GG_CHECK(bPeriodic == S.bPeriodic, GG_EINVAL, "bPeriodic differs from gg_build_local");
+    GG_CHECK(iSmoothType == GG_SMX_DENSITY || iSmoothType == GG_SMX_DENDVDX, GG_EUNSUP,
+             "gg_smooth implements SMX_DENSITY and SMX_DENDVDX");
+    GG_CHECK(bSymmetric == 0, GG_EUNSUP, "the density operators run in gather form (bSymmetric=0)");
+    GG_CHECK(nSmooth >= 1 && nSmooth <= 128, GG_EINVAL, "nSmooth %d outside 1..128", nSmooth);
+    GG_CHECK(nSmooth <= S.n, GG_EINVAL, "nSmooth %d exceeds the %d particles", nSmooth, S.n);
+    hipStream_t st = pkd->stream;
+    const int n = S.n, K = nSmooth;
+    timer_mark(pkd, 0);
+    GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
+    GG_TRY(S.nbr.reserve(sizeof(int) * (size_t)K * n));
+    GG_TRY(S.fb2s.reserve(sizeof(double) * n));
+    GG_TRY(S.err.reserve(sizeof(unsigned) * 2));
+    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 2, st));
+    GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
+    QArgs Q = query_args(pkd, 0, K);
+    Q.guess = S.guess.as<double>();
+    Q.nbr = S.nbr.as<int>();
+    hipLaunchKernelGGL(k_query, dim3(n), dim3(64), 0, st, Q);
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 1);
+    GG_TRY(S.off.reserve(sizeof(int64_t) * (n + 1)));
+    hipLaunchKernelGGL(k_fixed_off, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, K,
+                       S.off.as<int64_t>());
+    GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
+    GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
+    GG_TRY(S.o_rho.reserve(sizeof(double) * n));
+    GG_TRY(S.o_divv.reserve(sizeof(double) * n));
+    GG_TRY(S.o_curl.reserve(sizeof(double) * 3 * n));
+    GG_TRY(S.o_dvds.reserve(sizeof(double) * n));
+    GG_TRY(S.o_divvt.reserve(sizeof(double) * n));
+    GG_TRY(S.o_bal.reserve(sizeof(double) * n));
+    GG_TRY(S.o_diff.reserve(sizeof(double) * n));
+    DArgs D;
+    D.n = n;
+    D.op = iSmoothType;
+    D.per = S.bPeriodic;
+    for (int d = 0; d < 3; ++d) D.L[d] = S.L[d];
+    D.sx = S.sx.as<double>();
+    D.sy = S.sy.as<double>();
+    D.sz = S.sz.as<double>();
+    D.fb2 = S.fb2s.as<double>();
+    D.off = S.off.as<int64_t>();
+    D.idx = S.nbr.as<int>();
+    D.qflag = S.qf.as<uint32_t>();
+    D.m = S.m.as<double>();
+    D.vx = S.vx.as<double>();
+    D.vy = S.vy.as<double>();
+    D.vz = S.vz.as<double>();
+    D.u = S.u.as<double>();
+    D.act = S.act.as<uint32_t>();
+    D.H = smf->H;
+    D.a = smf->a;
+    D.gamma = smf->gamma;
+    D.bCannonical = smf->bCannonical;
+    D.iViscosityLimiter = smf->iViscosityLimiter;
+    D.bConstantDiffusion = smf->bConstantDiffusion;
+    D.rho = S.o_rho.as<double>();
+    D.divv = S.o_divv.as<double>();
+    D.curl = S.o_curl.as<double>();
+    D.dvds = S.o_dvds.as<double>();
+    D.divvt = S.o_divvt.as<double>();
+    D.bal = S.o_bal.as<double>();
+    D.diff = S.o_diff.as<double>();
+    D.nbract = S.nba.as<uint32_t>();
+    const int grid = nblk(n * 16, 256);
+    if (pkd->prec == GG_PREC_FP32)
+        hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
+    else
+        hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 2);
+    unsigned e[2];
+    GG_TRY(read_err(pkd, e));
+    S.fails = e[0];
+    S.ties = e[1];
+    GG_CHECK(e[0] == 0, GG_EUNSUP,
+             "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
+    const uint32_t *qf = S.qf.as<uint32_t>();
+    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, qf));
+    if (iSmoothType == GG_SMX_DENDVDX) {
+        GG_TRY(scatter_field(pkd, S.o_divv, GG_DIVV, 1, qf));
+        GG_TRY(scatter_field(pkd, S.o_curl, GG_CURLV, 3, qf));
+        GG_TRY(scatter_field(pkd, S.o_dvds, GG_DVDS, 1, qf));
+        GG_TRY(scatter_field(pkd, S.o_divvt, GG_FDIVV_T, 1, qf));
+        GG_TRY(scatter_field(pkd, S.o_bal, GG_BALSARA, 1, qf));
+        GG_TRY(scatter_field(pkd, S.o_diff, GG_DIFF, 1, qf));
+    }
+    hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                       S.qf.as<uint32_t>(), S.nba.as<uint32_t>(), pkd->f[GG_IACTIVE].as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    // remember which lists are exact ball lists for gg_resmooth
+    GG_TRY(S.fb2knn.reserve(sizeof(double) * n));
+    GG_TRY(S.qfknn.reserve(sizeof(uint32_t) * n));
+    GG_HIP(hipMemcpyAsync(S.fb2knn.p, S.fb2s.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    S.listK = K;
+    S.listValid = (S.ties == 0);
+    int64_t nq = 0;
+    GG_TRY(exclusive_scan_i64(st, (const int *)S.qf.as<uint32_t>(), S.pos.as<int64_t>(), n,
+                              S.sortTmp, &nq));
+    S.listQueries = (int)nq;
+    if (pnSmoothed) *pnSmoothed = (int)nq;
+    timer_mark(pkd, 3);
+    GG_HIP(hipStreamSynchronize(st));
+    const int ph[3] = {3, 4, 7};
+    timer_collect(pkd, 0, 3, ph);
+    return GG_OK;
+}
+
+extern "C" int gg_gas_pressure(GGPKD pkd, const gg_smf *smf)
+{
+    GG_CHECK(pkd && smf, GG_EINVAL, "null argument");
+    const int n = pkd->nLocal;
+    const int fl[] = {GG_UPRED, GG_DENSITY, GG_FBALL2, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF,
+                      GG_POVERRHO2, GG_C, GG_DT, GG_IACTIVE};
+    for (int f : fl) GG_TRY(ensure_field(pkd, f));
+    uint32_t mask;
+    GG_TRY(gas_mask(pkd, &mask));
+    hipLaunchKernelGGL(k_gas_pressure, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), mask, pkd->f[GG_UPRED].as<double>(),
+                       pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_FBALL2].as<double>(),
+                       pkd->f[GG_UDOTPDV].as<double>(), pkd->f[GG_UDOTAV].as<double>(),
+                       pkd->f[GG_UDOTDIFF].as<double>(), smf->gamma, smf->dtFacCourant,
+                       smf->dResolveJeans, pkd->f[GG_POVERRHO2].as<double>(),
+                       pkd->f[GG_C].as<double>(), pkd->f[GG_DT].as<double>());
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+                           int bSymmetric, int iSmoothType)
+{
+    (void)nSmooth;
+    GG_CHECK(pkd && smf, GG_EINVAL, "null argument");
+    SphDev &S = pkd->sph;
+    GG_CHECK(S.built, GG_ESTATE, "gg_resmooth before gg_build_local");
+    GG_CHECK(bPeriodic == S.bPeriodic, GG_EINVAL, "bPeriodic differs from gg_build_local");
+    GG_CHECK(iSmoothType == GG_SMX_SPHPRESSURETERMS, GG_EUNSUP,
+             "gg_resmooth implements SMX_SPHPRESSURETERMS");
+    GG_CHECK(bSymmetric == 1, GG_EUNSUP, "SphPressureTerms is the symmetric operator (bSymmetric=1)");
+    hipStream_t st = pkd->stream;
+    const int n = S.n;
+    timer_mark(pkd, 0);
+    GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
+    GG_TRY(S.fb2s.reserve(sizeof(double) * n));
+    GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.fb2s));
+    GG_TRY(S.err.reserve(sizeof(unsigned) * 2));
+    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 2, st));
+    // reuse the kNN lists when they are exactly the ball lists of the
+    // stored radii (same queries, same fBall2, no boundary ties)
+    bool reuse = false;
+    if (S.listValid) {
+        GG_TRY(S.diffcnt.reserve(sizeof(unsigned)));
+        GG_HIP(hipMemsetAsync(S.diffcnt.p, 0, sizeof(unsigned), st));
+        hipLaunchKernelGGL(k_count_diff, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           S.fb2s.as<double>(), S.fb2knn.as<double>(), S.qf.as<uint32_t>(),
+                           S.qfknn.as<uint32_t>(), S.diffcnt.as<unsigned>());
+        unsigned nd = 1;
+        GG_HIP(hipMemcpyAsync(&nd, S.diffcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        reuse = (nd == 0);
+    }
+    const int64_t *off;
+    const int *idx;
+    if (reuse) {
+        off = S.off.as<int64_t>();
+        idx = S.nbr.as<int>();
+    } else {
+        GG_TRY(S.gcnt.reserve(sizeof(int) * (n + 1)));
+        GG_TRY(S.goff.reserve(sizeof(int64_t) * (n + 1)));
+        QArgs Q = query_args(pkd, 1, 0);
+        Q.cnt = S.gcnt.as<int>();
+        hipLaunchKernelGGL(k_query, dim3(n), dim3(64), 0, st, Q);
+        GG_HIP(hipGetLastError());
+        int64_t tot = 0;
+        GG_TRY(exclusive_scan_i64(st, S.gcnt.as<int>(), S.goff.as<int64_t>(), n, S.sortTmp, &tot));
+        GG_TRY(S.gent.reserve(sizeof(int) * std::max<int64_t>(tot, 1)));
+        Q = query_args(pkd, 2, 0);
+        Q.off = S.goff.as<int64_t>();
+        Q.nbr = S.gent.as<int>();
+        hipLaunchKernelGGL(k_query, dim3(n), dim3(64), 0, st, Q);
+        GG_HIP(hipGetLastError());
+        unsigned e[2];
+        GG_TRY(read_err(pkd, e));
+        GG_CHECK(e[0] == 0, GG_EUNSUP,
+                 "ball gather overflowed its %d-entry buffer for %u particles", CAP, e[0]);
+        off = S.goff.as<int64_t>();
+        idx = S.gent.as<int>();
+    }
+    timer_mark(pkd, 1);
+    // reverse lists
+    GG_TRY(S.rcnt.reserve(sizeof(int) * (n + 1)));
+    GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
+    GG_TRY(S.rcur.reserve(sizeof(int) * (n + 1)));
+    GG_HIP(hipMemsetAsync(S.rcnt.p, 0, sizeof(int) * (n + 1), st));
+    GG_HIP(hipMemsetAsync(S.rcur.p, 0, sizeof(int) * (n + 1), st));
+    hipLaunchKernelGGL(k_rev_count, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.qf.as<uint32_t>(),
+                       off, idx, S.rcnt.as<int>());
+    int64_t rtot = 0;
+    GG_TRY(exclusive_scan_i64(st, S.rcnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &rtot));
+    GG_TRY(S.ridx.reserve(sizeof(int) * std::max<int64_t>(rtot, 1)));
+    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.qf.as<uint32_t>(),
+                       off, idx, S.roff.as<int64_t>(), S.rcur.as<int>(), S.ridx.as<int>());
+    hipLaunchKernelGGL(k_rev_sort, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.roff.as<int64_t>(),
+                       S.ridx.as<int>());
+    GG_HIP(hipGetLastError());
+    // inputs in slot order
+    GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
+    GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
+    GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
+    GG_TRY(gather_field(pkd, GG_DENSITY, 1, 0, S.rho));
+    GG_TRY(gather_field(pkd, GG_POVERRHO2, 1, 0, S.pr2));
+    GG_TRY(gather_field(pkd, GG_C, 1, 0, S.c));
+    GG_TRY(gather_field(pkd, GG_BALSARA, 1, 0, S.bal));
+    GG_TRY(gather_field(pkd, GG_DIFF, 1, 0, S.dif));
+    GG_TRY(gather_field(pkd, GG_DT, 1, 0, S.dt));
+    GG_TRY(S.res.reserve(sizeof(double) * 8 * n));
+    HArgs H;
+    H.n = n;
+    H.per = S.bPeriodic;
+    for (int d = 0; d < 3; ++d) H.L[d] = S.L[d];
+    H.sx = S.sx.as<double>();
+    H.sy = S.sy.as<double>();
+    H.sz = S.sz.as<double>();
+    H.fb2 = S.fb2s.as<double>();
+    H.off = off;
+    H.roff = S.roff.as<int64_t>();
+    H.idx = idx;
+    H.ridx = S.ridx.as<int>();
+    H.qflag = S.qf.as<uint32_t>();
+    H.act = S.act.as<uint32_t>();
+    H.m = S.m.as<double>();
+    H.vx = S.vx.as<double>();
+    H.vy = S.vy.as<double>();
+    H.vz = S.vz.as<double>();
+    H.u = S.u.as<double>();
+    H.rho = S.rho.as<double>();
+    H.pr2 = S.pr2.as<double>();
+    H.c = S.c.as<double>();
+    H.bal = S.bal.as<double>();
+    H.diff = S.dif.as<double>();
+    H.dt = S.dt.as<double>();
+    H.a = smf->a;
+    H.H = smf->H;
+    H.alpha = smf->alpha;
+    H.beta = smf->beta;
+    H.dtFacCourant = smf->dtFacCourant;
+    H.dtFacDiffusion = smf->dtFacDiffusion;
+    H.dThermalDiffusionCoeff = smf->dThermalDiffusionCoeff;
+    H.bCannonical = smf->bCannonical;
+    H.out = S.res.as<double>();
+    const int grid = nblk(n * 16, 256);
+    if (pkd->prec == GG_PREC_FP32)
+        hipLaunchKernelGGL(k_hydro<float>, dim3(grid), dim3(256), 0, st, H);
+    else
+        hipLaunchKernelGGL(k_hydro<double>, dim3(grid), dim3(256), 0, st, H);
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 2);
+    const int fl[] = {GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW};
+    for (int f : fl) GG_TRY(ensure_field(pkd, f));
+    hipLaunchKernelGGL(k_hydro_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                       S.order.as<int>(), S.qf.as<uint32_t>(), S.res.as<double>(),
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), pkd->f[GG_A].as<double>(),
+                       pkd->f[GG_UDOTPDV].as<double>(), pkd->f[GG_UDOTAV].as<double>(),
+                       pkd->f[GG_UDOTDIFF].as<double>(), pkd->f[GG_MUMAX].as<double>(),
+                       pkd->f[GG_DTNEW].as<double>(), 0);
+    hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                       S.qf.as<uint32_t>(), (const uint32_t *)nullptr,
+                       pkd->f[GG_IACTIVE].as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 3);
+    GG_HIP(hipStreamSynchronize(st));
+    const int ph[3] = {5, 6, 7};
+    timer_collect(pkd, 0, 3, ph);
+    return GG_OK;
+}
+
+extern "C" int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr)
+{
+    GG_CHECK(pkd && nbr, GG_EINVAL, "null argument");
+    SphDev &S = pkd->sph;
+    GG_CHECK(S.built && S.listK == nSmooth, GG_ESTATE, "no kNN lists with nSmooth %d", nSmooth);
+    const int n = S.n, K = nSmooth, N = pkd->nLocal;
+    std::vector<int> lst((size_t)n * K), ord(n);
+    std::vector<uint32_t> qf(n);
+    GG_HIP(hipMemcpy(lst.data(), S.nbr.p, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
+    GG_HIP(hipMemcpy(ord.data(), S.order.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    GG_HIP(hipMemcpy(qf.data(), S.qfknn.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < (size_t)N * K; ++k) nbr[k] = -1;
+    for (int s = 0; s < n; ++s) {
+        if (!qf[s]) continue;
+        for (int k = 0; k < K; ++k) nbr[(size_t)ord[s] * K + k] = ord[lst[(size_t)s * K + k]];
+    }
+    return GG_OK;
+}

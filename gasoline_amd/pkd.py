@@ -1,0 +1,242 @@
+"""ctypes binding of libgasoline_amd.so and a thin Pkd class over a GGPKD
+handle.
+
+Pkd mirrors the per-processor calls of gasoline that the C ABI replaces:
+build_binary (pkdBuildBinary, pkd.c:2627), grav_all (pkdGravAll,
+pkd.c:2869), build_local (pkdBuildLocal, pkd.c:2732), smooth (smInitialize +
+smSmooth + smFinish, smooth.c:54/1439/505), resmooth (smReSmooth,
+smooth.c:1716) and gas_pressure (pkdGasPressure, pkd.c:5750).  Every call
+goes to the HIP library; a failure raises GGError with the library's message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libgasoline_amd.so")
+
+GG_R, GG_V, GG_VPRED, GG_A, GG_MASS, GG_SOFT, GG_POT, GG_DTGRAV, GG_U, GG_UPRED, \
+    GG_FBALL2, GG_DENSITY, GG_DIVV, GG_CURLV, GG_DVDS, GG_FDIVV_T, GG_BALSARA, GG_DIFF, \
+    GG_POVERRHO2, GG_C, GG_DT, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW, \
+    GG_IACTIVE = range(27)
+
+FIELDS = {
+    "r": (GG_R, 3), "v": (GG_V, 3), "vPred": (GG_VPRED, 3), "a": (GG_A, 3),
+    "fMass": (GG_MASS, 1), "fSoft": (GG_SOFT, 1), "fPot": (GG_POT, 1),
+    "dtGrav": (GG_DTGRAV, 1), "u": (GG_U, 1), "uPred": (GG_UPRED, 1),
+    "fBall2": (GG_FBALL2, 1), "fDensity": (GG_DENSITY, 1), "divv": (GG_DIVV, 1),
+    "curlv": (GG_CURLV, 3), "dvds": (GG_DVDS, 1), "fDivv_t": (GG_FDIVV_T, 1),
+    "BalsaraSwitch": (GG_BALSARA, 1), "diff": (GG_DIFF, 1),
+    "PoverRho2": (GG_POVERRHO2, 1), "c": (GG_C, 1), "dt": (GG_DT, 1),
+    "uDotPdV": (GG_UDOTPDV, 1), "uDotAV": (GG_UDOTAV, 1), "uDotDiff": (GG_UDOTDIFF, 1),
+    "mumax": (GG_MUMAX, 1), "dtNew": (GG_DTNEW, 1), "iActive": (GG_IACTIVE, 1),
+}
+
+PREC_FP64, PREC_FP32 = 0, 1
+SMX_DENSITY, SMX_SPHPRESSURETERMS, SMX_DENDVDX = 1, 21, 22
+TYPE_ACTIVE, TYPE_TREEACTIVE, TYPE_SMOOTHACTIVE, TYPE_SMOOTHDONE = 1, 2, 4, 8
+TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
+
+# exported entry points (include/gasoline_amd.h)
+ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
+       "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
+       "gg_grav_lists", "gg_tree_info", "gg_build_local", "gg_smooth", "gg_resmooth",
+       "gg_gas_pressure", "gg_smooth_lists", "gg_timings"]
+
+
+class GGError(RuntimeError):
+    pass
+
+
+class Smf(C.Structure):
+    """gg_smf: SMF parameters of the density and pressure-force operators
+    (smoothfcn.h:22-130)."""
+    _fields_ = [("H", C.c_double), ("a", C.c_double), ("alpha", C.c_double),
+                ("beta", C.c_double), ("gamma", C.c_double),
+                ("dtFacCourant", C.c_double), ("dtFacDiffusion", C.c_double),
+                ("dMetalDiffusionCoeff", C.c_double),
+                ("dThermalDiffusionCoeff", C.c_double), ("dResolveJeans", C.c_double),
+                ("bCannonical", C.c_int), ("iViscosityLimiter", C.c_int),
+                ("bConstantDiffusion", C.c_int), ("pad", C.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        """H=0, a=1, alpha=1, beta=2, gamma=5/3, dEtaCourant=0.4
+        (dtFacCourant = 0.4*2/1.6, pkd.c:5893), no viscosity limiter,
+        no diffusion."""
+        s = cls(H=0.0, a=1.0, alpha=1.0, beta=2.0, gamma=5.0 / 3.0,
+                dtFacCourant=0.4 * 2 / 1.6, dtFacDiffusion=2 * 0.1,
+                dMetalDiffusionCoeff=0.0, dThermalDiffusionCoeff=0.0, dResolveJeans=0.0,
+                bCannonical=1, iViscosityLimiter=0, bConstantDiffusion=0, pad=0)
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+
+
+_LIB = None
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+
+
+def lib():
+    """Load libgasoline_amd.so; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise GGError("libgasoline_amd.so is not built: run __graft_entry__.build() "
+                      "(make -C gasoline_amd/csrc)")
+    L = C.CDLL(LIBPATH)
+    L.gg_last_error.restype = C.c_char_p
+    L.gg_initialize.argtypes = [C.POINTER(_vp), C.c_int, C.c_int]
+    L.gg_finish.argtypes = [_vp]
+    L.gg_set_local.argtypes = [_vp, C.c_int]
+    L.gg_local.argtypes = [_vp]
+    L.gg_put.argtypes = [_vp, C.c_int, _vp, C.c_int]
+    L.gg_get.argtypes = [_vp, C.c_int, _vp, C.c_int]
+    L.gg_set_precision.argtypes = [_vp, C.c_int]
+    L.gg_build_binary.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.gg_grav_all.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                              C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, _dp,
+                              _ip, _dp, _dp, _dp, _dp]
+    L.gg_grav_lists.argtypes = [_vp, _ip, _i64p, _ip, _i64p, _ip, _ip]
+    L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
+    L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
+    L.gg_smooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int, _ip]
+    L.gg_resmooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int]
+    L.gg_gas_pressure.argtypes = [_vp, C.POINTER(Smf)]
+    L.gg_smooth_lists.argtypes = [_vp, C.c_int, _ip]
+    L.gg_timings.argtypes = [_vp, _dp, C.c_int]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().gg_last_error().decode(errors="replace")
+        raise GGError(f"libgasoline_amd error {rc}: {msg}")
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Pkd:
+    """A GGPKD handle: device-resident particle store of nStore particles."""
+
+    def __init__(self, n, device=0, prec=PREC_FP32):
+        L = lib()
+        h = _vp()
+        _check(L.gg_initialize(C.byref(h), int(device), int(n)))
+        self.h = h
+        self.n = int(n)
+        self.set_precision(prec)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gg_finish(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_precision(self, prec):
+        _check(lib().gg_set_precision(self.h, int(prec)))
+
+    # ---------------------------------------------------------- fields
+    def put(self, name, arr):
+        """Copy a field in, from a numpy array or a device tensor (torch)."""
+        fid, w = FIELDS[name]
+        if hasattr(arr, "data_ptr"):             # device tensor
+            _check(lib().gg_put(self.h, fid, _vp(arr.data_ptr()), 1))
+            return
+        dt = np.uint32 if name == "iActive" else np.float64
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(-1)
+        if a.size != self.n * w:
+            raise ValueError(f"{name}: expected {self.n * w} values, got {a.size}")
+        _check(lib().gg_put(self.h, fid, a.ctypes.data_as(_vp), 0))
+
+    def get(self, name):
+        fid, w = FIELDS[name]
+        dt = np.uint32 if name == "iActive" else np.float64
+        a = np.empty(self.n * w, dtype=dt)
+        _check(lib().gg_get(self.h, fid, a.ctypes.data_as(_vp), 0))
+        return a.reshape(self.n, w) if w > 1 else a
+
+    # --------------------------------------------------------- gravity
+    def build_binary(self, nBucket=16, iOpenType=1, dCrit=0.7, iOrder=2):
+        _check(lib().gg_build_binary(self.h, int(nBucket), int(iOpenType), float(dCrit),
+                                     int(iOrder), 0, 1))
+
+    def grav_all(self, iOrder=2):
+        na = C.c_int()
+        ps, cs, ss, fl = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        _check(lib().gg_grav_all(self.h, 0, 0, int(iOrder), 0, 4, 0.0, 0.0, 0, 0.0, 0, 0.0,
+                                 None, C.byref(na), C.byref(ps), C.byref(cs), C.byref(ss),
+                                 C.byref(fl)))
+        return dict(nActive=na.value, dPartSum=ps.value, dCellSum=cs.value,
+                    dSoftSum=ss.value, dFlop=fl.value)
+
+    def grav_lists(self):
+        nb = C.c_int()
+        ne = C.c_int64()
+        _check(lib().gg_grav_lists(self.h, C.byref(nb), C.byref(ne), None, None, None, None))
+        bk = np.empty(max(nb.value, 1), np.int32)
+        off = np.empty(3 * max(nb.value, 1), np.int64)
+        cnt = np.empty(3 * max(nb.value, 1), np.int32)
+        ent = np.empty(max(ne.value, 1), np.int32)
+        _check(lib().gg_grav_lists(self.h, C.byref(nb), C.byref(ne), _p(bk, _ip),
+                                   _p(off, _i64p), _p(cnt, _ip), _p(ent, _ip)))
+        n = nb.value
+        return bk[:n], off[:3 * n].reshape(n, 3), cnt[:3 * n].reshape(n, 3), ent[:ne.value]
+
+    def tree_info(self):
+        nn, root = C.c_int(), C.c_int()
+        _check(lib().gg_tree_info(self.h, C.byref(nn), C.byref(root), None, None, None))
+        perm = np.empty(self.n, np.int32)
+        ints = np.empty((nn.value, 6), np.int32)
+        dbls = np.empty((nn.value, 8))
+        _check(lib().gg_tree_info(self.h, C.byref(nn), C.byref(root), _p(perm, _ip),
+                                  _p(ints, _ip), _p(dbls)))
+        return root.value, perm, ints, dbls
+
+    # ------------------------------------------------------------- SPH
+    def build_local(self, period=None, nBucket=32):
+        if period is None:
+            _check(lib().gg_build_local(self.h, int(nBucket), 0, None))
+        else:
+            L = np.ascontiguousarray(period, dtype=np.float64)
+            _check(lib().gg_build_local(self.h, int(nBucket), 1, _p(L)))
+
+    def smooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_DENDVDX):
+        ns = C.c_int()
+        _check(lib().gg_smooth(self.h, C.byref(smf), int(nSmooth), int(bool(periodic)), 0,
+                               int(iSmoothType), C.byref(ns)))
+        return ns.value
+
+    def resmooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_SPHPRESSURETERMS):
+        _check(lib().gg_resmooth(self.h, C.byref(smf), int(nSmooth), int(bool(periodic)), 1,
+                                 int(iSmoothType)))
+
+    def gas_pressure(self, smf):
+        _check(lib().gg_gas_pressure(self.h, C.byref(smf)))
+
+    def smooth_lists(self, nSmooth=64):
+        out = np.empty((self.n, nSmooth), np.int32)
+        _check(lib().gg_smooth_lists(self.h, int(nSmooth), _p(out, _ip)))
+        return out
+
+    def timings(self):
+        t = np.zeros(8)
+        _check(lib().gg_timings(self.h, _p(t), 8))
+        return dict(zip(["walk", "interact", "sph_tree", "knn", "density", "gather",
+                         "hydro", "other"], t))

@@ -1,0 +1,221 @@
+/*
+ * gasoline_amd.h -- C ABI of the MI355X force hot path for gasoline.
+ *
+ * The library (gasoline_amd/libgasoline_amd.so) replaces, for one process,
+ * the leaf calls that gasoline's PST layer makes into its per-processor code:
+ *
+ *   pstBuildTree leaf  (pst.c:2955-2973) -> pkdBuildBinary / pkdBuildLocal
+ *   pstGravity leaf    (pst.c:3305-3336) -> pkdGravAll
+ *   pstSmooth leaf     (pst.c:3012-3039) -> smInitialize, smSmooth, smFinish
+ *   pstReSmooth leaf   (pst.c:3169-3196) -> smInitialize, smReSmooth, smFinish
+ *   pstGetGasPressure                    -> pkdGasPressure
+ *
+ * A GGPKD handle plays the role of gasoline's PKD: it owns a device-resident
+ * particle store (structure of arrays, FP64), the trees and the interaction
+ * and neighbour lists.  Fields are exchanged in the caller's particle order
+ * (the order of the arrays given to gg_put); the trees keep their own
+ * locality orders internally, so unlike pkdBuildBinary the caller's order is
+ * never permuted.
+ *
+ * Conventions: plain pointers and sizes; every entry point returns 0 on
+ * success and a negative GG_E* code on error, with a message available from
+ * gg_last_error() (gasoline itself asserts; errors here are returned).
+ * All calls on one GGPKD are issued on that handle's HIP stream and are
+ * synchronous with respect to the host unless stated otherwise.
+ */
+#ifndef GASOLINE_AMD_H
+#define GASOLINE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gg_pkd_s *GGPKD;
+
+/* error codes */
+#define GG_OK          0
+#define GG_EINVAL     -1   /* bad argument */
+#define GG_ENOMEM     -2   /* device or host allocation failed */
+#define GG_EHIP       -3   /* HIP runtime error */
+#define GG_ESTATE     -4   /* call out of order (e.g. no tree built) */
+#define GG_EUNSUP     -5   /* feature outside this library's scope */
+
+/* Particle fields (one FP64 value per particle unless noted).  Names follow
+ * gasoline's PARTICLE (pkd.h:90-225). */
+enum gg_field {
+    GG_R = 0,        /* r[3], x y z interleaved (3 doubles) */
+    GG_V,            /* v[3] */
+    GG_VPRED,        /* vPred[3] */
+    GG_A,            /* a[3] (accumulated by gravity and hydro) */
+    GG_MASS,         /* fMass */
+    GG_SOFT,         /* fSoft */
+    GG_POT,          /* fPot */
+    GG_DTGRAV,       /* dtGrav: max (m_i+m_j) b, a 1/dt^2 estimate */
+    GG_U,            /* u */
+    GG_UPRED,        /* uPred */
+    GG_FBALL2,       /* fBall2 = (2h)^2 */
+    GG_DENSITY,      /* fDensity */
+    GG_DIVV,         /* divv */
+    GG_CURLV,        /* curlv[3] */
+    GG_DVDS,         /* dvds */
+    GG_FDIVV_T,      /* fDivv_t */
+    GG_BALSARA,      /* BalsaraSwitch */
+    GG_DIFF,         /* diff */
+    GG_POVERRHO2,    /* PoverRho2 = (P/rho)/rho */
+    GG_C,            /* c, sound speed */
+    GG_DT,           /* dt, Courant time step (in/out of gas pressure) */
+    GG_UDOTPDV,      /* uDotPdV */
+    GG_UDOTAV,       /* uDotAV */
+    GG_UDOTDIFF,     /* uDotDiff */
+    GG_MUMAX,        /* mumax */
+    GG_DTNEW,        /* dtNew */
+    GG_IACTIVE,      /* iActive type bits (uint32 per particle, pkd.h:288-329) */
+    GG_NFIELDS
+};
+
+/* iActive bits used by this path (values as pkd.h:288-308) */
+#define GG_TYPE_ACTIVE        (1u << 0)
+#define GG_TYPE_TREEACTIVE    (1u << 1)
+#define GG_TYPE_SMOOTHACTIVE  (1u << 2)
+#define GG_TYPE_SMOOTHDONE    (1u << 3)
+#define GG_TYPE_NbrOfACTIVE   (1u << 5)
+#define GG_TYPE_GAS           (1u << 8)
+
+/* Arithmetic of the interaction kernels. */
+#define GG_PREC_FP64  0   /* plain FP64 throughout */
+#define GG_PREC_FP32  1   /* FP32 on cell-relative coordinates, compensated sums */
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a handle on HIP device `device` for up to nStore particles.
+ * Replaces pkdInitialize (pkd.c:150-296) for the fields this path uses. */
+int gg_initialize(GGPKD *ppkd, int device, int nStore);
+
+/* Release every device and host buffer of the handle (pkdFinish). */
+int gg_finish(GGPKD pkd);
+
+/* Message of the last error on this thread. */
+const char *gg_last_error(void);
+
+/* Number of particles currently in the store (pkdLocal). */
+int gg_set_local(GGPKD pkd, int nLocal);
+int gg_local(GGPKD pkd);
+
+/* Copy a field in (gg_put) or out (gg_get), in the caller's particle order.
+ * src/dst are host pointers when bDevice == 0, device pointers otherwise.
+ * Stands in for the particle I/O the reference does through pStore
+ * (pkdReadTipsy pkd.c:297, pkdOutVector outtype.c:806). */
+int gg_put(GGPKD pkd, int field, const void *src, int bDevice);
+int gg_get(GGPKD pkd, int field, void *dst, int bDevice);
+
+/* Select FP32 or FP64 interaction arithmetic (default GG_PREC_FP32). */
+int gg_set_precision(GGPKD pkd, int prec);
+
+/* ------------------------------------------------------------ gravity */
+
+/* Build the gravity tree: spatial bisection at the midpoint of the squeezed
+ * bounding box's longest axis, buckets of <= nBucket particles, FP64 c.o.m.,
+ * mass-weighted softening, multipoles to iOrder and OPEN_JOSH opening radii.
+ * Replaces pkdBuildBinary (pkd.h:794, pkd.c:2627-2723).  iOpenType must be 1
+ * (OPEN_JOSH, opentype.h:5) or 0 (minimal Bmax); bGravity must be 1. */
+int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCrit,
+                    int iOrder, int bTreeActiveOnly, int bGravity);
+
+/* Gravity on all active particles: per-bucket interaction lists from the
+ * threaded-tree walk (pkdBucketWalk walk.c:306-439), then particle-particle,
+ * softened-cell and Newtonian-cell interactions and the intra-bucket pairs
+ * (pkdBucketInteract grav.c:23-249).  a, fPot and dtGrav of active particles
+ * are ACCUMULATED onto their stored values (gasoline zeroes them first in
+ * msrInitAccel).  Replaces pkdGravAll (pkd.h:797-801, pkd.c:2869-3060) with
+ * the same signature; nReps must be 0 and bEwald/bComove/bDoSun 0 (periodic
+ * Ewald gravity is outside this library).  Counters follow pkd.c:2946-2949
+ * and the flop model of grav.c:246-247. */
+int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
+                int iEwOrder, double fEwCut, double fEwhCut, int bComove,
+                double dRhoFac, int bDoSun, double dSunSoft, double *aSun,
+                int *nActive, double *pdPartSum, double *pdCellSum,
+                double *pdSoftSum, double *pdFlop);
+
+/* Copy the interaction lists of the last gg_grav_all to the host, for
+ * parity checks.  Lists are per active bucket b (buckets in tree order):
+ * bucket_node[b], then for kind k in {PP, soft cell, Newtonian cell}
+ * off[3*b+k] .. off[3*b+k] + cnt[3*b+k] index into ent[].  PP entries are
+ * particle slots in the tree's storage order, cell entries are node ids.
+ * Call with ent == NULL to get the sizes (*nBuckets, *nEnt). */
+int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucket_node,
+                  int64_t *off, int *cnt, int *ent);
+
+/* Tree shape of the last gg_build_binary, for parity checks: perm[k] is
+ * the caller index of storage slot k; per node ints[6*i..] = iDim, pLower,
+ * pUpper, iLower, iUpper (threaded next), 0 and dbls[8*i..] = fMass, fSoft,
+ * r[3], fOpen2, Bmax, fSplit. */
+int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
+                 double *dbls);
+
+/* ----------------------------------------------------------------- SPH */
+
+/* Smoothing parameters: the subset of gasoline's SMF (smoothfcn.h:22-130)
+ * the density and pressure-force operators read. */
+typedef struct gg_smf {
+    double H, a;
+    double alpha, beta, gamma;
+    double dtFacCourant, dtFacDiffusion;
+    double dMetalDiffusionCoeff, dThermalDiffusionCoeff;
+    double dResolveJeans;
+    int bCannonical, iViscosityLimiter, bConstantDiffusion, pad;
+} gg_smf;
+
+/* Smooth operator ids, numbered as enum smx_smoothtype (smoothfcn.h:147-170)
+ * of a -DGASOLINE build,
+ * for the three this library implements. */
+#define GG_SMX_DENSITY           1
+#define GG_SMX_SPHPRESSURETERMS  21
+#define GG_SMX_DENDVDX           22
+
+/* Build the SPH search structure over the gas particles (TYPE_GAS, or all
+ * particles if none is flagged).  Replaces pkdBuildLocal (pkd.h:795,
+ * pkd.c:2732-2848); the neighbour sets found below do not depend on the
+ * tree's shape.  fPeriod[3] is the box when bPeriodic. */
+int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const double *fPeriod);
+
+/* kNN smooth: for every smooth-active particle, the nSmooth nearest
+ * neighbours by FP64 distance (minimum periodic image), fBall2 = largest
+ * squared distance, then the operator iSmoothType (GG_SMX_DENSITY or
+ * GG_SMX_DENDVDX) over that list.  Replaces smInitialize + smSmooth + smFinish
+ * (smooth.h:616-617,638; smooth.c:1439-1713) for those operators.
+ * *pnSmoothed receives smf->nSmoothed. */
+int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+              int bSymmetric, int iSmoothType, int *pnSmoothed);
+
+/* Ball-gather re-smooth at each particle's stored fBall2 (d2 <= fBall2) and
+ * the operator iSmoothType (GG_SMX_SPHPRESSURETERMS).  The symmetric
+ * operator is evaluated in gather form over the neighbour and reverse-
+ * neighbour lists, so no scatter or atomics are used.  Replaces smInitialize
+ * + smReSmooth + smFinish (smooth.h:636; smooth.c:1716-1749) with
+ * SphPressureTermsSym (smoothfcn.c:2276-2385). */
+int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+                int bSymmetric, int iSmoothType);
+
+/* Adiabatic equation of state: PoverRho2, c and the Courant dt update for
+ * gas particles.  Replaces pkdGasPressure (pkd.h:912, pkd.c:5750-5785)
+ * without cooling. */
+int gg_gas_pressure(GGPKD pkd, const gg_smf *smf);
+
+/* Neighbour lists of the last gg_smooth, for parity checks: nbr[i*nSmooth+k]
+ * in caller order (sorted by distance, then index). */
+int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr);
+
+/* ------------------------------------------------------------ timing */
+
+/* Device time in milliseconds of the kernels of the last call, by phase:
+ * t[0] walk, t[1] interact, t[2] SPH structure, t[3] kNN, t[4] density,
+ * t[5] gather lists, t[6] hydro, t[7] other.  Measured with HIP events on
+ * the handle's stream. */
+int gg_timings(GGPKD pkd, double *t, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
