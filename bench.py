@@ -1,0 +1,223 @@
+"""Benchmark of the force hot path on one MI355X.
+
+Workload (the SPH adiabatic box): N = 2^20 gas particles uniform in a
+periodic unit cube, equal mass, u = 1, gamma = 5/3, Gaussian velocities
+(sigma 0.1), M4 kernel, nSmooth = 64, Monaghan viscosity alpha = 1,
+beta = 2.  One step = gg_build_local + gg_smooth(DenDVDX) + gg_gas_pressure
++ gg_resmooth(SphPressureTermsSym), inputs resident in HBM, the previous
+step's smoothing lengths discarded so every kNN search starts cold.
+
+value = SPH pair evaluations per second (density pass + force pass, each
+pass counted as sum_p |N(p)|), FP32 pair arithmetic.  The same step in plain
+FP64 is timed beside it.  A gravity leg (Plummer sphere N = 2^20, theta 0.7,
+quadrupole, bucket 16; walk + interaction, tree build excluded) reports
+interactions/s in both precisions.  The CPU restatement of the same path is
+timed on a bounded sample on the host cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grav interactions/s + SPH pair-evals/s at rms force err<=1e-4, vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+DENS_BYTES_PER_PAIR = 36     # j-side FP32 record: r 12, m 4, vPred 12, uPred 4, flags 4
+HYDRO_BYTES_PER_PAIR = 52    # + rho, P/rho^2, c, Balsara, dt
+
+
+def sph_setup(gg, n, prec, seed=2):
+    from gasoline_amd import ics
+    d = ics.sph_box(n, seed=seed)
+    pk = gg.Pkd(n, prec=prec)
+    pk.put("r", d["r"])
+    pk.put("fMass", d["m"])
+    pk.put("vPred", d["v"])
+    pk.put("uPred", d["u"])
+    return pk
+
+
+def sph_step(pk, smf, zero_fb2, K=64):
+    pk.put("fBall2", zero_fb2)           # device-to-device: cold kNN search
+    pk.build_local(period=[1.0, 1.0, 1.0])
+    pk.smooth(smf, K, True)
+    pk.gas_pressure(smf)
+    pk.resmooth(smf, K, True)
+
+
+def time_sph(gg, torch, n, prec, steps, warmup):
+    smf = gg.Smf.default()
+    pk = sph_setup(gg, n, prec)
+    zero = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        sph_step(pk, smf, zero)
+    torch.cuda.synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sph_step(pk, smf, zero)
+        t = pk.timings()
+        phases.append(t)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    pk.close()
+    return dt, ph
+
+
+def time_grav(gg, torch, n, prec, steps, warmup):
+    from gasoline_amd import ics
+    r, v, m = ics.plummer(n, seed=1)
+    soft = np.full(n, 0.01)
+    pk = gg.Pkd(n, prec=prec)
+    pk.put("r", r)
+    pk.put("fMass", m)
+    pk.put("fSoft", soft)
+    pk.build_binary(16, 1, 0.7, 2)
+    st = None
+    for _ in range(warmup):
+        st = pk.grav_all(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    walk, inter = [], []
+    for _ in range(steps):
+        st = pk.grav_all(2)
+        t = pk.timings()
+        walk.append(t["walk"])
+        inter.append(t["interact"])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    pk.close()
+    nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
+    return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter))
+
+
+def cpu_baseline_sph(n=1 << 17):
+    """The oracle's SPH step (kNN + DenDVDX + EOS + SphPressureTermsSym) on a
+    periodic box of n particles, pair-evals/s on the host cores."""
+    from gasoline_amd import ics
+    from oracle import oracle as O
+    d = ics.sph_box(n, seed=3)
+    smf = O.default_smf()
+    t0 = time.perf_counter()
+    nbr, fb2, _ = O.knn(d["r"], 64, period=[1, 1, 1])
+    lists = O.ball_gather(d["r"], fb2, period=[1, 1, 1])
+    den = O.dendvdx(d["m"], d["v"], d["u"], fb2, lists, smf)
+    pr2, c, dt = O.gas_pressure(d["u"], den["rho"], fb2, smf)
+    O.sph_pressure_sym(d["m"], d["v"], d["u"], den["rho"], pr2, c, den["balsara"],
+                       den["diff"], dt, fb2, lists, smf)
+    el = time.perf_counter() - t0
+    pairs = 2 * len(lists[1])
+    return pairs / el, el
+
+
+def cpu_baseline_grav(n=32768):
+    from gasoline_amd import ics
+    from oracle import oracle as O
+    r, v, m = ics.plummer(n, seed=1)
+    soft = np.full(n, 0.01)
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=2)
+    t0 = time.perf_counter()
+    _, _, _, st = T.grav_all()
+    el = time.perf_counter() - t0
+    return (st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]) / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--ngrav", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gasoline_amd as gg
+
+    n = args.n
+    K = 64
+    pairs = 2 * K * n                      # density pass + force pass
+    t32, ph32 = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup)
+    t64, ph64 = time_sph(gg, torch, n, gg.PREC_FP64, max(2, args.steps // 2), 1)
+    g32 = time_grav(gg, torch, args.ngrav, gg.PREC_FP32, args.steps, args.warmup)
+    g64 = time_grav(gg, torch, args.ngrav, gg.PREC_FP64, max(2, args.steps // 2), 1)
+
+    # roofline of the dominant SPH kernel (HBM-bound gathers): algorithmic
+    # bytes of the j-side records over its HIP-event duration
+    kern = max(("hydro", HYDRO_BYTES_PER_PAIR), ("density", DENS_BYTES_PER_PAIR),
+               key=lambda kv: ph32[kv[0]])
+    kname, bpp = kern
+    kms = ph32[kname]
+    achieved = K * n * bpp / (kms * 1e-3) / 1e9
+    g_inter_s = g32[1] / (g32[0])
+    line = {
+        "metric": METRIC,
+        "value": pairs / t32,
+        "unit": "SPH pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "SPH adiabatic box: N=%d periodic unit cube, nSmooth=64, M4, "
+                        "alpha=1 beta=2, one DenDVDX + one SphPressureTermsSym pass "
+                        "(tree build, kNN, EOS and reverse lists inside the step)" % n,
+            "n": n, "nSmooth": K, "pairs_per_step": pairs,
+            "gravity_workload": "Plummer N=%d, eps=0.01, theta=0.7, iOrder=2, nBucket=16; "
+                                "walk+interact (tree build excluded)" % args.ngrav,
+        },
+        "fp64_value": pairs / t64,
+        "speedup_vs_fp64": t64 / t32,
+        "phases_ms_fp32": ph32,
+        "phases_ms_fp64": ph64,
+        "grav_interactions_per_s": g_inter_s,
+        "grav_interactions_per_s_fp64": g64[1] / g64[0],
+        "grav_speedup_vs_fp64": g64[0] / g32[0],
+        "grav_ms_walk": g32[3], "grav_ms_interact_fp32": g32[4], "grav_ms_interact_fp64": g64[4],
+        "roofline": {
+            "bound": "hbm", "kernel": "k_%s" % kname, "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "roofline_gravity": {
+            "bound": "valu", "kernel": "k_interact<float,2>",
+            "achieved": g32[2] / (g32[4] * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": g32[2] / (g32[4] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        v_sph, el_sph = cpu_baseline_sph()
+        v_gr, el_gr = cpu_baseline_grav()
+        line["cpu_baseline"] = {
+            "value": v_sph, "unit": "SPH pair-evals/s", "cores": cores, "kind": "port",
+            "sample": "oracle SPH step on a periodic box of 131072 particles (%.1f s): "
+                      "kNN and DenDVDX on %d OpenMP threads, the symmetric force pass "
+                      "serial as in the reference" % (el_sph, cores),
+            "grav_interactions_per_s": v_gr,
+            "grav_sample": "oracle gravity (1 thread) on Plummer N=32768 (%.2f s)" % el_gr,
+        }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

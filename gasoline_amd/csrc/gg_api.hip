@@ -189,10 +189,11 @@ void timer_mark(GGPKD pkd, int slot)
 // the phase slots (the stream must have been synchronised).
 void timer_collect(GGPKD pkd, int first, int nphases, const int *phase)
 {
+    for (int k = 0; k < nphases; ++k) pkd->tm.ms[phase[k]] = 0;
     for (int k = 0; k < nphases; ++k) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, pkd->tm.ev[first + k], pkd->tm.ev[first + k + 1]) == hipSuccess)
-            pkd->tm.ms[phase[k]] = ms;
+            pkd->tm.ms[phase[k]] += ms;
     }
 }
 
@@ -301,6 +302,6 @@ extern "C" int gg_get(GGPKD pkd, int field, void *dst, int bDevice)
 extern "C" int gg_timings(GGPKD pkd, double *t, int n)
 {
     GG_CHECK(pkd && t, GG_EINVAL, "null argument");
-    for (int k = 0; k < n && k < 8; ++k) t[k] = pkd->tm.ms[k];
+    for (int k = 0; k < n && k < 12; ++k) t[k] = pkd->tm.ms[k];
     return GG_OK;
 }

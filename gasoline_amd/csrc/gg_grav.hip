@@ -773,7 +773,7 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
         timer_mark(pkd, 3);
         GG_HIP(hipMemcpyAsync(hs, G.stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
-        const int phase[3] = {0, 1, 7};
+        const int phase[3] = {0, 1, 8};
         timer_collect(pkd, 0, 3, phase);
     }
     if (nActive) *nActive = (int)hs[0];

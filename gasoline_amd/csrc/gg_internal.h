@@ -95,7 +95,7 @@ struct SphDev {
 struct Timers {
     hipEvent_t ev[16];
     int created = 0;
-    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace gg

@@ -1057,7 +1057,6 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_CHECK(nSmooth <= S.n, GG_EINVAL, "nSmooth %d exceeds the %d particles", nSmooth, S.n);
     hipStream_t st = pkd->stream;
     const int n = S.n, K = nSmooth;
-    timer_mark(pkd, 0);
     GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
     GG_TRY(S.nbr.reserve(sizeof(int) * (size_t)K * n));
     GG_TRY(S.fb2s.reserve(sizeof(double) * n));
@@ -1067,6 +1066,7 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     QArgs Q = query_args(pkd, 0, K);
     Q.guess = S.guess.as<double>();
     Q.nbr = S.nbr.as<int>();
+    timer_mark(pkd, 0);
     hipLaunchKernelGGL(k_query, dim3(n), dim3(64), 0, st, Q);
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 1);
@@ -1118,12 +1118,13 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     D.diff = S.o_diff.as<double>();
     D.nbract = S.nba.as<uint32_t>();
     const int grid = nblk(n * 16, 256);
+    timer_mark(pkd, 2);
     if (pkd->prec == GG_PREC_FP32)
         hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
     else
         hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 2);
+    timer_mark(pkd, 3);
     unsigned e[2];
     GG_TRY(read_err(pkd, e));
     S.fails = e[0];
@@ -1156,10 +1157,10 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
                               S.sortTmp, &nq));
     S.listQueries = (int)nq;
     if (pnSmoothed) *pnSmoothed = (int)nq;
-    timer_mark(pkd, 3);
+    timer_mark(pkd, 4);
     GG_HIP(hipStreamSynchronize(st));
-    const int ph[3] = {3, 4, 7};
-    timer_collect(pkd, 0, 3, ph);
+    const int ph[4] = {3, 8, 4, 8};
+    timer_collect(pkd, 0, 4, ph);
     return GG_OK;
 }
 
@@ -1261,6 +1262,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     hipLaunchKernelGGL(k_rev_sort, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.roff.as<int64_t>(),
                        S.ridx.as<int>());
     GG_HIP(hipGetLastError());
+    timer_mark(pkd, 2);
     // inputs in slot order
     GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
     GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
@@ -1309,12 +1311,13 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.bCannonical = smf->bCannonical;
     H.out = S.res.as<double>();
     const int grid = nblk(n * 16, 256);
+    timer_mark(pkd, 3);
     if (pkd->prec == GG_PREC_FP32)
         hipLaunchKernelGGL(k_hydro<float>, dim3(grid), dim3(256), 0, st, H);
     else
         hipLaunchKernelGGL(k_hydro<double>, dim3(grid), dim3(256), 0, st, H);
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 2);
+    timer_mark(pkd, 4);
     const int fl[] = {GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW};
     for (int f : fl) GG_TRY(ensure_field(pkd, f));
     hipLaunchKernelGGL(k_hydro_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n,
@@ -1327,10 +1330,10 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
                        S.qf.as<uint32_t>(), (const uint32_t *)nullptr,
                        pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 3);
+    timer_mark(pkd, 5);
     GG_HIP(hipStreamSynchronize(st));
-    const int ph[3] = {5, 6, 7};
-    timer_collect(pkd, 0, 3, ph);
+    const int ph[5] = {5, 6, 8, 7, 8};
+    timer_collect(pkd, 0, 5, ph);
     return GG_OK;
 }
 

@@ -236,7 +236,7 @@ class Pkd:
         return out
 
     def timings(self):
-        t = np.zeros(8)
-        _check(lib().gg_timings(self.h, _p(t), 8))
-        return dict(zip(["walk", "interact", "sph_tree", "knn", "density", "gather",
-                         "hydro", "other"], t))
+        t = np.zeros(9)
+        _check(lib().gg_timings(self.h, _p(t), 9))
+        return dict(zip(["walk", "interact", "sph_tree", "knn", "density", "lists",
+                         "revlists", "hydro", "other"], t))

@@ -209,10 +209,11 @@ int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr);
 
 /* ------------------------------------------------------------ timing */
 
-/* Device time in milliseconds of the kernels of the last call, by phase:
- * t[0] walk, t[1] interact, t[2] SPH structure, t[3] kNN, t[4] density,
- * t[5] gather lists, t[6] hydro, t[7] other.  Measured with HIP events on
- * the handle's stream. */
+/* Device time in milliseconds of the last call of each kind, by phase:
+ * t[0] tree walk (list build), t[1] interaction kernel, t[2] SPH structure,
+ * t[3] kNN kernel, t[4] density kernel, t[5] resmooth lists, t[6] reverse
+ * lists, t[7] hydro kernel, t[8] other (gathers, scatters, counters).
+ * Measured with HIP events on the handle's stream; n <= 12. */
 int gg_timings(GGPKD pkd, double *t, int n);
 
 #ifdef __cplusplus

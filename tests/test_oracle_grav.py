@@ -1,0 +1,222 @@
+"""Pins of the gravity oracle (oracle/grav_oracle.c) that do not retype its
+formulas: the reference's own v_sqrt1.c, closed forms and derivative
+identities of the spline kernels, tree invariants, the walk's coverage
+invariant and convergence to an O(N^2) direct sum."""
+import numpy as np
+import pytest
+
+from gasoline_amd import ics
+from oracle import oracle as O
+
+
+# ------------------------------------------------------------- v_sqrt1
+def _r2_samples():
+    rng = np.random.default_rng(0)
+    x = 10.0 ** rng.uniform(-300, 300, 20000)
+    edge = np.array([1.0, 2.0, 4.0, 0.25, np.nextafter(1.0, 2), np.nextafter(2.0, 1), 1e-308,
+                     3.0, 1.5, 1e308])
+    return np.concatenate([x, edge])
+
+
+def test_vsqrt1_bitexact_vs_reference_source():
+    r2 = _r2_samples()
+    ref = O.ref_vsqrt1(r2)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    mine = O.vsqrt1(r2)
+    assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
+
+
+def test_vsqrt1_accuracy():
+    r2 = _r2_samples()
+    r2 = r2[r2 >= np.finfo(np.float64).tiny]     # subnormals are outside its range
+    rel = np.abs(O.vsqrt1(r2) * np.sqrt(r2) - 1.0)
+    assert rel.max() < 1e-9
+
+
+# ------------------------------------------------------------- splines
+def test_spline_newtonian_beyond_twoh():
+    for r in [0.02, 0.05, 1.0, 7.5]:
+        a, b = O.spline(r * r, 0.02)
+        assert a == pytest.approx(1 / r, rel=1e-15)
+        assert b == pytest.approx(1 / r ** 3, rel=1e-15)
+
+
+def test_spline_origin_values():
+    # grav.h:81-83: a(0) = 1.4 * dih with dih = 2/twoh; b(0) = 4/3 dih^3
+    twoh = 0.3
+    dih = 2 / twoh
+    a, b = O.spline(0.0, twoh)
+    assert a == pytest.approx(1.4 * dih, rel=1e-15)
+    assert b == pytest.approx(4.0 / 3.0 * dih ** 3, rel=1e-15)
+
+
+@pytest.mark.parametrize("twoh", [0.1, 1.0])
+def test_spline_continuity_and_force_identity(twoh):
+    # continuity at u = 1 and u = 2, and b = -(1/r) da/dr (finite difference)
+    for r0 in [0.5 * twoh, twoh]:
+        lo = O.spline((r0 * (1 - 1e-9)) ** 2, twoh)
+        hi = O.spline((r0 * (1 + 1e-9)) ** 2, twoh)
+        assert lo[0] == pytest.approx(hi[0], rel=1e-7)
+        assert lo[1] == pytest.approx(hi[1], rel=1e-7)
+    for r in np.linspace(0.05, 1.5, 31) * twoh:
+        h = 1e-6 * twoh
+        da = (O.spline((r + h) ** 2, twoh)[0] - O.spline((r - h) ** 2, twoh)[0]) / (2 * h)
+        assert O.spline(r * r, twoh)[1] == pytest.approx(-da / r, rel=1e-5)
+
+
+def test_splineq_derivative_chain():
+    # SPLINEQ: a,b as SPLINEM; c = -(1/r) db/dr; d = -(1/r) dc/dr
+    twoh = 0.4
+    for r in np.linspace(0.05, 1.6, 25) * twoh:
+        a, b, c, d = O.splineq(1 / r, r * r, twoh)
+        a2, b2 = O.spline(r * r, twoh)
+        assert a == pytest.approx(a2, rel=1e-12)
+        assert b == pytest.approx(b2, rel=1e-12)
+        h = 1e-6 * twoh
+        qp = O.splineq(1 / (r + h), (r + h) ** 2, twoh)
+        qm = O.splineq(1 / (r - h), (r - h) ** 2, twoh)
+        assert c == pytest.approx(-(qp[1] - qm[1]) / (2 * h) / r, rel=1e-5)
+        assert d == pytest.approx(-(qp[2] - qm[2]) / (2 * h) / r, rel=1e-5)
+
+
+# ---------------------------------------------------------------- tree
+@pytest.fixture(scope="module")
+def plummer():
+    r, v, m = ics.plummer(8192, seed=3)
+    return r, m, np.full(len(m), 0.01)
+
+
+def test_tree_invariants(plummer):
+    r, m, soft = plummer
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=4)
+    perm = T.perm()
+    assert np.array_equal(np.sort(perm), np.arange(len(m)))
+    ints, dbls = T.nodes()
+    rs, ms = r[perm], m[perm]
+    nn = len(ints)
+    visited = []
+    ic = T.root
+    while ic != -1:                 # threaded walk visits every node once
+        visited.append(ic)
+        ic = ints[ic, 3] if ints[ic, 0] != -1 else ints[ic, 4]
+    assert sorted(visited) == list(range(nn))
+    for c in range(nn):
+        idim, lo, hi, il, inext = ints[c, :5]
+        sl = slice(lo, hi + 1)
+        pts, mm = rs[sl], ms[sl]
+        assert np.array_equal(dbls[c, 1:4], pts.min(0)) and np.array_equal(dbls[c, 4:7], pts.max(0))
+        assert dbls[c, 7] == pytest.approx(mm.sum(), rel=1e-13)
+        com = (mm[:, None] * pts).sum(0) / mm.sum()
+        assert np.allclose(dbls[c, 9:12], com, rtol=0, atol=1e-13 * (1 + np.abs(com).max()))
+        d = pts - dbls[c, 9:12]
+        d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        bmax = np.sqrt(d2).max()
+        assert dbls[c, 44] == bmax
+        dopen = max(2 / np.sqrt(3.0) * bmax / 0.7, bmax)
+        assert dbls[c, 12] == dopen * dopen
+        Q = [(mm * d[:, 0] * d[:, 0]).sum(), (mm * d[:, 1] * d[:, 1]).sum(),
+             (mm * d[:, 2] * d[:, 2]).sum(), (mm * d[:, 0] * d[:, 1]).sum(),
+             (mm * d[:, 0] * d[:, 2]).sum(), (mm * d[:, 1] * d[:, 2]).sum()]
+        scale = (mm * d2).sum() + 1e-300
+        assert np.allclose(dbls[c, 13:19], Q, rtol=0, atol=1e-12 * scale)
+        if idim == -1:
+            ext = (pts.max(0) - pts.min(0)).max()
+            assert hi - lo + 1 <= 16 or ext == 0
+        else:
+            assert il == c + 1
+            # children split the parent at its midpoint plane
+            lo_child = ints[il]
+            up = lo_child[4]
+            assert lo_child[1] == lo and ints[up, 2] == hi and ints[up, 1] == lo_child[2] + 1
+            assert (rs[lo:lo_child[2] + 1, idim] < dbls[c, 0]).all()
+            assert (rs[ints[up, 1]:hi + 1, idim] >= dbls[c, 0]).all()
+
+
+def test_walk_covers_every_particle_once(plummer):
+    r, m, soft = plummer
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=2)
+    ints, dbls = T.nodes()
+    n = len(m)
+    perm = T.perm()
+    rs = r[perm]
+    for ib in T.buckets()[::7]:
+        part, scell, ncell = T.walk(ib)
+        cover = np.zeros(n, np.int64)
+        np.add.at(cover, part, 1)
+        lo, hi = ints[ib, 1], ints[ib, 2]
+        cover[lo:hi + 1] += 1
+        bmin, bmax = rs[lo:hi + 1].min(0), rs[lo:hi + 1].max(0)
+        for c in np.concatenate([scell, ncell]):
+            cover[ints[c, 1]:ints[c, 2] + 1] += 1
+            # accepted cells: c.o.m. outside the opening sphere of the bucket
+            x = dbls[c, 9:12]
+            dd = np.where(x < bmin, bmin - x, np.where(x > bmax, x - bmax, 0.0))
+            assert (dd * dd).sum() > dbls[c, 12]
+            assert ints[c, 2] - ints[c, 1] >= 3
+        assert (cover == 1).all()
+
+
+def _rms_err(a, ad):
+    e = np.linalg.norm(a - ad, axis=1) / np.linalg.norm(ad, axis=1)
+    return np.sqrt(np.mean(e ** 2))
+
+
+def test_tree_converges_to_direct_sum(plummer):
+    r, m, soft = plummer
+    idx = np.arange(0, len(m), 8, dtype=np.int32)
+    ad, pd = O.direct_sum(r, m, soft, idx)
+    errs = {}
+    for theta, order in [(0.7, 1), (0.7, 2), (0.7, 4), (0.4, 4)]:
+        T = O.Tree(r, m, soft, nBucket=16, dTheta=theta, iOrder=order)
+        a, pot, dtg, st = T.grav_all()
+        errs[(theta, order)] = _rms_err(a[idx], ad)
+        assert st["nActive"] == len(m)
+    # each multipole order and smaller opening angle must reduce the error:
+    # a wrong sign or index in a multipole term breaks this ordering
+    assert errs[(0.7, 2)] < 0.7 * errs[(0.7, 1)]
+    assert errs[(0.7, 4)] < 0.7 * errs[(0.7, 2)]
+    assert errs[(0.4, 4)] < 0.5 * errs[(0.7, 4)]
+    assert errs[(0.7, 2)] < 5e-3
+    assert errs[(0.4, 4)] < 2e-4
+
+
+def test_single_particle_buckets_equal_point_masses():
+    # with nBucket = 1 every cell holding one particle is exact: a tree with
+    # tiny theta reproduces the direct sum to rounding
+    r, v, m = ics.plummer(600, seed=4)
+    soft = np.full(len(m), 0.01)
+    T = O.Tree(r, m, soft, nBucket=1, dTheta=0.05, iOrder=2)
+    a, pot, _, _ = T.grav_all()
+    ad, pd = O.direct_sum(r, m, soft)
+    assert _rms_err(a, ad) < 1e-6
+    assert np.max(np.abs(pot / pd - 1)) < 1e-6
+
+
+def test_direct_sum_matches_plummer_potential():
+    r, v, m = ics.plummer(32768, seed=5)
+    soft = np.full(len(m), 1e-3)
+    rng = np.random.default_rng(1)
+    idx = rng.choice(len(m), 256, replace=False).astype(np.int32)
+    _, pd = O.direct_sum(r, m, soft, idx)
+    rad = np.linalg.norm(r[idx], axis=1)
+    inner = rad < 3
+    phi = -1 / np.sqrt(rad ** 2 + 1) * 0.999 ** 0   # truncation at 0.999 in mass
+    assert np.median(np.abs(pd[inner] / phi[inner] - 1)) < 0.02
+
+
+def test_counters_follow_reference_formula(plummer):
+    r, m, soft = plummer
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=2)
+    _, _, _, st = T.grav_all()
+    ints, _ = T.nodes()
+    part = cell = soft_n = flop = 0
+    for ib in T.buckets():
+        p, s, c = T.walk(ib)
+        nb = ints[ib, 2] - ints[ib, 1] + 1
+        part += nb * len(p) + nb * (2 * (ints[ib, 2] - ints[ib, 1]) - nb + 1) // 2
+        cell += nb * len(c)
+        soft_n += nb * len(s)
+        flop += nb * ((len(p) + nb) * 38 + len(s) * 82 + len(c) * (35 + 41))
+    assert st["dPartSum"] == part and st["dCellSum"] == cell and st["dSoftSum"] == soft_n
+    assert st["dFlop"] == flop
