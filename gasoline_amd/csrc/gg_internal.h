@@ -71,12 +71,14 @@ struct SphDev {
     int built = 0, n = 0, bPeriodic = 0, nLeaf = 0, nL2 = 1, nBucket = 0;
     double L[3] = {0, 0, 0};
     int listK = 0, listValid = 0, listQueries = 0;
-    unsigned ties = 0, fails = 0;
+    unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0};
+    int64_t nExtra = 0;
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
     DevBuf sx, sy, sz, nodeBox;                // slot order positions, boxes
     DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
     DevBuf gcnt, goff, gent;                   // ball-gather CSR
-    DevBuf rcnt, roff, rcur, ridx;             // reverse lists
+    DevBuf rcnt, roff, rcur, ridx;             // extras lists
+    DevBuf retry, nretry, hrec;                // kNN retries, force records
     DevBuf qf, act, nba, err, guess, diffcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
@@ -85,7 +87,7 @@ struct SphDev {
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
                          &sx, &sy, &sz, &nodeBox, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &qf, &act, &nba,
-                         &err, &guess, &diffcnt, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
+                         &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();

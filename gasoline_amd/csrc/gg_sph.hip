@@ -139,7 +139,7 @@ __global__ void k_morton(int n, const int *__restrict__ gidx, const double *__re
     uint64_t q[3];
     for (int d = 0; d < 3; ++d) {
         double t = (r[3 * (size_t)i + d] - lo[d]) * sc[d];
-        t = fmin(fmax(t, 0.0), 2097151.0);
+        t = fmin(fmax(t, 0.0), 65535.0);
         q[d] = (uint64_t)t;
     }
     key[k] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
@@ -206,6 +206,7 @@ struct QArgs {
     int *cnt;                          // gather count out
     const int64_t *off;                // gather fill offsets
     unsigned int *err;                 // [0] failures, [1] boundary ties
+    const int *qlist;                  // optional: block b serves slot qlist[b]
 };
 
 __device__ __forceinline__ bool kv_less(double da, int ca, double db, int cb)
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
     __shared__ double cd[CAP];
     __shared__ int cc[CAP], cs[CAP];
     const int lane = threadIdx.x;
-    const int q = blockIdx.x;
+    const int q = A.qlist ? A.qlist[blockIdx.x] : (int)blockIdx.x;
     if (!A.qflag[q]) {
         if (A.mode == 1 && lane == 0) A.cnt[q] = 0;
         return;
@@ -434,6 +435,351 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
         atomicAdd(&A.err[0], 1u);
         if (A.mode == 0) A.fb2[q] = -1.0;
         if (A.mode == 1) A.cnt[q] = 0;
+    }
+}
+
+// ---------------------------------------------- kNN by leaf groups
+// One 256-lane workgroup per leaf of 32 query particles.  The leaves within
+// a group radius RL of the leaf's box are found once and their positions
+// staged in LDS as FP32 offsets from the leaf-box centre.  Each wavefront
+// then settles its queries: FP32 squared distances (kept in registers) and
+// ballot counts choose a squared radius T <= RL^2 whose ball holds a little
+// more than K candidates; every candidate whose FP32 distance is within T
+// plus a bound on the FP32 rounding error is re-measured in FP64 from the
+// global positions, so the set S handed to the sort contains every particle
+// with FP64 d2 <= T.  If at least K of them have d2 <= T, the first K of S
+// sorted by (d2, caller index) are the exact K nearest.  Queries that cannot
+// be settled this way go to a retry list served by k_query.
+constexpr int KL_CM = 4096;            // staged candidates per group
+constexpr int KL_ML = KL_CM / LEAF;    // staged leaves per group
+constexpr int KL_NV = KL_CM / 64;      // candidates per lane
+
+struct KLArgs {
+    int n, nLeaf, nL2, K, per;
+    double L[3];
+    const double *sx, *sy, *sz, *box;
+    const int *order;
+    const uint32_t *qflag;
+    const double *guess;               // previous fBall2 (<= 0: none)
+    double *fb2;
+    int *nbr;
+    int *retry;
+    unsigned *nretry;
+    unsigned *err;
+};
+
+// Squared gap between two boxes, minimised over periodic images of a.
+__device__ __forceinline__ double boxbox_d2(const double *a, const double *b, int per,
+                                            const double *L)
+{
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double g = fmax(0.0, fmax(a[k] - b[3 + k], b[k] - a[3 + k]));
+        if (per) {
+            const double g1 = fmax(0.0, fmax(a[k] + L[k] - b[3 + k], b[k] - a[3 + k] - L[k]));
+            const double g2 = fmax(0.0, fmax(a[k] - L[k] - b[3 + k], b[k] - a[3 + k] + L[k]));
+            g = fmin(g, fmin(g1, g2));
+        }
+        d2 += g * g;
+    }
+    return d2;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// minimum image of an offset (periodic) or the offset itself
+__device__ __forceinline__ double wrap(double t, int per, double L)
+{
+    if (per) {
+        if (t > 0.5 * L) t -= L;
+        else if (t < -0.5 * L) t += L;
+    }
+    return t;
+}
+
+// Bitonic sort of 128 (d2, caller, slot) triples held 2 per lane
+// (element e = lane + 64 k), ascending in (d2, caller).
+__device__ void bitonic128(double (&d)[2], int (&c)[2], int (&s)[2], int lane)
+{
+    for (int size = 2; size <= 128; size <<= 1) {
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int e = lane;
+                const bool asc = (e & size) == 0;
+                const bool sw = asc ? kv_less(d[1], c[1], d[0], c[0]) : kv_less(d[0], c[0], d[1], c[1]);
+                if (sw) {
+                    const double td = d[0]; d[0] = d[1]; d[1] = td;
+                    const int tc = c[0]; c[0] = c[1]; c[1] = tc;
+                    const int ts = s[0]; s[0] = s[1]; s[1] = ts;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int e = lane + 64 * k;
+                    const double pd = __shfl_xor(d[k], j);
+                    const int pc = __shfl_xor(c[k], j);
+                    const int ps = __shfl_xor(s[k], j);
+                    const bool lower = (lane & j) == 0;
+                    const bool asc = (e & size) == 0;
+                    const bool take = (lower == asc) ? kv_less(pd, pc, d[k], c[k])
+                                                     : kv_less(d[k], c[k], pd, pc);
+                    if (take) {
+                        d[k] = pd;
+                        c[k] = pc;
+                        s[k] = ps;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_knn_leaf(KLArgs A)
+{
+    __shared__ int fr[2][FCAP];
+    __shared__ int lf[KL_ML];
+    __shared__ float cx[KL_CM], cy[KL_CM], cz[KL_CM];
+    __shared__ double bd[4][CAP];
+    __shared__ int bc[4][CAP], bs[4][CAP];
+    __shared__ int s_cnt[2];
+    __shared__ double s_R2;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int leaf = blockIdx.x;
+    const int s0 = leaf * LEAF, nq = min(LEAF, A.n - s0);
+    const int K = A.K;
+    const double *lb = A.box + 6 * (size_t)(A.nL2 + leaf);
+    const double ox = 0.5 * (lb[0] + lb[3]), oy = 0.5 * (lb[1] + lb[4]), oz = 0.5 * (lb[2] + lb[5]);
+    if (w == 0) {
+        double g = 0.0;
+        bool none = false;
+        if (lane < nq) {
+            const double v = A.guess ? A.guess[s0 + lane] : 0.0;
+            none = !(v > 0);
+            g = none ? 0.0 : v;
+        }
+        for (int o = 32; o > 0; o >>= 1) g = fmax(g, __shfl_xor(g, o));
+        const unsigned long long nh = __ballot(none);
+        if (lane == 0) {
+            double R2;
+            if (nh == 0) {
+                R2 = g * 1.44;
+            } else {   // leaf density estimate, ~2K particles in the ball
+                const double ex = lb[3] - lb[0], ey = lb[4] - lb[1], ez = lb[5] - lb[2];
+                double vol = ex * ey * ez;
+                const double emax = fmax(ex, fmax(ey, ez));
+                if (!(vol > 0)) vol = emax * emax * emax;
+                double r3 = vol * (2.0 * K) / fmax(nq, 1) * (3.0 / (4.0 * M_PI));
+                if (!(r3 > 0)) r3 = 1e-30;
+                R2 = cbrt(r3 * r3);
+            }
+            s_R2 = R2;
+            s_cnt[0] = 0;
+            s_cnt[1] = 0;
+            fr[0][0] = 1;
+        }
+    }
+    __syncthreads();
+    const double RL2 = s_R2;
+    const double RL2b = RL2 * (1.0 + 1e-12);
+    int nf = 1, cur = 0;
+    bool ovf = false;
+    while (nf > 0) {
+        for (int k = t; k < nf; k += 256) {
+            const int node = fr[cur][k];
+            if (boxbox_d2(A.box + 6 * (size_t)node, lb, A.per, A.L) <= RL2b) {
+                if (node >= A.nL2) {
+                    const int p = atomicAdd(&s_cnt[1], 1);
+                    if (p < KL_ML) lf[p] = node - A.nL2;
+                } else {
+                    const int p = atomicAdd(&s_cnt[0], 2);
+                    if (p + 1 < FCAP) {
+                        fr[cur ^ 1][p] = 2 * node;
+                        fr[cur ^ 1][p + 1] = 2 * node + 1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        nf = s_cnt[0];
+        const int nlv = s_cnt[1];
+        __syncthreads();
+        if (t == 0) s_cnt[0] = 0;
+        if (nf > FCAP || nlv > KL_ML) {
+            ovf = true;
+            break;
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    if (ovf) {             // group too large to stage: every query retries
+        if (t == 0) atomicAdd(&A.err[2], 1u);
+        if (t < nq && A.qflag[s0 + t]) {
+            const unsigned p = atomicAdd(A.nretry, 1u);
+            A.retry[p] = s0 + t;
+        }
+        return;
+    }
+    const int nl = s_cnt[1];
+    if (t == 0) atomicMax(&A.err[5], (unsigned)nl);
+    const int nc = nl * LEAF;
+    // staged offsets from the leaf-box centre; M bounds their magnitude
+    float mloc = 0.f;
+    for (int c = t; c < nc; c += 256) {
+        const int slot = lf[c / LEAF] * LEAF + (c % LEAF);
+        float fx = 3.0e30f, fy = 3.0e30f, fz = 3.0e30f;
+        if (slot < A.n) {
+            fx = (float)wrap(A.sx[slot] - ox, A.per, A.L[0]);
+            fy = (float)wrap(A.sy[slot] - oy, A.per, A.L[1]);
+            fz = (float)wrap(A.sz[slot] - oz, A.per, A.L[2]);
+            mloc = fmaxf(mloc, fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))));
+        }
+        cx[c] = fx;
+        cy[c] = fy;
+        cz[c] = fz;
+    }
+    for (int o = 32; o > 0; o >>= 1) mloc = fmaxf(mloc, __shfl_xor(mloc, o));
+    if (lane == 0) atomicMax((int *)&s_cnt[0], __float_as_int(mloc));   // s_cnt[0] is 0 here
+    __syncthreads();
+    const double M = (double)__int_as_float(s_cnt[0]);
+    for (int j = w; j < nq; j += 4) {
+        const int q = s0 + j;
+        if (!A.qflag[q]) continue;
+        const double x = A.sx[q], y = A.sy[q], z = A.sz[q];
+        const float qx = (float)wrap(x - ox, A.per, A.L[0]);
+        const float qy = (float)wrap(y - oy, A.per, A.L[1]);
+        const float qz = (float)wrap(z - oz, A.per, A.L[2]);
+        float dd[KL_NV];
+#pragma unroll
+        for (int k = 0; k < KL_NV; ++k) {
+            const int c = lane + 64 * k;
+            float d2 = 3.0e38f;
+            if (c < nc) {
+                const float dx = qx - cx[c], dy = qy - cy[c], dz = qz - cz[c];
+                d2 = dx * dx + dy * dy + dz * dz;
+            }
+            dd[k] = d2;
+        }
+        const double gq = A.guess ? A.guess[q] : 0.0;
+        double T = gq > 0 ? fmin(gq * 1.15, RL2) : RL2;
+        // FP32 rounding bound: |d32^2 - d64^2| <= tol(T) for d near sqrt(T)
+        bool ok = false;
+        int cnt = 0;
+        for (int it = 0; it < 16; ++it) {
+            const float Tf = (float)(T + 1e-6 * (M * sqrt(T) + T));
+            cnt = 0;
+#pragma unroll
+            for (int k = 0; k < KL_NV; ++k) cnt += __popcll(__ballot(dd[k] <= Tf));
+            if (cnt >= K && cnt <= CAP) {
+                ok = true;
+                break;
+            }
+            if (cnt < K) {
+                if (T >= RL2) {
+                    if (lane == 0) atomicAdd(&A.err[3], 1u);
+                    break;
+                }
+                const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
+                T = fmin(RL2, T * fmin(fmax(f, 1.2), 16.0));
+            } else {
+                const double f = cbrt((1.25 * K / cnt) * (1.25 * K / cnt));
+                T *= fmin(fmax(f, 0.05), 0.95);
+            }
+        }
+        if (ok) {   // compact the superset with exact FP64 distances
+            const float Tf = (float)(T + 1e-6 * (M * sqrt(T) + T));
+            int base = 0, in64 = 0;
+#pragma unroll
+            for (int k = 0; k < KL_NV; ++k) {
+                const bool in = dd[k] <= Tf;
+                const unsigned long long m = __ballot(in);
+                if (m) {
+                    double d2 = DBL_MAX;
+                    int slot = 0;
+                    if (in) {
+                        const int c = lane + 64 * k;
+                        slot = lf[c / LEAF] * LEAF + (c % LEAF);
+                        const double dx = img_sep(x, A.sx[slot], A.per, A.L[0]);
+                        const double dy = img_sep(y, A.sy[slot], A.per, A.L[1]);
+                        const double dz = img_sep(z, A.sz[slot], A.per, A.L[2]);
+                        d2 = sq_dist(dx, dy, dz);
+                        const int p = base + __popcll(m & ((1ull << lane) - 1));
+                        bd[w][p] = d2;
+                        bs[w][p] = slot;
+                        bc[w][p] = A.order[slot];
+                    }
+                    in64 += __popcll(__ballot(in && d2 <= T));
+                    base += __popcll(m);
+                }
+            }
+            ok = in64 >= K;
+            cnt = base;
+            if (!ok && lane == 0) atomicAdd(&A.err[4], 1u);
+        } else if (cnt >= K && lane == 0) {
+            atomicAdd(&A.err[4], 1u);
+        }
+        if (!ok) {
+            if (lane == 0) {
+                const unsigned p = atomicAdd(A.nretry, 1u);
+                A.retry[p] = q;
+            }
+            continue;
+        }
+        wave_sync();
+        double dK = 0, dK1 = DBL_MAX;
+        const int eK = K - 1;
+        if (cnt <= 128) {
+            double d[2];
+            int c[2], s[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int e = lane + 64 * k;
+                d[k] = e < cnt ? bd[w][e] : DBL_MAX;
+                c[k] = e < cnt ? bc[w][e] : 0x7fffffff;
+                s[k] = e < cnt ? bs[w][e] : -1;
+            }
+            wave_sync();
+            bitonic128(d, c, s, lane);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int e = lane + 64 * k;
+                if (e < K) A.nbr[(size_t)q * K + e] = s[k];
+                const double a = __shfl(d[k], eK & 63);
+                if (k == (eK >> 6)) dK = a;
+                const double b = __shfl(d[k], K & 63);
+                if (k == (K >> 6) && K < 128) dK1 = b;
+            }
+        } else {
+            double d[4];
+            int c[4], s[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = lane + 64 * k;
+                d[k] = e < cnt ? bd[w][e] : DBL_MAX;
+                c[k] = e < cnt ? bc[w][e] : 0x7fffffff;
+                s[k] = e < cnt ? bs[w][e] : -1;
+            }
+            wave_sync();
+            bitonic256(d, c, s, lane);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = lane + 64 * k;
+                if (e < K) A.nbr[(size_t)q * K + e] = s[k];
+                const double a = __shfl(d[k], eK & 63);
+                if (k == (eK >> 6)) dK = a;
+                const double b = __shfl(d[k], K & 63);
+                if (k == (K >> 6) && K < CAP) dK1 = b;
+            }
+        }
+        if (lane == 0) {
+            A.fb2[q] = dK;
+            if (dK1 == dK) atomicAdd(&A.err[1], 1u);
+        }
+        wave_sync();
     }
 }
 
@@ -743,6 +1089,244 @@ __global__ void k_hydro_scatter(int n, const int *__restrict__ order,
     }
 }
 
+// ------------------------------------------- hydro, own list + extras
+// Per-particle record of the force pass, slot order, in the pair precision:
+// velocity, mass, density, P/rho^2, sound speed, Balsara switch, h = 2h/2,
+// the kernel normalisation 0.5/pi * ih2^2 / h (SphPressureTermsSym,
+// smoothfcn.c:2302-2305), 4/fBall2, Courant dt, uPred, diff, activity.
+template <typename T> struct alignas(16) HRec {
+    T vx, vy, vz, m;
+    T rho, pr2, c, bal;
+    T ph, fn1, ih2, dt;
+    T u, diff, act, pad;
+};
+
+template <typename T>
+__global__ void k_hpack(int n, const int *__restrict__ order, const double *__restrict__ fb2,
+                        const double *__restrict__ m, const double *__restrict__ vp,
+                        const double *__restrict__ u, const double *__restrict__ rho,
+                        const double *__restrict__ pr2, const double *__restrict__ c,
+                        const double *__restrict__ bal, const double *__restrict__ diff,
+                        const double *__restrict__ dt, const uint32_t *__restrict__ bits,
+                        HRec<T> *__restrict__ rec)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    const double f = fb2[s];
+    const double ph = sqrt(0.25 * f), ih2 = 4.0 / f;
+    const double fNorm = 0.5 * M_1_PI * ih2 / ph;
+    HRec<T> r;
+    r.vx = (T)vp[3 * (size_t)i];
+    r.vy = (T)vp[3 * (size_t)i + 1];
+    r.vz = (T)vp[3 * (size_t)i + 2];
+    r.m = (T)m[i];
+    r.rho = (T)rho[i];
+    r.pr2 = (T)pr2[i];
+    r.c = (T)c[i];
+    r.bal = (T)bal[i];
+    r.ph = (T)ph;
+    r.fn1 = (T)(fNorm * ih2);
+    r.ih2 = (T)ih2;
+    r.dt = (T)dt[i];
+    r.u = (T)u[i];
+    r.diff = (T)diff[i];
+    r.act = (T)((bits[i] & 1u) ? 1 : 0);
+    r.pad = 0;
+    rec[s] = r;
+}
+
+template <typename T> struct HPar {
+    T a, H, alpha, beta, dtFacCourant, vFac;
+    double dtFacDiffusion, kDiff;
+};
+
+struct HAcc {
+    double ax = 0, ay = 0, az = 0, pdv = 0, av = 0, dif = 0, mu = 0;
+    double dtn = GG_FLT_MAX;
+};
+
+// One pair of SphPressureTermsSym (SphPressureTerms.h:97-152) as it affects
+// particle "self": role 0, self is p (the list owner, whose smoothing length
+// is used); role 1, self is q.  dx = x_p - x_q (minimum image from p).
+template <typename T>
+__device__ __forceinline__ void hpair(const HRec<T> &P, const HRec<T> &Q, T dx, T dy, T dz,
+                                      T d2, int role, const HPar<T> &hp, HAcc &acc)
+{
+    const bool pa = P.act > T(0), qa = Q.act > T(0);
+    if (!pa && !qa) return;
+    const bool sa = role ? qa : pa;
+    const T r2 = d2 * P.ih2;
+    const T rs1 = m4_dw<T>(r2) * P.fn1;
+    const T rp = rs1 * P.m, rq = rs1 * Q.m;
+    const T dvdotdr = hp.vFac * ((P.vx - Q.vx) * dx + (P.vy - Q.vy) * dy + (P.vz - Q.vz) * dz)
+                      + d2 * hp.H;
+    const T rs = role ? rp : rq;
+    // PRES_PDV(a,b) = a, PRES_ACC(a,b) = a+b (smoothfcn.c:20-23)
+    if (sa) acc.pdv += (double)(rs * (role ? Q.pr2 : P.pr2) * dvdotdr);
+    T Acc = P.pr2 + Q.pr2;
+    T dtv;
+    if (dvdotdr >= T(0)) {
+        dtv = hp.dtFacCourant * P.ph * trcp<T>(T(2) * (P.c > Q.c ? P.c : Q.c));
+    } else {
+        const T absmu = -dvdotdr * hp.a * trcp<T>(tsqrt<T>(d2));
+        acc.mu = fmax(acc.mu, (double)absmu);
+        T visc = hp.alpha * (P.c + Q.c) + hp.beta * T(1.5) * absmu;
+        dtv = hp.dtFacCourant * P.ph * trcp<T>(T(0.625) * (P.c + Q.c) + T(0.375) * visc);
+        visc = (T(0.5) * (P.bal + Q.bal)) * visc * absmu * trcp<T>(P.rho + Q.rho);
+        if (sa) acc.av += (double)(rs * (T(0.5) * visc) * dvdotdr);
+        Acc += visc;
+    }
+    if (sa) {
+        Acc *= rs * hp.a;
+        const T f = role ? Acc : -Acc;     // a_p -= Acc dx, a_q += Acc dx
+        acc.ax += (double)(f * dx);
+        acc.ay += (double)(f * dy);
+        acc.az += (double)(f * dz);
+    }
+    if (hp.kDiff > 0) {   // thermal diffusion (SphPressureTerms.h:32-70)
+        const double dsum = (double)P.diff + (double)Q.diff;
+        if (dsum != 0) {
+            const double diffTh = 2 * hp.kDiff * dsum / ((double)P.rho + (double)Q.rho);
+            const double dtd = hp.dtFacDiffusion * (double)P.ph * (double)P.ph /
+                               (diffTh * (double)P.rho);
+            if (dtd < (double)dtv) dtv = (T)dtd;
+            const double du = diffTh * ((double)P.u - (double)Q.u);
+            if (sa) acc.dif += role ? -du * (double)rp : du * (double)rq;
+        }
+    }
+    // SETDTNEW_PQ (SphPressureTerms.h:97-100)
+    acc.dtn = fmin(acc.dtn, (double)dtv);
+    acc.dtn = fmin(acc.dtn, 4.0 * (double)(role ? P.dt : Q.dt));
+}
+
+struct H2Args {
+    int n, per;
+    double L[3];
+    const double *sx, *sy, *sz, *fb2;
+    const int64_t *off, *eoff;
+    const int *idx, *eidx;
+    const uint32_t *qflag;
+    double a, H, alpha, beta, dtFacCourant, dtFacDiffusion, kDiff;
+    int bCannonical;
+    double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
+};
+
+// Gather form of the symmetric force pass, 16 lanes per particle i.  Over
+// i's own list: the pair as p = i, and, when i lies in the neighbour's ball,
+// the pair as q = i.  Over i's extras (particles whose balls hold i but which
+// are not in i's list): the pair as q = i.
+template <typename T>
+__global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restrict__ rec)
+{
+    const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (i >= A.n) return;
+    HPar<T> hp;
+    hp.a = (T)A.a;
+    hp.H = (T)A.H;
+    hp.alpha = (T)A.alpha;
+    hp.beta = (T)A.beta;
+    hp.dtFacCourant = (T)A.dtFacCourant;
+    hp.vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
+    hp.dtFacDiffusion = A.dtFacDiffusion;
+    hp.kDiff = A.kDiff;
+    const HRec<T> I = rec[i];
+    const double xi = A.sx[i], yi = A.sy[i], zi = A.sz[i];
+    HAcc acc;
+    if (A.qflag[i]) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16) {
+            const int q = A.idx[e];
+            const HRec<T> Q = rec[q];
+            const double xq = A.sx[q], yq = A.sy[q], zq = A.sz[q];
+            const double dx = img_sep(xi, xq, A.per, A.L[0]);
+            const double dy = img_sep(yi, yq, A.per, A.L[1]);
+            const double dz = img_sep(zi, zq, A.per, A.L[2]);
+            hpair<T>(I, Q, (T)dx, (T)dy, (T)dz, (T)sq_dist(dx, dy, dz), 0, hp, acc);
+            if (A.qflag[q]) {
+                const double ex = img_sep(xq, xi, A.per, A.L[0]);
+                const double ey = img_sep(yq, yi, A.per, A.L[1]);
+                const double ez = img_sep(zq, zi, A.per, A.L[2]);
+                const double d2q = sq_dist(ex, ey, ez);
+                if (d2q <= A.fb2[q])
+                    hpair<T>(Q, I, (T)ex, (T)ey, (T)ez, (T)d2q, 1, hp, acc);
+            }
+        }
+    }
+    for (int64_t e = A.eoff[i] + sub; e < A.eoff[i + 1]; e += 16) {
+        const int p = A.eidx[e];
+        const HRec<T> P = rec[p];
+        const double ex = img_sep(A.sx[p], xi, A.per, A.L[0]);
+        const double ey = img_sep(A.sy[p], yi, A.per, A.L[1]);
+        const double ez = img_sep(A.sz[p], zi, A.per, A.L[2]);
+        hpair<T>(P, I, (T)ex, (T)ey, (T)ez, (T)sq_dist(ex, ey, ez), 1, hp, acc);
+    }
+    for (int w = 1; w < 16; w <<= 1) {
+        acc.ax += __shfl_xor(acc.ax, w);
+        acc.ay += __shfl_xor(acc.ay, w);
+        acc.az += __shfl_xor(acc.az, w);
+        acc.pdv += __shfl_xor(acc.pdv, w);
+        acc.av += __shfl_xor(acc.av, w);
+        acc.dif += __shfl_xor(acc.dif, w);
+        acc.mu = fmax(acc.mu, __shfl_xor(acc.mu, w));
+        acc.dtn = fmin(acc.dtn, __shfl_xor(acc.dtn, w));
+    }
+    if (sub == 0) {
+        double *o = A.out + 8 * (size_t)i;
+        o[0] = acc.ax; o[1] = acc.ay; o[2] = acc.az; o[3] = acc.pdv; o[4] = acc.av;
+        o[5] = acc.dif; o[6] = acc.mu; o[7] = acc.dtn;
+    }
+}
+
+// Extras: q receives p when q is in p's list but p is not in q's ball (or q
+// has no list).  16 lanes per list owner p.  mode 0 counts, mode 1 fills at
+// eoff + cursor.
+__global__ __launch_bounds__(256) void k_extra(int n, int mode, int per, double Lx, double Ly,
+                                               double Lz, const uint32_t *__restrict__ qflag,
+                                               const int64_t *__restrict__ off,
+                                               const int *__restrict__ idx,
+                                               const double *__restrict__ sx,
+                                               const double *__restrict__ sy,
+                                               const double *__restrict__ sz,
+                                               const double *__restrict__ fb2,
+                                               int *__restrict__ cnt,
+                                               const int64_t *__restrict__ eoff,
+                                               int *__restrict__ eidx)
+{
+    const int p = (blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (p >= n || !qflag[p]) return;
+    const double xp = sx[p], yp = sy[p], zp = sz[p];
+    for (int64_t e = off[p] + sub; e < off[p + 1]; e += 16) {
+        const int q = idx[e];
+        bool extra = !qflag[q];
+        if (!extra) {
+            const double ex = img_sep(sx[q], xp, per, Lx);
+            const double ey = img_sep(sy[q], yp, per, Ly);
+            const double ez = img_sep(sz[q], zp, per, Lz);
+            extra = sq_dist(ex, ey, ez) > fb2[q];
+        }
+        if (!extra) continue;
+        const int k = atomicAdd(&cnt[q], 1);
+        if (mode) eidx[eoff[q] + k] = p;
+    }
+}
+
+// Bitwise OR of the selection bits over all particles (is any gas flagged?).
+__global__ void k_any_bits(int n, const uint32_t *__restrict__ bits, uint32_t mask,
+                           unsigned *__restrict__ out)
+{
+    __shared__ unsigned s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    unsigned v = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        v |= bits[i] & mask;
+    if (v) atomicOr(&s, v);
+    __syncthreads();
+    if (threadIdx.x == 0 && s) atomicOr(out, s);
+}
+
 // ------------------------------------------------------- list plumbing
 __global__ void k_rev_count(int n, const uint32_t *__restrict__ qflag,
                             const int64_t *__restrict__ off, const int *__restrict__ idx,
@@ -885,18 +1469,16 @@ static int scatter_field(GGPKD pkd, const DevBuf &src, int field, int width,
 static int gas_mask(GGPKD pkd, uint32_t *mask)
 {
     const int n = pkd->nLocal;
-    std::vector<uint32_t> b(n);
-    if (n) {
-        GG_HIP(hipMemcpyAsync(b.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n,
-                              hipMemcpyDeviceToHost, pkd->stream));
-        GG_HIP(hipStreamSynchronize(pkd->stream));
-    }
-    *mask = 0;
-    for (int i = 0; i < n; ++i)
-        if (b[i] & GG_TYPE_GAS) {
-            *mask = GG_TYPE_GAS;
-            break;
-        }
+    SphDev &S = pkd->sph;
+    GG_TRY(S.diffcnt.reserve(sizeof(unsigned)));
+    GG_HIP(hipMemsetAsync(S.diffcnt.p, 0, sizeof(unsigned), pkd->stream));
+    hipLaunchKernelGGL(k_any_bits, dim3(std::max(1, std::min(nblk(n, 256), 1024))), dim3(256), 0,
+                       pkd->stream, n, pkd->f[GG_IACTIVE].as<uint32_t>(), (uint32_t)GG_TYPE_GAS,
+                       S.diffcnt.as<unsigned>());
+    unsigned any = 0;
+    GG_HIP(hipMemcpyAsync(&any, S.diffcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    *mask = any ? GG_TYPE_GAS : 0;
     return GG_OK;
 }
 
@@ -938,6 +1520,7 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.cnt = nullptr;
     Q.off = nullptr;
     Q.err = S.err.as<unsigned>();
+    Q.qlist = nullptr;
     return Q;
 }
 
@@ -997,7 +1580,7 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     double ext = 0;
     for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
     if (!(ext > 0)) ext = 1.0;
-    const double sc = 2097151.0 / ext;
+    const double sc = 65535.0 / ext;
     GG_TRY(S.key.reserve(sizeof(uint64_t) * n));
     GG_TRY(S.order.reserve(sizeof(int) * n));
     GG_TRY(S.k2.reserve(sizeof(uint64_t) * n));
@@ -1007,10 +1590,10 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
                        S.k2.as<uint64_t>(), S.v2.as<int>());
     size_t tmpb = 0;
     GG_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, S.k2.as<uint64_t>(), S.key.as<uint64_t>(),
-                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 63, st));
+                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 48, st));
     GG_TRY(S.sortTmp.reserve(std::max<size_t>(tmpb, 16)));
     GG_HIP(rocprim::radix_sort_pairs(S.sortTmp.p, tmpb, S.k2.as<uint64_t>(), S.key.as<uint64_t>(),
-                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 63, st));
+                                     S.v2.as<int>(), S.order.as<int>(), n, 0, 48, st));
     GG_TRY(S.sx.reserve(sizeof(double) * n));
     GG_TRY(S.sy.reserve(sizeof(double) * n));
     GG_TRY(S.sz.reserve(sizeof(double) * n));
@@ -1060,15 +1643,46 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
     GG_TRY(S.nbr.reserve(sizeof(int) * (size_t)K * n));
     GG_TRY(S.fb2s.reserve(sizeof(double) * n));
-    GG_TRY(S.err.reserve(sizeof(unsigned) * 2));
-    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 2, st));
+    GG_TRY(S.err.reserve(sizeof(unsigned) * 8));
+    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 8, st));
     GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
-    QArgs Q = query_args(pkd, 0, K);
-    Q.guess = S.guess.as<double>();
-    Q.nbr = S.nbr.as<int>();
+    GG_TRY(S.retry.reserve(sizeof(int) * n));
+    GG_TRY(S.nretry.reserve(sizeof(unsigned)));
+    GG_HIP(hipMemsetAsync(S.nretry.p, 0, sizeof(unsigned), st));
+    KLArgs KL;
+    KL.n = n;
+    KL.nLeaf = S.nLeaf;
+    KL.nL2 = S.nL2;
+    KL.K = K;
+    KL.per = S.bPeriodic;
+    for (int d = 0; d < 3; ++d) KL.L[d] = S.L[d];
+    KL.sx = S.sx.as<double>();
+    KL.sy = S.sy.as<double>();
+    KL.sz = S.sz.as<double>();
+    KL.box = S.nodeBox.as<double>();
+    KL.order = S.order.as<int>();
+    KL.qflag = S.qf.as<uint32_t>();
+    KL.guess = S.guess.as<double>();
+    KL.fb2 = S.fb2s.as<double>();
+    KL.nbr = S.nbr.as<int>();
+    KL.retry = S.retry.as<int>();
+    KL.nretry = S.nretry.as<unsigned>();
+    KL.err = S.err.as<unsigned>();
     timer_mark(pkd, 0);
-    hipLaunchKernelGGL(k_query, dim3(n), dim3(64), 0, st, Q);
+    hipLaunchKernelGGL(k_knn_leaf, dim3(S.nLeaf), dim3(256), 0, st, KL);
     GG_HIP(hipGetLastError());
+    unsigned nretry = 0;
+    GG_HIP(hipMemcpyAsync(&nretry, S.nretry.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    S.retried = nretry;
+    if (nretry > 0) {   // queries the leaf groups could not settle
+        QArgs Q = query_args(pkd, 0, K);
+        Q.guess = S.guess.as<double>();
+        Q.nbr = S.nbr.as<int>();
+        Q.qlist = S.retry.as<int>();
+        hipLaunchKernelGGL(k_query, dim3(nretry), dim3(64), 0, st, Q);
+        GG_HIP(hipGetLastError());
+    }
     timer_mark(pkd, 1);
     GG_TRY(S.off.reserve(sizeof(int64_t) * (n + 1)));
     hipLaunchKernelGGL(k_fixed_off, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, K,
@@ -1125,10 +1739,12 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
         hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 3);
-    unsigned e[2];
-    GG_TRY(read_err(pkd, e));
+    unsigned e[8];
+    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 8, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
     S.fails = e[0];
     S.ties = e[1];
+    for (int k = 0; k < 4; ++k) S.diag[k] = e[2 + k];
     GG_CHECK(e[0] == 0, GG_EUNSUP,
              "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
     const uint32_t *qf = S.qf.as<uint32_t>();
@@ -1246,37 +1862,46 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         idx = S.gent.as<int>();
     }
     timer_mark(pkd, 1);
-    // reverse lists
+    // extras: particles whose balls hold i but which are not in i's list
     GG_TRY(S.rcnt.reserve(sizeof(int) * (n + 1)));
     GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
-    GG_TRY(S.rcur.reserve(sizeof(int) * (n + 1)));
     GG_HIP(hipMemsetAsync(S.rcnt.p, 0, sizeof(int) * (n + 1), st));
-    GG_HIP(hipMemsetAsync(S.rcur.p, 0, sizeof(int) * (n + 1), st));
-    hipLaunchKernelGGL(k_rev_count, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.qf.as<uint32_t>(),
-                       off, idx, S.rcnt.as<int>());
-    int64_t rtot = 0;
-    GG_TRY(exclusive_scan_i64(st, S.rcnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &rtot));
-    GG_TRY(S.ridx.reserve(sizeof(int) * std::max<int64_t>(rtot, 1)));
-    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.qf.as<uint32_t>(),
-                       off, idx, S.roff.as<int64_t>(), S.rcur.as<int>(), S.ridx.as<int>());
+    hipLaunchKernelGGL(k_extra, dim3(nblk(n * 16, 256)), dim3(256), 0, st, n, 0, S.bPeriodic,
+                       S.L[0], S.L[1], S.L[2], S.qf.as<uint32_t>(), off, idx, S.sx.as<double>(),
+                       S.sy.as<double>(), S.sz.as<double>(), S.fb2s.as<double>(),
+                       S.rcnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
+    int64_t etot = 0;
+    GG_TRY(exclusive_scan_i64(st, S.rcnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &etot));
+    S.nExtra = etot;
+    GG_TRY(S.ridx.reserve(sizeof(int) * std::max<int64_t>(etot, 1)));
+    GG_HIP(hipMemsetAsync(S.rcnt.p, 0, sizeof(int) * (n + 1), st));
+    hipLaunchKernelGGL(k_extra, dim3(nblk(n * 16, 256)), dim3(256), 0, st, n, 1, S.bPeriodic,
+                       S.L[0], S.L[1], S.L[2], S.qf.as<uint32_t>(), off, idx, S.sx.as<double>(),
+                       S.sy.as<double>(), S.sz.as<double>(), S.fb2s.as<double>(),
+                       S.rcnt.as<int>(), S.roff.as<int64_t>(), S.ridx.as<int>());
     hipLaunchKernelGGL(k_rev_sort, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.roff.as<int64_t>(),
                        S.ridx.as<int>());
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 2);
-    // inputs in slot order
-    GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
-    GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
-    GG_TRY(gather_field(pkd, GG_DENSITY, 1, 0, S.rho));
-    GG_TRY(gather_field(pkd, GG_POVERRHO2, 1, 0, S.pr2));
-    GG_TRY(gather_field(pkd, GG_C, 1, 0, S.c));
-    GG_TRY(gather_field(pkd, GG_BALSARA, 1, 0, S.bal));
-    GG_TRY(gather_field(pkd, GG_DIFF, 1, 0, S.dif));
-    GG_TRY(gather_field(pkd, GG_DT, 1, 0, S.dt));
+    // per-particle records in slot order
+    const int in[] = {GG_MASS, GG_VPRED, GG_UPRED, GG_DENSITY, GG_POVERRHO2, GG_C, GG_BALSARA,
+                      GG_DIFF, GG_DT};
+    for (int f : in) GG_TRY(ensure_field(pkd, f));
+    const bool f32 = pkd->prec == GG_PREC_FP32;
+    GG_TRY(S.hrec.reserve((f32 ? sizeof(HRec<float>) : sizeof(HRec<double>)) * (size_t)n));
     GG_TRY(S.res.reserve(sizeof(double) * 8 * n));
-    HArgs H;
+#define GG_HPACK(T)                                                                           \
+    hipLaunchKernelGGL(k_hpack<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(), \
+                       S.fb2s.as<double>(), pkd->f[GG_MASS].as<double>(),                     \
+                       pkd->f[GG_VPRED].as<double>(), pkd->f[GG_UPRED].as<double>(),          \
+                       pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_POVERRHO2].as<double>(),     \
+                       pkd->f[GG_C].as<double>(), pkd->f[GG_BALSARA].as<double>(),             \
+                       pkd->f[GG_DIFF].as<double>(), pkd->f[GG_DT].as<double>(),               \
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), S.hrec.as<HRec<T>>())
+    if (f32) GG_HPACK(float);
+    else GG_HPACK(double);
+#undef GG_HPACK
+    H2Args H;
     H.n = n;
     H.per = S.bPeriodic;
     for (int d = 0; d < 3; ++d) H.L[d] = S.L[d];
@@ -1285,37 +1910,27 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.sz = S.sz.as<double>();
     H.fb2 = S.fb2s.as<double>();
     H.off = off;
-    H.roff = S.roff.as<int64_t>();
+    H.eoff = S.roff.as<int64_t>();
     H.idx = idx;
-    H.ridx = S.ridx.as<int>();
+    H.eidx = S.ridx.as<int>();
     H.qflag = S.qf.as<uint32_t>();
-    H.act = S.act.as<uint32_t>();
-    H.m = S.m.as<double>();
-    H.vx = S.vx.as<double>();
-    H.vy = S.vy.as<double>();
-    H.vz = S.vz.as<double>();
-    H.u = S.u.as<double>();
-    H.rho = S.rho.as<double>();
-    H.pr2 = S.pr2.as<double>();
-    H.c = S.c.as<double>();
-    H.bal = S.bal.as<double>();
-    H.diff = S.dif.as<double>();
-    H.dt = S.dt.as<double>();
     H.a = smf->a;
     H.H = smf->H;
     H.alpha = smf->alpha;
     H.beta = smf->beta;
     H.dtFacCourant = smf->dtFacCourant;
     H.dtFacDiffusion = smf->dtFacDiffusion;
-    H.dThermalDiffusionCoeff = smf->dThermalDiffusionCoeff;
+    H.kDiff = smf->dThermalDiffusionCoeff;
     H.bCannonical = smf->bCannonical;
     H.out = S.res.as<double>();
     const int grid = nblk(n * 16, 256);
     timer_mark(pkd, 3);
-    if (pkd->prec == GG_PREC_FP32)
-        hipLaunchKernelGGL(k_hydro<float>, dim3(grid), dim3(256), 0, st, H);
+    if (f32)
+        hipLaunchKernelGGL(k_hydro2<float>, dim3(grid), dim3(256), 0, st, H,
+                           (const HRec<float> *)S.hrec.as<HRec<float>>());
     else
-        hipLaunchKernelGGL(k_hydro<double>, dim3(grid), dim3(256), 0, st, H);
+        hipLaunchKernelGGL(k_hydro2<double>, dim3(grid), dim3(256), 0, st, H,
+                           (const HRec<double> *)S.hrec.as<HRec<double>>());
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 4);
     const int fl[] = {GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW};
@@ -1353,5 +1968,16 @@ extern "C" int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr)
         if (!qf[s]) continue;
         for (int k = 0; k < K; ++k) nbr[(size_t)ord[s] * K + k] = ord[lst[(size_t)s * K + k]];
     }
+    return GG_OK;
+}
+
+extern "C" int gg_smooth_stats(GGPKD pkd, int64_t *s, int n)
+{
+    GG_CHECK(pkd && s, GG_EINVAL, "null argument");
+    const SphDev &S = pkd->sph;
+    const int64_t v[8] = {(int64_t)S.retried, (int64_t)S.ties, S.nExtra, (int64_t)S.listQueries,
+                          (int64_t)S.diag[0], (int64_t)S.diag[1], (int64_t)S.diag[2],
+                          (int64_t)S.diag[3]};
+    for (int k = 0; k < n && k < 8; ++k) s[k] = v[k];
     return GG_OK;
 }

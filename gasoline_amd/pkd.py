@@ -42,7 +42,7 @@ TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
        "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_build_local", "gg_smooth", "gg_resmooth",
-       "gg_gas_pressure", "gg_smooth_lists", "gg_timings"]
+       "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings"]
 
 
 class GGError(RuntimeError):
@@ -110,6 +110,7 @@ def lib():
     L.gg_gas_pressure.argtypes = [_vp, C.POINTER(Smf)]
     L.gg_smooth_lists.argtypes = [_vp, C.c_int, _ip]
     L.gg_timings.argtypes = [_vp, _dp, C.c_int]
+    L.gg_smooth_stats.argtypes = [_vp, _i64p, C.c_int]
     _LIB = L
     return L
 
@@ -234,6 +235,13 @@ class Pkd:
         out = np.empty((self.n, nSmooth), np.int32)
         _check(lib().gg_smooth_lists(self.h, int(nSmooth), _p(out, _ip)))
         return out
+
+    def smooth_stats(self):
+        s = np.zeros(8, np.int64)
+        _check(lib().gg_smooth_stats(self.h, _p(s, _i64p), 8))
+        return dict(retried=int(s[0]), ties=int(s[1]), extras=int(s[2]), queries=int(s[3]),
+                    groups_overflowed=int(s[4]), short_at_RL=int(s[5]), unsettled=int(s[6]),
+                    max_leaves=int(s[7]))
 
     def timings(self):
         t = np.zeros(9)

@@ -207,6 +207,17 @@ int gg_gas_pressure(GGPKD pkd, const gg_smf *smf);
  * in caller order (sorted by distance, then index). */
 int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr);
 
+/* Counters of the last gg_smooth / gg_resmooth: s[0] kNN queries that the
+ * leaf-group search handed to the single-query search, s[1] particles whose
+ * nSmooth-th and (nSmooth+1)-th distances tie (their neighbour set is not
+ * unique; gasoline's priority queue keeps the one met first), s[2] entries
+ * of the force pass's extras lists, s[3] smooth-active particles; search
+ * diagnostics of the leaf-group kNN: s[4] groups whose neighbourhood did not
+ * fit the staging buffer, s[5] queries with fewer than nSmooth candidates in
+ * the group radius, s[6] queries whose radius search did not settle, s[7]
+ * largest number of staged leaves.  n <= 8. */
+int gg_smooth_stats(GGPKD pkd, int64_t *s, int n);
+
 /* ------------------------------------------------------------ timing */
 
 /* Device time in milliseconds of the last call of each kind, by phase:

@@ -148,3 +148,16 @@ def test_small_and_degenerate_inputs(gpu):
     with setup_pkd(gpu, r, np.ones(100), np.full(100, 0.1), gpu.PREC_FP64) as pk:
         with pytest.raises(gpu.GGError):
             pk.build_binary(16, 1, 0.7, 2)
+
+
+def test_c_host_driver(gpu):
+    # the C host sequences (gasoline_amd/host) drive a gravity and an SPH
+    # step through the C ABI and check conservation
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "gasoline_amd", "host", "ggforce")
+    assert os.path.exists(exe), "ggforce not built"
+    out = subprocess.run([exe, "32768", "32768"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "Gravity Calculated" in out.stdout and "ggforce ok" in out.stdout
