@@ -68,13 +68,16 @@ struct GravDev {
 
 // SPH state on the device (search structure, lists and per-call scratch).
 struct SphDev {
-    int built = 0, n = 0, bPeriodic = 0, nLeaf = 0, nL2 = 1, nBucket = 0;
+    int built = 0, n = 0, bPeriodic = 0, nb = 0, nBucket = 0;
     double L[3] = {0, 0, 0};
     int listK = 0, listValid = 0, listQueries = 0;
     unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0};
     int64_t nExtra = 0;
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
-    DevBuf sx, sy, sz, nodeBox;                // slot order positions, boxes
+    DevBuf sx, sy, sz;                         // slot order positions
+    DevBuf child, rng, sparse;                 // radix tree, bucket-box sparse table
+    DevBuf bflag, bcode, bfirst, bcnt, bbox, bof; // buckets
+    DevBuf crec, grpf, grpc, grpn, grpr;       // child records, kNN groups
     DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
     DevBuf gcnt, goff, gent;                   // ball-gather CSR
     DevBuf rcnt, roff, rcur, ridx;             // extras lists
@@ -85,7 +88,8 @@ struct SphDev {
     void release_all()
     {
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
-                         &sx, &sy, &sz, &nodeBox, &nbr, &fb2s, &fb2knn, &qfknn, &off,
+                         &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode,
+                         &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,

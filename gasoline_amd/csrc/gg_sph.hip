@@ -24,9 +24,8 @@
 
 namespace gg {
 
-constexpr int LEAF = 32;          // particles per leaf
 constexpr int FCAP = 1024;        // BFS frontier capacity (nodes)
-constexpr int LCAP = 512;         // leaves per query capacity
+         // leaves per query capacity
 constexpr int CAP = 256;          // candidate buffer (4 per lane)
 constexpr int MAXTRY = 48;        // radius adjustments per query
 constexpr double GG_FLT_MAX = (double)3.402823466E+38F;   // floattype.h:9
@@ -117,6 +116,29 @@ __global__ void k_bbox(int n, const int *__restrict__ gidx, const double *__rest
         for (int d = 0; d < 6; ++d) part[6 * (size_t)blockIdx.x + d] = s[d][0];
 }
 
+// Box of slots [0, n) for n <= 64 (one wavefront).
+__global__ void k_bbox_slots(int n, const double *__restrict__ sx, const double *__restrict__ sy,
+                             const double *__restrict__ sz, double *__restrict__ box)
+{
+    const int l = threadIdx.x;
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, u[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (l < n) {
+        v[0] = u[0] = sx[l];
+        v[1] = u[1] = sy[l];
+        v[2] = u[2] = sz[l];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            v[d] = fmin(v[d], __shfl_xor(v[d], o));
+            u[d] = fmax(u[d], __shfl_xor(u[d], o));
+        }
+    if (l == 0)
+        for (int d = 0; d < 3; ++d) {
+            box[d] = v[d];
+            box[3 + d] = u[d];
+        }
+}
+
 __device__ __forceinline__ uint64_t spread21(uint64_t v)
 {
     v &= 0x1fffffull;
@@ -158,46 +180,187 @@ __global__ void k_gather_pos(int n, const int *__restrict__ order, const double 
     sz[s] = r[3 * (size_t)i + 2];
 }
 
-__global__ void k_leaf_box(int n, int nLeaf, int nL2, const double *__restrict__ sx,
-                           const double *__restrict__ sy, const double *__restrict__ sz,
-                           double *__restrict__ box)
+// ------------------------------------------------- radix tree (LBVH)
+// Binary radix tree over the sorted Morton keys (Karras 2012): internal
+// node i in [0, n-2] covers the slot range rng[2i]..rng[2i+1]; a child code
+// c >= 0 is internal node c, c < 0 is the single particle ~c.  Equal keys are
+// ordered by slot.  Buckets are the maximal subtrees of at most BKT
+// particles: every bucket is one Morton-prefix cell, so its box is tight.
+// A node of more than BKT particles covers whole consecutive buckets, so its
+// box is the union of a bucket range, read from a sparse table of bucket
+// boxes (FP32, rounded outwards); no bottom-up pass is needed.
+constexpr int BKT = 32;
+
+__device__ __forceinline__ int lcp_delta(const uint64_t *__restrict__ k, int n, int i, int j)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nL2) return;
-    double *b = box + 6 * (size_t)(nL2 + k);
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    if (k < nLeaf) {
-        const int s0 = k * LEAF, s1 = min(s0 + LEAF, n);
-        for (int s = s0; s < s1; ++s) {
-            lo[0] = fmin(lo[0], sx[s]); hi[0] = fmax(hi[0], sx[s]);
-            lo[1] = fmin(lo[1], sy[s]); hi[1] = fmax(hi[1], sy[s]);
-            lo[2] = fmin(lo[2], sz[s]); hi[2] = fmax(hi[2], sz[s]);
-        }
-    }
-    for (int d = 0; d < 3; ++d) {
-        b[d] = lo[d];
-        b[3 + d] = hi[d];
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll(a ^ b);
+    return 64 + __clz(i ^ j);
+}
+
+__global__ void k_lbvh(int n, const uint64_t *__restrict__ key, int *__restrict__ child,
+                       int *__restrict__ rng)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (lcp_delta(key, n, i, i + 1) - lcp_delta(key, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = lcp_delta(key, n, i, i - d);
+    int lmax = 2;
+    while (lcp_delta(key, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (lcp_delta(key, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = lcp_delta(key, n, i, j);
+    int sp = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (lcp_delta(key, n, i, i + (sp + t) * d) > dnode) sp += t;
+    } while (t > 1);
+    const int g = i + sp * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    child[2 * i] = (lo == g) ? ~g : g;
+    child[2 * i + 1] = (hi == g + 1) ? ~(g + 1) : g + 1;
+    rng[2 * i] = lo;
+    rng[2 * i + 1] = hi;
+}
+
+__device__ __forceinline__ void child_info(int c, const int *__restrict__ rng, int &first, int &cnt)
+{
+    if (c >= 0) {
+        first = rng[2 * c];
+        cnt = rng[2 * c + 1] - first + 1;
+    } else {
+        first = ~c;
+        cnt = 1;
     }
 }
 
-__global__ void k_level_box(int first, int count, double *__restrict__ box)
+// Mark bucket starts: children of nodes larger than BKT that are not.
+__global__ void k_bucket_mark(int n, const int *__restrict__ child, const int *__restrict__ rng,
+                              int *__restrict__ flag, int *__restrict__ cnt)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    const int i = first + k;
-    double *b = box + 6 * (size_t)i;
-    const double *l = box + 6 * (size_t)(2 * i), *u = box + 6 * (size_t)(2 * i + 1);
-    for (int d = 0; d < 3; ++d) {
-        b[d] = fmin(l[d], u[d]);
-        b[3 + d] = fmax(l[3 + d], u[3 + d]);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || rng[2 * i + 1] - rng[2 * i] + 1 <= BKT) return;
+    for (int k = 0; k < 2; ++k) {
+        int f, m;
+        child_info(child[2 * i + k], rng, f, m);
+        if (m <= BKT) {
+            flag[f] = 1;
+            cnt[f] = m;
+        }
     }
+}
+
+__device__ __forceinline__ float f_down(double d)
+{
+    float f = (float)d;
+    if ((double)f > d) {
+        int b = __float_as_int(f);
+        b = f > 0 ? b - 1 : (f == 0 ? (int)0x80000001u : b + 1);
+        f = __int_as_float(b);
+    }
+    return f;
+}
+
+__device__ __forceinline__ float f_up(double d)
+{
+    float f = (float)d;
+    if ((double)f < d) {
+        int b = __float_as_int(f);
+        b = f > 0 ? b + 1 : (f == 0 ? 1 : b - 1);
+        f = __int_as_float(b);
+    }
+    return f;
+}
+
+// Bucket b: first slot, count, FP64 box, FP32 outward box (sparse table
+// level 0) and the bucket of each of its slots.
+__global__ void k_bucket_write(int n, const int *__restrict__ flag, const int *__restrict__ cnt,
+                               const int64_t *__restrict__ pos, const double *__restrict__ sx,
+                               const double *__restrict__ sy, const double *__restrict__ sz,
+                               int *__restrict__ bfirst, int *__restrict__ bcnt,
+                               double *__restrict__ bbox, float *__restrict__ fbox,
+                               int *__restrict__ bof)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const int b = (int)pos[i], m = cnt[i];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = i; k < i + m; ++k) {
+        const double q[3] = {sx[k], sy[k], sz[k]};
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], q[d]);
+            hi[d] = fmax(hi[d], q[d]);
+        }
+        bof[k] = b;
+    }
+    bfirst[b] = i;
+    bcnt[b] = m;
+    for (int d = 0; d < 3; ++d) {
+        bbox[6 * (size_t)b + d] = lo[d];
+        bbox[6 * (size_t)b + 3 + d] = hi[d];
+        fbox[6 * (size_t)b + d] = f_down(lo[d]);
+        fbox[6 * (size_t)b + 3 + d] = f_up(hi[d]);
+    }
+}
+
+// Sparse table level k: union of the two level k-1 boxes it spans.
+__global__ void k_sparse_level(int nb, int half, const float *__restrict__ prev,
+                               float *__restrict__ cur)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b + 2 * half > nb) return;
+    const float *a = prev + 6 * (size_t)b, *c = prev + 6 * (size_t)(b + half);
+    float *o = cur + 6 * (size_t)b;
+    for (int d = 0; d < 3; ++d) {
+        o[d] = fminf(a[d], c[d]);
+        o[3 + d] = fmaxf(a[3 + d], c[3 + d]);
+    }
+}
+
+// Child record of an internal node: the child's particle count, its bucket
+// index when it is a bucket (count <= BKT) or its internal-node index
+// otherwise, and its box in FP32 rounded outwards, so a box test on the
+// record never excludes what the FP64 box would include.
+struct __align__(16) CRec {
+    int code, cnt;
+    float lo[3], hi[3];
+};
+
+__global__ void k_child_rec(int n, int nb, const int *__restrict__ child,
+                            const int *__restrict__ rng, const int *__restrict__ bof,
+                            const float *__restrict__ sparse, CRec *__restrict__ rec)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * (n - 1)) return;
+    const int c = child[e];
+    int f, m;
+    child_info(c, rng, f, m);
+    const int b0 = bof[f], b1 = bof[f + m - 1];
+    const int len = b1 - b0 + 1, lv = 31 - __clz(len);
+    const float *t0 = sparse + 6 * ((size_t)lv * nb + b0);
+    const float *t1 = sparse + 6 * ((size_t)lv * nb + b1 - (1 << lv) + 1);
+    CRec r;
+    r.code = m > BKT ? c : b0;
+    r.cnt = m;
+    for (int d = 0; d < 3; ++d) {
+        r.lo[d] = fminf(t0[d], t1[d]);
+        r.hi[d] = fmaxf(t0[3 + d], t1[3 + d]);
+    }
+    rec[e] = r;
 }
 
 // -------------------------------------------------------- kNN / gather
 struct QArgs {
-    int n, nLeaf, nL2, K, per, mode;   // mode 0 kNN, 1 gather count, 2 gather fill
+    int n, K, per, mode;               // mode 0 kNN, 1 gather count, 2 gather fill
     double L[3];
-    const double *sx, *sy, *sz, *box;
+    const double *sx, *sy, *sz;
+    const CRec *rec;                   // radix tree child records
+    const int *bof;                    // slot -> bucket
+    const double *bbox;                // bucket boxes
+    const int *bfirst, *bcnt;          // bucket first slot, count
     const int *order;                  // slot -> caller (tie key)
     const uint32_t *qflag;             // slot order: nonzero = query
     const double *guess;               // slot order: fBall2 guess (<= 0: none)
@@ -258,13 +421,117 @@ __device__ void bitonic256(double (&d)[4], int (&c)[4], int (&s)[4], int lane)
     }
 }
 
+// Bitonic sort of 128 triples held 2 per lane.
+__device__ void bitonic128(double (&d)[2], int (&c)[2], int (&s)[2], int lane)
+{
+    for (int size = 2; size <= 128; size <<= 1) {
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const bool asc = (lane & size) == 0;
+                const bool sw = asc ? kv_less(d[1], c[1], d[0], c[0]) : kv_less(d[0], c[0], d[1], c[1]);
+                if (sw) {
+                    const double td = d[0]; d[0] = d[1]; d[1] = td;
+                    const int tc = c[0]; c[0] = c[1]; c[1] = tc;
+                    const int ts = s[0]; s[0] = s[1]; s[1] = ts;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int e = lane + 64 * k;
+                    const double pd = __shfl_xor(d[k], j);
+                    const int pc = __shfl_xor(c[k], j);
+                    const int ps = __shfl_xor(s[k], j);
+                    const bool lower = (lane & j) == 0;
+                    const bool asc = (e & size) == 0;
+                    const bool take = (lower == asc) ? kv_less(pd, pc, d[k], c[k])
+                                                     : kv_less(d[k], c[k], pd, pc);
+                    if (take) {
+                        d[k] = pd;
+                        c[k] = pc;
+                        s[k] = ps;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront-wide breadth-first search of the radix tree: appends to
+// (bf, bc) the buckets whose box passes test(box) and returns their number
+// (their particle total in *tot), or -1 when the frontier or the bucket
+// list overflows.  fr holds 2*FC ints; bf and bc may be LDS or global.
+template <typename Test>
+__device__ int wave_bfs(const QArgs &A, Test test, int *fr, int FC, int *bf, int *bc, int BC,
+                        int lane, int *tot)
+{
+    if (A.n <= BKT) {   // the root is the only bucket
+        if (lane == 0) {
+            bf[0] = 0;
+            bc[0] = A.n;
+        }
+        *tot = A.n;
+        wave_sync();
+        return 1;
+    }
+    int nb = 0, nf = 1, cur = 0, np = 0;
+    if (lane == 0) fr[0] = 0;
+    wave_sync();
+    while (nf > 0) {
+        int nn = 0;
+        for (int base = 0; base < 2 * nf; base += 64) {
+            const int k = base + lane;
+            bool hit = false, big = false;
+            CRec r;
+            r.code = r.cnt = 0;
+            if (k < 2 * nf) {
+                r = A.rec[2 * fr[cur * FC + (k >> 1)] + (k & 1)];
+                const double b[6] = {r.lo[0], r.lo[1], r.lo[2], r.hi[0], r.hi[1], r.hi[2]};
+                hit = test(b);
+                big = r.cnt > BKT;
+            }
+            const unsigned long long mb = __ballot(hit && !big);
+            const unsigned long long mi = __ballot(hit && big);
+            if (hit && !big) {
+                const int p = nb + __popcll(mb & ((1ull << lane) - 1));
+                if (p < BC) {
+                    bf[p] = r.code;
+                    bc[p] = r.cnt;
+                }
+                np += r.cnt;
+            }
+            if (hit && big) {
+                const int p = nn + __popcll(mi & ((1ull << lane) - 1));
+                if (p < FC) fr[(cur ^ 1) * FC + p] = r.code;
+            }
+            nb += __popcll(mb);
+            nn += __popcll(mi);
+        }
+        wave_sync();
+        if (nn > FC || nb > BC) return -1;
+        nf = nn;
+        cur ^= 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o);
+    *tot = np;
+    return nb;
+}
+
+constexpr int QB = 512;   // buckets per single-query search
+
 // One 64-lane workgroup per query slot.  mode 0: exact kNN by (d2, caller
-// index) with a radius that is grown or shrunk until the ball holds between
-// K and CAP particles; mode 1/2: ball gather at fBall2 (d2 <= fBall2).
+// index) with a radius grown or shrunk until the ball holds between K and
+// CAP particles; mode 1/2: ball gather at fBall2 (d2 <= fBall2).
 __global__ __launch_bounds__(64) void k_query(QArgs A)
 {
-    __shared__ int fr[2][FCAP];
-    __shared__ int lf[LCAP];
+    __shared__ int fr[2 * FCAP];
+    __shared__ int bfst[QB], bcnt[QB];
     __shared__ double cd[CAP];
     __shared__ int cc[CAP], cs[CAP];
     const int lane = threadIdx.x;
@@ -280,82 +547,43 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
         R2 = A.fb2[q];
     } else if (A.guess && A.guess[q] > 0) {
         R2 = A.guess[q] * 1.2;
-    } else {
-        // leaf density estimate, aiming at ~2K particles in the ball
-        const double *b = A.box + 6 * (size_t)(A.nL2 + q / LEAF);
-        const double ex = b[3] - b[0], ey = b[4] - b[1], ez = b[5] - b[2];
+    } else {   // bucket density estimate, ~2K particles in the ball
+        const int b = A.bof[q];
+        const double *bb = A.bbox + 6 * (size_t)b;
+        const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
         double vol = ex * ey * ez;
         const double emax = fmax(ex, fmax(ey, ez));
         if (!(vol > 0)) vol = emax * emax * emax;
-        const int nl = min(LEAF, A.n - (q / LEAF) * LEAF);
-        double r3 = vol * (2.0 * K) / fmax(nl, 1) * (3.0 / (4.0 * M_PI));
+        double r3 = vol * (2.0 * K) / fmax(A.bcnt[b], 1) * (3.0 / (4.0 * M_PI));
         if (!(r3 > 0)) r3 = 1e-30;
-        R2 = pow(r3, 2.0 / 3.0);
+        R2 = cbrt(r3 * r3);
     }
     for (int attempt = 0; attempt < MAXTRY; ++attempt) {
-        // breadth-first search of the box hierarchy for leaves within R
         const double R2b = R2 * (1.0 + 1e-12);
-        int nf = 1, cur = 0, nl = 0;
-        bool ovf = false;
-        if (lane == 0) fr[0][0] = 1;
-        __syncthreads();
-        while (nf > 0) {
-            int nn = 0;
-            for (int base = 0; base < nf; base += 64) {
-                const int k = base + lane;
-                int node = 0;
-                bool hit = false;
-                if (k < nf) {
-                    node = fr[cur][k];
-                    hit = box_d2(A.box + 6 * (size_t)node, x, y, z, A.per, A.L) <= R2b;
-                }
-                const bool leaf = node >= A.nL2;
-                const unsigned long long ml = __ballot(hit && leaf);
-                const unsigned long long mi = __ballot(hit && !leaf);
-                if (hit && leaf) {
-                    const int p = nl + __popcll(ml & ((1ull << lane) - 1));
-                    if (p < LCAP) lf[p] = node - A.nL2;
-                }
-                if (hit && !leaf) {
-                    const int p = nn + 2 * __popcll(mi & ((1ull << lane) - 1));
-                    if (p + 1 < FCAP) {
-                        fr[cur ^ 1][p] = 2 * node;
-                        fr[cur ^ 1][p + 1] = 2 * node + 1;
-                    }
-                }
-                nl += __popcll(ml);
-                nn += 2 * __popcll(mi);
-            }
-            __syncthreads();
-            if (nn > FCAP || nl > LCAP) {
-                ovf = true;
-                break;
-            }
-            nf = nn;
-            cur ^= 1;
-        }
-        if (ovf) {
+        int np;
+        const int nb = wave_bfs(
+            A, [&](const double *b) { return box_d2(b, x, y, z, A.per, A.L) <= R2b; }, fr, FCAP,
+            bfst, bcnt, QB, lane, &np);
+        if (nb < 0) {
             if (A.mode != 0) break;   // a ball gather cannot change its radius
-            R2 *= 0.25;               // kNN: ball far too large for the buffers
+            R2 *= 0.25;
             continue;
         }
-        // candidates within R, two leaves per pass
+        for (int k = lane; k < nb; k += 64) bfst[k] = A.bfirst[bfst[k]];
+        wave_sync();
         int total = 0;
-        for (int li = 0; li < nl; li += 2) {
-            const int leaf = li + (lane >> 5);
+        for (int bi = 0; bi < nb; bi += 2) {
+            const int b = bi + (lane >> 5), o = lane & 31;
             bool ok = false;
             double d2 = 0.0;
             int slot = 0;
-            if (leaf < nl) {
-                const int lk = lf[leaf];
-                slot = lk * LEAF + (lane & 31);
-                if (slot < A.n && (lane & 31) < LEAF) {
-                    const double dx = img_sep(x, A.sx[slot], A.per, A.L[0]);
-                    const double dy = img_sep(y, A.sy[slot], A.per, A.L[1]);
-                    const double dz = img_sep(z, A.sz[slot], A.per, A.L[2]);
-                    d2 = sq_dist(dx, dy, dz);
-                    ok = d2 <= R2;
-                }
+            if (b < nb && o < bcnt[b]) {
+                slot = bfst[b] + o;
+                const double dx = img_sep(x, A.sx[slot], A.per, A.L[0]);
+                const double dy = img_sep(y, A.sy[slot], A.per, A.L[1]);
+                const double dz = img_sep(z, A.sz[slot], A.per, A.L[2]);
+                d2 = sq_dist(dx, dy, dz);
+                ok = d2 <= R2;
             }
             const unsigned long long m = __ballot(ok);
             if (ok) {
@@ -368,15 +596,15 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
             }
             total += __popcll(m);
         }
-        __syncthreads();
+        wave_sync();
         if (A.mode == 0) {
             if (total < K) {
-                const double f = total > 0 ? pow(2.0 * K / total, 2.0 / 3.0) : 4.0;
+                const double f = total > 0 ? cbrt((2.0 * K / total) * (2.0 * K / total)) : 4.0;
                 R2 *= fmin(fmax(f, 1.6), 16.0);
                 continue;
             }
             if (total > CAP) {
-                const double f = pow(1.5 * K / total, 2.0 / 3.0);
+                const double f = cbrt((1.5 * K / total) * (1.5 * K / total));
                 R2 *= fmin(fmax(f, 0.05), 0.9);
                 continue;
             }
@@ -405,7 +633,6 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
                 const int e = lane + 64 * k;
                 if (e < K) A.nbr[(size_t)q * K + e] = s[k];
             }
-            // fBall2 = K-th smallest; a tie with the (K+1)-th is reported
             const int eK = K - 1;
             double dK = 0, dK1 = DBL_MAX;
 #pragma unroll
@@ -438,34 +665,33 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
     }
 }
 
-// ---------------------------------------------- kNN by leaf groups
-// One 256-lane workgroup per leaf of 32 query particles.  The leaves within
-// a group radius RL of the leaf's box are found once and their positions
-// staged in LDS as FP32 offsets from the leaf-box centre.  Each wavefront
-// then settles its queries: FP32 squared distances (kept in registers) and
-// ballot counts choose a squared radius T <= RL^2 whose ball holds a little
-// more than K candidates; every candidate whose FP32 distance is within T
-// plus a bound on the FP32 rounding error is re-measured in FP64 from the
-// global positions, so the set S handed to the sort contains every particle
-// with FP64 d2 <= T.  If at least K of them have d2 <= T, the first K of S
-// sorted by (d2, caller index) are the exact K nearest.  Queries that cannot
-// be settled this way go to a retry list served by k_query.
-constexpr int KL_CM = 4096;            // staged candidates per group
-constexpr int KL_ML = KL_CM / LEAF;    // staged leaves per group
+// ---------------------------------------------- kNN by bucket groups
+// Group of a bucket: every bucket within a group radius RL of the bucket's
+// box.  k_group_lists finds the groups (one wavefront per bucket, a
+// breadth-first descent of the radix tree) and k_knn_group settles the
+// bucket's queries against its group.  The group's FP64 positions are
+// staged in LDS; each wavefront settles one query at a time: squared
+// distances, computed exactly as the reference computes them, stay in
+// registers and ballot counts choose a squared radius T <= RL^2 whose ball
+// holds between K and CAP particles; the ball is compacted and sorted by
+// (d2, slot), and sorted again by (d2, caller index) if equal distances
+// meet among its first K+1.  Every particle within RL of a query is staged,
+// so the first K are the exact K nearest.  Queries that cannot be settled
+// this way go to a retry list served by k_query.
+constexpr int KL_CM = 2240;            // staged candidates per group
+constexpr int KL_MB = 256;             // buckets per group
 constexpr int KL_NV = KL_CM / 64;      // candidates per lane
+constexpr int KL_FC = 512;             // descent frontier per wavefront
 
 struct KLArgs {
-    int n, nLeaf, nL2, K, per;
-    double L[3];
-    const double *sx, *sy, *sz, *box;
-    const int *order;
-    const uint32_t *qflag;
-    const double *guess;               // previous fBall2 (<= 0: none)
-    double *fb2;
-    int *nbr;
+    QArgs Q;
+    int nb;                            // buckets
+    const int *bfirst, *bcnt;
+    int *gfirst, *gcnt;                // group lists, KL_MB per bucket
+    int *gnb;                          // buckets in the group, -1: too large
+    double *grl2;                      // group radius^2
     int *retry;
     unsigned *nretry;
-    unsigned *err;
 };
 
 // Squared gap between two boxes, minimised over periodic images of a.
@@ -485,195 +711,264 @@ __device__ __forceinline__ double boxbox_d2(const double *a, const double *b, in
     return d2;
 }
 
-__device__ __forceinline__ void wave_sync()
+// One wavefront per bucket: group radius from the fBall2 guesses of the
+// bucket's particles (else from the bucket's density) and the group list.
+__global__ __launch_bounds__(256) void k_group_lists(KLArgs KA)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// minimum image of an offset (periodic) or the offset itself
-__device__ __forceinline__ double wrap(double t, int per, double L)
-{
-    if (per) {
-        if (t > 0.5 * L) t -= L;
-        else if (t < -0.5 * L) t += L;
-    }
-    return t;
-}
-
-// Bitonic sort of 128 (d2, caller, slot) triples held 2 per lane
-// (element e = lane + 64 k), ascending in (d2, caller).
-__device__ void bitonic128(double (&d)[2], int (&c)[2], int (&s)[2], int lane)
-{
-    for (int size = 2; size <= 128; size <<= 1) {
-        for (int j = size >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-                const int e = lane;
-                const bool asc = (e & size) == 0;
-                const bool sw = asc ? kv_less(d[1], c[1], d[0], c[0]) : kv_less(d[0], c[0], d[1], c[1]);
-                if (sw) {
-                    const double td = d[0]; d[0] = d[1]; d[1] = td;
-                    const int tc = c[0]; c[0] = c[1]; c[1] = tc;
-                    const int ts = s[0]; s[0] = s[1]; s[1] = ts;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int e = lane + 64 * k;
-                    const double pd = __shfl_xor(d[k], j);
-                    const int pc = __shfl_xor(c[k], j);
-                    const int ps = __shfl_xor(s[k], j);
-                    const bool lower = (lane & j) == 0;
-                    const bool asc = (e & size) == 0;
-                    const bool take = (lower == asc) ? kv_less(pd, pc, d[k], c[k])
-                                                     : kv_less(d[k], c[k], pd, pc);
-                    if (take) {
-                        d[k] = pd;
-                        c[k] = pc;
-                        s[k] = ps;
-                    }
-                }
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_knn_leaf(KLArgs A)
-{
-    __shared__ int fr[2][FCAP];
-    __shared__ int lf[KL_ML];
-    __shared__ float cx[KL_CM], cy[KL_CM], cz[KL_CM];
-    __shared__ double bd[4][CAP];
-    __shared__ int bc[4][CAP], bs[4][CAP];
-    __shared__ int s_cnt[2];
-    __shared__ double s_R2;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int leaf = blockIdx.x;
-    const int s0 = leaf * LEAF, nq = min(LEAF, A.n - s0);
+    const QArgs &A = KA.Q;
+    __shared__ int fr[4][2 * KL_FC];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = blockIdx.x * 4 + w;
+    if (g >= KA.nb) return;
+    const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
     const int K = A.K;
-    const double *lb = A.box + 6 * (size_t)(A.nL2 + leaf);
-    const double ox = 0.5 * (lb[0] + lb[3]), oy = 0.5 * (lb[1] + lb[4]), oz = 0.5 * (lb[2] + lb[5]);
-    if (w == 0) {
-        double g = 0.0;
-        bool none = false;
-        if (lane < nq) {
-            const double v = A.guess ? A.guess[s0 + lane] : 0.0;
-            none = !(v > 0);
-            g = none ? 0.0 : v;
-        }
-        for (int o = 32; o > 0; o >>= 1) g = fmax(g, __shfl_xor(g, o));
-        const unsigned long long nh = __ballot(none);
-        if (lane == 0) {
-            double R2;
-            if (nh == 0) {
-                R2 = g * 1.44;
-            } else {   // leaf density estimate, ~2K particles in the ball
-                const double ex = lb[3] - lb[0], ey = lb[4] - lb[1], ez = lb[5] - lb[2];
-                double vol = ex * ey * ez;
-                const double emax = fmax(ex, fmax(ey, ez));
-                if (!(vol > 0)) vol = emax * emax * emax;
-                double r3 = vol * (2.0 * K) / fmax(nq, 1) * (3.0 / (4.0 * M_PI));
-                if (!(r3 > 0)) r3 = 1e-30;
-                R2 = cbrt(r3 * r3);
+    const double *lb = A.bbox + 6 * (size_t)g;
+    double gmax = 0.0;
+    bool none = false;
+    if (lane < nq) {
+        const double v = A.guess ? A.guess[s0 + lane] : 0.0;
+        none = !(v > 0);
+        gmax = none ? 0.0 : v;
+    }
+    for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
+    const unsigned long long nh = __ballot(none);
+    double R2;
+    if (nh == 0) {
+        R2 = gmax * 1.44;
+    } else {   // bucket density estimate, ~2K particles in the ball
+        const double ex = lb[3] - lb[0], ey = lb[4] - lb[1], ez = lb[5] - lb[2];
+        double vol = ex * ey * ez;
+        const double emax = fmax(ex, fmax(ey, ez));
+        if (!(vol > 0)) vol = emax * emax * emax;
+        double r3 = vol * (2.0 * K) / fmax(nq, 1) * (3.0 / (4.0 * M_PI));
+        if (!(r3 > 0)) r3 = 1e-30;
+        R2 = cbrt(r3 * r3);
+    }
+    const double bx[6] = {lb[0], lb[1], lb[2], lb[3], lb[4], lb[5]};
+    const double RL2b = R2 * (1.0 + 1e-12);
+    int tot = 0;
+    int nb = wave_bfs(
+        A, [&](const double *b) { return boxbox_d2(b, bx, A.per, A.L) <= RL2b; }, fr[w], KL_FC,
+        KA.gfirst + (size_t)g * KL_MB, KA.gcnt + (size_t)g * KL_MB, KL_MB, lane, &tot);
+    if (nb == 0 || tot > KL_CM) nb = -1;   // none only for a NaN radius
+    if (lane == 0) {
+        KA.gnb[g] = nb;
+        KA.grl2[g] = R2;
+        if (nb >= 0) atomicMax(&A.err[5], (unsigned)nb);
+    }
+}
+
+// r-th smallest (0-based) of the values v[k] held by the elements
+// lane + 64 k with ok[k] set, by a wavefront quickselect whose pivot is the
+// lowest-numbered element still in range; *nlt and *neq count the values
+// below and equal to it.  Returns ~0 if the range empties (never for
+// 0 <= r < number of elements).
+template <int KE>
+__device__ uint64_t wave_select(const uint64_t (&v)[KE], const bool (&ok)[KE], int r, int *nlt,
+                                int *neq)
+{
+    uint64_t lo = 0, hi = ~0ull;
+    int base = 0;
+    for (int it = 0; it < 64 * KE + 1; ++it) {
+        uint64_t p = 0;
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const unsigned long long m = __ballot(ok[k] && v[k] >= lo && v[k] <= hi);
+            if (!found && m) {
+                const int src = __ffsll((long long)m) - 1;
+                const uint32_t plo = __builtin_amdgcn_readlane((uint32_t)v[k], src);
+                const uint32_t phi = __builtin_amdgcn_readlane((uint32_t)(v[k] >> 32), src);
+                p = ((uint64_t)phi << 32) | plo;
+                found = true;
             }
-            s_R2 = R2;
-            s_cnt[0] = 0;
-            s_cnt[1] = 0;
-            fr[0][0] = 1;
+        }
+        if (!found) break;
+        int cl = 0, ce = 0;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            cl += __popcll(__ballot(ok[k] && v[k] >= lo && v[k] < p));
+            ce += __popcll(__ballot(ok[k] && v[k] == p));
+        }
+        if (r < base + cl) {
+            hi = p - 1;
+        } else if (r < base + cl + ce) {
+            *nlt = base + cl;
+            *neq = ce;
+            return p;
+        } else {
+            base += cl + ce;
+            lo = p + 1;
         }
     }
-    __syncthreads();
-    const double RL2 = s_R2;
-    const double RL2b = RL2 * (1.0 + 1e-12);
-    int nf = 1, cur = 0;
-    bool ovf = false;
-    while (nf > 0) {
-        for (int k = t; k < nf; k += 256) {
-            const int node = fr[cur][k];
-            if (boxbox_d2(A.box + 6 * (size_t)node, lb, A.per, A.L) <= RL2b) {
-                if (node >= A.nL2) {
-                    const int p = atomicAdd(&s_cnt[1], 1);
-                    if (p < KL_ML) lf[p] = node - A.nL2;
-                } else {
-                    const int p = atomicAdd(&s_cnt[0], 2);
-                    if (p + 1 < FCAP) {
-                        fr[cur ^ 1][p] = 2 * node;
-                        fr[cur ^ 1][p + 1] = 2 * node + 1;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        nf = s_cnt[0];
-        const int nlv = s_cnt[1];
-        __syncthreads();
-        if (t == 0) s_cnt[0] = 0;
-        if (nf > FCAP || nlv > KL_ML) {
-            ovf = true;
-            break;
-        }
-        cur ^= 1;
-        __syncthreads();
+    *nlt = *neq = 0;
+    return ~0ull;
+}
+
+// The K nearest of a settled ball of cnt <= 64 KE particles held in LDS as
+// (d2, slot): the K-th smallest d2 by quickselect on its bit pattern
+// (non-negative doubles order as their bits), then every particle below it
+// and, when equal distances straddle the K-th place, those of the smallest
+// caller indices among the equal ones.  The list is written in ball order.
+// Returns false (nothing written) if the selection does not come to K.
+template <int KE>
+__device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, const int *bs,
+                           int lane)
+{
+    const int K = A.K;
+    uint64_t v[KE];
+    bool ok[KE];
+    int s[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = lane + 64 * k;
+        ok[k] = e < cnt;
+        v[k] = ok[k] ? (uint64_t)__double_as_longlong(bd[e]) : ~0ull;
+        s[k] = ok[k] ? bs[e] : -1;
     }
-    if (ovf) {             // group too large to stage: every query retries
+    int nlt, neq;
+    const uint64_t pK = wave_select<KE>(v, ok, K - 1, &nlt, &neq);
+    bool take[KE];
+    if (nlt + neq == K) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) take[k] = ok[k] && v[k] <= pK;
+    } else {   // equal distances straddle the K-th place: caller index decides
+        uint64_t c[KE];
+        bool eq[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            eq[k] = ok[k] && v[k] == pK;
+            c[k] = eq[k] ? (uint64_t)(uint32_t)A.order[s[k]] : ~0ull;
+        }
+        int l2, q2;
+        const uint64_t cK = wave_select<KE>(c, eq, K - nlt - 1, &l2, &q2);
+#pragma unroll
+        for (int k = 0; k < KE; ++k) take[k] = ok[k] && (v[k] < pK || (eq[k] && c[k] <= cK));
+        if (lane == 0) atomicAdd(&A.err[1], 1u);
+    }
+    unsigned long long m[KE];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        m[k] = __ballot(take[k]);
+        tot += __popcll(m[k]);
+    }
+    if (tot != K) return false;   // not reached for a settled ball
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        if (take[k]) A.nbr[(size_t)q * K + base + __popcll(m[k] & ((1ull << lane) - 1))] = s[k];
+        base += __popcll(m[k]);
+    }
+    if (lane == 0) A.fb2[q] = __longlong_as_double((long long)pK);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
+{
+    const QArgs &A = KA.Q;
+    __shared__ double cx[KL_CM], cy[KL_CM], cz[KL_CM];
+    __shared__ int csl[KL_CM];
+    __shared__ double bd[4][CAP];
+    __shared__ int bs[4][CAP];
+    __shared__ int lfst[KL_MB], lcnt[KL_MB], loff[KL_MB];
+    __shared__ double qx[BKT], qy[BKT], qz[BKT];
+    __shared__ int qfl[BKT];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int g = blockIdx.x;
+    const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
+    const int K = A.K;
+    const int nb = KA.gnb[g];
+    if (nb < 0) {          // group too large to stage: every query retries
         if (t == 0) atomicAdd(&A.err[2], 1u);
         if (t < nq && A.qflag[s0 + t]) {
-            const unsigned p = atomicAdd(A.nretry, 1u);
-            A.retry[p] = s0 + t;
+            const unsigned p = atomicAdd(KA.nretry, 1u);
+            KA.retry[p] = s0 + t;
         }
         return;
     }
-    const int nl = s_cnt[1];
-    if (t == 0) atomicMax(&A.err[5], (unsigned)nl);
-    const int nc = nl * LEAF;
-    // staged offsets from the leaf-box centre; M bounds their magnitude
-    float mloc = 0.f;
-    for (int c = t; c < nc; c += 256) {
-        const int slot = lf[c / LEAF] * LEAF + (c % LEAF);
-        float fx = 3.0e30f, fy = 3.0e30f, fz = 3.0e30f;
-        if (slot < A.n) {
-            fx = (float)wrap(A.sx[slot] - ox, A.per, A.L[0]);
-            fy = (float)wrap(A.sy[slot] - oy, A.per, A.L[1]);
-            fz = (float)wrap(A.sz[slot] - oz, A.per, A.L[2]);
-            mloc = fmaxf(mloc, fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))));
-        }
-        cx[c] = fx;
-        cy[c] = fy;
-        cz[c] = fz;
+    const double RL2 = KA.grl2[g];
+    if (t < nb) {
+        lfst[t] = A.bfirst[KA.gfirst[(size_t)g * KL_MB + t]];
+        lcnt[t] = KA.gcnt[(size_t)g * KL_MB + t];
     }
-    for (int o = 32; o > 0; o >>= 1) mloc = fmaxf(mloc, __shfl_xor(mloc, o));
-    if (lane == 0) atomicMax((int *)&s_cnt[0], __float_as_int(mloc));   // s_cnt[0] is 0 here
+    if (t >= 64 && t < 64 + nq) {
+        const int j = t - 64;
+        qx[j] = A.sx[s0 + j];
+        qy[j] = A.sy[s0 + j];
+        qz[j] = A.sz[s0 + j];
+        qfl[j] = A.qflag[s0 + j] != 0;
+    }
     __syncthreads();
-    const double M = (double)__int_as_float(s_cnt[0]);
+    if (w == 0) {          // staging offsets: exclusive prefix of the counts
+        int run = 0;
+        for (int base = 0; base < nb; base += 64) {
+            const int k = base + lane;
+            const int v = k < nb ? lcnt[k] : 0;
+            int incl = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o);
+                if (lane >= o) incl += u;
+            }
+            if (k < nb) loff[k] = run + incl - v;
+            run += __shfl(incl, 63);
+        }
+    }
+    __syncthreads();
+    const int nc = loff[nb - 1] + lcnt[nb - 1];
+    for (int k = w; k < nb; k += 4)
+        if (lane < lcnt[k]) csl[loff[k] + lane] = lfst[k] + lane;
+    __syncthreads();
+    for (int c0 = t; c0 < nc; c0 += 4 * 256) {
+        int sl[4];
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = c0 + 256 * u;
+            sl[u] = c < nc ? csl[c] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sl[u] >= 0) {
+                px[u] = A.sx[sl[u]];
+                py[u] = A.sy[sl[u]];
+                pz[u] = A.sz[sl[u]];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sl[u] >= 0) {
+                const int c = c0 + 256 * u;
+                cx[c] = px[u];
+                cy[c] = py[u];
+                cz[c] = pz[u];
+            }
+    }
+    __syncthreads();
     for (int j = w; j < nq; j += 4) {
+        if (!qfl[j]) continue;
         const int q = s0 + j;
-        if (!A.qflag[q]) continue;
-        const double x = A.sx[q], y = A.sy[q], z = A.sz[q];
-        const float qx = (float)wrap(x - ox, A.per, A.L[0]);
-        const float qy = (float)wrap(y - oy, A.per, A.L[1]);
-        const float qz = (float)wrap(z - oz, A.per, A.L[2]);
-        float dd[KL_NV];
+        const double x = qx[j], y = qy[j], z = qz[j];
+        double dd[KL_NV];
 #pragma unroll
         for (int k = 0; k < KL_NV; ++k) {
             const int c = lane + 64 * k;
-            float d2 = 3.0e38f;
+            double d2 = DBL_MAX;
             if (c < nc) {
-                const float dx = qx - cx[c], dy = qy - cy[c], dz = qz - cz[c];
-                d2 = dx * dx + dy * dy + dz * dz;
+                const double dx = img_sep(x, cx[c], A.per, A.L[0]);
+                const double dy = img_sep(y, cy[c], A.per, A.L[1]);
+                const double dz = img_sep(z, cz[c], A.per, A.L[2]);
+                d2 = sq_dist(dx, dy, dz);
             }
             dd[k] = d2;
         }
         const double gq = A.guess ? A.guess[q] : 0.0;
         double T = gq > 0 ? fmin(gq * 1.15, RL2) : RL2;
-        // FP32 rounding bound: |d32^2 - d64^2| <= tol(T) for d near sqrt(T)
         bool ok = false;
         int cnt = 0;
         for (int it = 0; it < 16; ++it) {
-            const float Tf = (float)(T + 1e-6 * (M * sqrt(T) + T));
             cnt = 0;
 #pragma unroll
-            for (int k = 0; k < KL_NV; ++k) cnt += __popcll(__ballot(dd[k] <= Tf));
+            for (int k = 0; k < KL_NV; ++k) cnt += __popcll(__ballot(dd[k] <= T));
             if (cnt >= K && cnt <= CAP) {
                 ok = true;
                 break;
@@ -690,94 +985,33 @@ __global__ __launch_bounds__(256) void k_knn_leaf(KLArgs A)
                 T *= fmin(fmax(f, 0.05), 0.95);
             }
         }
-        if (ok) {   // compact the superset with exact FP64 distances
-            const float Tf = (float)(T + 1e-6 * (M * sqrt(T) + T));
-            int base = 0, in64 = 0;
-#pragma unroll
-            for (int k = 0; k < KL_NV; ++k) {
-                const bool in = dd[k] <= Tf;
-                const unsigned long long m = __ballot(in);
-                if (m) {
-                    double d2 = DBL_MAX;
-                    int slot = 0;
-                    if (in) {
-                        const int c = lane + 64 * k;
-                        slot = lf[c / LEAF] * LEAF + (c % LEAF);
-                        const double dx = img_sep(x, A.sx[slot], A.per, A.L[0]);
-                        const double dy = img_sep(y, A.sy[slot], A.per, A.L[1]);
-                        const double dz = img_sep(z, A.sz[slot], A.per, A.L[2]);
-                        d2 = sq_dist(dx, dy, dz);
-                        const int p = base + __popcll(m & ((1ull << lane) - 1));
-                        bd[w][p] = d2;
-                        bs[w][p] = slot;
-                        bc[w][p] = A.order[slot];
-                    }
-                    in64 += __popcll(__ballot(in && d2 <= T));
-                    base += __popcll(m);
-                }
-            }
-            ok = in64 >= K;
-            cnt = base;
-            if (!ok && lane == 0) atomicAdd(&A.err[4], 1u);
-        } else if (cnt >= K && lane == 0) {
-            atomicAdd(&A.err[4], 1u);
-        }
         if (!ok) {
             if (lane == 0) {
-                const unsigned p = atomicAdd(A.nretry, 1u);
-                A.retry[p] = q;
+                if (cnt >= K) atomicAdd(&A.err[4], 1u);
+                const unsigned p = atomicAdd(KA.nretry, 1u);
+                KA.retry[p] = q;
             }
             continue;
         }
-        wave_sync();
-        double dK = 0, dK1 = DBL_MAX;
-        const int eK = K - 1;
-        if (cnt <= 128) {
-            double d[2];
-            int c[2], s[2];
+        int base = 0;      // compact the ball
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int e = lane + 64 * k;
-                d[k] = e < cnt ? bd[w][e] : DBL_MAX;
-                c[k] = e < cnt ? bc[w][e] : 0x7fffffff;
-                s[k] = e < cnt ? bs[w][e] : -1;
+        for (int k = 0; k < KL_NV; ++k) {
+            const bool in = dd[k] <= T;
+            const unsigned long long m = __ballot(in);
+            if (in) {
+                const int p = base + __popcll(m & ((1ull << lane) - 1));
+                bd[w][p] = dd[k];
+                bs[w][p] = csl[lane + 64 * k];
             }
-            wave_sync();
-            bitonic128(d, c, s, lane);
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int e = lane + 64 * k;
-                if (e < K) A.nbr[(size_t)q * K + e] = s[k];
-                const double a = __shfl(d[k], eK & 63);
-                if (k == (eK >> 6)) dK = a;
-                const double b = __shfl(d[k], K & 63);
-                if (k == (K >> 6) && K < 128) dK1 = b;
-            }
-        } else {
-            double d[4];
-            int c[4], s[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int e = lane + 64 * k;
-                d[k] = e < cnt ? bd[w][e] : DBL_MAX;
-                c[k] = e < cnt ? bc[w][e] : 0x7fffffff;
-                s[k] = e < cnt ? bs[w][e] : -1;
-            }
-            wave_sync();
-            bitonic256(d, c, s, lane);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int e = lane + 64 * k;
-                if (e < K) A.nbr[(size_t)q * K + e] = s[k];
-                const double a = __shfl(d[k], eK & 63);
-                if (k == (eK >> 6)) dK = a;
-                const double b = __shfl(d[k], K & 63);
-                if (k == (K >> 6) && K < CAP) dK1 = b;
-            }
+            base += __popcll(m);
         }
-        if (lane == 0) {
-            A.fb2[q] = dK;
-            if (dK1 == dK) atomicAdd(&A.err[1], 1u);
+        wave_sync();
+        const bool sel = cnt <= 128 ? knn_select<2>(A, q, cnt, bd[w], bs[w], lane)
+                                    : knn_select<4>(A, q, cnt, bd[w], bs[w], lane);
+        if (!sel && lane == 0) {
+            atomicAdd(&A.err[4], 1u);
+            const unsigned p = atomicAdd(KA.nretry, 1u);
+            KA.retry[p] = q;
         }
         wave_sync();
     }
@@ -1502,8 +1736,6 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     SphDev &S = pkd->sph;
     QArgs Q;
     Q.n = S.n;
-    Q.nLeaf = S.nLeaf;
-    Q.nL2 = S.nL2;
     Q.K = K;
     Q.per = S.bPeriodic;
     Q.mode = mode;
@@ -1511,7 +1743,11 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.sx = S.sx.as<double>();
     Q.sy = S.sy.as<double>();
     Q.sz = S.sz.as<double>();
-    Q.box = S.nodeBox.as<double>();
+    Q.rec = S.crec.as<CRec>();
+    Q.bof = S.bof.as<int>();
+    Q.bbox = S.bbox.as<double>();
+    Q.bfirst = S.bfirst.as<int>();
+    Q.bcnt = S.bcnt.as<int>();
     Q.order = S.order.as<int>();
     Q.qflag = S.qf.as<uint32_t>();
     Q.guess = nullptr;
@@ -1600,23 +1836,62 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     hipLaunchKernelGGL(k_gather_pos, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
                        pkd->f[GG_R].as<double>(), S.sx.as<double>(), S.sy.as<double>(),
                        S.sz.as<double>());
-    const int nLeaf = (n + LEAF - 1) / LEAF;
-    int nL2 = 1;
-    while (nL2 < nLeaf) nL2 <<= 1;
-    GG_TRY(S.nodeBox.reserve(sizeof(double) * 6 * 2 * (size_t)nL2));
-    hipLaunchKernelGGL(k_leaf_box, dim3(nblk(nL2, 256)), dim3(256), 0, st, n, nLeaf, nL2,
-                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),
-                       S.nodeBox.as<double>());
-    for (int first = nL2 >> 1; first >= 1; first >>= 1)
-        hipLaunchKernelGGL(k_level_box, dim3(nblk(first, 256)), dim3(256), 0, st, first, first,
-                           S.nodeBox.as<double>());
+    // radix tree over the sorted keys, its buckets, their boxes and the
+    // child records the searches descend
+    const int ni = std::max(n - 1, 1);
+    GG_TRY(S.child.reserve(sizeof(int) * 2 * ni));
+    GG_TRY(S.rng.reserve(sizeof(int) * 2 * ni));
+    GG_TRY(S.bflag.reserve(sizeof(int) * n));
+    GG_TRY(S.bcode.reserve(sizeof(int) * n));
+    GG_TRY(S.bof.reserve(sizeof(int) * n));
+    int nb = 1;
+    if (n > BKT) {
+        GG_HIP(hipMemsetAsync(S.bflag.p, 0, sizeof(int) * n, st));
+        hipLaunchKernelGGL(k_lbvh, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
+                           S.key.as<uint64_t>(), S.child.as<int>(), S.rng.as<int>());
+        hipLaunchKernelGGL(k_bucket_mark, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
+                           S.child.as<int>(), S.rng.as<int>(), S.bflag.as<int>(),
+                           S.bcode.as<int>());
+        int64_t tb = 0;
+        GG_TRY(exclusive_scan_i64(st, S.bflag.as<int>(), S.pos.as<int64_t>(), n, S.sortTmp, &tb));
+        nb = (int)tb;
+        int nlev = 1;
+        while ((2 << (nlev - 1)) <= nb) ++nlev;   // levels 0 .. floor(log2 nb)
+        GG_TRY(S.bfirst.reserve(sizeof(int) * nb));
+        GG_TRY(S.bcnt.reserve(sizeof(int) * nb));
+        GG_TRY(S.bbox.reserve(sizeof(double) * 6 * nb));
+        GG_TRY(S.sparse.reserve(sizeof(float) * 6 * (size_t)nb * nlev));
+        float *sp = S.sparse.as<float>();
+        hipLaunchKernelGGL(k_bucket_write, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           S.bflag.as<int>(), S.bcode.as<int>(), S.pos.as<int64_t>(),
+                           S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),
+                           S.bfirst.as<int>(), S.bcnt.as<int>(), S.bbox.as<double>(), sp,
+                           S.bof.as<int>());
+        for (int lv = 1; lv < nlev; ++lv)
+            hipLaunchKernelGGL(k_sparse_level, dim3(nblk(nb, 256)), dim3(256), 0, st, nb,
+                               1 << (lv - 1), sp + 6 * (size_t)(lv - 1) * nb,
+                               sp + 6 * (size_t)lv * nb);
+        GG_TRY(S.crec.reserve(sizeof(CRec) * 2 * (size_t)(n - 1)));
+        hipLaunchKernelGGL(k_child_rec, dim3(nblk(2 * (n - 1), 256)), dim3(256), 0, st, n, nb,
+                           S.child.as<int>(), S.rng.as<int>(), S.bof.as<int>(), sp,
+                           S.crec.as<CRec>());
+    } else {   // a single bucket: the whole set; the searches need no tree
+        GG_TRY(S.bfirst.reserve(sizeof(int)));
+        GG_TRY(S.bcnt.reserve(sizeof(int)));
+        GG_TRY(S.bbox.reserve(sizeof(double) * 6));
+        GG_HIP(hipMemsetAsync(S.bof.p, 0, sizeof(int) * n, st));
+        const int one[2] = {0, n};
+        GG_HIP(hipMemcpyAsync(S.bfirst.p, &one[0], sizeof(int), hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(S.bcnt.p, &one[1], sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bbox_slots, dim3(1), dim3(64), 0, st, n, S.sx.as<double>(),
+                           S.sy.as<double>(), S.sz.as<double>(), S.bbox.as<double>());
+    }
     GG_HIP(hipGetLastError());
     S.n = n;
-    S.nLeaf = nLeaf;
-    S.nL2 = nL2;
+    S.nb = nb;
     S.bPeriodic = bPeriodic;
     for (int d = 0; d < 3; ++d) S.L[d] = bPeriodic ? fPeriod[d] : 0.0;
-    S.nBucket = LEAF;
+    S.nBucket = BKT;
     S.built = 1;
     S.listValid = 0;
     timer_mark(pkd, 1);
@@ -1650,26 +1925,25 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(S.nretry.reserve(sizeof(unsigned)));
     GG_HIP(hipMemsetAsync(S.nretry.p, 0, sizeof(unsigned), st));
     KLArgs KL;
-    KL.n = n;
-    KL.nLeaf = S.nLeaf;
-    KL.nL2 = S.nL2;
-    KL.K = K;
-    KL.per = S.bPeriodic;
-    for (int d = 0; d < 3; ++d) KL.L[d] = S.L[d];
-    KL.sx = S.sx.as<double>();
-    KL.sy = S.sy.as<double>();
-    KL.sz = S.sz.as<double>();
-    KL.box = S.nodeBox.as<double>();
-    KL.order = S.order.as<int>();
-    KL.qflag = S.qf.as<uint32_t>();
-    KL.guess = S.guess.as<double>();
-    KL.fb2 = S.fb2s.as<double>();
-    KL.nbr = S.nbr.as<int>();
+    KL.Q = query_args(pkd, 0, K);
+    KL.Q.guess = S.guess.as<double>();
+    KL.Q.nbr = S.nbr.as<int>();
+    KL.nb = S.nb;
+    KL.bfirst = S.bfirst.as<int>();
+    KL.bcnt = S.bcnt.as<int>();
+    GG_TRY(S.grpf.reserve(sizeof(int) * KL_MB * (size_t)S.nb));
+    GG_TRY(S.grpc.reserve(sizeof(int) * KL_MB * (size_t)S.nb));
+    GG_TRY(S.grpn.reserve(sizeof(int) * S.nb));
+    GG_TRY(S.grpr.reserve(sizeof(double) * S.nb));
+    KL.gfirst = S.grpf.as<int>();
+    KL.gcnt = S.grpc.as<int>();
+    KL.gnb = S.grpn.as<int>();
+    KL.grl2 = S.grpr.as<double>();
     KL.retry = S.retry.as<int>();
     KL.nretry = S.nretry.as<unsigned>();
-    KL.err = S.err.as<unsigned>();
     timer_mark(pkd, 0);
-    hipLaunchKernelGGL(k_knn_leaf, dim3(S.nLeaf), dim3(256), 0, st, KL);
+    hipLaunchKernelGGL(k_group_lists, dim3((S.nb + 3) / 4), dim3(256), 0, st, KL);
+    hipLaunchKernelGGL(k_knn_group, dim3(S.nb), dim3(256), 0, st, KL);
     GG_HIP(hipGetLastError());
     unsigned nretry = 0;
     GG_HIP(hipMemcpyAsync(&nretry, S.nretry.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));

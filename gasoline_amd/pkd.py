@@ -133,12 +133,13 @@ class Pkd:
         h = _vp()
         _check(L.gg_initialize(C.byref(h), int(device), int(n)))
         self.h = h
+        self._finish = L.gg_finish   # held so that close() works at interpreter exit
         self.n = int(n)
         self.set_precision(prec)
 
     def close(self):
         if getattr(self, "h", None):
-            lib().gg_finish(self.h)
+            self._finish(self.h)
             self.h = None
 
     def __del__(self):

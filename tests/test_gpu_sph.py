@@ -1,8 +1,10 @@
 """GPU parity of the SPH path (gg_build_local, gg_smooth, gg_gas_pressure,
 gg_resmooth) against the CPU oracle, through the C ABI.
 
-Bars: neighbour lists and fBall2 bit-exact (the oracle orders lists by
-(d2, caller index) as the library does); FP64 operator outputs within 1e-12
+Bars: neighbour sets and fBall2 bit-exact, with ties at the K-th distance
+broken by the smaller caller index in both (a list's order is not compared:
+the library writes it in ball order, the oracle by (d2, caller index), the
+reference in its priority-queue order); FP64 operator outputs within 1e-12
 relative; FP32 within 1e-5 relative to the field's scale (1e-4 for the
 second-derivative quantities dvds and for single values near zero)."""
 import numpy as np
@@ -66,7 +68,28 @@ def test_knn_bitexact(gpu, case, request):
         nbr, fb2 = pk.smooth_lists(K), pk.get("fBall2")
     assert ns == len(d["m"])
     assert np.array_equal(fb2, ref["fb2"])
-    assert np.array_equal(nbr, ref["nbr"])
+    assert np.array_equal(np.sort(nbr, axis=1), np.sort(ref["nbr"], axis=1))
+
+
+def test_knn_lattice_ties(gpu):
+    # periodic 16^3 lattice, K = 32: 27 particles lie within sqrt(3) spacings
+    # and 6 at two spacings, so every list takes 5 of 6 equal distances and
+    # the caller index decides which
+    g = np.arange(16) / 16.0
+    r = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    r = r[np.random.default_rng(4).permutation(len(r))]
+    nbr_o, fb2_o, ties_o = O.knn(r, 32, period=[1, 1, 1])
+    assert ties_o == len(r)
+    with gpu.Pkd(len(r)) as pk:
+        pk.put("r", r)
+        pk.put("fMass", np.full(len(r), 1.0 / len(r)))
+        pk.build_local(period=[1, 1, 1])
+        pk.smooth(gpu.Smf.default(), 32, True)
+        nbr, fb2 = pk.smooth_lists(32), pk.get("fBall2")
+        st = pk.smooth_stats()
+    assert st["ties"] == len(r)
+    assert np.array_equal(fb2, fb2_o)
+    assert np.array_equal(np.sort(nbr, axis=1), np.sort(nbr_o, axis=1))
 
 
 @pytest.mark.parametrize("prec", ["FP64", "FP32"])
