@@ -76,7 +76,7 @@ struct SphDev {
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
     DevBuf sx, sy, sz;                         // slot order positions
     DevBuf child, rng, sparse;                 // radix tree, bucket-box sparse table
-    DevBuf bflag, bcode, bfirst, bcnt, bbox, bof; // buckets
+    DevBuf bflag, bcode, bpar, bfirst, bcnt, bbox, bden, bof; // buckets
     DevBuf crec, grpf, grpc, grpn, grpr;       // child records, kNN groups
     DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
     DevBuf gcnt, goff, gent;                   // ball-gather CSR
@@ -88,7 +88,7 @@ struct SphDev {
     void release_all()
     {
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
-                         &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode,
+                         &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,

@@ -51,6 +51,29 @@ __device__ __forceinline__ double sq_dist(double dx, double dy, double dz)
     return dx * dx + dy * dy + dz * dz;
 }
 
+// FP32 bounds rounded outwards: f_down(d) <= d <= f_up(d).
+__device__ __forceinline__ float f_down(double d)
+{
+    float f = (float)d;
+    if ((double)f > d) {
+        int b = __float_as_int(f);
+        b = f > 0 ? b - 1 : (f == 0 ? (int)0x80000001u : b + 1);
+        f = __int_as_float(b);
+    }
+    return f;
+}
+
+__device__ __forceinline__ float f_up(double d)
+{
+    float f = (float)d;
+    if ((double)f < d) {
+        int b = __float_as_int(f);
+        b = f > 0 ? b + 1 : (f == 0 ? 1 : b - 1);
+        f = __int_as_float(b);
+    }
+    return f;
+}
+
 // Squared distance from a point to a box, minimised over periodic images.
 __device__ __forceinline__ double box_d2(const double *b, double x, double y,
                                          double z, int per, const double *L)
@@ -116,9 +139,12 @@ __global__ void k_bbox(int n, const int *__restrict__ gidx, const double *__rest
         for (int d = 0; d < 6; ++d) part[6 * (size_t)blockIdx.x + d] = s[d][0];
 }
 
-// Box of slots [0, n) for n <= 64 (one wavefront).
-__global__ void k_bbox_slots(int n, const double *__restrict__ sx, const double *__restrict__ sy,
-                             const double *__restrict__ sz, double *__restrict__ box)
+// The one bucket of slots [0, n), n <= 64 (one wavefront): its box, the box
+// in FP32 rounded outwards, and its density over the quantisation cube.
+__global__ void k_bbox_slots(int n, double sc, const double *__restrict__ sx,
+                             const double *__restrict__ sy, const double *__restrict__ sz,
+                             double *__restrict__ box, float *__restrict__ fbox,
+                             double *__restrict__ den)
 {
     const int l = threadIdx.x;
     double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, u[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
@@ -132,11 +158,15 @@ __global__ void k_bbox_slots(int n, const double *__restrict__ sx, const double 
             v[d] = fmin(v[d], __shfl_xor(v[d], o));
             u[d] = fmax(u[d], __shfl_xor(u[d], o));
         }
-    if (l == 0)
+    if (l == 0) {
         for (int d = 0; d < 3; ++d) {
             box[d] = v[d];
             box[3 + d] = u[d];
+            fbox[d] = f_down(v[d]);
+            fbox[3 + d] = f_up(u[d]);
         }
+        den[0] = ldexp((double)n * sc * sc * sc, -48);
+    }
 }
 
 __device__ __forceinline__ uint64_t spread21(uint64_t v)
@@ -237,56 +267,47 @@ __device__ __forceinline__ void child_info(int c, const int *__restrict__ rng, i
     }
 }
 
-// Mark bucket starts: children of nodes larger than BKT that are not.
-__global__ void k_bucket_mark(int n, const int *__restrict__ child, const int *__restrict__ rng,
-                              int *__restrict__ flag, int *__restrict__ cnt)
+// Mark bucket starts: children of nodes larger than BKT that are not.  For
+// each, also note the parent's particle count and the length of the key
+// prefix its particles share (the parent's Morton cell).
+__global__ void k_bucket_mark(int n, const uint64_t *__restrict__ key,
+                              const int *__restrict__ child, const int *__restrict__ rng,
+                              int *__restrict__ flag, int *__restrict__ cnt,
+                              int2 *__restrict__ par)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1 || rng[2 * i + 1] - rng[2 * i] + 1 <= BKT) return;
+    if (i >= n - 1) return;
+    const int lo = rng[2 * i], hi = rng[2 * i + 1];
+    if (hi - lo + 1 <= BKT) return;
+    const int pre = min(lcp_delta(key, n, lo, hi), 64) - 16;   // of the 48 key bits
     for (int k = 0; k < 2; ++k) {
         int f, m;
         child_info(child[2 * i + k], rng, f, m);
         if (m <= BKT) {
             flag[f] = 1;
             cnt[f] = m;
+            par[f] = make_int2(hi - lo + 1, pre);
         }
     }
 }
 
-__device__ __forceinline__ float f_down(double d)
-{
-    float f = (float)d;
-    if ((double)f > d) {
-        int b = __float_as_int(f);
-        b = f > 0 ? b - 1 : (f == 0 ? (int)0x80000001u : b + 1);
-        f = __int_as_float(b);
-    }
-    return f;
-}
-
-__device__ __forceinline__ float f_up(double d)
-{
-    float f = (float)d;
-    if ((double)f < d) {
-        int b = __float_as_int(f);
-        b = f > 0 ? b + 1 : (f == 0 ? 1 : b - 1);
-        f = __int_as_float(b);
-    }
-    return f;
-}
-
 // Bucket b: first slot, count, FP64 box, FP32 outward box (sparse table
-// level 0) and the bucket of each of its slots.
+// level 0), the bucket of each of its slots, and a number density: the
+// parent's particle count over the volume of its Morton cell (a key prefix
+// of p bits spans 2^(48-p) quantisation cells of volume 1/sc^3).
 __global__ void k_bucket_write(int n, const int *__restrict__ flag, const int *__restrict__ cnt,
+                               const int2 *__restrict__ par, double sc,
                                const int64_t *__restrict__ pos, const double *__restrict__ sx,
                                const double *__restrict__ sy, const double *__restrict__ sz,
                                int *__restrict__ bfirst, int *__restrict__ bcnt,
                                double *__restrict__ bbox, float *__restrict__ fbox,
-                               int *__restrict__ bof)
+                               double *__restrict__ bden, int *__restrict__ bof)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !flag[i]) return;
     const int b = (int)pos[i], m = cnt[i];
+    const int2 pp = par[i];
+    bden[b] = ldexp((double)pp.x * sc * sc * sc, pp.y - 48);
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int k = i; k < i + m; ++k) {
         const double q[3] = {sx[k], sy[k], sz[k]};
@@ -361,6 +382,8 @@ struct QArgs {
     const int *bof;                    // slot -> bucket
     const double *bbox;                // bucket boxes
     const int *bfirst, *bcnt;          // bucket first slot, count
+    const double *bden;                // bucket number density estimate
+    const float *fbox;                 // bucket boxes, FP32 rounded outwards
     const int *order;                  // slot -> caller (tie key)
     const uint32_t *qflag;             // slot order: nonzero = query
     const double *guess;               // slot order: fBall2 guess (<= 0: none)
@@ -371,6 +394,15 @@ struct QArgs {
     unsigned int *err;                 // [0] failures, [1] boundary ties
     const int *qlist;                  // optional: block b serves slot qlist[b]
 };
+
+// Squared radius of a ball expected to hold 2K particles at number density
+// den (a cold start, with no fBall2 to go by).
+__device__ __forceinline__ double cold_r2(double den, int K)
+{
+    double r3 = (2.0 * K) / den * (3.0 / (4.0 * M_PI));
+    if (!(r3 > 0) || !(r3 < DBL_MAX)) r3 = 1e-30;
+    return cbrt(r3 * r3);
+}
 
 __device__ __forceinline__ bool kv_less(double da, int ca, double db, int cb)
 {
@@ -547,16 +579,8 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
         R2 = A.fb2[q];
     } else if (A.guess && A.guess[q] > 0) {
         R2 = A.guess[q] * 1.2;
-    } else {   // bucket density estimate, ~2K particles in the ball
-        const int b = A.bof[q];
-        const double *bb = A.bbox + 6 * (size_t)b;
-        const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
-        double vol = ex * ey * ez;
-        const double emax = fmax(ex, fmax(ey, ez));
-        if (!(vol > 0)) vol = emax * emax * emax;
-        double r3 = vol * (2.0 * K) / fmax(A.bcnt[b], 1) * (3.0 / (4.0 * M_PI));
-        if (!(r3 > 0)) r3 = 1e-30;
-        R2 = cbrt(r3 * r3);
+    } else {
+        R2 = cold_r2(A.bden[A.bof[q]], K);
     }
     for (int attempt = 0; attempt < MAXTRY; ++attempt) {
         const double R2b = R2 * (1.0 + 1e-12);
@@ -678,16 +702,17 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
 // meet among its first K+1.  Every particle within RL of a query is staged,
 // so the first K are the exact K nearest.  Queries that cannot be settled
 // this way go to a retry list served by k_query.
-constexpr int KL_CM = 2240;            // staged candidates per group
-constexpr int KL_MB = 256;             // buckets per group
-constexpr int KL_NV = KL_CM / 64;      // candidates per lane
+constexpr int KL_CM = 2048;            // staged candidates per group
+constexpr int KL_MB = 192;             // buckets per group
+constexpr int KL_SB = 32;              // buckets of a query's own ball
+constexpr int KL_NS = KL_SB / 2;       // candidates per lane
 constexpr int KL_FC = 512;             // descent frontier per wavefront
 
 struct KLArgs {
     QArgs Q;
     int nb;                            // buckets
     const int *bfirst, *bcnt;
-    int *gfirst, *gcnt;                // group lists, KL_MB per bucket
+    int *gfirst, *gcnt;                // group lists (bucket, count), KL_MB per bucket
     int *gnb;                          // buckets in the group, -1: too large
     double *grl2;                      // group radius^2
     int *retry;
@@ -733,17 +758,8 @@ __global__ __launch_bounds__(256) void k_group_lists(KLArgs KA)
     for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
     const unsigned long long nh = __ballot(none);
     double R2;
-    if (nh == 0) {
-        R2 = gmax * 1.44;
-    } else {   // bucket density estimate, ~2K particles in the ball
-        const double ex = lb[3] - lb[0], ey = lb[4] - lb[1], ez = lb[5] - lb[2];
-        double vol = ex * ey * ez;
-        const double emax = fmax(ex, fmax(ey, ez));
-        if (!(vol > 0)) vol = emax * emax * emax;
-        double r3 = vol * (2.0 * K) / fmax(nq, 1) * (3.0 / (4.0 * M_PI));
-        if (!(r3 > 0)) r3 = 1e-30;
-        R2 = cbrt(r3 * r3);
-    }
+    if (nh == 0) R2 = gmax * 1.44;
+    else R2 = cold_r2(A.bden[g], K);
     const double bx[6] = {lb[0], lb[1], lb[2], lb[3], lb[4], lb[5]};
     const double RL2b = R2 * (1.0 + 1e-12);
     int tot = 0;
@@ -864,6 +880,152 @@ __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, con
     return true;
 }
 
+// Squared distance of a staged candidate from a query, as the reference
+// computes it; IMG = false when no periodic image can apply to the pair
+// (then img_sep is a plain difference).
+template <bool IMG>
+__device__ __forceinline__ double pair_d2(double x, double y, double z, double px, double py,
+                                          double pz, const double *L)
+{
+    if (IMG)
+        return sq_dist(img_sep(x, px, 1, L[0]), img_sep(y, py, 1, L[1]), img_sep(z, pz, 1, L[2]));
+    return sq_dist(x - px, y - py, z - pz);
+}
+
+// LDS state of a kNN group shared by its wavefronts.
+struct KLGroup {
+    const double *cx, *cy, *cz;        // staged positions
+    const int *csl;                    // staged slot
+    const float (*lbx)[6];             // bucket boxes (FP32, outwards)
+    const int *lcnt, *loff;            // bucket count, staging offset
+    int nb, nc;
+};
+
+// Settles one query over the candidates of the nsel <= KL_SB buckets listed
+// in sel (two buckets per 64 lanes): finds T <= Tsel with
+// K <= |{d2 <= T}| <= CAP and compacts that ball into (bd, bs).  Returns 0
+// when settled (*cnt = ball size), 1 when fewer than K lie within Tsel
+// (*cnt = their number), 2 when no such T was found.
+template <bool IMG>
+__device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char *sel, int nsel,
+                          double x, double y, double z, double Tsel, double *bd, int *bs,
+                          int lane, int *cnt_out)
+{
+    const int K = A.K;
+    const int ns = (nsel + 1) >> 1;    // slots in use (wave-uniform)
+    double dd[KL_NS];
+    int ci[KL_NS];
+#pragma unroll
+    for (int k = 0; k < KL_NS; ++k) {
+        ci[k] = -1;
+        dd[k] = DBL_MAX;
+        if (k < ns) {
+            const int e = 2 * k + (lane >> 5), o = lane & 31;
+            if (e < nsel) {
+                const int b = sel[e];
+                if (o < G.lcnt[b]) {
+                    const int c = G.loff[b] + o;
+                    ci[k] = c;
+                    dd[k] = pair_d2<IMG>(x, y, z, G.cx[c], G.cy[c], G.cz[c], A.L);
+                }
+            }
+        }
+    }
+    double T = Tsel;
+    int cnt = 0;
+    bool ok = false;
+    for (int it = 0; it < 16; ++it) {
+        cnt = 0;
+#pragma unroll
+        for (int k = 0; k < KL_NS; ++k)
+            if (k < ns) cnt += __popcll(__ballot(dd[k] <= T));
+        if (cnt >= K && cnt <= CAP) {
+            ok = true;
+            break;
+        }
+        if (cnt < K) {
+            if (T >= Tsel) {
+                *cnt_out = cnt;
+                return 1;
+            }
+            const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
+            T = fmin(Tsel, T * fmin(fmax(f, 1.2), 16.0));
+        } else {
+            const double f = cbrt((1.25 * K / cnt) * (1.25 * K / cnt));
+            T *= fmin(fmax(f, 0.05), 0.95);
+        }
+    }
+    if (!ok) return 2;
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < KL_NS; ++k) {
+        if (k >= ns) break;
+        const bool in = dd[k] <= T;
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const int p = base + __popcll(m & ((1ull << lane) - 1));
+            bd[p] = dd[k];
+            bs[p] = G.csl[ci[k]];
+        }
+        base += __popcll(m);
+    }
+    *cnt_out = cnt;
+    return 0;
+}
+
+// One query: choose the buckets within Tsel of it, settle, and widen Tsel
+// (up to the group radius) while fewer than K lie within it, or narrow it
+// while the ball meets more than KL_SB buckets.  Returns false if the query
+// must be retried by k_query.
+template <bool IMG>
+__device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, int q, double x,
+                          double y, double z, double Tsel, double RL2, double *bd, int *bs,
+                          int lane)
+{
+    const int K = A.K;
+    for (int pass = 0; pass < 8; ++pass) {
+        const double Tb = Tsel * (1.0 + 1e-12);
+        int nsel = 0;
+        for (int kb = 0; kb < G.nb; kb += 64) {
+            const int k = kb + lane;
+            bool in = false;
+            if (k < G.nb) {
+                const double b[6] = {G.lbx[k][0], G.lbx[k][1], G.lbx[k][2],
+                                     G.lbx[k][3], G.lbx[k][4], G.lbx[k][5]};
+                in = box_d2(b, x, y, z, A.per, A.L) <= Tb;
+            }
+            const unsigned long long m = __ballot(in);
+            const int p = nsel + __popcll(m & ((1ull << lane) - 1));
+            if (in && p < KL_SB) sel[p] = (unsigned char)k;
+            nsel += __popcll(m);
+        }
+        wave_sync();
+        if (nsel > KL_SB) {            // too many buckets: a smaller ball first
+            Tsel *= 0.5;
+            continue;
+        }
+        int cnt = 0;
+        const int st = knn_settle<IMG>(A, G, sel, nsel, x, y, z, Tsel, bd, bs, lane, &cnt);
+        if (st == 1 && Tsel < RL2) {   // widen the ball towards the group radius
+            const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
+            Tsel = fmin(RL2, Tsel * fmin(fmax(f, 1.2), 16.0));
+            wave_sync();
+            continue;
+        }
+        if (st != 0) {
+            if (lane == 0) atomicAdd(&A.err[st == 1 ? 3 : 4], 1u);
+            return false;
+        }
+        wave_sync();
+        const bool done = cnt <= 128 ? knn_select<2>(A, q, cnt, bd, bs, lane)
+                                     : knn_select<4>(A, q, cnt, bd, bs, lane);
+        if (!done && lane == 0) atomicAdd(&A.err[4], 1u);
+        return done;
+    }
+    if (lane == 0) atomicAdd(&A.err[4], 1u);
+    return false;
+}
+
 __global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
 {
     const QArgs &A = KA.Q;
@@ -871,13 +1033,17 @@ __global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
     __shared__ int csl[KL_CM];
     __shared__ double bd[4][CAP];
     __shared__ int bs[4][CAP];
-    __shared__ int lfst[KL_MB], lcnt[KL_MB], loff[KL_MB];
-    __shared__ double qx[BKT], qy[BKT], qz[BKT];
+    __shared__ float lbx[KL_MB][6];
+    __shared__ int lcnt[KL_MB], loff[KL_MB];
+    __shared__ unsigned char sel[4][KL_SB];
+    __shared__ double qx[BKT], qy[BKT], qz[BKT], qg[BKT];
     __shared__ int qfl[BKT];
+    __shared__ double ext[4][6];
+    __shared__ int s_img;
+    int *lfst = reinterpret_cast<int *>(&bd[0][0]);   // staging only
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = blockIdx.x;
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
-    const int K = A.K;
     const int nb = KA.gnb[g];
     if (nb < 0) {          // group too large to stage: every query retries
         if (t == 0) atomicAdd(&A.err[2], 1u);
@@ -889,14 +1055,17 @@ __global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
     }
     const double RL2 = KA.grl2[g];
     if (t < nb) {
-        lfst[t] = A.bfirst[KA.gfirst[(size_t)g * KL_MB + t]];
+        const int b = KA.gfirst[(size_t)g * KL_MB + t];
+        lfst[t] = A.bfirst[b];
         lcnt[t] = KA.gcnt[(size_t)g * KL_MB + t];
+        for (int d = 0; d < 6; ++d) lbx[t][d] = A.fbox[6 * (size_t)b + d];
     }
     if (t >= 64 && t < 64 + nq) {
         const int j = t - 64;
         qx[j] = A.sx[s0 + j];
         qy[j] = A.sy[s0 + j];
         qz[j] = A.sz[s0 + j];
+        qg[j] = A.guess ? A.guess[s0 + j] : 0.0;
         qfl[j] = A.qflag[s0 + j] != 0;
     }
     __syncthreads();
@@ -919,6 +1088,7 @@ __global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
     for (int k = w; k < nb; k += 4)
         if (lane < lcnt[k]) csl[loff[k] + lane] = lfst[k] + lane;
     __syncthreads();
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int c0 = t; c0 < nc; c0 += 4 * 256) {
         int sl[4];
         double px[4], py[4], pz[4];
@@ -941,75 +1111,66 @@ __global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
                 cx[c] = px[u];
                 cy[c] = py[u];
                 cz[c] = pz[u];
+                lo[0] = fmin(lo[0], px[u]);
+                lo[1] = fmin(lo[1], py[u]);
+                lo[2] = fmin(lo[2], pz[u]);
+                hi[0] = fmax(hi[0], px[u]);
+                hi[1] = fmax(hi[1], py[u]);
+                hi[2] = fmax(hi[2], pz[u]);
             }
     }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+        }
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) {
+            ext[w][d] = lo[d];
+            ext[w][3 + d] = hi[d];
+        }
     __syncthreads();
+    if (t == 0) {
+        // img_sep(p, q) takes an image only when fl(p - q) leaves
+        // [-L/2, L/2]; rounding is monotone, so no pair of a query p in the
+        // bucket box and a staged q does when the extreme differences stay in
+        int img = 0;
+        if (A.per) {
+            const double *qb = A.bbox + 6 * (size_t)g;
+            for (int d = 0; d < 3; ++d) {
+                double cmin = ext[0][d], cmax = ext[0][3 + d];
+                for (int v = 1; v < 4; ++v) {
+                    cmin = fmin(cmin, ext[v][d]);
+                    cmax = fmax(cmax, ext[v][3 + d]);
+                }
+                const double h = 0.5 * A.L[d];
+                if (!(qb[3 + d] - cmin <= h) || !(qb[d] - cmax >= -h)) img = 1;
+            }
+        }
+        s_img = img;
+    }
+    __syncthreads();
+    const int img = s_img;
+    KLGroup G;
+    G.cx = cx;
+    G.cy = cy;
+    G.cz = cz;
+    G.csl = csl;
+    G.lbx = lbx;
+    G.lcnt = lcnt;
+    G.loff = loff;
+    G.nb = nb;
+    G.nc = nc;
     for (int j = w; j < nq; j += 4) {
         if (!qfl[j]) continue;
         const int q = s0 + j;
-        const double x = qx[j], y = qy[j], z = qz[j];
-        double dd[KL_NV];
-#pragma unroll
-        for (int k = 0; k < KL_NV; ++k) {
-            const int c = lane + 64 * k;
-            double d2 = DBL_MAX;
-            if (c < nc) {
-                const double dx = img_sep(x, cx[c], A.per, A.L[0]);
-                const double dy = img_sep(y, cy[c], A.per, A.L[1]);
-                const double dz = img_sep(z, cz[c], A.per, A.L[2]);
-                d2 = sq_dist(dx, dy, dz);
-            }
-            dd[k] = d2;
-        }
-        const double gq = A.guess ? A.guess[q] : 0.0;
-        double T = gq > 0 ? fmin(gq * 1.15, RL2) : RL2;
-        bool ok = false;
-        int cnt = 0;
-        for (int it = 0; it < 16; ++it) {
-            cnt = 0;
-#pragma unroll
-            for (int k = 0; k < KL_NV; ++k) cnt += __popcll(__ballot(dd[k] <= T));
-            if (cnt >= K && cnt <= CAP) {
-                ok = true;
-                break;
-            }
-            if (cnt < K) {
-                if (T >= RL2) {
-                    if (lane == 0) atomicAdd(&A.err[3], 1u);
-                    break;
-                }
-                const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
-                T = fmin(RL2, T * fmin(fmax(f, 1.2), 16.0));
-            } else {
-                const double f = cbrt((1.25 * K / cnt) * (1.25 * K / cnt));
-                T *= fmin(fmax(f, 0.05), 0.95);
-            }
-        }
-        if (!ok) {
-            if (lane == 0) {
-                if (cnt >= K) atomicAdd(&A.err[4], 1u);
-                const unsigned p = atomicAdd(KA.nretry, 1u);
-                KA.retry[p] = q;
-            }
-            continue;
-        }
-        int base = 0;      // compact the ball
-#pragma unroll
-        for (int k = 0; k < KL_NV; ++k) {
-            const bool in = dd[k] <= T;
-            const unsigned long long m = __ballot(in);
-            if (in) {
-                const int p = base + __popcll(m & ((1ull << lane) - 1));
-                bd[w][p] = dd[k];
-                bs[w][p] = csl[lane + 64 * k];
-            }
-            base += __popcll(m);
-        }
-        wave_sync();
-        const bool sel = cnt <= 128 ? knn_select<2>(A, q, cnt, bd[w], bs[w], lane)
-                                    : knn_select<4>(A, q, cnt, bd[w], bs[w], lane);
-        if (!sel && lane == 0) {
-            atomicAdd(&A.err[4], 1u);
+        const double gq = qg[j];
+        // first ball: 1.15 fBall2 from the hint, else 0.8 RL^2 (~1.6K expected)
+        const double Tsel = gq > 0 ? fmin(gq * 1.15, RL2) : 0.8 * RL2;
+        const bool done =
+            img ? knn_query<true>(A, G, sel[w], q, qx[j], qy[j], qz[j], Tsel, RL2, bd[w], bs[w], lane)
+                : knn_query<false>(A, G, sel[w], q, qx[j], qy[j], qz[j], Tsel, RL2, bd[w], bs[w], lane);
+        if (!done && lane == 0) {
             const unsigned p = atomicAdd(KA.nretry, 1u);
             KA.retry[p] = q;
         }
@@ -1748,6 +1909,8 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.bbox = S.bbox.as<double>();
     Q.bfirst = S.bfirst.as<int>();
     Q.bcnt = S.bcnt.as<int>();
+    Q.bden = S.bden.as<double>();
+    Q.fbox = S.sparse.as<float>();
     Q.order = S.order.as<int>();
     Q.qflag = S.qf.as<uint32_t>();
     Q.guess = nullptr;
@@ -1849,9 +2012,10 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         GG_HIP(hipMemsetAsync(S.bflag.p, 0, sizeof(int) * n, st));
         hipLaunchKernelGGL(k_lbvh, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
                            S.key.as<uint64_t>(), S.child.as<int>(), S.rng.as<int>());
+        GG_TRY(S.bpar.reserve(sizeof(int2) * n));
         hipLaunchKernelGGL(k_bucket_mark, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
-                           S.child.as<int>(), S.rng.as<int>(), S.bflag.as<int>(),
-                           S.bcode.as<int>());
+                           S.key.as<uint64_t>(), S.child.as<int>(), S.rng.as<int>(),
+                           S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>());
         int64_t tb = 0;
         GG_TRY(exclusive_scan_i64(st, S.bflag.as<int>(), S.pos.as<int64_t>(), n, S.sortTmp, &tb));
         nb = (int)tb;
@@ -1862,11 +2026,12 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         GG_TRY(S.bbox.reserve(sizeof(double) * 6 * nb));
         GG_TRY(S.sparse.reserve(sizeof(float) * 6 * (size_t)nb * nlev));
         float *sp = S.sparse.as<float>();
+        GG_TRY(S.bden.reserve(sizeof(double) * nb));
         hipLaunchKernelGGL(k_bucket_write, dim3(nblk(n, 256)), dim3(256), 0, st, n,
-                           S.bflag.as<int>(), S.bcode.as<int>(), S.pos.as<int64_t>(),
-                           S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),
-                           S.bfirst.as<int>(), S.bcnt.as<int>(), S.bbox.as<double>(), sp,
-                           S.bof.as<int>());
+                           S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>(), sc,
+                           S.pos.as<int64_t>(), S.sx.as<double>(), S.sy.as<double>(),
+                           S.sz.as<double>(), S.bfirst.as<int>(), S.bcnt.as<int>(),
+                           S.bbox.as<double>(), sp, S.bden.as<double>(), S.bof.as<int>());
         for (int lv = 1; lv < nlev; ++lv)
             hipLaunchKernelGGL(k_sparse_level, dim3(nblk(nb, 256)), dim3(256), 0, st, nb,
                                1 << (lv - 1), sp + 6 * (size_t)(lv - 1) * nb,
@@ -1879,12 +2044,15 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         GG_TRY(S.bfirst.reserve(sizeof(int)));
         GG_TRY(S.bcnt.reserve(sizeof(int)));
         GG_TRY(S.bbox.reserve(sizeof(double) * 6));
+        GG_TRY(S.sparse.reserve(sizeof(float) * 6));
+        GG_TRY(S.bden.reserve(sizeof(double)));
         GG_HIP(hipMemsetAsync(S.bof.p, 0, sizeof(int) * n, st));
         const int one[2] = {0, n};
         GG_HIP(hipMemcpyAsync(S.bfirst.p, &one[0], sizeof(int), hipMemcpyHostToDevice, st));
         GG_HIP(hipMemcpyAsync(S.bcnt.p, &one[1], sizeof(int), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_bbox_slots, dim3(1), dim3(64), 0, st, n, S.sx.as<double>(),
-                           S.sy.as<double>(), S.sz.as<double>(), S.bbox.as<double>());
+        hipLaunchKernelGGL(k_bbox_slots, dim3(1), dim3(64), 0, st, n, sc, S.sx.as<double>(),
+                           S.sy.as<double>(), S.sz.as<double>(), S.bbox.as<double>(),
+                           S.sparse.as<float>(), S.bden.as<double>());
     }
     GG_HIP(hipGetLastError());
     S.n = n;
