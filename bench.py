@@ -30,8 +30,11 @@ sys.path.insert(0, ROOT)
 METRIC = "grav interactions/s + SPH pair-evals/s at rms force err<=1e-4, vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-DENS_BYTES_PER_PAIR = 36     # j-side FP32 record: r 12, m 4, vPred 12, uPred 4, flags 4
-HYDRO_BYTES_PER_PAIR = 52    # + rho, P/rho^2, c, Balsara, dt
+# Algorithmic bytes per unit of work (the data the operator must touch once):
+KNN_BYTES_PER_QUERY = 24 + 8     # + 4 K: position read, fBall2 and K list entries written
+DENS_BYTES_PER_PAIR = 64         # j side, slot-ordered FP64 arrays: r 24, m 8, vPred 24, uPred 8
+HYDRO_BYTES_PER_PAIR = 64        # j side, packed FP32 force record (16 floats)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def sph_setup(gg, n, prec, seed=2):
@@ -139,7 +142,19 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--ngrav", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run five device-to-device copies of MIB MiB (the known-size "
+                         "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "copies": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -155,13 +170,27 @@ def main():
     g32 = time_grav(gg, torch, args.ngrav, gg.PREC_FP32, args.steps, args.warmup)
     g64 = time_grav(gg, torch, args.ngrav, gg.PREC_FP64, max(2, args.steps // 2), 1)
 
-    # roofline of the dominant SPH kernel (HBM-bound gathers): algorithmic
-    # bytes of the j-side records over its HIP-event duration
-    kern = max(("hydro", HYDRO_BYTES_PER_PAIR), ("density", DENS_BYTES_PER_PAIR),
-               key=lambda kv: ph32[kv[0]])
-    kname, bpp = kern
-    kms = ph32[kname]
-    achieved = K * n * bpp / (kms * 1e-3) / 1e9
+    # roofline of each SPH kernel: algorithmic bytes per launch over its
+    # HIP-event duration on the handle's stream; the dominant one is reported
+    kern = {
+        "k_knn_group": (ph32["knn"], n * (KNN_BYTES_PER_QUERY + 4 * K)),
+        "k_dendvdx<float>": (ph32["density"], n * K * DENS_BYTES_PER_PAIR),
+        "k_hydro2<float>": (ph32["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
+    }
+    traffic = {}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f)
+
+    def roof(name):
+        ms, by = kern[name]
+        a = by / (ms * 1e-3) / 1e9
+        t = traffic.get(name)
+        return {"bound": "hbm", "kernel": name, "achieved": a, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": a / HBM_PEAK_GBS, "ms": ms, "algorithmic_bytes": by,
+                "traffic": (t["fetch_bytes"] + t["write_bytes"]) if t else None}
+
+    dom = max(kern, key=lambda k: kern[k][0])
     g_inter_s = g32[1] / (g32[0])
     line = {
         "metric": METRIC,
@@ -192,11 +221,8 @@ def main():
         "grav_interactions_per_s_fp64": g64[1] / g64[0],
         "grav_speedup_vs_fp64": g64[0] / g32[0],
         "grav_ms_walk": g32[3], "grav_ms_interact_fp32": g32[4], "grav_ms_interact_fp64": g64[4],
-        "roofline": {
-            "bound": "hbm", "kernel": "k_%s" % kname, "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-        },
+        "roofline": roof(dom),
+        "rooflines_other": [roof(k) for k in kern if k != dom],
         "roofline_gravity": {
             "bound": "valu", "kernel": "k_interact<float,2>",
             "achieved": g32[2] / (g32[4] * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,

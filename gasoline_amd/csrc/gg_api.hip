@@ -236,9 +236,10 @@ extern "C" int gg_finish(GGPKD pkd)
     GravDev &G = pkd->grav;
     DevBuf *gb[] = {&G.px, &G.py, &G.pz, &G.pm, &G.ph, &G.pact, &G.perm, &G.wnode, &G.cell,
                     &G.momOH, &G.bucket, &G.bbox, &G.cnt, &G.off, &G.ent, &G.acc, &G.stats,
-                    &G.scan};
+                    &G.scan, &G.werr};
     for (DevBuf *b : gb) b->release();
     sph_release(pkd);
+    pkd->mlp.release_all();
     if (pkd->tm.created)
         for (int k = 0; k < 16; ++k) (void)hipEventDestroy(pkd->tm.ev[k]);
     (void)hipStreamDestroy(pkd->stream);

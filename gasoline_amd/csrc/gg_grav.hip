@@ -14,7 +14,7 @@ namespace gg {
 struct WalkNode {        // 64 bytes: what the walk reads per visited cell
     double cx, cy, cz, fOpen2, fSoft;
     int pLower, pUpper, iLower, iNext;
-    int pad0, pad1;
+    int iUpper, pad1;       // second child (-1 for a bucket)
 };
 
 // Per node FP64 record read when a cell is on an interaction list.
@@ -96,6 +96,113 @@ __global__ void k_walk(int nb, const int *__restrict__ bucket,
         }
     }
     if (!fill) {
+        cnt[3 * (size_t)b] = nP;
+        cnt[3 * (size_t)b + 1] = nS;
+        cnt[3 * (size_t)b + 2] = nN;
+    }
+}
+
+// The same walk, one wavefront per bucket: a breadth-first descent that
+// takes the decisions of pkdLocalWalk (walk.c:58-178) for up to 64 cells at
+// a time.  Every cell gets the decision the threaded walk gives it (open:
+// the bucket's active box within fOpen2 of the c.o.m., or fewer than 4
+// particles; an opened bucket's particles go on the PP list unless it is
+// this bucket; an accepted cell goes on the softened or the Newtonian list),
+// so each list holds the reference's members; within a list they come in
+// breadth-first rather than preorder order.  Pass fill = 0 counts, fill = 1
+// writes; *err is set if a frontier outgrows WK_FC (the caller then uses
+// k_walk).
+constexpr int WK_FC = 1024;
+
+__global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict__ bucket,
+                                                  const double *__restrict__ bbox,
+                                                  const WalkNode *__restrict__ wn, int iRoot,
+                                                  int fill, int *__restrict__ cnt,
+                                                  const int64_t *__restrict__ off,
+                                                  int *__restrict__ ent, int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+    __shared__ int fr[4][2][WK_FC];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = blockIdx.x * 4 + w;
+    if (b >= nb) return;
+    const int ib = bucket[b];
+    double bb[6];
+    for (int k = 0; k < 6; ++k) bb[k] = bbox[8 * (size_t)b + k];
+    const double fSoftMax = bbox[8 * (size_t)b + 6];
+    int64_t oP = 0, oS = 0, oN = 0;
+    if (fill) {
+        oP = off[3 * (size_t)b];
+        oS = off[3 * (size_t)b + 1];
+        oN = off[3 * (size_t)b + 2];
+    }
+    const unsigned long long below = (1ull << lane) - 1;
+    int nP = 0, nS = 0, nN = 0, nf = 1, cur = 0;
+    if (lane == 0) fr[w][0][0] = iRoot;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    while (nf > 0) {
+        int nn = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int k = base + lane;
+            bool kids = false, pp = false, soft = false, newt = false;
+            int c = 0, cl = 0, cu = 0, pl = 0, np = 0;
+            if (k < nf) {
+                c = fr[w][cur][k];
+                const WalkNode n = wn[c];
+                bool open = box_within(bb, n.fOpen2, n.cx, n.cy, n.cz);
+                if (n.pUpper - n.pLower < 3) open = true;   // < 4 particles: open
+                if (open) {
+                    if (n.iLower != -1) {
+                        kids = true;
+                        cl = n.iLower;
+                        cu = n.iUpper;
+                    } else if (c != ib) {
+                        pp = true;
+                        pl = n.pLower;
+                        np = n.pUpper - n.pLower + 1;
+                    }
+                } else {
+                    double twoh2 = n.fSoft + fSoftMax;
+                    twoh2 *= twoh2;
+                    soft = (twoh2 < n.fOpen2) ? false : box_within(bb, twoh2, n.cx, n.cy, n.cz);
+                    newt = !soft;
+                }
+            }
+            const unsigned long long mk = __ballot(kids);
+            if (kids) {
+                const int p = nn + 2 * __popcll(mk & below);
+                if (p + 1 < WK_FC) {
+                    fr[w][cur ^ 1][p] = cl;
+                    fr[w][cur ^ 1][p + 1] = cu;
+                }
+            }
+            nn += 2 * __popcll(mk);
+            int incl = np;     // PP: exclusive prefix of the particle counts
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o);
+                if (lane >= o) incl += u;
+            }
+            if (fill && pp)
+                for (int j = 0; j < np; ++j) ent[oP + nP + incl - np + j] = pl + j;
+            nP += __shfl(incl, 63);
+            const unsigned long long ms = __ballot(soft), mn = __ballot(newt);
+            if (fill && soft) ent[oS + nS + __popcll(ms & below)] = c;
+            if (fill && newt) ent[oN + nN + __popcll(mn & below)] = c;
+            nS += __popcll(ms);
+            nN += __popcll(mn);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (nn > WK_FC) {
+            if (lane == 0) atomicOr(err, 1);
+            return;
+        }
+        nf = nn;
+        cur ^= 1;
+    }
+    if (!fill && lane == 0) {
         cnt[3 * (size_t)b] = nP;
         cnt[3 * (size_t)b + 1] = nS;
         cnt[3 * (size_t)b + 2] = nN;
@@ -583,7 +690,8 @@ int grav_upload(GGPKD pkd)
         w.pUpper = t.pUpper[c];
         w.iLower = t.iLower[c];
         w.iNext = t.iNext[c];
-        w.pad0 = w.pad1 = 0;
+        w.iUpper = t.iLower[c] >= 0 ? t.iNext[t.iLower[c]] : -1;
+        w.pad1 = 0;
         double *r = &cell[(size_t)CR_N * c];
         r[CR_M] = t.mass[c];
         for (int d = 0; d < 3; ++d) r[CR_X + d] = t.com[3 * c + d];
@@ -725,18 +833,37 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
     unsigned long long hs[5] = {0, 0, 0, 0, 0};
     if (nb > 0) {
         timer_mark(pkd, 0);
-        // walk: count, scan, fill
-        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+        // walk: count, scan, fill; one wavefront per bucket, or one thread
+        // per bucket if a breadth-first frontier outgrows its buffer
+        GG_TRY(G.werr.reserve(sizeof(int)));
+        GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
                            G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
+                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr,
+                           G.werr.as<int>());
         GG_HIP(hipGetLastError());
+        int werr = 0;
+        GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        if (werr)
+            hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                               G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr,
+                               (int *)nullptr);
         int64_t total = 0;
         GG_TRY(exclusive_scan_i64(st, G.cnt.as<int>(), G.off.as<int64_t>(), 3 * nb, G.scan, &total));
         G.nEnt = total;
         GG_TRY(G.ent.reserve(sizeof(int) * std::max<int64_t>(total, 1)));
-        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
-                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
+        if (werr)
+            hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                               G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
+        else
+            hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
+                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                               G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>(),
+                               G.werr.as<int>());
+        G.serialWalk = werr;
         GG_HIP(hipGetLastError());
         GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
         GG_HIP(hipMemsetAsync(G.stats.p, 0, sizeof(unsigned long long) * 5, st));

@@ -63,6 +63,8 @@ struct GravDev {
     DevBuf acc;            // per slot: ax ay az pot dtg (FP64)
     DevBuf stats;          // reduction scratch
     DevBuf scan;           // scan scratch
+    DevBuf werr;           // walk frontier overflow flag
+    int serialWalk = 0;    // the last walk fell back to one thread per bucket
     HostGravTree host;
 };
 
@@ -71,7 +73,9 @@ struct SphDev {
     int built = 0, n = 0, bPeriodic = 0, nb = 0, nBucket = 0;
     double L[3] = {0, 0, 0};
     int listK = 0, listValid = 0, listQueries = 0;
-    unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0};
+    unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0}, passes = 0;
+    int64_t cand = 0;                          // kNN candidates examined
+    double seedSlack = 1.15;                   // first trial ball / fBall2 hint
     int64_t nExtra = 0;
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
     DevBuf sx, sy, sz;                         // slot order positions
@@ -81,6 +85,8 @@ struct SphDev {
     DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
     DevBuf gcnt, goff, gent;                   // ball-gather CSR
     DevBuf rcnt, roff, rcur, ridx;             // extras lists
+    DevBuf ecnt, eidx;                         // extras found by the density pass
+    int exValid = 0;
     DevBuf retry, nretry, hrec;                // kNN retries, force records
     DevBuf qf, act, nba, err, guess, diffcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
@@ -90,10 +96,28 @@ struct SphDev {
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
-                         &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &qf, &act, &nba,
+                         &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
+        for (DevBuf *b : all) b->release();
+    }
+};
+
+// h-proposal MLP state on the device.
+constexpr int HMLP_H = 32;                                  // hidden width
+constexpr int HMLP_NW = HMLP_H * 4 + HMLP_H + HMLP_H * HMLP_H + HMLP_H + HMLP_H + 1;   // 1249
+constexpr int HMLP_NP = HMLP_NW + 8;                        // + input mean, scale
+static_assert(HMLP_NW == GG_HMLP_NW && HMLP_NP == GG_HMLP_NP, "header constants");
+struct HmlpDev {
+    int init = 0, normSet = 0, nHist = 0, step = 0;
+    DevBuf par, m, v, grad, gpart, lpart;   // parameters, Adam moments, gradients
+    DevBuf hx, hh, hv;                      // history: features, log h, valid (caller order)
+    DevBuf sel, pos, samp, scan, stat;      // sample list, scratch
+    void release_all()
+    {
+        DevBuf *all[] = {&par, &m, &v, &grad, &gpart, &lpart, &hx, &hh, &hv,
+                         &sel, &pos, &samp, &scan, &stat};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -114,6 +138,7 @@ struct gg_pkd_s {
     gg::DevBuf f[GG_NFIELDS];      // caller-order fields
     gg::GravDev grav;
     gg::SphDev sph;
+    gg::HmlpDev mlp;
     gg::Timers tm;
 };
 

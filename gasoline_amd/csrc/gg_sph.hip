@@ -26,7 +26,8 @@ namespace gg {
 
 constexpr int FCAP = 1024;        // BFS frontier capacity (nodes)
          // leaves per query capacity
-constexpr int CAP = 256;          // candidate buffer (4 per lane)
+constexpr int CAP = 256;
+constexpr int ECAP = 48;           // force-pass extras kept per particle by the density pass          // candidate buffer (4 per lane)
 constexpr int MAXTRY = 48;        // radius adjustments per query
 constexpr double GG_FLT_MAX = (double)3.402823466E+38F;   // floattype.h:9
 
@@ -384,6 +385,7 @@ struct QArgs {
     const int *bfirst, *bcnt;          // bucket first slot, count
     const double *bden;                // bucket number density estimate
     const float *fbox;                 // bucket boxes, FP32 rounded outwards
+    double slack;                      // first trial ball = slack * fBall2 hint
     const int *order;                  // slot -> caller (tie key)
     const uint32_t *qflag;             // slot order: nonzero = query
     const double *guess;               // slot order: fBall2 guess (<= 0: none)
@@ -578,7 +580,7 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
     if (A.mode != 0) {
         R2 = A.fb2[q];
     } else if (A.guess && A.guess[q] > 0) {
-        R2 = A.guess[q] * 1.2;
+        R2 = A.guess[q] * A.slack;
     } else {
         R2 = cold_r2(A.bden[A.bof[q]], K);
     }
@@ -692,17 +694,19 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
 // ---------------------------------------------- kNN by bucket groups
 // Group of a bucket: every bucket within a group radius RL of the bucket's
 // box.  k_group_lists finds the groups (one wavefront per bucket, a
-// breadth-first descent of the radix tree) and k_knn_group settles the
-// bucket's queries against its group.  The group's FP64 positions are
-// staged in LDS; each wavefront settles one query at a time: squared
-// distances, computed exactly as the reference computes them, stay in
-// registers and ballot counts choose a squared radius T <= RL^2 whose ball
-// holds between K and CAP particles; the ball is compacted and sorted by
-// (d2, slot), and sorted again by (d2, caller index) if equal distances
-// meet among its first K+1.  Every particle within RL of a query is staged,
-// so the first K are the exact K nearest.  Queries that cannot be settled
-// this way go to a retry list served by k_query.
-constexpr int KL_CM = 2048;            // staged candidates per group
+// breadth-first descent of the radix tree) and k_knn_bucket settles the
+// bucket's queries against its group, one wavefront per bucket holding the
+// group's bucket boxes and ranges in LDS.  For each query the wavefront
+// takes the group's buckets within a trial radius of it and reads their
+// positions from the slot-ordered arrays (each bucket is 32 consecutive
+// slots at most, one load per half-wavefront); squared distances, computed
+// exactly as the reference computes them, stay in registers and ballot
+// counts choose a squared radius T whose ball holds between K and CAP
+// particles.  The ball is compacted and the K nearest selected from it by
+// (d2, caller index).  Every particle within RL of a query is in the group,
+// so the selection is exact.  Queries that cannot be settled this way go to
+// a retry list served by k_query.
+constexpr int KL_CM = 4096;            // particles per group
 constexpr int KL_MB = 192;             // buckets per group
 constexpr int KL_SB = 32;              // buckets of a query's own ball
 constexpr int KL_NS = KL_SB / 2;       // candidates per lane
@@ -758,7 +762,7 @@ __global__ __launch_bounds__(256) void k_group_lists(KLArgs KA)
     for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
     const unsigned long long nh = __ballot(none);
     double R2;
-    if (nh == 0) R2 = gmax * 1.44;
+    if (nh == 0) R2 = gmax * A.slack * 1.25;   // 1.44 fBall2 at the default slack
     else R2 = cold_r2(A.bden[g], K);
     const double bx[6] = {lb[0], lb[1], lb[2], lb[3], lb[4], lb[5]};
     const double RL2b = R2 * (1.0 + 1e-12);
@@ -880,7 +884,7 @@ __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, con
     return true;
 }
 
-// Squared distance of a staged candidate from a query, as the reference
+// Squared distance of a candidate from a query, as the reference
 // computes it; IMG = false when no periodic image can apply to the pair
 // (then img_sep is a plain difference).
 template <bool IMG>
@@ -892,43 +896,43 @@ __device__ __forceinline__ double pair_d2(double x, double y, double z, double p
     return sq_dist(x - px, y - py, z - pz);
 }
 
-// LDS state of a kNN group shared by its wavefronts.
+// LDS state of a kNN group (one wavefront).
 struct KLGroup {
-    const double *cx, *cy, *cz;        // staged positions
-    const int *csl;                    // staged slot
     const float (*lbx)[6];             // bucket boxes (FP32, outwards)
-    const int *lcnt, *loff;            // bucket count, staging offset
-    int nb, nc;
+    const int *lfst, *lcnt;            // bucket first slot, count
+    int nb;
 };
 
-// Settles one query over the candidates of the nsel <= KL_SB buckets listed
-// in sel (two buckets per 64 lanes): finds T <= Tsel with
-// K <= |{d2 <= T}| <= CAP and compacts that ball into (bd, bs).  Returns 0
-// when settled (*cnt = ball size), 1 when fewer than K lie within Tsel
-// (*cnt = their number), 2 when no such T was found.
+// Settles one query over the nsel <= KL_SB buckets listed in sel (two
+// buckets per 64 lanes): finds T <= Tsel with K <= |{d2 <= T}| <= CAP and
+// compacts that ball into (bd, bs).  Returns 0 when settled (*cnt = ball
+// size), 1 when fewer than K lie within Tsel (*cnt = their number), 2 when
+// no such T was found.
 template <bool IMG>
 __device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char *sel, int nsel,
                           double x, double y, double z, double Tsel, double *bd, int *bs,
-                          int lane, int *cnt_out)
+                          int lane, int *cnt_out, unsigned &ncand)
 {
     const int K = A.K;
     const int ns = (nsel + 1) >> 1;    // slots in use (wave-uniform)
-    double dd[KL_NS];
     int ci[KL_NS];
 #pragma unroll
     for (int k = 0; k < KL_NS; ++k) {
         ci[k] = -1;
+        const int e = 2 * k + (lane >> 5), o = lane & 31;
+        if (k < ns && e < nsel) {
+            const int b = sel[e];
+            if (o < G.lcnt[b]) ci[k] = G.lfst[b] + o;
+        }
+    }
+    double dd[KL_NS];
+#pragma unroll
+    for (int k = 0; k < KL_NS; ++k) {
         dd[k] = DBL_MAX;
-        if (k < ns) {
-            const int e = 2 * k + (lane >> 5), o = lane & 31;
-            if (e < nsel) {
-                const int b = sel[e];
-                if (o < G.lcnt[b]) {
-                    const int c = G.loff[b] + o;
-                    ci[k] = c;
-                    dd[k] = pair_d2<IMG>(x, y, z, G.cx[c], G.cy[c], G.cz[c], A.L);
-                }
-            }
+        if (k < ns && ci[k] >= 0) {
+            const int c = ci[k];
+            dd[k] = pair_d2<IMG>(x, y, z, A.sx[c], A.sy[c], A.sz[c], A.L);
+            ++ncand;                   // per lane; summed over the wavefront at the end
         }
     }
     double T = Tsel;
@@ -959,31 +963,33 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char 
     int base = 0;
 #pragma unroll
     for (int k = 0; k < KL_NS; ++k) {
-        if (k >= ns) break;
-        const bool in = dd[k] <= T;
-        const unsigned long long m = __ballot(in);
-        if (in) {
-            const int p = base + __popcll(m & ((1ull << lane) - 1));
-            bd[p] = dd[k];
-            bs[p] = G.csl[ci[k]];
+        if (k < ns) {
+            const bool in = dd[k] <= T;
+            const unsigned long long m = __ballot(in);
+            if (in) {
+                const int p = base + __popcll(m & ((1ull << lane) - 1));
+                bd[p] = dd[k];
+                bs[p] = ci[k];
+            }
+            base += __popcll(m);
         }
-        base += __popcll(m);
     }
     *cnt_out = cnt;
     return 0;
 }
 
-// One query: choose the buckets within Tsel of it, settle, and widen Tsel
-// (up to the group radius) while fewer than K lie within it, or narrow it
-// while the ball meets more than KL_SB buckets.  Returns false if the query
-// must be retried by k_query.
+// One query: choose the group's buckets within Tsel of it, settle, and
+// widen Tsel (up to the group radius) while fewer than K lie within it, or
+// narrow it while the ball meets more than KL_SB buckets.  Returns false if
+// the query must be retried by k_query.
 template <bool IMG>
 __device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, int q, double x,
                           double y, double z, double Tsel, double RL2, double *bd, int *bs,
-                          int lane)
+                          int lane, unsigned &npass, unsigned &ncand)
 {
     const int K = A.K;
     for (int pass = 0; pass < 8; ++pass) {
+        npass += pass > 0;
         const double Tb = Tsel * (1.0 + 1e-12);
         int nsel = 0;
         for (int kb = 0; kb < G.nb; kb += 64) {
@@ -992,7 +998,8 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, 
             if (k < G.nb) {
                 const double b[6] = {G.lbx[k][0], G.lbx[k][1], G.lbx[k][2],
                                      G.lbx[k][3], G.lbx[k][4], G.lbx[k][5]};
-                in = box_d2(b, x, y, z, A.per, A.L) <= Tb;
+                // without images for the group the plain box distance is exact
+                in = box_d2(b, x, y, z, IMG ? A.per : 0, A.L) <= Tb;
             }
             const unsigned long long m = __ballot(in);
             const int p = nsel + __popcll(m & ((1ull << lane) - 1));
@@ -1005,7 +1012,7 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, 
             continue;
         }
         int cnt = 0;
-        const int st = knn_settle<IMG>(A, G, sel, nsel, x, y, z, Tsel, bd, bs, lane, &cnt);
+        const int st = knn_settle<IMG>(A, G, sel, nsel, x, y, z, Tsel, bd, bs, lane, &cnt, ncand);
         if (st == 1 && Tsel < RL2) {   // widen the ball towards the group radius
             const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
             Tsel = fmin(RL2, Tsel * fmin(fmax(f, 1.2), 16.0));
@@ -1026,155 +1033,103 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, 
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_knn_group(KLArgs KA)
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One wavefront per bucket.
+__global__ __launch_bounds__(64) void k_knn_bucket(KLArgs KA)
 {
     const QArgs &A = KA.Q;
-    __shared__ double cx[KL_CM], cy[KL_CM], cz[KL_CM];
-    __shared__ int csl[KL_CM];
-    __shared__ double bd[4][CAP];
-    __shared__ int bs[4][CAP];
     __shared__ float lbx[KL_MB][6];
-    __shared__ int lcnt[KL_MB], loff[KL_MB];
-    __shared__ unsigned char sel[4][KL_SB];
-    __shared__ double qx[BKT], qy[BKT], qz[BKT], qg[BKT];
-    __shared__ int qfl[BKT];
-    __shared__ double ext[4][6];
-    __shared__ int s_img;
-    int *lfst = reinterpret_cast<int *>(&bd[0][0]);   // staging only
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ int lfst[KL_MB], lcnt[KL_MB];
+    __shared__ unsigned char sel[KL_SB];
+    __shared__ double bd[CAP];
+    __shared__ int bs[CAP];
+    const int lane = threadIdx.x;
     const int g = blockIdx.x;
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
     const int nb = KA.gnb[g];
-    if (nb < 0) {          // group too large to stage: every query retries
-        if (t == 0) atomicAdd(&A.err[2], 1u);
-        if (t < nq && A.qflag[s0 + t]) {
+    if (nb < 0) {          // group too large: every query retries
+        if (lane == 0) atomicAdd(&A.err[2], 1u);
+        if (lane < nq && A.qflag[s0 + lane]) {
             const unsigned p = atomicAdd(KA.nretry, 1u);
-            KA.retry[p] = s0 + t;
+            KA.retry[p] = s0 + lane;
         }
         return;
     }
     const double RL2 = KA.grl2[g];
-    if (t < nb) {
-        const int b = KA.gfirst[(size_t)g * KL_MB + t];
-        lfst[t] = A.bfirst[b];
-        lcnt[t] = KA.gcnt[(size_t)g * KL_MB + t];
-        for (int d = 0; d < 6; ++d) lbx[t][d] = A.fbox[6 * (size_t)b + d];
+    // the bucket's queries, one per lane, broadcast as each is served
+    double qx = 0, qy = 0, qz = 0, qg = 0;
+    int qf = 0;
+    if (lane < nq) {
+        qx = A.sx[s0 + lane];
+        qy = A.sy[s0 + lane];
+        qz = A.sz[s0 + lane];
+        qg = A.guess ? A.guess[s0 + lane] : 0.0;
+        qf = A.qflag[s0 + lane] != 0;
     }
-    if (t >= 64 && t < 64 + nq) {
-        const int j = t - 64;
-        qx[j] = A.sx[s0 + j];
-        qy[j] = A.sy[s0 + j];
-        qz[j] = A.sz[s0 + j];
-        qg[j] = A.guess ? A.guess[s0 + j] : 0.0;
-        qfl[j] = A.qflag[s0 + j] != 0;
-    }
-    __syncthreads();
-    if (w == 0) {          // staging offsets: exclusive prefix of the counts
-        int run = 0;
-        for (int base = 0; base < nb; base += 64) {
-            const int k = base + lane;
-            const int v = k < nb ? lcnt[k] : 0;
-            int incl = v;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int u = __shfl_up(incl, o);
-                if (lane >= o) incl += u;
-            }
-            if (k < nb) loff[k] = run + incl - v;
-            run += __shfl(incl, 63);
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int k = lane; k < nb; k += 64) {
+        const int b = KA.gfirst[(size_t)g * KL_MB + k];
+        lfst[k] = A.bfirst[b];
+        lcnt[k] = KA.gcnt[(size_t)g * KL_MB + k];
+        for (int d = 0; d < 3; ++d) {
+            const float l = A.fbox[6 * (size_t)b + d], h = A.fbox[6 * (size_t)b + 3 + d];
+            lbx[k][d] = l;
+            lbx[k][3 + d] = h;
+            lo[d] = fminf(lo[d], l);
+            hi[d] = fmaxf(hi[d], h);
         }
-    }
-    __syncthreads();
-    const int nc = loff[nb - 1] + lcnt[nb - 1];
-    for (int k = w; k < nb; k += 4)
-        if (lane < lcnt[k]) csl[loff[k] + lane] = lfst[k] + lane;
-    __syncthreads();
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int c0 = t; c0 < nc; c0 += 4 * 256) {
-        int sl[4];
-        double px[4], py[4], pz[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int c = c0 + 256 * u;
-            sl[u] = c < nc ? csl[c] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (sl[u] >= 0) {
-                px[u] = A.sx[sl[u]];
-                py[u] = A.sy[sl[u]];
-                pz[u] = A.sz[sl[u]];
-            }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (sl[u] >= 0) {
-                const int c = c0 + 256 * u;
-                cx[c] = px[u];
-                cy[c] = py[u];
-                cz[c] = pz[u];
-                lo[0] = fmin(lo[0], px[u]);
-                lo[1] = fmin(lo[1], py[u]);
-                lo[2] = fmin(lo[2], pz[u]);
-                hi[0] = fmax(hi[0], px[u]);
-                hi[1] = fmax(hi[1], py[u]);
-                hi[2] = fmax(hi[2], pz[u]);
-            }
     }
     for (int o = 32; o > 0; o >>= 1)
         for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+            lo[d] = fminf(lo[d], __shfl_xor(lo[d], o));
+            hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o));
         }
-    if (lane == 0)
+    // img_sep(p, q) takes an image only when fl(p - q) leaves [-L/2, L/2];
+    // rounding is monotone, so no pair of a query p in the bucket box and a
+    // group particle q (inside the outward boxes) does when the extreme
+    // differences stay inside
+    bool img = false;
+    if (A.per) {
+        const double *qb = A.bbox + 6 * (size_t)g;
         for (int d = 0; d < 3; ++d) {
-            ext[w][d] = lo[d];
-            ext[w][3 + d] = hi[d];
+            const double h = 0.5 * A.L[d];
+            if (!(qb[3 + d] - (double)lo[d] <= h) || !(qb[d] - (double)hi[d] >= -h)) img = true;
         }
-    __syncthreads();
-    if (t == 0) {
-        // img_sep(p, q) takes an image only when fl(p - q) leaves
-        // [-L/2, L/2]; rounding is monotone, so no pair of a query p in the
-        // bucket box and a staged q does when the extreme differences stay in
-        int img = 0;
-        if (A.per) {
-            const double *qb = A.bbox + 6 * (size_t)g;
-            for (int d = 0; d < 3; ++d) {
-                double cmin = ext[0][d], cmax = ext[0][3 + d];
-                for (int v = 1; v < 4; ++v) {
-                    cmin = fmin(cmin, ext[v][d]);
-                    cmax = fmax(cmax, ext[v][3 + d]);
-                }
-                const double h = 0.5 * A.L[d];
-                if (!(qb[3 + d] - cmin <= h) || !(qb[d] - cmax >= -h)) img = 1;
-            }
-        }
-        s_img = img;
     }
-    __syncthreads();
-    const int img = s_img;
+    wave_sync();
     KLGroup G;
-    G.cx = cx;
-    G.cy = cy;
-    G.cz = cz;
-    G.csl = csl;
     G.lbx = lbx;
+    G.lfst = lfst;
     G.lcnt = lcnt;
-    G.loff = loff;
     G.nb = nb;
-    G.nc = nc;
-    for (int j = w; j < nq; j += 4) {
-        if (!qfl[j]) continue;
+    unsigned npass = 0, ncand = 0;     // search work: passes (uniform), distances (per lane)
+    for (int j = 0; j < nq; ++j) {
+        if (!__builtin_amdgcn_readlane(qf, j)) continue;
         const int q = s0 + j;
-        const double gq = qg[j];
-        // first ball: 1.15 fBall2 from the hint, else 0.8 RL^2 (~1.6K expected)
-        const double Tsel = gq > 0 ? fmin(gq * 1.15, RL2) : 0.8 * RL2;
+        const double x = readlane_d(qx, j), y = readlane_d(qy, j), z = readlane_d(qz, j),
+                     gq = readlane_d(qg, j);
+        // first ball: slack * fBall2 from the hint, else 0.8 RL^2 (~1.6K expected)
+        const double Tsel = gq > 0 ? fmin(gq * A.slack, RL2) : 0.8 * RL2;
         const bool done =
-            img ? knn_query<true>(A, G, sel[w], q, qx[j], qy[j], qz[j], Tsel, RL2, bd[w], bs[w], lane)
-                : knn_query<false>(A, G, sel[w], q, qx[j], qy[j], qz[j], Tsel, RL2, bd[w], bs[w], lane);
+            img ? knn_query<true>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand)
+                : knn_query<false>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand);
         if (!done && lane == 0) {
             const unsigned p = atomicAdd(KA.nretry, 1u);
             KA.retry[p] = q;
         }
         wave_sync();
+    }
+    for (int o = 32; o > 0; o >>= 1) ncand += __shfl_xor(ncand, o);
+    if (lane == 0) {
+        if (npass) atomicAdd(&A.err[6], npass);
+        atomicAdd(reinterpret_cast<unsigned long long *>(A.err + 8), (unsigned long long)ncand);
     }
 }
 
@@ -1214,7 +1169,10 @@ struct DArgs {
     double H, a, gamma;
     int bCannonical, iViscosityLimiter, bConstantDiffusion;
     double *rho, *divv, *curl, *dvds, *divvt, *bal, *diff;   // slot order
-    uint32_t *nbract;                        // slot order flags (atomicOr)
+    uint32_t *nbract;                        // slot order: 32 = neighbour of an active particle
+    int *ecnt, *eidx;                        // force-pass extras, ecap per particle
+    int ecap;
+    unsigned *err;                           // err[7]: an extras list overflowed
 };
 
 // DenDVDX (smoothfcn.c:3390-3627): 16 lanes per particle over its list,
@@ -1245,8 +1203,25 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
         const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
         const T r2 = (T)(sq_dist(dxd, dyd, dzd) * ih2d);
         const T mq = (T)A.m[q];
-        if (pact) atomicOr(&A.nbract[q], 32u);
+        if (pact) A.nbract[q] = 32u;   // every writer stores the same flag
         qact |= A.act[q];
+        {   // extras of the force pass: q's own list misses i when i lies
+            // outside q's ball, seen from q (or q has no list)
+            bool extra = !A.qflag[q];
+            if (!extra) {
+                double d2q = sq_dist(dxd, dyd, dzd);
+                if (A.per && (fabs(x - A.sx[q]) > 0.5 * A.L[0] || fabs(y - A.sy[q]) > 0.5 * A.L[1] ||
+                              fabs(z - A.sz[q]) > 0.5 * A.L[2]))
+                    d2q = sq_dist(img_sep(A.sx[q], x, 1, A.L[0]), img_sep(A.sy[q], y, 1, A.L[1]),
+                                  img_sep(A.sz[q], z, 1, A.L[2]));
+                extra = d2q > A.fb2[q];
+            }
+            if (extra) {
+                const int k = atomicAdd(&A.ecnt[q], 1);
+                if (k < A.ecap) A.eidx[(size_t)q * A.ecap + k] = i;
+                else atomicOr(&A.err[7], 1u);
+            }
+        }
         rho += (double)(m4_w<T>(r2) * mq);
         if (A.op == GG_SMX_DENDVDX) {
             const T rs1 = m4_dw<T>(r2) * mq;
@@ -1275,7 +1250,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     if (sub != 0) return;
     const double ih = sqrt(ih2d);
     const double fNorm = M_1_PI * ih2d * ih;
-    if (qact & 1u) atomicOr(&A.nbract[i], 32u);
+    if (qact & 1u) A.nbract[i] = 32u;
     A.rho[i] = rho * fNorm;
     if (A.op != GG_SMX_DENDVDX) return;
     const double trace = xx + yy + zz;
@@ -1599,8 +1574,10 @@ struct H2Args {
     int n, per;
     double L[3];
     const double *sx, *sy, *sz, *fb2;
-    const int64_t *off, *eoff;
-    const int *idx, *eidx;
+    const int64_t *off, *eoff;         // eoff: CSR extras, or nullptr for
+    const int *idx, *eidx;             // ecap-strided lists of ecnt entries
+    const int *ecnt;
+    int ecap;
     const uint32_t *qflag;
     double a, H, alpha, beta, dtFacCourant, dtFacDiffusion, kDiff;
     int bCannonical;
@@ -1648,7 +1625,9 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
             }
         }
     }
-    for (int64_t e = A.eoff[i] + sub; e < A.eoff[i + 1]; e += 16) {
+    const int64_t e0 = A.eoff ? A.eoff[i] : (int64_t)i * A.ecap;
+    const int64_t e1 = A.eoff ? A.eoff[i + 1] : e0 + A.ecnt[i];
+    for (int64_t e = e0 + sub; e < e1; e += 16) {
         const int p = A.eidx[e];
         const HRec<T> P = rec[p];
         const double ex = img_sep(A.sx[p], xi, A.per, A.L[0]);
@@ -1747,6 +1726,24 @@ __global__ void k_rev_fill(int n, const uint32_t *__restrict__ qflag,
 }
 
 // Sort each reverse list so the summation order is deterministic.
+__global__ void k_rev_sort_fixed(int n, int ecap, const int *__restrict__ ecnt,
+                                 int *__restrict__ eidx)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int *a = eidx + (size_t)q * ecap;
+    const int m = ecnt[q];
+    for (int i = 1; i < m; ++i) {
+        const int v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) {
+            a[j + 1] = a[j];
+            --j;
+        }
+        a[j + 1] = v;
+    }
+}
+
 __global__ void k_rev_sort(int n, const int64_t *__restrict__ roff, int *__restrict__ ridx)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1911,6 +1908,7 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.bcnt = S.bcnt.as<int>();
     Q.bden = S.bden.as<double>();
     Q.fbox = S.sparse.as<float>();
+    Q.slack = S.seedSlack;
     Q.order = S.order.as<int>();
     Q.qflag = S.qf.as<uint32_t>();
     Q.guess = nullptr;
@@ -2086,8 +2084,8 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
     GG_TRY(S.nbr.reserve(sizeof(int) * (size_t)K * n));
     GG_TRY(S.fb2s.reserve(sizeof(double) * n));
-    GG_TRY(S.err.reserve(sizeof(unsigned) * 8));
-    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 8, st));
+    GG_TRY(S.err.reserve(sizeof(unsigned) * 12));
+    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 12, st));
     GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
     GG_TRY(S.retry.reserve(sizeof(int) * n));
     GG_TRY(S.nretry.reserve(sizeof(unsigned)));
@@ -2111,7 +2109,9 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     KL.nretry = S.nretry.as<unsigned>();
     timer_mark(pkd, 0);
     hipLaunchKernelGGL(k_group_lists, dim3((S.nb + 3) / 4), dim3(256), 0, st, KL);
-    hipLaunchKernelGGL(k_knn_group, dim3(S.nb), dim3(256), 0, st, KL);
+    timer_mark(pkd, 1);
+    hipLaunchKernelGGL(k_knn_bucket, dim3(S.nb), dim3(64), 0, st, KL);
+    timer_mark(pkd, 2);
     GG_HIP(hipGetLastError());
     unsigned nretry = 0;
     GG_HIP(hipMemcpyAsync(&nretry, S.nretry.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -2125,7 +2125,7 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
         hipLaunchKernelGGL(k_query, dim3(nretry), dim3(64), 0, st, Q);
         GG_HIP(hipGetLastError());
     }
-    timer_mark(pkd, 1);
+    timer_mark(pkd, 3);
     GG_TRY(S.off.reserve(sizeof(int64_t) * (n + 1)));
     hipLaunchKernelGGL(k_fixed_off, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, K,
                        S.off.as<int64_t>());
@@ -2173,20 +2173,32 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     D.bal = S.o_bal.as<double>();
     D.diff = S.o_diff.as<double>();
     D.nbract = S.nba.as<uint32_t>();
+    GG_TRY(S.ecnt.reserve(sizeof(int) * (size_t)n));
+    GG_TRY(S.eidx.reserve(sizeof(int) * (size_t)n * ECAP));
+    GG_HIP(hipMemsetAsync(S.ecnt.p, 0, sizeof(int) * (size_t)n, st));
+    D.ecnt = S.ecnt.as<int>();
+    D.eidx = S.eidx.as<int>();
+    D.ecap = ECAP;
+    D.err = S.err.as<unsigned>();
     const int grid = nblk(n * 16, 256);
-    timer_mark(pkd, 2);
+    timer_mark(pkd, 4);
     if (pkd->prec == GG_PREC_FP32)
         hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
     else
         hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 3);
-    unsigned e[8];
-    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 8, hipMemcpyDeviceToHost, st));
+    timer_mark(pkd, 5);
+    unsigned e[12];
+    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
     S.fails = e[0];
     S.ties = e[1];
     for (int k = 0; k < 4; ++k) S.diag[k] = e[2 + k];
+    S.passes = e[6];
+    S.exValid = (e[7] == 0);   // the extras lists are whole
+    uint64_t cand;
+    std::memcpy(&cand, &e[8], sizeof(cand));
+    S.cand = (int64_t)cand;
     GG_CHECK(e[0] == 0, GG_EUNSUP,
              "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
     const uint32_t *qf = S.qf.as<uint32_t>();
@@ -2215,10 +2227,10 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
                               S.sortTmp, &nq));
     S.listQueries = (int)nq;
     if (pnSmoothed) *pnSmoothed = (int)nq;
-    timer_mark(pkd, 4);
+    timer_mark(pkd, 6);
     GG_HIP(hipStreamSynchronize(st));
-    const int ph[4] = {3, 8, 4, 8};
-    timer_collect(pkd, 0, 4, ph);
+    const int ph[6] = {9, 3, 10, 8, 4, 8};
+    timer_collect(pkd, 0, 6, ph);
     return GG_OK;
 }
 
@@ -2304,6 +2316,15 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         idx = S.gent.as<int>();
     }
     timer_mark(pkd, 1);
+    const bool fixedExtras = reuse && S.exValid;
+    if (fixedExtras) {   // the density pass found them: sort each list
+        hipLaunchKernelGGL(k_rev_sort_fixed, dim3(nblk(n, 256)), dim3(256), 0, st, n, ECAP,
+                           S.ecnt.as<int>(), S.eidx.as<int>());
+        int64_t etot = 0;
+        GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
+        GG_TRY(exclusive_scan_i64(st, S.ecnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &etot));
+        S.nExtra = etot;
+    } else {
     // extras: particles whose balls hold i but which are not in i's list
     GG_TRY(S.rcnt.reserve(sizeof(int) * (n + 1)));
     GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
@@ -2323,6 +2344,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
                        S.rcnt.as<int>(), S.roff.as<int64_t>(), S.ridx.as<int>());
     hipLaunchKernelGGL(k_rev_sort, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.roff.as<int64_t>(),
                        S.ridx.as<int>());
+    }
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 2);
     // per-particle records in slot order
@@ -2352,9 +2374,11 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.sz = S.sz.as<double>();
     H.fb2 = S.fb2s.as<double>();
     H.off = off;
-    H.eoff = S.roff.as<int64_t>();
+    H.eoff = fixedExtras ? nullptr : S.roff.as<int64_t>();
+    H.ecnt = fixedExtras ? S.ecnt.as<int>() : nullptr;
+    H.ecap = ECAP;
     H.idx = idx;
-    H.eidx = S.ridx.as<int>();
+    H.eidx = fixedExtras ? S.eidx.as<int>() : S.ridx.as<int>();
     H.qflag = S.qf.as<uint32_t>();
     H.a = smf->a;
     H.H = smf->H;
@@ -2413,13 +2437,21 @@ extern "C" int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr)
     return GG_OK;
 }
 
+extern "C" int gg_set_seed_slack(GGPKD pkd, double slack)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GG_CHECK(slack >= 1.0 && slack <= 2.0, GG_EINVAL, "slack %g outside [1, 2]", slack);
+    pkd->sph.seedSlack = slack;
+    return GG_OK;
+}
+
 extern "C" int gg_smooth_stats(GGPKD pkd, int64_t *s, int n)
 {
     GG_CHECK(pkd && s, GG_EINVAL, "null argument");
     const SphDev &S = pkd->sph;
-    const int64_t v[8] = {(int64_t)S.retried, (int64_t)S.ties, S.nExtra, (int64_t)S.listQueries,
-                          (int64_t)S.diag[0], (int64_t)S.diag[1], (int64_t)S.diag[2],
-                          (int64_t)S.diag[3]};
-    for (int k = 0; k < n && k < 8; ++k) s[k] = v[k];
+    const int64_t v[10] = {(int64_t)S.retried, (int64_t)S.ties, S.nExtra, (int64_t)S.listQueries,
+                           (int64_t)S.diag[0], (int64_t)S.diag[1], (int64_t)S.diag[2],
+                           (int64_t)S.diag[3], (int64_t)S.passes, S.cand};
+    for (int k = 0; k < n && k < 10; ++k) s[k] = v[k];
     return GG_OK;
 }

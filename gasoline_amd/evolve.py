@@ -1,0 +1,73 @@
+"""Single-rung kick-drift-kick stepping over the device-resident store: the
+host sequence of msrTopStepKDK (master.c:8242) for one rung, calling the
+force path (gravity tree + SPH) and the device kick/drift through the C ABI.
+
+    step: open kick (v, u by dt/2; vPred, uPred predicted to the step's end)
+          -> drift (r by dt) -> forces at the new positions -> close kick.
+
+Forces: gg_init_accel, then gravity (gg_build_binary + gg_grav_all) if
+enabled, then SPH (gg_build_local, gg_smooth(DenDVDX), gg_gas_pressure,
+gg_resmooth(SphPressureTermsSym), which adds the hydro acceleration and the
+uDot terms).  The h-proposal network, when given, seeds the density pass's
+search radius from the recorded history and is trained after it.
+"""
+import numpy as np
+
+from . import pkd as P
+
+
+class KDK:
+    def __init__(self, pk, smf, nSmooth=64, gravity=True, theta=0.7, iOrder=2, nBucket=16,
+                 period=None, hmlp=None):
+        self.pk, self.smf, self.K = pk, smf, nSmooth
+        self.gravity, self.theta, self.iOrder, self.nBucket = gravity, theta, iOrder, nBucket
+        self.period = period
+        self.hmlp = hmlp            # dict(train_iters, batch, lr) or None
+        self.dt_prev = None
+        self.stats = []             # kNN counters of each density pass
+
+    def seed_search(self):
+        """Before a density pass: the network's proposal, if trained."""
+        if self.hmlp and self.dt_prev is not None and self.hmlp.get("predict", True):
+            self.pk.hmlp_predict()
+
+    def forces(self, dt):
+        pk = self.pk
+        pk.init_accel()
+        if self.gravity:
+            pk.build_binary(self.nBucket, 1, self.theta, self.iOrder)
+            pk.grav_all(self.iOrder)
+        pk.build_local(period=self.period)
+        self.seed_search()
+        pk.smooth(self.smf, self.K, self.period is not None)
+        self.stats.append(pk.smooth_stats())
+        if self.hmlp:
+            if self.dt_prev is not None:
+                h = self.hmlp
+                self.last_loss = pk.hmlp_train(h.get("train_iters", 50), h.get("batch", 8192),
+                                               h.get("lr", 3e-3), h.get("seed", 1))
+            pk.hmlp_record(dt)
+        pk.gas_pressure(self.smf)
+        pk.resmooth(self.smf, self.K, self.period is not None)
+
+    def step(self, dt):
+        pk = self.pk
+        pk.kick(1.0, 0.5 * dt, 1.0, dt, 0.5 * dt, dt)
+        pk.drift(dt, period=self.period)
+        self.forces(dt)
+        self.dt_prev = dt
+        pk.kick(1.0, 0.5 * dt, 1.0, 0.5 * dt, 0.5 * dt, 0.5 * dt)
+
+
+def start(pk, d, soft, bits=None):
+    """Put initial conditions: r, v, fMass, fSoft, u (vPred = v, uPred = u)."""
+    pk.put("r", d["r"])
+    pk.put("v", d["v"])
+    pk.put("vPred", d["v"])
+    pk.put("fMass", d["m"])
+    pk.put("fSoft", np.broadcast_to(np.asarray(soft, float), d["m"].shape).copy())
+    pk.put("u", d["u"])
+    pk.put("uPred", d["u"])
+    if bits is not None:
+        pk.put("iActive", bits)
+    return pk

@@ -42,7 +42,12 @@ TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
        "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_build_local", "gg_smooth", "gg_resmooth",
-       "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings"]
+       "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
+       "gg_set_seed_slack", "gg_init_accel", "gg_kick", "gg_drift",
+       "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
+       "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
+
+HMLP_NW, HMLP_NP = 1249, 1257
 
 
 class GGError(RuntimeError):
@@ -111,6 +116,18 @@ def lib():
     L.gg_smooth_lists.argtypes = [_vp, C.c_int, _ip]
     L.gg_timings.argtypes = [_vp, _dp, C.c_int]
     L.gg_smooth_stats.argtypes = [_vp, _i64p, C.c_int]
+    L.gg_set_seed_slack.argtypes = [_vp, C.c_double]
+    L.gg_init_accel.argtypes = [_vp]
+    L.gg_kick.argtypes = [_vp] + [C.c_double] * 6 + [C.c_int]
+    L.gg_drift.argtypes = [_vp, C.c_double, _dp, C.c_int, _dp]
+    _fp = C.POINTER(C.c_float)
+    L.gg_hmlp_init.argtypes = [_vp, C.c_uint]
+    L.gg_hmlp_record.argtypes = [_vp, C.c_double]
+    L.gg_hmlp_train.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_uint, _dp, _dp, _ip]
+    L.gg_hmlp_predict.argtypes = [_vp]
+    L.gg_hmlp_params.argtypes = [_vp, _fp, C.c_int, C.c_int]
+    L.gg_hmlp_eval.argtypes = [_vp, _fp, C.c_int, _fp]
+    L.gg_hmlp_grad.argtypes = [_vp, _fp, _fp, C.c_int, _fp, _dp]
     _LIB = L
     return L
 
@@ -238,14 +255,80 @@ class Pkd:
         return out
 
     def smooth_stats(self):
-        s = np.zeros(8, np.int64)
-        _check(lib().gg_smooth_stats(self.h, _p(s, _i64p), 8))
+        s = np.zeros(10, np.int64)
+        _check(lib().gg_smooth_stats(self.h, _p(s, _i64p), 10))
         return dict(retried=int(s[0]), ties=int(s[1]), extras=int(s[2]), queries=int(s[3]),
                     groups_overflowed=int(s[4]), short_at_RL=int(s[5]), unsettled=int(s[6]),
-                    max_leaves=int(s[7]))
+                    max_buckets=int(s[7]), extra_passes=int(s[8]), candidates=int(s[9]))
+
+    def set_seed_slack(self, slack):
+        _check(lib().gg_set_seed_slack(self.h, float(slack)))
+
+    # --- kick, drift, accel reset (pkdInitAccel, pkdKick, pkdDrift)
+    def init_accel(self):
+        _check(lib().gg_init_accel(self.h))
+
+    def kick(self, dvFacOne, dvFacTwo, dvPredFacOne, dvPredFacTwo, duDelta, duPredDelta,
+             bIsothermal=False):
+        _check(lib().gg_kick(self.h, dvFacOne, dvFacTwo, dvPredFacOne, dvPredFacTwo, duDelta,
+                             duPredDelta, int(bIsothermal)))
+
+    def drift(self, dDelta, period=None, center=(0.0, 0.0, 0.0)):
+        if period is None:
+            _check(lib().gg_drift(self.h, dDelta, None, 0, None))
+        else:
+            c = np.ascontiguousarray(center, np.float64)
+            L = np.ascontiguousarray(period, np.float64)
+            _check(lib().gg_drift(self.h, dDelta, _p(c), 1, _p(L)))
+
+    # --- h-proposal MLP
+    def hmlp_init(self, seed=1):
+        _check(lib().gg_hmlp_init(self.h, int(seed)))
+
+    def hmlp_record(self, dt):
+        _check(lib().gg_hmlp_record(self.h, float(dt)))
+
+    def hmlp_train(self, nIter, nBatch=4096, lr=1e-3, seed=1):
+        l0, l1, ns = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().gg_hmlp_train(self.h, int(nIter), int(nBatch), float(lr), int(seed),
+                                   C.byref(l0), C.byref(l1), C.byref(ns)))
+        return l0.value, l1.value, ns.value
+
+    def hmlp_predict(self):
+        _check(lib().gg_hmlp_predict(self.h))
+
+    def hmlp_params(self, w=None):
+        fp = C.POINTER(C.c_float)
+        if w is None:
+            out = np.zeros(HMLP_NP, np.float32)
+            _check(lib().gg_hmlp_params(self.h, out.ctypes.data_as(fp), HMLP_NP, 0))
+            return out
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib().gg_hmlp_params(self.h, w.ctypes.data_as(fp), len(w), 1))
+
+    def hmlp_eval(self, x):
+        """x: (n, 4) features -> y (n,)."""
+        fp = C.POINTER(C.c_float)
+        xs = np.ascontiguousarray(np.asarray(x, np.float32).T)
+        y = np.zeros(xs.shape[1], np.float32)
+        _check(lib().gg_hmlp_eval(self.h, xs.ctypes.data_as(fp), xs.shape[1],
+                                  y.ctypes.data_as(fp)))
+        return y
+
+    def hmlp_grad(self, x, t):
+        """Gradient of mean (y - t)^2 over samples x (n, 4): (g[HMLP_NW], loss)."""
+        fp = C.POINTER(C.c_float)
+        xs = np.ascontiguousarray(np.asarray(x, np.float32).T)
+        ts = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(HMLP_NW, np.float32)
+        loss = C.c_double()
+        _check(lib().gg_hmlp_grad(self.h, xs.ctypes.data_as(fp), ts.ctypes.data_as(fp),
+                                  xs.shape[1], g.ctypes.data_as(fp), C.byref(loss)))
+        return g, loss.value
 
     def timings(self):
-        t = np.zeros(9)
-        _check(lib().gg_timings(self.h, _p(t), 9))
+        t = np.zeros(11)
+        _check(lib().gg_timings(self.h, _p(t), 11))
         return dict(zip(["walk", "interact", "sph_tree", "knn", "density", "lists",
-                         "revlists", "hydro", "other"], t))
+                         "revlists", "hydro", "other", "knn_lists", "knn_retry"],
+                        (float(x) for x in t)))

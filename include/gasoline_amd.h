@@ -203,27 +203,102 @@ int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
  * without cooling. */
 int gg_gas_pressure(GGPKD pkd, const gg_smf *smf);
 
+/* The kNN search tries first a ball of slack * fBall2 around a particle
+ * whose stored fBall2 is positive (a hint: the previous step's value, or a
+ * proposal of the h-proposal network), widening it if it holds fewer than
+ * nSmooth particles; the result does not depend on it.  Default 1.15; an
+ * accurate hint does with less.  No counterpart in the reference, whose
+ * search starts from the bucket's own particles (smooth.c:1204-1305). */
+int gg_set_seed_slack(GGPKD pkd, double slack);
+
 /* Neighbour lists of the last gg_smooth, for parity checks: nbr[i*nSmooth+k]
- * in caller order (sorted by distance, then index). */
+ * in caller order.  The order within a list is the search's own (the
+ * reference's is its priority-queue order); the set is exact. */
 int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr);
 
 /* Counters of the last gg_smooth / gg_resmooth: s[0] kNN queries that the
- * leaf-group search handed to the single-query search, s[1] particles whose
+ * bucket-group search handed to the single-query search, s[1] particles whose
  * nSmooth-th and (nSmooth+1)-th distances tie (their neighbour set is not
  * unique; gasoline's priority queue keeps the one met first), s[2] entries
  * of the force pass's extras lists, s[3] smooth-active particles; search
- * diagnostics of the leaf-group kNN: s[4] groups whose neighbourhood did not
- * fit the staging buffer, s[5] queries with fewer than nSmooth candidates in
+ * diagnostics of the bucket-group kNN: s[4] groups too large to search
+ * (more than 192 buckets or 4096 particles), s[5] queries with fewer than nSmooth candidates in
  * the group radius, s[6] queries whose radius search did not settle, s[7]
- * largest number of staged leaves.  n <= 8. */
+ * largest number of buckets in a group; search work of the last gg_smooth:
+ * s[8] trial balls beyond the first (widened or narrowed), s[9] candidate
+ * particles whose distance was computed.  n <= 10. */
 int gg_smooth_stats(GGPKD pkd, int64_t *s, int n);
+
+/* ------------------------------------------- kick, drift, accel reset */
+
+/* pkdInitAccel (pkd.c:5443): a, fPot, dtGrav = 0 for TYPE_ACTIVE particles. */
+int gg_init_accel(GGPKD pkd);
+/* pkdKick (pkd.c:3780, NOCOOLING build): for TYPE_ACTIVE particles
+ * vPred = v dvPredFacOne + a dvPredFacTwo, v = v dvFacOne + a dvFacTwo; for
+ * TYPE_GAS ones unless bIsothermal, uPred = u + uDot duPredDelta and
+ * u = u + uDot duDelta with uDot = uDotPdV + uDotAV + uDotDiff (pkd.h:65). */
+int gg_kick(GGPKD pkd, double dvFacOne, double dvFacTwo, double dvPredFacOne,
+            double dvPredFacTwo, double duDelta, double duPredDelta, int bIsothermal);
+/* pkdDrift (pkd.c:3686): r += dDelta v, wrapped into the box
+ * [fCenter - fPeriod/2, fCenter + fPeriod/2) when bPeriodic; GG_EINVAL if a
+ * coordinate is still outside (the reference asserts).  The trees are stale
+ * afterwards: build again before the next force call. */
+int gg_drift(GGPKD pkd, double dDelta, const double *fCenter, int bPeriodic,
+             const double *fPeriod);
+
+/* ------------------------------------------------- h-proposal MLP */
+
+/* A 4-32-32-1 tanh network (FP32) that proposes each gas particle's next
+ * smoothing length from its previous h, density, velocity divergence and
+ * the step dt, and writes the proposal as the fBall2 hint gg_smooth starts
+ * its search from.  The kNN search stays exact whatever the hint, so the
+ * network changes how much work the search does, never its result.  No
+ * counterpart in the reference: it stands beside smSmooth (smooth.c:1439),
+ * whose search starts from the bucket's own particles.
+ *
+ * Use per step: gg_smooth(DENDVDX) -> gg_hmlp_train (previous history ->
+ * the fBall2 just found) -> gg_hmlp_record(dt) -> ... forces, kick, drift
+ * ... -> gg_hmlp_predict -> next gg_smooth.  The history is kept by store
+ * index: a caller that reorders its particles records again before
+ * predicting.
+ *
+ * Parameter vector (GG_HMLP_NP floats): W1[32][4], b1[32], W2[32][32],
+ * b2[32], W3[32], b3, input mean[4], input scale[4].  Inputs
+ * x = (log h, log rho, divv dt, log dt), standardised by mean and scale
+ * (taken from the first training set); output y = log h_next - log h. */
+#define GG_HMLP_NW 1249
+#define GG_HMLP_NP 1257
+
+/* Deterministic initialisation from seed (Glorot-uniform W1, W2; W3 = b3 = 0,
+ * so the untrained network proposes h_next = h); clears Adam state and
+ * history. */
+int gg_hmlp_init(GGPKD pkd, unsigned seed);
+/* Snapshot the features of the particles the last gg_smooth smoothed. */
+int gg_hmlp_record(GGPKD pkd, double dt);
+/* nIter Adam steps (beta 0.9/0.999) on minibatches of nBatch samples drawn
+ * (seeded, reproducibly: sample k of the handle's Adam step s is drawn by a
+ * hash of (seed, s, k)) from the particles with a history that the last
+ * gg_smooth smoothed again; target t = log h_now - log h_recorded.  Returns
+ * the first and last minibatch losses (mean (y-t)^2) and the sample count. */
+int gg_hmlp_train(GGPKD pkd, int nIter, int nBatch, double lr, unsigned seed,
+                  double *pLossFirst, double *pLossLast, int *pnSamples);
+/* Write fBall2 = (2 h exp(y))^2 from the recorded features, for the
+ * particles with a history; others keep their fBall2. */
+int gg_hmlp_predict(GGPKD pkd);
+/* Get (bSet = 0) or set the GG_HMLP_NP parameters. */
+int gg_hmlp_params(GGPKD pkd, float *w, int n, int bSet);
+/* Forward pass on n host samples x[4][n] (feature-major): y[n]. */
+int gg_hmlp_eval(GGPKD pkd, const float *x, int n, float *y);
+/* Gradient of mean (y - t)^2 over n host samples: g[GG_HMLP_NW], loss. */
+int gg_hmlp_grad(GGPKD pkd, const float *x, const float *t, int n, float *g, double *loss);
 
 /* ------------------------------------------------------------ timing */
 
 /* Device time in milliseconds of the last call of each kind, by phase:
  * t[0] tree walk (list build), t[1] interaction kernel, t[2] SPH structure,
  * t[3] kNN kernel, t[4] density kernel, t[5] resmooth lists, t[6] reverse
- * lists, t[7] hydro kernel, t[8] other (gathers, scatters, counters).
+ * lists, t[7] hydro kernel, t[8] other (gathers, scatters, counters), t[9]
+ * kNN group lists, t[10] kNN retries (with the count read-back before them).
  * Measured with HIP events on the handle's stream; n <= 12. */
 int gg_timings(GGPKD pkd, double *t, int n);
 

@@ -1,7 +1,9 @@
 """GPU parity of the gravity path (gg_build_binary, gg_grav_all) against the
 CPU oracle, through the C ABI.
 
-Bars: tree and interaction lists bit-exact; counters exact; FP64
+Bars: tree bit-exact; interaction lists bit-exact as sets per bucket and
+list kind (the library's walk is breadth-first, so members come in another
+order than the reference's preorder walk); counters exact; FP64
 accelerations/potentials within 1e-8 relative (the oracle uses gasoline's
 v_sqrt1, itself accurate to 2e-10); FP32 within 1e-5 rms / 1e-4 max relative
 to the FP64 oracle."""
@@ -64,7 +66,7 @@ def test_interaction_lists_bitexact(gpu, plummer32k):
         part, scell, ncell = T.walk(ib)
         for k, ref in enumerate((part, scell, ncell)):
             got = ent[off[b, k]:off[b, k] + cnt[b, k]]
-            assert np.array_equal(got, ref), (ib, k)
+            assert np.array_equal(np.sort(got), np.sort(ref)), (ib, k)
     for key in ("nActive", "dPartSum", "dCellSum", "dSoftSum", "dFlop"):
         assert st[key] == sto[key], key
 

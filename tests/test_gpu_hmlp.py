@@ -1,0 +1,194 @@
+"""GPU checks of the h-proposal MLP (gg_hmlp_*) against the CPU oracle
+(oracle/hmlp_oracle.py), through the C ABI, and what the trained network
+gains over the search without it.
+
+Bars: forward pass within 2e-5 absolute of the FP64 oracle on the same FP32
+inputs; gradient within 1e-4 of the largest component; an Adam training run
+replayed by the oracle (same minibatch draws) to within 1e-2 lr per step;
+training bit-reproducible; neighbour sets identical whatever the seed."""
+import numpy as np
+import pytest
+
+from gasoline_amd import evolve, ics
+from oracle import hmlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def random_params(rng):
+    p = rng.normal(0, 0.4, M.NP)
+    p[M.NW:M.NW + 4] = rng.normal(0, 1, 4)
+    p[M.NW + 4:] = rng.uniform(0.5, 2.0, 4)
+    return p.astype(np.float32)
+
+
+def test_forward_and_gradient_vs_oracle(gpu):
+    rng = np.random.default_rng(11)
+    p = random_params(rng)
+    x = rng.normal(0, 1.5, (3000, 4)).astype(np.float32)
+    t = rng.normal(0, 0.3, 3000).astype(np.float32)
+    with gpu.Pkd(16) as pk:
+        pk.hmlp_init(3)
+        pk.hmlp_params(p)
+        assert np.array_equal(pk.hmlp_params(), p)
+        y = pk.hmlp_eval(x)
+        g, loss = pk.hmlp_grad(x, t)
+    yo, _ = M.forward(p, x)
+    go, lo = M.grad(p, x, t)
+    assert np.max(np.abs(y - yo)) < 2e-5
+    assert np.max(np.abs(g - go)) < 1e-4 * np.max(np.abs(go))
+    assert abs(loss - lo) < 1e-5 * lo
+
+
+def test_init_proposes_previous_h(gpu):
+    # W3 = b3 = 0 after init: y = 0 for any input
+    with gpu.Pkd(16) as pk:
+        pk.hmlp_init(9)
+        w = pk.hmlp_params()
+        assert np.all(w[M.OW3:M.NW] == 0)
+        assert np.all(pk.hmlp_eval(np.random.default_rng(0).normal(size=(100, 4))) == 0)
+
+
+def _homologous(gpu, n=16384, seed=3):
+    """A sphere smoothed, recorded, then shrunk homologously (v = -0.5 r, dt
+    0.1) and smoothed again: a history and a new fBall2 to train on."""
+    d = ics.evrard(n, seed=seed)
+    d["v"] = -0.5 * d["r"]
+    pk = gpu.Pkd(n)
+    evolve.start(pk, d, 0.02)
+    smf = gpu.Smf.default(iViscosityLimiter=1)
+    pk.build_local()
+    pk.smooth(smf, 32, False)
+    fb2, rho, divv = pk.get("fBall2"), pk.get("fDensity"), pk.get("divv")
+    pk.hmlp_init(5)
+    pk.hmlp_record(0.1)
+    pk.put("r", d["r"] * np.exp(-0.05))
+    pk.build_local()
+    pk.smooth(smf, 32, False)
+    return pk, M.features(fb2, rho, divv, 0.1).astype(np.float32), pk.get("fBall2")
+
+
+def test_training_matches_oracle_replay(gpu):
+    pk, feat, fb2_now = _homologous(gpu)
+    with pk:
+        l0, l1, ns = pk.hmlp_train(4, 512, 1e-2, seed=5)
+        w_gpu = pk.hmlp_params()
+    assert ns == len(feat)
+    t = 0.5 * np.log(fb2_now) - np.log(2.0) - feat[:, 0].astype(np.float64)
+    p = np.zeros(M.NP)
+    with gpu.Pkd(16) as pk0:       # the initial weights for seed 5
+        pk0.hmlp_init(5)
+        p[:] = pk0.hmlp_params()
+    mu = feat.astype(np.float64).mean(0)
+    sd = np.maximum(feat.astype(np.float64).std(0), 1e-3)
+    assert np.allclose(w_gpu[M.NW:], np.r_[mu, sd], rtol=1e-4, atol=1e-5)
+    p[M.NW:] = w_gpu[M.NW:]          # replay with the standardisation the library took
+    m = np.zeros(M.NW)
+    v = np.zeros(M.NW)
+    losses = []
+    for it in range(4):
+        idx = M.draw(5, it, 512, len(feat))
+        g, l = M.grad(p, feat[idx], t[idx])
+        losses.append(l)
+        p, m, v = M.adam(p, g, m, v, it + 1, 1e-2)
+    assert abs(l0 - losses[0]) < 1e-4 * losses[0] and abs(l1 - losses[-1]) < 1e-4 * losses[-1]
+    assert np.max(np.abs(w_gpu[:M.NW] - p[:M.NW])) < 1e-2 * 1e-2 * 4
+
+
+def test_training_reproducible_and_learns(gpu):
+    out = []
+    for _ in range(2):
+        pk, feat, fb2_now = _homologous(gpu)
+        with pk:
+            l0, l1, _ = pk.hmlp_train(300, 4096, 3e-3, seed=2)
+            out.append((l0, l1, pk.hmlp_params()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+    # homologous contraction by e^-0.05: the target log h ratio is close to
+    # -0.05 everywhere; the trained network must fit it far better than
+    # the untrained one (y = 0)
+    assert out[0][1] < 0.05 * out[0][0]
+
+
+def _seed_gain(gpu, d, gravity, dt, steps, slack_mlp=1.08, K=32):
+    """Steps a system (KDK, single rung; gravity optional).  Every density
+    pass is run with three seeds on the same positions: none (cold), the
+    previous fBall2 (slack 1.15), and the network's proposal (slack_mlp),
+    the network being trained on the run's own earlier steps.  Checks that
+    the three give the same neighbour sets and returns the search work per
+    step: {mode: (candidates, extra trial balls)} and the rms error of the
+    proposed log h."""
+    n = len(d["m"])
+    smf = gpu.Smf.default(iViscosityLimiter=1)
+    work, errs = [], []
+    with gpu.Pkd(n) as pk:
+        evolve.start(pk, d, 0.02)
+        pk.hmlp_init(1)
+        run = evolve.KDK(pk, smf, nSmooth=K, gravity=gravity,
+                         hmlp=dict(train_iters=200, batch=8192, lr=3e-3, predict=False))
+        run.forces(dt)
+        for step in range(steps):
+            pk.kick(1.0, 0.5 * dt, 1.0, dt, 0.5 * dt, dt)
+            pk.drift(dt)
+            pk.build_local()
+            fb2_prev = pk.get("fBall2")
+            res = {}
+            for mode in ("cold", "prev", "mlp"):
+                pk.set_seed_slack(slack_mlp if mode == "mlp" else 1.15)
+                pk.put("fBall2", np.zeros(n) if mode == "cold" else fb2_prev)
+                if mode == "mlp":
+                    pk.hmlp_predict()
+                    seed = pk.get("fBall2")
+                pk.smooth(smf, K, False)
+                st = pk.smooth_stats()
+                res[mode] = (st["candidates"], st["extra_passes"], pk.smooth_lists(K).copy(),
+                             pk.get("fBall2"))
+            for mode in ("prev", "mlp"):
+                assert np.array_equal(res[mode][3], res["cold"][3])
+                assert np.array_equal(np.sort(res[mode][2], 1), np.sort(res["cold"][2], 1))
+            errs.append(float(np.sqrt(np.mean((0.5 * np.log(seed / res["cold"][3])) ** 2))))
+            work.append({k: res[k][:2] for k in res})
+            # the rest of the step, on the density pass just made
+            pk.hmlp_train(200, 8192, 3e-3, 1)
+            pk.hmlp_record(dt)
+            pk.init_accel()
+            if gravity:
+                pk.build_binary(16, 1, 0.7, 2)
+                pk.grav_all(2)
+            pk.gas_pressure(smf)
+            pk.resmooth(smf, K, False)
+            pk.kick(1.0, 0.5 * dt, 1.0, 0.5 * dt, 0.5 * dt, 0.5 * dt)
+    print("work per step (cold, prev, mlp) as (candidates, extra balls):",
+          [(w["cold"], w["prev"], w["mlp"]) for w in work], "rms log-h error of the proposals:",
+          [round(e, 4) for e in errs])
+    return work, errs
+
+
+def test_gain_in_a_collapse(gpu):
+    """Evrard collapse (gravity + SPH), 32k particles: h shrinks, so the
+    previous fBall2 over-estimates and costs candidates; the trained
+    network's proposal costs fewer."""
+    work, errs = _seed_gain(gpu, ics.evrard(32768, seed=8), True, 0.03, 8)
+    late = work[3:]
+    assert sum(w["mlp"][0] for w in late) < 0.98 * sum(w["prev"][0] for w in late)
+    assert max(errs[3:]) < 0.02
+
+
+def test_gain_in_an_expansion(gpu):
+    """A hot uniform sphere expanding homologously (v = r, no gravity), 32k
+    particles: h grows about 10% a step, beyond the 1.15 slack the previous
+    fBall2 is tried with, so most searches need a second, wider ball; the
+    trained network's proposal needs few."""
+    rng = np.random.default_rng(21)
+    n = 32768
+    rad = rng.uniform(0, 1, n) ** (1 / 3)
+    v = rng.normal(size=(n, 3))
+    r = rad[:, None] * v / np.linalg.norm(v, axis=1)[:, None]
+    d = dict(r=ics._f32(r), v=r.copy(), m=np.full(n, 1.0 / n), u=np.full(n, 0.1))
+    work, errs = _seed_gain(gpu, d, False, 0.1, 6)
+    late = work[2:]
+    prev_extra = sum(w["prev"][1] for w in late)
+    assert prev_extra > n            # the stale seed needs a second ball for most
+    assert sum(w["mlp"][1] for w in late) < 0.1 * prev_extra
+    assert sum(w["mlp"][0] for w in late) < 0.9 * sum(w["prev"][0] for w in late)
