@@ -114,6 +114,41 @@ __global__ void k_walk(int nb, const int *__restrict__ bucket,
 // k_walk).
 constexpr int WK_FC = 1024;
 
+// pkdLocalWalk's decision for cell c seen from bucket ib (active box bb,
+// largest softening fSoftMax): open (children, or the bucket's particles
+// for the PP list unless c is the bucket itself), or accept as a softened
+// or a Newtonian cell (walk.c:77-171).
+struct WalkDec {
+    bool kids, pp, soft, newt;
+    int cl, cu, pl, np;
+};
+
+__device__ __forceinline__ WalkDec walk_decide(const WalkNode &n, int c, int ib, const double *bb,
+                                               double fSoftMax)
+{
+#pragma clang fp contract(off)
+    WalkDec d{false, false, false, false, 0, 0, 0, 0};
+    bool open = box_within(bb, n.fOpen2, n.cx, n.cy, n.cz);
+    if (n.pUpper - n.pLower < 3) open = true;   // < 4 particles: open
+    if (open) {
+        if (n.iLower != -1) {
+            d.kids = true;
+            d.cl = n.iLower;
+            d.cu = n.iUpper;
+        } else if (c != ib) {
+            d.pp = true;
+            d.pl = n.pLower;
+            d.np = n.pUpper - n.pLower + 1;
+        }
+    } else {
+        double twoh2 = n.fSoft + fSoftMax;
+        twoh2 *= twoh2;
+        d.soft = (twoh2 < n.fOpen2) ? false : box_within(bb, twoh2, n.cx, n.cy, n.cz);
+        d.newt = !d.soft;
+    }
+    return d;
+}
+
 __global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict__ bucket,
                                                   const double *__restrict__ bbox,
                                                   const WalkNode *__restrict__ wn, int iRoot,
@@ -145,30 +180,14 @@ __global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict_
         int nn = 0;
         for (int base = 0; base < nf; base += 64) {
             const int k = base + lane;
-            bool kids = false, pp = false, soft = false, newt = false;
-            int c = 0, cl = 0, cu = 0, pl = 0, np = 0;
+            int c = 0;
+            WalkDec dc{false, false, false, false, 0, 0, 0, 0};
             if (k < nf) {
                 c = fr[w][cur][k];
-                const WalkNode n = wn[c];
-                bool open = box_within(bb, n.fOpen2, n.cx, n.cy, n.cz);
-                if (n.pUpper - n.pLower < 3) open = true;   // < 4 particles: open
-                if (open) {
-                    if (n.iLower != -1) {
-                        kids = true;
-                        cl = n.iLower;
-                        cu = n.iUpper;
-                    } else if (c != ib) {
-                        pp = true;
-                        pl = n.pLower;
-                        np = n.pUpper - n.pLower + 1;
-                    }
-                } else {
-                    double twoh2 = n.fSoft + fSoftMax;
-                    twoh2 *= twoh2;
-                    soft = (twoh2 < n.fOpen2) ? false : box_within(bb, twoh2, n.cx, n.cy, n.cz);
-                    newt = !soft;
-                }
+                dc = walk_decide(wn[c], c, ib, bb, fSoftMax);
             }
+            const bool kids = dc.kids, pp = dc.pp, soft = dc.soft, newt = dc.newt;
+            const int cl = dc.cl, cu = dc.cu, pl = dc.pl, np = dc.np;
             const unsigned long long mk = __ballot(kids);
             if (kids) {
                 const int p = nn + 2 * __popcll(mk & below);
@@ -363,256 +382,260 @@ struct IArgs {
 // One 64-lane workgroup per active bucket.  Lane = (target t, slice s):
 // TPW = next power of two >= bucket size targets, S = 64/TPW slices that
 // split each staged chunk; slice partials are reduced at the end.
-template <typename T, int ORDER>
-__global__ __launch_bounds__(64) void k_interact(IArgs A)
-{
-    __shared__ Stage<T, ORDER> L;
-    const int lane = threadIdx.x;
-    const int b = blockIdx.x;
-    const int ib = A.bucket[b];
-    const WalkNode bn = A.wn[ib];
-    const int pL = bn.pLower, nT = bn.pUpper - bn.pLower + 1;
-    int TPW = 1;
-    while (TPW < nT) TPW <<= 1;
-    const int S = 64 / TPW;
-    const int t = lane & (TPW - 1), s = lane / TPW;
-    const bool valid = t < nT;
-    const int it = pL + (valid ? t : 0);
-    const bool act = valid && (A.pact[it] & 1u);
-    const double cx = bn.cx, cy = bn.cy, cz = bn.cz;     // bucket centre of mass
-    const T xi = (T)(A.px[it] - cx), yi = (T)(A.py[it] - cy), zi = (T)(A.pz[it] - cz);
-    const T hi = (T)A.ph[it], mi = (T)A.pm[it];
+// One target of an interaction workgroup, and its running sums.
+template <typename T> struct Tgt {
+    T xi, yi, zi, hi, mi;   // position relative to the bucket's c.o.m., softening, mass
+    int s, S;               // slice of the staged chunk this lane evaluates
+    bool act;
+};
+template <typename T> struct Acc {
     Sum<T> SX, SY, SZ, SP;
     T dtg = 0;
+};
 
-    // (0) the bucket's own particles, every pair i != j (grav.c:211-242)
-    if (lane < nT) {
-        L.f[0][lane] = (T)(A.px[pL + lane] - cx);
-        L.f[1][lane] = (T)(A.py[pL + lane] - cy);
-        L.f[2][lane] = (T)(A.pz[pL + lane] - cz);
-        L.f[3][lane] = (T)A.pm[pL + lane];
-        L.f[4][lane] = (T)A.ph[pL + lane];
-    }
-    __syncthreads();
-    if (act) {
-        Part<T> P;
-        for (int e = s; e < nT; e += S) {
-            if (e == t) continue;
-            const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
-            const T mj = L.f[3][e];
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            T a, bb;
-            spline_ab<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb);
-            const T idt2 = (mi + mj) * bb;
-            dtg = idt2 > dtg ? idt2 : dtg;
-            a *= mj;
-            bb *= mj;
-            P.pot -= a;
-            P.ax -= dx * bb;
-            P.ay -= dy * bb;
-            P.az -= dz * bb;
-        }
-        SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
-    }
-    __syncthreads();
+// Staging of one interaction per lane into chunk slot `lane`, relative to
+// the bucket's centre of mass (cx, cy, cz).
+template <typename T, int ORDER>
+__device__ __forceinline__ void stage_pp(Stage<T, ORDER> &L, int lane, int j, const IArgs &A,
+                                         double cx, double cy, double cz)
+{
+    L.f[0][lane] = (T)(A.px[j] - cx);
+    L.f[1][lane] = (T)(A.py[j] - cy);
+    L.f[2][lane] = (T)(A.pz[j] - cz);
+    L.f[3][lane] = (T)A.pm[j];
+    L.f[4][lane] = (T)A.ph[j];
+}
 
-    const size_t b3 = 3 * (size_t)b;
-    // (1) particle-particle list (grav.c:81-108)
-    {
-        const int n = A.cnt[b3];
-        const int *E = A.ent + A.off[b3];
-        for (int base = 0; base < n; base += 64) {
-            const int m = min(64, n - base);
-            if (lane < m) {
-                const int j = E[base + lane];
-                L.f[0][lane] = (T)(A.px[j] - cx);
-                L.f[1][lane] = (T)(A.py[j] - cy);
-                L.f[2][lane] = (T)(A.pz[j] - cz);
-                L.f[3][lane] = (T)A.pm[j];
-                L.f[4][lane] = (T)A.ph[j];
-            }
-            __syncthreads();
-            if (act) {
-                Part<T> P;
-                for (int e = s; e < m; e += S) {
-                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
-                    const T mj = L.f[3][e];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    T a, bb;
-                    spline_ab<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb);
-                    const T idt2 = (mi + mj) * bb;
-                    dtg = idt2 > dtg ? idt2 : dtg;
-                    a *= mj;
-                    bb *= mj;
-                    P.pot -= a;
-                    P.ax -= dx * bb;
-                    P.ay -= dy * bb;
-                    P.az -= dz * bb;
-                }
-                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
-            }
-            __syncthreads();
-        }
+template <typename T, int ORDER>
+__device__ __forceinline__ void stage_soft(Stage<T, ORDER> &L, int lane, int ic, const IArgs &A,
+                                           double cx, double cy, double cz)
+{
+    const double *c = A.cell + (size_t)CR_N * ic;
+    L.f[0][lane] = (T)(c[CR_X] - cx);
+    L.f[1][lane] = (T)(c[CR_X + 1] - cy);
+    L.f[2][lane] = (T)(c[CR_X + 2] - cz);
+    L.f[3][lane] = (T)c[CR_M];
+    L.f[4][lane] = (T)c[CR_H];
+    for (int k = 0; k < 6; ++k) L.f[5 + k][lane] = (T)c[CR_QR + k];
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void stage_newt(Stage<T, ORDER> &L, int lane, int ic, const IArgs &A,
+                                           double cx, double cy, double cz)
+{
+    const double *c = A.cell + (size_t)CR_N * ic;
+    L.f[0][lane] = (T)(c[CR_X] - cx);
+    L.f[1][lane] = (T)(c[CR_X + 1] - cy);
+    L.f[2][lane] = (T)(c[CR_X + 2] - cz);
+    L.f[3][lane] = (T)c[CR_M];
+    for (int k = 0; k < 6; ++k) L.f[4 + k][lane] = (T)c[CR_QT + k];
+    if constexpr (ORDER >= 3) {
+        const double *oh = A.momOH + (size_t)OH_N * ic;
+        for (int k = 0; k < 10; ++k) L.f[10 + k][lane] = (T)oh[k];
+        if constexpr (ORDER >= 4)
+            for (int k = 0; k < 15; ++k) L.f[20 + k][lane] = (T)oh[10 + k];
     }
-    // (2) softened cells: monopole + raw quadrupole (grav.c:117-150)
-    {
-        const int n = A.cnt[b3 + 1];
-        const int *E = A.ent + A.off[b3 + 1];
-        for (int base = 0; base < n; base += 64) {
-            const int m = min(64, n - base);
-            if (lane < m) {
-                const double *c = A.cell + (size_t)CR_N * E[base + lane];
-                L.f[0][lane] = (T)(c[CR_X] - cx);
-                L.f[1][lane] = (T)(c[CR_X + 1] - cy);
-                L.f[2][lane] = (T)(c[CR_X + 2] - cz);
-                L.f[3][lane] = (T)c[CR_M];
-                L.f[4][lane] = (T)c[CR_H];
-                for (int k = 0; k < 6; ++k) L.f[5 + k][lane] = (T)c[CR_QR + k];
-            }
-            __syncthreads();
-            if (act) {
-                Part<T> P;
-                for (int e = s; e < m; e += S) {
-                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
-                    const T mc = L.f[3][e];
-                    const T xx = L.f[5][e], yy = L.f[6][e], zz = L.f[7][e];
-                    const T xy = L.f[8][e], xz = L.f[9][e], yz = L.f[10][e];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    T a, bb, c, d;
-                    spline_abcd<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb, c, d);
-                    const T qirx = xx * dx + xy * dy + xz * dz;
-                    const T qiry = xy * dx + yy * dy + yz * dz;
-                    const T qirz = xz * dx + yz * dy + zz * dz;
-                    const T qir = T(0.5) * (qirx * dx + qiry * dy + qirz * dz);
-                    const T tr = T(0.5) * (xx + yy + zz);
-                    const T qir3 = bb * mc + d * qir - c * tr;
-                    P.pot -= a * mc + c * qir - bb * tr;
-                    P.ax -= qir3 * dx - c * qirx;
-                    P.ay -= qir3 * dy - c * qiry;
-                    P.az -= qir3 * dz - c * qirz;
-                    const T idt2 = (mi + mc) * bb;
-                    dtg = idt2 > dtg ? idt2 : dtg;
-                }
-                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
-            }
-            __syncthreads();
-        }
+}
+
+// Particle-particle interactions with the m staged particles
+// (grav.c:81-108; the intra-bucket pairs of grav.c:211-242 skip e == self).
+template <typename T, int ORDER>
+__device__ __forceinline__ void eval_pp(const Stage<T, ORDER> &L, int m, const Tgt<T> &g,
+                                        Acc<T> &R, int self = -1)
+{
+    if (!g.act) return;
+    const T xi = g.xi, yi = g.yi, zi = g.zi, hi = g.hi, mi = g.mi;
+    const int s = g.s, S = g.S;
+    T &dtg = R.dtg;
+    Part<T> P;
+    for (int e = s; e < m; e += S) {
+        if (e == self) continue;
+        const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+        const T mj = L.f[3][e];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T a, bb;
+        spline_ab<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb);
+        const T idt2 = (mi + mj) * bb;
+        dtg = idt2 > dtg ? idt2 : dtg;
+        a *= mj;
+        bb *= mj;
+        P.pot -= a;
+        P.ax -= dx * bb;
+        P.ay -= dy * bb;
+        P.az -= dz * bb;
     }
-    // (3) Newtonian cells: reduced multipoles to ORDER (grav.c:164-191,
-    // QEVAL qeval.h:21-64)
-    {
-        const int n = A.cnt[b3 + 2];
-        const int *E = A.ent + A.off[b3 + 2];
-        for (int base = 0; base < n; base += 64) {
-            const int m = min(64, n - base);
-            if (lane < m) {
-                const int ic = E[base + lane];
-                const double *c = A.cell + (size_t)CR_N * ic;
-                L.f[0][lane] = (T)(c[CR_X] - cx);
-                L.f[1][lane] = (T)(c[CR_X + 1] - cy);
-                L.f[2][lane] = (T)(c[CR_X + 2] - cz);
-                L.f[3][lane] = (T)c[CR_M];
-                for (int k = 0; k < 6; ++k) L.f[4 + k][lane] = (T)c[CR_QT + k];
-                if constexpr (ORDER >= 3) {
-                    const double *oh = A.momOH + (size_t)OH_N * ic;
-                    for (int k = 0; k < 10; ++k) L.f[10 + k][lane] = (T)oh[k];
-                    if constexpr (ORDER >= 4)
-                        for (int k = 0; k < 15; ++k) L.f[20 + k][lane] = (T)oh[10 + k];
-                }
-            }
-            __syncthreads();
-            if (act) {
-                Part<T> P;
-                for (int e = s; e < m; e += S) {
-                    const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
-                    const T mc = L.f[3][e];
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    T g[6];
-                    g[0] = rsq<T>(r2);
-                    const T dir2 = g[0] * g[0];
-                    g[1] = g[0] * dir2;
-                    g[2] = T(3) * g[1] * dir2;
-                    g[3] = T(5) * g[2] * dir2;
-                    g[4] = T(7) * g[3] * dir2;
-                    g[5] = T(9) * g[4] * dir2;
-                    T Qx, Qy, Qz, Qm, Qta = 0;
-                    if constexpr (ORDER >= 4) {
-                        const T *H = nullptr;
-                        (void)H;
-                        const T xxxx = L.f[20][e], xyyy = L.f[21][e], xxxy = L.f[22][e],
-                                yyyy = L.f[23][e], xxxz = L.f[24][e], yyyz = L.f[25][e],
-                                xxyy = L.f[26][e], xxyz = L.f[27][e], xyyz = L.f[28][e],
-                                xxzz = L.f[29][e], xyzz = L.f[30][e], xzzz = L.f[31][e],
-                                yyzz = L.f[32][e], yzzz = L.f[33][e], zzzz = L.f[34][e];
-                        Qx = T(1.0 / 6.0) * (xzzz * dz * dz * dz + T(3) * xyzz * dy * dz * dz
-                             + T(3) * xyyz * dy * dy * dz + xyyy * dy * dy * dy
-                             + T(3) * xxzz * dx * dz * dz + T(6) * xxyz * dx * dy * dz
-                             + T(3) * xxyy * dx * dy * dy + T(3) * xxxz * dx * dx * dz
-                             + T(3) * xxxy * dx * dx * dy + xxxx * dx * dx * dx);
-                        Qy = T(1.0 / 6.0) * (yzzz * dz * dz * dz + T(3) * xyzz * dx * dz * dz
-                             + T(3) * xxyz * dx * dx * dz + xxxy * dx * dx * dx
-                             + T(3) * yyzz * dy * dz * dz + T(6) * xyyz * dx * dy * dz
-                             + T(3) * xxyy * dx * dx * dy + T(3) * yyyz * dy * dy * dz
-                             + T(3) * xyyy * dx * dy * dy + yyyy * dy * dy * dy);
-                        Qz = T(1.0 / 6.0) * (yyyz * dy * dy * dy + T(3) * xyyz * dx * dy * dy
-                             + T(3) * xxyz * dx * dx * dy + xxxz * dx * dx * dx
-                             + T(3) * yyzz * dy * dy * dz + T(6) * xyzz * dx * dy * dz
-                             + T(3) * xxzz * dx * dx * dz + T(3) * yzzz * dy * dz * dz
-                             + T(3) * xzzz * dx * dz * dz + zzzz * dz * dz * dz);
-                        Qm = T(1.0 / 4.0) * (Qx * dx + Qy * dy + Qz * dz);
-                        P.pot -= g[4] * Qm;
-                        Qta += g[5] * Qm;
-                        P.ax += g[4] * Qx;
-                        P.ay += g[4] * Qy;
-                        P.az += g[4] * Qz;
-                    }
-                    if constexpr (ORDER >= 3) {
-                        const T xxx = L.f[10][e], xyy = L.f[11][e], xxy = L.f[12][e],
-                                yyy = L.f[13][e], xxz = L.f[14][e], yyz = L.f[15][e],
-                                xyz = L.f[16][e], xzz = L.f[17][e], yzz = L.f[18][e],
-                                zzz = L.f[19][e];
-                        Qx = T(0.5) * (xzz * dz * dz + T(2) * xyz * dy * dz + xyy * dy * dy
-                             + T(2) * xxz * dx * dz + T(2) * xxy * dx * dy + xxx * dx * dx);
-                        Qy = T(0.5) * (yzz * dz * dz + T(2) * xyz * dx * dz + xxy * dx * dx
-                             + T(2) * yyz * dy * dz + T(2) * xyy * dx * dy + yyy * dy * dy);
-                        Qz = T(0.5) * (yyz * dy * dy + T(2) * xyz * dx * dy + xxz * dx * dx
-                             + T(2) * yzz * dy * dz + T(2) * xzz * dx * dz + zzz * dz * dz);
-                        Qm = T(1.0 / 3.0) * (Qx * dx + Qy * dy + Qz * dz);
-                        P.pot -= g[3] * Qm;
-                        Qta += g[4] * Qm;
-                        P.ax += g[3] * Qx;
-                        P.ay += g[3] * Qy;
-                        P.az += g[3] * Qz;
-                    }
-                    if constexpr (ORDER >= 2) {
-                        const T xx = L.f[4][e], yy = L.f[5][e], zz = L.f[6][e];
-                        const T xy = L.f[7][e], xz = L.f[8][e], yz = L.f[9][e];
-                        Qx = xz * dz + xy * dy + xx * dx;
-                        Qy = yz * dz + xy * dx + yy * dy;
-                        Qz = yz * dy + xz * dx + zz * dz;
-                        Qm = T(0.5) * (Qx * dx + Qy * dy + Qz * dz);
-                        P.pot -= g[2] * Qm;
-                        Qta += g[3] * Qm;
-                        P.ax += g[2] * Qx;
-                        P.ay += g[2] * Qy;
-                        P.az += g[2] * Qz;
-                    }
-                    P.pot -= g[0] * mc;
-                    Qta += g[1] * mc;
-                    P.ax -= dx * Qta;
-                    P.ay -= dy * Qta;
-                    P.az -= dz * Qta;
-                    const T idt2 = (mi + mc) * g[1];
-                    dtg = idt2 > dtg ? idt2 : dtg;
-                }
-                SX.add(P.ax); SY.add(P.ay); SZ.add(P.az); SP.add(P.pot);
-            }
-            __syncthreads();
-        }
+    R.SX.add(P.ax); R.SY.add(P.ay); R.SZ.add(P.az); R.SP.add(P.pot);
+}
+
+// Softened cells: monopole + raw quadrupole (grav.c:117-150).
+template <typename T, int ORDER>
+__device__ __forceinline__ void eval_soft(const Stage<T, ORDER> &L, int m, const Tgt<T> &g,
+                                          Acc<T> &R)
+{
+    if (!g.act) return;
+    const T xi = g.xi, yi = g.yi, zi = g.zi, hi = g.hi, mi = g.mi;
+    const int s = g.s, S = g.S;
+    T &dtg = R.dtg;
+    Part<T> P;
+    for (int e = s; e < m; e += S) {
+        const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+        const T mc = L.f[3][e];
+        const T xx = L.f[5][e], yy = L.f[6][e], zz = L.f[7][e];
+        const T xy = L.f[8][e], xz = L.f[9][e], yz = L.f[10][e];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T a, bb, c, d;
+        spline_abcd<T>(r2, rsq<T>(r2), hi + L.f[4][e], a, bb, c, d);
+        const T qirx = xx * dx + xy * dy + xz * dz;
+        const T qiry = xy * dx + yy * dy + yz * dz;
+        const T qirz = xz * dx + yz * dy + zz * dz;
+        const T qir = T(0.5) * (qirx * dx + qiry * dy + qirz * dz);
+        const T tr = T(0.5) * (xx + yy + zz);
+        const T qir3 = bb * mc + d * qir - c * tr;
+        P.pot -= a * mc + c * qir - bb * tr;
+        P.ax -= qir3 * dx - c * qirx;
+        P.ay -= qir3 * dy - c * qiry;
+        P.az -= qir3 * dz - c * qirz;
+        const T idt2 = (mi + mc) * bb;
+        dtg = idt2 > dtg ? idt2 : dtg;
     }
-    // reduce the S slices of each target (lanes t, t+TPW, ...)
-    double ax = SX.value(), ay = SY.value(), az = SZ.value(), pot = SP.value();
-    double dg = (double)dtg;
+
+    R.SX.add(P.ax); R.SY.add(P.ay); R.SZ.add(P.az); R.SP.add(P.pot);
+}
+
+// Newtonian cells: reduced multipoles to ORDER (grav.c:164-191, QEVAL
+// qeval.h:21-64).
+template <typename T, int ORDER>
+__device__ __forceinline__ void eval_newt(const Stage<T, ORDER> &L, int m, const Tgt<T> &g,
+                                          Acc<T> &R)
+{
+    if (!g.act) return;
+    const T xi = g.xi, yi = g.yi, zi = g.zi, mi = g.mi;
+    const int s = g.s, S = g.S;
+    T &dtg = R.dtg;
+    Part<T> P;
+    for (int e = s; e < m; e += S) {
+        const T dx = xi - L.f[0][e], dy = yi - L.f[1][e], dz = zi - L.f[2][e];
+        const T mc = L.f[3][e];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T g[6];
+        g[0] = rsq<T>(r2);
+        const T dir2 = g[0] * g[0];
+        g[1] = g[0] * dir2;
+        g[2] = T(3) * g[1] * dir2;
+        g[3] = T(5) * g[2] * dir2;
+        g[4] = T(7) * g[3] * dir2;
+        g[5] = T(9) * g[4] * dir2;
+        T Qx, Qy, Qz, Qm, Qta = 0;
+        if constexpr (ORDER >= 4) {
+            const T *H = nullptr;
+            (void)H;
+            const T xxxx = L.f[20][e], xyyy = L.f[21][e], xxxy = L.f[22][e],
+                    yyyy = L.f[23][e], xxxz = L.f[24][e], yyyz = L.f[25][e],
+                    xxyy = L.f[26][e], xxyz = L.f[27][e], xyyz = L.f[28][e],
+                    xxzz = L.f[29][e], xyzz = L.f[30][e], xzzz = L.f[31][e],
+                    yyzz = L.f[32][e], yzzz = L.f[33][e], zzzz = L.f[34][e];
+            Qx = T(1.0 / 6.0) * (xzzz * dz * dz * dz + T(3) * xyzz * dy * dz * dz
+                 + T(3) * xyyz * dy * dy * dz + xyyy * dy * dy * dy
+                 + T(3) * xxzz * dx * dz * dz + T(6) * xxyz * dx * dy * dz
+                 + T(3) * xxyy * dx * dy * dy + T(3) * xxxz * dx * dx * dz
+                 + T(3) * xxxy * dx * dx * dy + xxxx * dx * dx * dx);
+            Qy = T(1.0 / 6.0) * (yzzz * dz * dz * dz + T(3) * xyzz * dx * dz * dz
+                 + T(3) * xxyz * dx * dx * dz + xxxy * dx * dx * dx
+                 + T(3) * yyzz * dy * dz * dz + T(6) * xyyz * dx * dy * dz
+                 + T(3) * xxyy * dx * dx * dy + T(3) * yyyz * dy * dy * dz
+                 + T(3) * xyyy * dx * dy * dy + yyyy * dy * dy * dy);
+            Qz = T(1.0 / 6.0) * (yyyz * dy * dy * dy + T(3) * xyyz * dx * dy * dy
+                 + T(3) * xxyz * dx * dx * dy + xxxz * dx * dx * dx
+                 + T(3) * yyzz * dy * dy * dz + T(6) * xyzz * dx * dy * dz
+                 + T(3) * xxzz * dx * dx * dz + T(3) * yzzz * dy * dz * dz
+                 + T(3) * xzzz * dx * dz * dz + zzzz * dz * dz * dz);
+            Qm = T(1.0 / 4.0) * (Qx * dx + Qy * dy + Qz * dz);
+            P.pot -= g[4] * Qm;
+            Qta += g[5] * Qm;
+            P.ax += g[4] * Qx;
+            P.ay += g[4] * Qy;
+            P.az += g[4] * Qz;
+        }
+        if constexpr (ORDER >= 3) {
+            const T xxx = L.f[10][e], xyy = L.f[11][e], xxy = L.f[12][e],
+                    yyy = L.f[13][e], xxz = L.f[14][e], yyz = L.f[15][e],
+                    xyz = L.f[16][e], xzz = L.f[17][e], yzz = L.f[18][e],
+                    zzz = L.f[19][e];
+            Qx = T(0.5) * (xzz * dz * dz + T(2) * xyz * dy * dz + xyy * dy * dy
+                 + T(2) * xxz * dx * dz + T(2) * xxy * dx * dy + xxx * dx * dx);
+            Qy = T(0.5) * (yzz * dz * dz + T(2) * xyz * dx * dz + xxy * dx * dx
+                 + T(2) * yyz * dy * dz + T(2) * xyy * dx * dy + yyy * dy * dy);
+            Qz = T(0.5) * (yyz * dy * dy + T(2) * xyz * dx * dy + xxz * dx * dx
+                 + T(2) * yzz * dy * dz + T(2) * xzz * dx * dz + zzz * dz * dz);
+            Qm = T(1.0 / 3.0) * (Qx * dx + Qy * dy + Qz * dz);
+            P.pot -= g[3] * Qm;
+            Qta += g[4] * Qm;
+            P.ax += g[3] * Qx;
+            P.ay += g[3] * Qy;
+            P.az += g[3] * Qz;
+        }
+        if constexpr (ORDER >= 2) {
+            const T xx = L.f[4][e], yy = L.f[5][e], zz = L.f[6][e];
+            const T xy = L.f[7][e], xz = L.f[8][e], yz = L.f[9][e];
+            Qx = xz * dz + xy * dy + xx * dx;
+            Qy = yz * dz + xy * dx + yy * dy;
+            Qz = yz * dy + xz * dx + zz * dz;
+            Qm = T(0.5) * (Qx * dx + Qy * dy + Qz * dz);
+            P.pot -= g[2] * Qm;
+            Qta += g[3] * Qm;
+            P.ax += g[2] * Qx;
+            P.ay += g[2] * Qy;
+            P.az += g[2] * Qz;
+        }
+        P.pot -= g[0] * mc;
+        Qta += g[1] * mc;
+        P.ax -= dx * Qta;
+        P.ay -= dy * Qta;
+        P.az -= dz * Qta;
+        const T idt2 = (mi + mc) * g[1];
+        dtg = idt2 > dtg ? idt2 : dtg;
+    }
+
+    R.SX.add(P.ax); R.SY.add(P.ay); R.SZ.add(P.az); R.SP.add(P.pot);
+}
+
+// Target set-up of a bucket's workgroup and its intra-bucket pairs.
+// Lane = (target t, slice s): TPW = next power of two >= bucket size
+// targets, S = 64/TPW slices that split each staged chunk.
+template <typename T, int ORDER>
+__device__ __forceinline__ void bucket_targets(const IArgs &A, const WalkNode &bn,
+                                               Stage<T, ORDER> &L, Tgt<T> &g, Acc<T> &R,
+                                               int &it, int &TPW)
+{
+    const int lane = threadIdx.x;
+    const int pL = bn.pLower, nT = bn.pUpper - bn.pLower + 1;
+    TPW = 1;
+    while (TPW < nT) TPW <<= 1;
+    g.S = 64 / TPW;
+    const int t = lane & (TPW - 1);
+    g.s = lane / TPW;
+    const bool valid = t < nT;
+    it = pL + (valid ? t : 0);
+    g.act = valid && (A.pact[it] & 1u);
+    const double cx = bn.cx, cy = bn.cy, cz = bn.cz;     // bucket centre of mass
+    g.xi = (T)(A.px[it] - cx);
+    g.yi = (T)(A.py[it] - cy);
+    g.zi = (T)(A.pz[it] - cz);
+    g.hi = (T)A.ph[it];
+    g.mi = (T)A.pm[it];
+    if (lane < nT) stage_pp(L, lane, pL + lane, A, cx, cy, cz);
+    __syncthreads();
+    eval_pp(L, nT, g, R, t);       // every pair i != j of the bucket
+    __syncthreads();
+}
+
+// Sum the S slices of each target and write the slot's results.
+template <typename T>
+__device__ __forceinline__ void bucket_store(const IArgs &A, const Tgt<T> &g, const Acc<T> &R,
+                                             int it, int TPW)
+{
+    double ax = R.SX.value(), ay = R.SY.value(), az = R.SZ.value(), pot = R.SP.value();
+    double dg = (double)R.dtg;
     for (int o = TPW; o < 64; o <<= 1) {
         ax += __shfl_xor(ax, o);
         ay += __shfl_xor(ay, o);
@@ -621,7 +644,7 @@ __global__ __launch_bounds__(64) void k_interact(IArgs A)
         const double od = __shfl_xor(dg, o);
         dg = od > dg ? od : dg;
     }
-    if (act && s == 0) {
+    if (g.act && g.s == 0) {
         double *o = A.acc + 5 * (size_t)it;
         o[0] = ax;
         o[1] = ay;
@@ -629,6 +652,202 @@ __global__ __launch_bounds__(64) void k_interact(IArgs A)
         o[3] = pot;
         o[4] = dg;
     }
+}
+
+// One 64-lane workgroup per active bucket over its three lists, in chunks
+// of 64 entries staged in LDS as cell-relative T.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void k_interact(IArgs A)
+{
+    __shared__ Stage<T, ORDER> L;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const WalkNode bn = A.wn[A.bucket[b]];
+    Tgt<T> g;
+    Acc<T> R;
+    int it, TPW;
+    bucket_targets(A, bn, L, g, R, it, TPW);
+    const size_t b3 = 3 * (size_t)b;
+    for (int kind = 0; kind < 3; ++kind) {
+        const int n = A.cnt[b3 + kind];
+        const int *E = A.ent + A.off[b3 + kind];
+        for (int base = 0; base < n; base += 64) {
+            const int m = min(64, n - base);
+            if (lane < m) {
+                const int v = E[base + lane];
+                if (kind == 0) stage_pp(L, lane, v, A, bn.cx, bn.cy, bn.cz);
+                else if (kind == 1) stage_soft(L, lane, v, A, bn.cx, bn.cy, bn.cz);
+                else stage_newt(L, lane, v, A, bn.cx, bn.cy, bn.cz);
+            }
+            __syncthreads();
+            if (kind == 0) eval_pp(L, m, g, R);
+            else if (kind == 1) eval_soft(L, m, g, R);
+            else eval_newt(L, m, g, R);
+            __syncthreads();
+        }
+    }
+    bucket_store(A, g, R, it, TPW);
+}
+
+// The walk and the interactions of one bucket in one 64-lane workgroup,
+// with no list in memory.  The walk is breadth-first with walk_decide's
+// decisions for up to 64 cells at a time; accepted cells and opened buckets
+// queue in LDS and are evaluated in chunks of 64 as soon as a chunk is full,
+// a PP chunk taking whole opened buckets (at most 64 particles).  Per-kind
+// counts go to cnt for the interaction counters.  A frontier that outgrows
+// WI_FC sets *err and leaves the bucket unevaluated (the caller then runs
+// the listed walk and k_interact).
+constexpr int WI_FC = 512;
+constexpr int WI_Q = 128;
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void flush_cells(Stage<T, ORDER> &L, int *q, int &nq, int m, int kind,
+                                            const IArgs &A, const WalkNode &bn, const Tgt<T> &g,
+                                            Acc<T> &R)
+{
+    const int lane = threadIdx.x;
+    if (lane < m) {
+        if (kind == 1) stage_soft(L, lane, q[lane], A, bn.cx, bn.cy, bn.cz);
+        else stage_newt(L, lane, q[lane], A, bn.cx, bn.cy, bn.cz);
+    }
+    __syncthreads();
+    if (kind == 1) eval_soft(L, m, g, R);
+    else eval_newt(L, m, g, R);
+    const int rest = nq - m;           // < 64 remain: move them to the front
+    const int v = lane < rest ? q[m + lane] : 0;
+    __syncthreads();
+    if (lane < rest) q[lane] = v;
+    __syncthreads();
+    nq = rest;
+}
+
+// One PP chunk: the longest run of pending buckets with at most 64 particles.
+template <typename T, int ORDER>
+__device__ __forceinline__ void flush_pp(Stage<T, ORDER> &L, int *qpl, int *qpn, int *pidx,
+                                         int &nr, int &np, const IArgs &A, const WalkNode &bn,
+                                         const Tgt<T> &g, Acc<T> &R)
+{
+    const int lane = threadIdx.x;
+    const int cntr = lane < nr ? qpn[lane] : 0;
+    int incl = cntr;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    const bool take = lane < nr && incl <= 64;
+    const int ntake = __popcll(__ballot(take));
+    const int m = __shfl(incl, ntake - 1);
+    if (take)
+        for (int j = 0; j < cntr; ++j) pidx[incl - cntr + j] = qpl[lane] + j;
+    __syncthreads();
+    if (lane < m) stage_pp(L, lane, pidx[lane], A, bn.cx, bn.cy, bn.cz);
+    __syncthreads();
+    eval_pp(L, m, g, R);
+    const int rest = nr - ntake;       // <= WI_Q - 1 ranges remain
+    int vl[2], vn[2];
+    for (int h = 0; h < 2; ++h) {
+        const int k = lane + 64 * h;
+        vl[h] = k < rest ? qpl[ntake + k] : 0;
+        vn[h] = k < rest ? qpn[ntake + k] : 0;
+    }
+    __syncthreads();
+    for (int h = 0; h < 2; ++h) {
+        const int k = lane + 64 * h;
+        if (k < rest) {
+            qpl[k] = vl[h];
+            qpn[k] = vn[h];
+        }
+    }
+    __syncthreads();
+    nr = rest;
+    np -= m;
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
+                                                      int iRoot, int *__restrict__ cnt,
+                                                      int *__restrict__ err)
+{
+    __shared__ Stage<T, ORDER> L;
+    __shared__ int fr[2][WI_FC];
+    __shared__ int qs[WI_Q], qn[WI_Q], qpl[WI_Q], qpn[WI_Q];
+    __shared__ int pidx[64];
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const int ib = A.bucket[b];
+    const WalkNode bn = A.wn[ib];
+    Tgt<T> g;
+    Acc<T> R;
+    int it, TPW;
+    bucket_targets(A, bn, L, g, R, it, TPW);
+    double bb[6];
+    for (int k = 0; k < 6; ++k) bb[k] = bbox[8 * (size_t)b + k];
+    const double fSoftMax = bbox[8 * (size_t)b + 6];
+    const unsigned long long below = (1ull << lane) - 1;
+    int nS = 0, nN = 0, nR = 0, nRp = 0;          // pending (wave-uniform)
+    int totP = 0, totS = 0, totN = 0;
+    int nf = 1, cur = 0;
+    if (lane == 0) fr[0][0] = iRoot;
+    __syncthreads();
+    while (nf > 0) {
+        int nn = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int k = base + lane;
+            int c = 0;
+            WalkDec dc{false, false, false, false, 0, 0, 0, 0};
+            if (k < nf) {
+                c = fr[cur][k];
+                dc = walk_decide(A.wn[c], c, ib, bb, fSoftMax);
+            }
+            const unsigned long long mk = __ballot(dc.kids);
+            if (dc.kids) {
+                const int p = nn + 2 * __popcll(mk & below);
+                if (p + 1 < WI_FC) {
+                    fr[cur ^ 1][p] = dc.cl;
+                    fr[cur ^ 1][p + 1] = dc.cu;
+                }
+            }
+            nn += 2 * __popcll(mk);
+            const unsigned long long ms = __ballot(dc.soft), mn = __ballot(dc.newt);
+            const unsigned long long mp = __ballot(dc.pp);
+            if (dc.soft) qs[nS + __popcll(ms & below)] = c;
+            if (dc.newt) qn[nN + __popcll(mn & below)] = c;
+            if (dc.pp) {
+                const int r = nR + __popcll(mp & below);
+                qpl[r] = dc.pl;
+                qpn[r] = dc.np;
+            }
+            int add = dc.np;
+            for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o);
+            nS += __popcll(ms);
+            nN += __popcll(mn);
+            nR += __popcll(mp);
+            nRp += add;
+            totS += __popcll(ms);
+            totN += __popcll(mn);
+            totP += add;
+            __syncthreads();
+            while (nS >= 64) flush_cells(L, qs, nS, 64, 1, A, bn, g, R);
+            while (nN >= 64) flush_cells(L, qn, nN, 64, 2, A, bn, g, R);
+            while (nRp >= 64) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, bn, g, R);
+        }
+        __syncthreads();
+        if (nn > WI_FC) {
+            if (lane == 0) atomicOr(err, 1);
+            return;
+        }
+        nf = nn;
+        cur ^= 1;
+    }
+    if (nS > 0) flush_cells(L, qs, nS, nS, 1, A, bn, g, R);
+    if (nN > 0) flush_cells(L, qn, nN, nN, 2, A, bn, g, R);
+    while (nR > 0) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, bn, g, R);
+    if (lane == 0) {
+        cnt[3 * (size_t)b] = totP;
+        cnt[3 * (size_t)b + 1] = totS;
+        cnt[3 * (size_t)b + 2] = totN;
+    }
+    bucket_store(A, g, R, it, TPW);
 }
 
 // Add the per-slot results to the caller-order fields of active particles.
@@ -757,6 +976,7 @@ int grav_upload(GGPKD pkd)
     G.iOrder = t.iOrder;
     G.nBucket = t.nBucket;
     G.built = 1;
+    G.listsCurrent = 0;
     return GG_OK;
 }
 
@@ -809,6 +1029,86 @@ extern "C" int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCr
     return grav_upload(pkd);
 }
 
+static IArgs make_iargs(GGPKD pkd, int iOrder)
+{
+    GravDev &G = pkd->grav;
+    IArgs A;
+    A.nb = G.nb;
+    A.iOrder = iOrder;
+    A.bucket = G.bucket.as<int>();
+    A.wn = G.wnode.as<WalkNode>();
+    A.cell = G.cell.as<double>();
+    A.momOH = G.momOH.as<double>();
+    A.px = G.px.as<double>();
+    A.py = G.py.as<double>();
+    A.pz = G.pz.as<double>();
+    A.pm = G.pm.as<double>();
+    A.ph = G.ph.as<double>();
+    A.pact = G.pact.as<uint32_t>();
+    A.cnt = G.cnt.as<int>();
+    A.off = G.off.as<int64_t>();
+    A.ent = G.ent.as<int>();
+    A.acc = G.acc.as<double>();
+    return A;
+}
+
+template <typename T>
+static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
+{
+    dim3 grid(A.nb), block(64);
+    const double *bb = G.bbox.as<double>();
+    int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
+    switch (order) {
+    case 1: hipLaunchKernelGGL((k_walk_interact<T, 1>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
+    case 2: hipLaunchKernelGGL((k_walk_interact<T, 2>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
+    case 3: hipLaunchKernelGGL((k_walk_interact<T, 3>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
+    case 4: hipLaunchKernelGGL((k_walk_interact<T, 4>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
+    default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
+    }
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// The listed walk (count, scan, fill) into G.cnt / G.off / G.ent: one
+// wavefront per bucket, or one thread per bucket if a breadth-first
+// frontier outgrows its buffer.
+static int walk_lists(GGPKD pkd)
+{
+    GravDev &G = pkd->grav;
+    hipStream_t st = pkd->stream;
+    const int nb = G.nb;
+    GG_TRY(G.werr.reserve(sizeof(int)));
+    GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb, G.bucket.as<int>(),
+                       G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.iRoot, 0, G.cnt.as<int>(),
+                       (const int64_t *)nullptr, (int *)nullptr, G.werr.as<int>());
+    GG_HIP(hipGetLastError());
+    int werr = 0;
+    GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    if (werr)
+        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
+    int64_t total = 0;
+    GG_TRY(exclusive_scan_i64(st, G.cnt.as<int>(), G.off.as<int64_t>(), 3 * nb, G.scan, &total));
+    G.nEnt = total;
+    GG_TRY(G.ent.reserve(sizeof(int) * std::max<int64_t>(total, 1)));
+    if (werr)
+        hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
+    else
+        hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>(),
+                           G.werr.as<int>());
+    GG_HIP(hipGetLastError());
+    G.serialWalk = werr;
+    G.listsCurrent = 1;
+    return GG_OK;
+}
+
 extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
                            int iEwOrder, double fEwCut, double fEwhCut, int bComove,
                            double dRhoFac, int bDoSun, double dSunSoft, double *aSun,
@@ -833,75 +1133,42 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
     unsigned long long hs[5] = {0, 0, 0, 0, 0};
     if (nb > 0) {
         timer_mark(pkd, 0);
-        // walk: count, scan, fill; one wavefront per bucket, or one thread
-        // per bucket if a breadth-first frontier outgrows its buffer
+        // walk and interactions fused, one workgroup per bucket; the listed
+        // walk and k_interact if a breadth-first frontier outgrew its buffer
+        const IArgs A = make_iargs(pkd, iOrder);
         GG_TRY(G.werr.reserve(sizeof(int)));
         GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
-                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr,
-                           G.werr.as<int>());
-        GG_HIP(hipGetLastError());
+        G.listsCurrent = 0;
+        if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_fused<float>(iOrder, A, G, st));
+        else GG_TRY(launch_fused<double>(iOrder, A, G, st));
         int werr = 0;
         GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
-        if (werr)
-            hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
-                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                               G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr,
-                               (int *)nullptr);
-        int64_t total = 0;
-        GG_TRY(exclusive_scan_i64(st, G.cnt.as<int>(), G.off.as<int64_t>(), 3 * nb, G.scan, &total));
-        G.nEnt = total;
-        GG_TRY(G.ent.reserve(sizeof(int) * std::max<int64_t>(total, 1)));
-        if (werr)
-            hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
-                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                               G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
-        else
-            hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
-                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                               G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>(),
-                               G.werr.as<int>());
-        G.serialWalk = werr;
-        GG_HIP(hipGetLastError());
+        G.fusedWalk = !werr;
+        if (werr) {
+            GG_TRY(walk_lists(pkd));
+            const IArgs B = make_iargs(pkd, iOrder);
+            if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_interact<float>(iOrder, B, st));
+            else GG_TRY(launch_interact<double>(iOrder, B, st));
+        }
+        timer_mark(pkd, 1);
         GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
         GG_HIP(hipMemsetAsync(G.stats.p, 0, sizeof(unsigned long long) * 5, st));
         hipLaunchKernelGGL(k_grav_stats, dim3(std::min((nb + 255) / 256, 1024)), dim3(256), 0, st,
                            nb, G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
                            G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
         GG_HIP(hipGetLastError());
-        timer_mark(pkd, 1);
-        IArgs A;
-        A.nb = nb;
-        A.iOrder = iOrder;
-        A.bucket = G.bucket.as<int>();
-        A.wn = G.wnode.as<WalkNode>();
-        A.cell = G.cell.as<double>();
-        A.momOH = G.momOH.as<double>();
-        A.px = G.px.as<double>();
-        A.py = G.py.as<double>();
-        A.pz = G.pz.as<double>();
-        A.pm = G.pm.as<double>();
-        A.ph = G.ph.as<double>();
-        A.pact = G.pact.as<uint32_t>();
-        A.cnt = G.cnt.as<int>();
-        A.off = G.off.as<int64_t>();
-        A.ent = G.ent.as<int>();
-        A.acc = G.acc.as<double>();
-        if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_interact<float>(iOrder, A, st));
-        else GG_TRY(launch_interact<double>(iOrder, A, st));
-        timer_mark(pkd, 2);
         hipLaunchKernelGGL(k_grav_scatter, dim3((G.n + 255) / 256), dim3(256), 0, st, G.n,
                            G.perm.as<int>(), G.pact.as<uint32_t>(), G.acc.as<double>(),
                            pkd->f[GG_A].as<double>(), pkd->f[GG_POT].as<double>(),
                            pkd->f[GG_DTGRAV].as<double>());
         GG_HIP(hipGetLastError());
-        timer_mark(pkd, 3);
+        timer_mark(pkd, 2);
         GG_HIP(hipMemcpyAsync(hs, G.stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
-        const int phase[3] = {0, 1, 8};
-        timer_collect(pkd, 0, 3, phase);
+        const int phase[2] = {1, 8};
+        timer_collect(pkd, 0, 2, phase);
+        pkd->tm.ms[0] = 0;     // the walk is inside the interaction kernel
     }
     if (nActive) *nActive = (int)hs[0];
     if (pdPartSum) *pdPartSum = (double)hs[1];
@@ -917,6 +1184,7 @@ extern "C" int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucke
     GG_CHECK(pkd, GG_EINVAL, "null handle");
     GravDev &G = pkd->grav;
     GG_CHECK(G.built, GG_ESTATE, "no gravity tree");
+    if (G.nb && !G.listsCurrent) GG_TRY(walk_lists(pkd));
     if (nBuckets) *nBuckets = G.nb;
     if (nEnt) *nEnt = G.nEnt;
     if (!ent) return GG_OK;

@@ -64,7 +64,9 @@ struct GravDev {
     DevBuf stats;          // reduction scratch
     DevBuf scan;           // scan scratch
     DevBuf werr;           // walk frontier overflow flag
-    int serialWalk = 0;    // the last walk fell back to one thread per bucket
+    int serialWalk = 0;    // the last listed walk fell back to one thread per bucket
+    int listsCurrent = 0;  // cnt/off/ent hold the lists of the current tree
+    int fusedWalk = 0;     // the last gg_grav_all walked inside the interaction kernel
     HostGravTree host;
 };
 

@@ -100,6 +100,29 @@ def test_multipole_orders_fp64(gpu, order):
     assert np.max(np.abs(pot / po - 1)) < 1e-7
 
 
+def test_small_opening_angle_uses_the_listed_walk(gpu):
+    # theta = 0.05 opens nearly every cell: breadth-first frontiers outgrow
+    # the fused kernel's buffer and the listed walk (and, deeper, the
+    # one-thread-per-bucket walk) takes over; results must not change
+    r, v, m = ics.plummer(8192, seed=12)
+    soft = np.full(len(m), 0.01)
+    T = O.Tree(r, m, soft, nBucket=8, dTheta=0.05, iOrder=2)
+    ao, po, _, sto = T.grav_all()
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64) as pk:
+        pk.build_binary(8, 1, 0.05, 2)
+        st = pk.grav_all(2)
+        a, pot = pk.get("a"), pk.get("fPot")
+        bk, off, cnt, ent = pk.grav_lists()
+    for key in ("dPartSum", "dCellSum", "dSoftSum"):
+        assert st[key] == sto[key], key
+    assert rel_err(a, ao).max() < 1e-7 and np.max(np.abs(pot / po - 1)) < 1e-7
+    for b in range(0, len(bk), 37):
+        part, scell, ncell = T.walk(bk[b])
+        for k, ref in enumerate((part, scell, ncell)):
+            got = ent[off[b, k]:off[b, k] + cnt[b, k]]
+            assert np.array_equal(np.sort(got), np.sort(ref))
+
+
 def test_softened_cells_and_partial_activity(gpu):
     # large softening puts cells on the softened list; a third of the
     # particles active: inactive ones must be left untouched
