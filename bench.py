@@ -10,8 +10,8 @@ step's smoothing lengths discarded so every kNN search starts cold.
 value = SPH pair evaluations per second (density pass + force pass, each
 pass counted as sum_p |N(p)|), FP32 pair arithmetic.  The same step in plain
 FP64 is timed beside it.  A gravity leg (Plummer sphere N = 2^20, theta 0.7,
-quadrupole, bucket 16; walk + interaction, tree build excluded) reports
-interactions/s in both precisions.  The CPU restatement of the same path is
+quadrupole, bucket 16; the fused walk + interaction kernel, tree build
+excluded) reports interactions/s in both precisions.  The CPU restatement of the same path is
 timed on a bounded sample on the host cores.
 
 Prints one JSON line.
@@ -29,7 +29,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grav interactions/s + SPH pair-evals/s at rms force err<=1e-4, vs plain FP64"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA)
+FP32_NONPACKED_TFLOPS = 78.6      # without packed FP32, which this build does not emit
 # Algorithmic bytes per unit of work (the data the operator must touch once):
 KNN_BYTES_PER_QUERY = 24 + 8     # + 4 K: position read, fBall2 and K list entries written
 DENS_BYTES_PER_PAIR = 64         # j side, slot-ordered FP64 arrays: r 24, m 8, vPred 24, uPred 8
@@ -103,7 +104,7 @@ def time_grav(gg, torch, n, prec, steps, warmup):
     return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter))
 
 
-def cpu_baseline_sph(n=1 << 17):
+def cpu_baseline_sph(n=1 << 20):
     """The oracle's SPH step (kNN + DenDVDX + EOS + SphPressureTermsSym) on a
     periodic box of n particles, pair-evals/s on the host cores."""
     from gasoline_amd import ics
@@ -122,7 +123,7 @@ def cpu_baseline_sph(n=1 << 17):
     return pairs / el, el
 
 
-def cpu_baseline_grav(n=32768):
+def cpu_baseline_grav(n=1 << 20):
     from gasoline_amd import ics
     from oracle import oracle as O
     r, v, m = ics.plummer(n, seed=1)
@@ -173,7 +174,7 @@ def main():
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream; the dominant one is reported
     kern = {
-        "k_knn_group": (ph32["knn"], n * (KNN_BYTES_PER_QUERY + 4 * K)),
+        "k_knn_bucket": (ph32["knn"], n * (KNN_BYTES_PER_QUERY + 4 * K)),
         "k_dendvdx<float>": (ph32["density"], n * K * DENS_BYTES_PER_PAIR),
         "k_hydro2<float>": (ph32["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
     }
@@ -211,7 +212,7 @@ def main():
                         "(tree build, kNN, EOS and reverse lists inside the step)" % n,
             "n": n, "nSmooth": K, "pairs_per_step": pairs,
             "gravity_workload": "Plummer N=%d, eps=0.01, theta=0.7, iOrder=2, nBucket=16; "
-                                "walk+interact (tree build excluded)" % args.ngrav,
+                                "fused walk+interact (tree build excluded)" % args.ngrav,
         },
         "fp64_value": pairs / t64,
         "speedup_vs_fp64": t64 / t32,
@@ -224,10 +225,13 @@ def main():
         "roofline": roof(dom),
         "rooflines_other": [roof(k) for k in kern if k != dom],
         "roofline_gravity": {
-            "bound": "valu", "kernel": "k_interact<float,2>",
+            "bound": "valu", "kernel": "k_walk_interact<float,2> (walk included)",
             "achieved": g32[2] / (g32[4] * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": g32[2] / (g32[4] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "frac_nonpacked": g32[2] / (g32[4] * 1e-3) / 1e12 / FP32_NONPACKED_TFLOPS,
+            "traffic": (traffic["k_walk_interact<float,2>"]["fetch_bytes"]
+                        + traffic["k_walk_interact<float,2>"]["write_bytes"])
+            if "k_walk_interact<float,2>" in traffic else None,
         },
     }
     if not args.no_cpu:
@@ -236,11 +240,12 @@ def main():
         v_gr, el_gr = cpu_baseline_grav()
         line["cpu_baseline"] = {
             "value": v_sph, "unit": "SPH pair-evals/s", "cores": cores, "kind": "port",
-            "sample": "oracle SPH step on a periodic box of 131072 particles (%.1f s): "
+            "sample": "oracle SPH step on the benchmark's box, 1048576 particles (%.1f s): "
                       "kNN and DenDVDX on %d OpenMP threads, the symmetric force pass "
                       "serial as in the reference" % (el_sph, cores),
             "grav_interactions_per_s": v_gr,
-            "grav_sample": "oracle gravity (1 thread) on Plummer N=32768 (%.2f s)" % el_gr,
+            "grav_sample": "oracle gravity (1 thread, walk + interactions) on Plummer "
+                           "N=1048576, theta 0.7, quadrupole, bucket 16 (%.1f s)" % el_gr,
         }
     print(json.dumps(line))
 
