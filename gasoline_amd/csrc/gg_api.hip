@@ -236,8 +236,10 @@ extern "C" int gg_finish(GGPKD pkd)
     GravDev &G = pkd->grav;
     DevBuf *gb[] = {&G.px, &G.py, &G.pz, &G.pm, &G.ph, &G.pact, &G.perm, &G.wnode, &G.cell,
                     &G.momOH, &G.bucket, &G.bbox, &G.cnt, &G.off, &G.ent, &G.acc, &G.stats,
-                    &G.scan, &G.werr};
+                    &G.scan, &G.werr, &G.broot};
     for (DevBuf *b : gb) b->release();
+    if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
     sph_release(pkd);
     pkd->mlp.release_all();
     if (pkd->tm.created)

@@ -46,7 +46,7 @@ __device__ __forceinline__ bool box_within(const double *bb, double fBall2,
 // Newtonian cells.  fill == 1: write them at the scanned offsets.
 __global__ void k_walk(int nb, const int *__restrict__ bucket,
                        const double *__restrict__ bbox,
-                       const WalkNode *__restrict__ wn, int iRoot, int fill,
+                       const WalkNode *__restrict__ wn, const int *__restrict__ broot, int fill,
                        int *__restrict__ cnt, const int64_t *__restrict__ off,
                        int *__restrict__ ent)
 {
@@ -64,7 +64,7 @@ __global__ void k_walk(int nb, const int *__restrict__ bucket,
         oS = off[3 * (size_t)b + 1];
         oN = off[3 * (size_t)b + 2];
     }
-    int ic = iRoot;
+    int ic = broot[b];
     while (ic != -1) {
         const WalkNode c = wn[ic];
         bool open = box_within(bb, c.fOpen2, c.cx, c.cy, c.cz);
@@ -151,7 +151,8 @@ __device__ __forceinline__ WalkDec walk_decide(const WalkNode &n, int c, int ib,
 
 __global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict__ bucket,
                                                   const double *__restrict__ bbox,
-                                                  const WalkNode *__restrict__ wn, int iRoot,
+                                                  const WalkNode *__restrict__ wn,
+                                                  const int *__restrict__ broot,
                                                   int fill, int *__restrict__ cnt,
                                                   const int64_t *__restrict__ off,
                                                   int *__restrict__ ent, int *__restrict__ err)
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict_
     }
     const unsigned long long below = (1ull << lane) - 1;
     int nP = 0, nS = 0, nN = 0, nf = 1, cur = 0;
-    if (lane == 0) fr[w][0][0] = iRoot;
+    if (lane == 0) fr[w][0][0] = broot[b];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     while (nf > 0) {
@@ -226,6 +227,14 @@ __global__ __launch_bounds__(256) void k_walk_bfs(int nb, const int *__restrict_
         cnt[3 * (size_t)b + 1] = nS;
         cnt[3 * (size_t)b + 2] = nN;
     }
+}
+
+// Zero n 32-bit words (a kernel, not a memset, so that it can sit in a
+// captured graph).
+__global__ void k_zero32(int n, uint32_t *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // Interaction counters of pkd.c:2946-2949 and the flop model of
@@ -765,7 +774,8 @@ __device__ __forceinline__ void flush_pp(Stage<T, ORDER> &L, int *qpl, int *qpn,
 
 template <typename T, int ORDER>
 __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
-                                                      int iRoot, int *__restrict__ cnt,
+                                                      const int *__restrict__ broot,
+                                                      int *__restrict__ cnt,
                                                       int *__restrict__ err)
 {
     __shared__ Stage<T, ORDER> L;
@@ -787,7 +797,7 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
     int nS = 0, nN = 0, nR = 0, nRp = 0;          // pending (wave-uniform)
     int totP = 0, totS = 0, totN = 0;
     int nf = 1, cur = 0;
-    if (lane == 0) fr[0][0] = iRoot;
+    if (lane == 0) fr[0][0] = broot[b];
     __syncthreads();
     while (nf > 0) {
         int nn = 0;
@@ -930,7 +940,7 @@ int grav_upload(GGPKD pkd)
     }
     // active buckets, their active-particle bounds and maximum softening
     // (pkd.c:2916-2937, walk.c:319-324)
-    std::vector<int> bk;
+    std::vector<int> bk, br;
     std::vector<double> bb;
     for (int c = 0; c < nn; ++c) {
         if (t.iDim[c] != -1 || t.pLower[c] > t.pUpper[c]) continue;
@@ -950,6 +960,7 @@ int grav_upload(GGPKD pkd)
         }
         if (na == 0) continue;
         bk.push_back(c);
+        br.push_back(G.nodeRoot.empty() ? t.iRoot : G.nodeRoot[c]);
         for (int d = 0; d < 3; ++d) bb.push_back(lo[d]);
         for (int d = 0; d < 3; ++d) bb.push_back(hi[d]);
         bb.push_back(fsm);
@@ -962,11 +973,13 @@ int grav_upload(GGPKD pkd)
                  "bucket %d holds %d particles (> 64): coincident positions?", c,
                  t.pUpper[c] - t.pLower[c] + 1);
     GG_TRY(G.bucket.reserve(sizeof(int) * std::max(G.nb, 1)));
+    GG_TRY(G.broot.reserve(sizeof(int) * std::max(G.nb, 1)));
     GG_TRY(G.bbox.reserve(sizeof(double) * 8 * std::max(G.nb, 1)));
     GG_TRY(G.cnt.reserve(sizeof(int) * 3 * std::max(G.nb, 1)));
     GG_TRY(G.off.reserve(sizeof(int64_t) * (3 * (size_t)std::max(G.nb, 1) + 1)));
     if (G.nb) {
         GG_HIP(hipMemcpyAsync(G.bucket.p, bk.data(), sizeof(int) * G.nb, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.broot.p, br.data(), sizeof(int) * G.nb, hipMemcpyHostToDevice, st));
         GG_HIP(hipMemcpyAsync(G.bbox.p, bb.data(), sizeof(double) * 8 * G.nb, hipMemcpyHostToDevice, st));
     }
     GG_HIP(hipStreamSynchronize(st));
@@ -977,6 +990,7 @@ int grav_upload(GGPKD pkd)
     G.nBucket = t.nBucket;
     G.built = 1;
     G.listsCurrent = 0;
+    ++G.treeId;
     return GG_OK;
 }
 
@@ -1026,6 +1040,42 @@ extern "C" int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCr
     }
     GG_TRY(build_binary_host(pkd->grav.host, n, r.data(), m.data(), h.data(), bits.data(),
                              nBucket, iOpenType, dCrit, iOrder));
+    pkd->grav.roots.assign(1, pkd->grav.host.iRoot);
+    pkd->grav.nodeRoot.assign((size_t)pkd->grav.host.nNodes, pkd->grav.host.iRoot);
+    return grav_upload(pkd);
+}
+
+extern "C" int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBucket,
+                               int iOpenType, double dCrit, int iOrder)
+{
+    GG_CHECK(pkd && sysOff, GG_EINVAL, "null argument");
+    GG_CHECK(nSys >= 1, GG_EINVAL, "nSys must be >= 1");
+    GG_CHECK(nBucket >= 1 && nBucket <= 64, GG_EINVAL, "nBucket %d outside 1..64", nBucket);
+    GG_CHECK(iOpenType == 0 || iOpenType == 1, GG_EUNSUP,
+             "iOpenType %d: only OPEN_JOSH (1) and Bmax (0) are implemented", iOpenType);
+    GG_CHECK(iOrder >= 1 && iOrder <= 4, GG_EINVAL, "iOrder %d outside 1..4", iOrder);
+    GG_CHECK(dCrit > 0, GG_EINVAL, "dCrit must be positive");
+    const int n = pkd->nLocal;
+    GG_CHECK(sysOff[0] == 0 && sysOff[nSys] == n, GG_EINVAL,
+             "system offsets must run from 0 to the %d particles", n);
+    for (int s = 0; s < nSys; ++s)
+        GG_CHECK(sysOff[s + 1] >= sysOff[s], GG_EINVAL, "system offsets decrease at %d", s);
+    GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
+             "positions, masses and softenings must be put before gg_build_forest");
+    GG_TRY(ensure_field(pkd, GG_IACTIVE));
+    std::vector<double> r(3 * (size_t)n), m(n), h(n);
+    std::vector<uint32_t> bits(n);
+    hipStream_t st = pkd->stream;
+    if (n > 0) {
+        GG_HIP(hipMemcpyAsync(r.data(), pkd->f[GG_R].p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(m.data(), pkd->f[GG_MASS].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(h.data(), pkd->f[GG_SOFT].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipMemcpyAsync(bits.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+    }
+    GravDev &G = pkd->grav;
+    GG_TRY(build_forest_host(G.host, G.roots, G.nodeRoot, nSys, sysOff, r.data(), m.data(),
+                             h.data(), bits.data(), nBucket, iOpenType, dCrit, iOrder));
     return grav_upload(pkd);
 }
 
@@ -1059,10 +1109,10 @@ static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
     const double *bb = G.bbox.as<double>();
     int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
     switch (order) {
-    case 1: hipLaunchKernelGGL((k_walk_interact<T, 1>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
-    case 2: hipLaunchKernelGGL((k_walk_interact<T, 2>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
-    case 3: hipLaunchKernelGGL((k_walk_interact<T, 3>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
-    case 4: hipLaunchKernelGGL((k_walk_interact<T, 4>), grid, block, 0, st, A, bb, G.iRoot, cnt, err); break;
+    case 1: hipLaunchKernelGGL((k_walk_interact<T, 1>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
+    case 2: hipLaunchKernelGGL((k_walk_interact<T, 2>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
+    case 3: hipLaunchKernelGGL((k_walk_interact<T, 3>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
+    case 4: hipLaunchKernelGGL((k_walk_interact<T, 4>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
     default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
     }
     GG_HIP(hipGetLastError());
@@ -1080,7 +1130,7 @@ static int walk_lists(GGPKD pkd)
     GG_TRY(G.werr.reserve(sizeof(int)));
     GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb, G.bucket.as<int>(),
-                       G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.iRoot, 0, G.cnt.as<int>(),
+                       G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.broot.as<int>(), 0, G.cnt.as<int>(),
                        (const int64_t *)nullptr, (int *)nullptr, G.werr.as<int>());
     GG_HIP(hipGetLastError());
     int werr = 0;
@@ -1088,20 +1138,17 @@ static int walk_lists(GGPKD pkd)
     GG_HIP(hipStreamSynchronize(st));
     if (werr)
         hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
-                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.broot.as<int>(), 0, G.cnt.as<int>(), (const int64_t *)nullptr, (int *)nullptr);
     int64_t total = 0;
     GG_TRY(exclusive_scan_i64(st, G.cnt.as<int>(), G.off.as<int64_t>(), 3 * nb, G.scan, &total));
     G.nEnt = total;
     GG_TRY(G.ent.reserve(sizeof(int) * std::max<int64_t>(total, 1)));
     if (werr)
         hipLaunchKernelGGL(k_walk, dim3((nb + 127) / 128), dim3(128), 0, st, nb,
-                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.broot.as<int>(), 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>());
     else
         hipLaunchKernelGGL(k_walk_bfs, dim3((nb + 3) / 4), dim3(256), 0, st, nb,
-                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
-                           G.iRoot, 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>(),
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(), G.broot.as<int>(), 1, G.cnt.as<int>(), G.off.as<int64_t>(), G.ent.as<int>(),
                            G.werr.as<int>());
     GG_HIP(hipGetLastError());
     G.serialWalk = werr;
@@ -1230,5 +1277,166 @@ extern "C" int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *
             o[6] = t.bmaxr[c];
             o[7] = t.fSplit[c];
         }
+    return GG_OK;
+}
+
+// ------------------------------------------------------- graph and tuner
+// The device work of one gg_grav_all (fused walk + interactions, counters,
+// scatter into a / fPot / dtGrav) as kernel launches only.
+static int grav_enqueue(GGPKD pkd, int iOrder)
+{
+    GravDev &G = pkd->grav;
+    hipStream_t st = pkd->stream;
+    const int nb = G.nb;
+    hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, 1, G.werr.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, 10, G.stats.as<uint32_t>());
+    const IArgs A = make_iargs(pkd, iOrder);
+    if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_fused<float>(iOrder, A, G, st));
+    else GG_TRY(launch_fused<double>(iOrder, A, G, st));
+    hipLaunchKernelGGL(k_grav_stats, dim3(std::min((nb + 255) / 256, 1024)), dim3(256), 0, st, nb,
+                       G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                       G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
+    hipLaunchKernelGGL(k_grav_scatter, dim3((G.n + 255) / 256), dim3(256), 0, st, G.n,
+                       G.perm.as<int>(), G.pact.as<uint32_t>(), G.acc.as<double>(),
+                       pkd->f[GG_A].as<double>(), pkd->f[GG_POT].as<double>(),
+                       pkd->f[GG_DTGRAV].as<double>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+static void graph_release(GravDev &G)
+{
+    if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.gexec = nullptr;
+    G.graph = nullptr;
+    G.graphOrder = 0;
+}
+
+extern "C" int gg_grav_graph(GGPKD pkd, int iOrder)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.built, GG_ESTATE, "gg_grav_graph before gg_build_binary / gg_build_forest");
+    GG_CHECK(G.n == pkd->nLocal, GG_ESTATE, "particle count changed since the tree was built");
+    GG_CHECK(iOrder >= 1 && iOrder <= G.iOrder, GG_EINVAL,
+             "iOrder %d exceeds the tree's moments (%d)", iOrder, G.iOrder);
+    GG_CHECK(G.nb > 0, GG_EINVAL, "no active bucket");
+    const int f[] = {GG_A, GG_POT, GG_DTGRAV};
+    for (int k : f) GG_TRY(ensure_field(pkd, k));
+    GG_TRY(G.werr.reserve(sizeof(int) * 11));
+    GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
+    graph_release(G);
+    hipStream_t st = pkd->stream;
+    {   // the fused walk must hold every frontier of this tree: try it once
+        hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, 1, G.werr.as<uint32_t>());
+        const IArgs A = make_iargs(pkd, iOrder);
+        if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_fused<float>(iOrder, A, G, st));
+        else GG_TRY(launch_fused<double>(iOrder, A, G, st));
+        int werr = 0;
+        GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        GG_CHECK(!werr, GG_EUNSUP,
+                 "this tree needs the listed walk (a frontier outgrows the fused kernel): "
+                 "use gg_grav_all");
+    }
+    GG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = grav_enqueue(pkd, iOrder);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    GG_CHECK(rc == GG_OK, rc, "capture of the gravity step failed: %s", gg_last_error());
+    GG_HIP(e);
+    G.graph = g;
+    GG_HIP(hipGraphInstantiate(&G.gexec, g, nullptr, nullptr, 0));
+    G.graphOrder = iOrder;
+    G.graphTree = G.treeId;
+    return GG_OK;
+}
+
+extern "C" int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double *pdCellSum,
+                              double *pdSoftSum, double *pdFlop)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.gexec && G.graphTree == G.treeId && G.built, GG_ESTATE,
+             "no graph for the current tree: gg_grav_graph after building");
+    hipStream_t st = pkd->stream;
+    GG_HIP(hipGraphLaunch(G.gexec, st));
+    int werr = 0;
+    unsigned long long hs[5];
+    GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipMemcpyAsync(hs, G.stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    G.listsCurrent = 0;
+    // gg_grav_graph checked that no frontier of this tree overflows
+    GG_CHECK(!werr, GG_EHIP, "a walk frontier overflowed inside the graph");
+    if (nActive) *nActive = (int)hs[0];
+    if (pdPartSum) *pdPartSum = (double)hs[1];
+    if (pdCellSum) *pdCellSum = (double)hs[2];
+    if (pdSoftSum) *pdSoftSum = (double)hs[3];
+    if (pdFlop) *pdFlop = (double)hs[4];
+    return GG_OK;
+}
+
+// Bucket-size autotuner: build with each candidate nBucket, time the device
+// work of gg_grav_all (second of two runs), keep the fastest.  The caller's
+// a, fPot and dtGrav are left as they were.
+extern "C" int gg_tune_bucket(GGPKD pkd, int nSys, const int *sysOff, int iOpenType,
+                              double dCrit, int iOrder, const int *cand, int nCand, double *ms,
+                              int *pBest)
+{
+    GG_CHECK(pkd && cand && nCand >= 1, GG_EINVAL, "null argument or no candidate");
+    const int n = pkd->nLocal;
+    const int f[] = {GG_A, GG_POT, GG_DTGRAV};
+    for (int k : f) GG_TRY(ensure_field(pkd, k));
+    hipStream_t st = pkd->stream;
+    DevBuf save;
+    GG_TRY(save.reserve(sizeof(double) * 5 * (size_t)std::max(n, 1)));
+    double *sv = save.as<double>();
+    int rc = GG_OK;
+    auto copy_fields = [&](bool out) -> int {
+        const size_t w3 = sizeof(double) * 3 * (size_t)n, w1 = sizeof(double) * (size_t)n;
+        void *fa = pkd->f[GG_A].p, *fp = pkd->f[GG_POT].p, *fd = pkd->f[GG_DTGRAV].p;
+        char *b = reinterpret_cast<char *>(sv);
+        if (out) {
+            GG_HIP(hipMemcpyAsync(b, fa, w3, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(b + w3, fp, w1, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(b + w3 + w1, fd, w1, hipMemcpyDeviceToDevice, st));
+        } else {
+            GG_HIP(hipMemcpyAsync(fa, b, w3, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(fp, b + w3, w1, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(fd, b + w3 + w1, w1, hipMemcpyDeviceToDevice, st));
+        }
+        GG_HIP(hipStreamSynchronize(st));
+        return GG_OK;
+    };
+    auto build = [&](int nb) -> int {
+        return sysOff ? gg_build_forest(pkd, nSys, sysOff, nb, iOpenType, dCrit, iOrder)
+                      : gg_build_binary(pkd, nb, iOpenType, dCrit, iOrder, 0, 1);
+    };
+    if (n > 0) rc = copy_fields(true);
+    int best = -1;
+    double bestMs = 0;
+    for (int k = 0; k < nCand && rc == GG_OK; ++k) {
+        rc = build(cand[k]);
+        for (int rep = 0; rep < 2 && rc == GG_OK; ++rep)
+            rc = gg_grav_all(pkd, 0, 0, iOrder, 0, 0, 0, 0, 0, 0, 0, 0, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr);
+        if (rc != GG_OK) break;
+        const double t = pkd->tm.ms[1] + pkd->tm.ms[8];
+        if (ms) ms[k] = t;
+        if (best < 0 || t < bestMs) {
+            best = k;
+            bestMs = t;
+        }
+    }
+    if (rc == GG_OK) rc = build(cand[best]);
+    if (n > 0) {
+        const int rc2 = copy_fields(false);
+        if (rc == GG_OK) rc = rc2;
+    }
+    save.release();
+    if (rc != GG_OK) return rc;
+    if (pBest) *pBest = cand[best];
     return GG_OK;
 }

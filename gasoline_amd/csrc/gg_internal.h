@@ -64,6 +64,12 @@ struct GravDev {
     DevBuf stats;          // reduction scratch
     DevBuf scan;           // scan scratch
     DevBuf werr;           // walk frontier overflow flag
+    DevBuf broot;          // per active bucket: the root of its tree
+    std::vector<int> roots, nodeRoot;   // per system / per node (host)
+    hipGraph_t graph = nullptr;         // captured step (gg_grav_graph)
+    hipGraphExec_t gexec = nullptr;
+    int graphOrder = 0;
+    long treeId = 0, graphTree = -1;
     int serialWalk = 0;    // the last listed walk fell back to one thread per bucket
     int listsCurrent = 0;  // cnt/off/ent hold the lists of the current tree
     int fusedWalk = 0;     // the last gg_grav_all walked inside the interaction kernel

@@ -316,4 +316,62 @@ int build_binary_host(HostGravTree &t, int n, const double *r,
     return 0;
 }
 
+int build_forest_host(HostGravTree &t, std::vector<int> &roots, std::vector<int> &nodeRoot,
+                      int nSys, const int *off, const double *r, const double *m,
+                      const double *soft, const uint32_t *iActive, int nBucket,
+                      int iOpenType, double dCrit, int iOrder)
+{
+    std::vector<HostGravTree> part((size_t)nSys);
+    std::vector<int> rc((size_t)nSys, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nSys; ++s) {
+        const int o = off[s], n = off[s + 1] - off[s];
+        rc[s] = build_binary_host(part[s], n, r + 3 * (size_t)o, m + o, soft + o,
+                                  iActive ? iActive + o : nullptr, nBucket, iOpenType, dCrit,
+                                  iOrder);
+    }
+    for (int s = 0; s < nSys; ++s)
+        if (rc[s]) return rc[s];
+    t = HostGravTree();
+    t.nBucket = nBucket;
+    t.iOrder = iOrder;
+    roots.assign((size_t)nSys, -1);
+    nodeRoot.clear();
+    int nb = 0, pb = 0;
+    for (int s = 0; s < nSys; ++s) {
+        const HostGravTree &q = part[s];
+        roots[s] = q.nNodes ? nb + q.iRoot : -1;
+        for (int i = 0; i < q.n; ++i) t.perm.push_back(q.perm[i] + off[s]);
+        t.x.insert(t.x.end(), q.x.begin(), q.x.end());
+        t.y.insert(t.y.end(), q.y.begin(), q.y.end());
+        t.z.insert(t.z.end(), q.z.begin(), q.z.end());
+        t.m.insert(t.m.end(), q.m.begin(), q.m.end());
+        t.h.insert(t.h.end(), q.h.begin(), q.h.end());
+        t.act.insert(t.act.end(), q.act.begin(), q.act.end());
+        for (int c = 0; c < q.nNodes; ++c) {
+            t.iDim.push_back(q.iDim[c]);
+            t.pLower.push_back(q.pLower[c] + pb);
+            t.pUpper.push_back(q.pUpper[c] + pb);
+            t.iLower.push_back(q.iLower[c] >= 0 ? q.iLower[c] + nb : -1);
+            t.iNext.push_back(q.iNext[c] >= 0 ? q.iNext[c] + nb : -1);
+            nodeRoot.push_back(roots[s]);
+        }
+        t.fSplit.insert(t.fSplit.end(), q.fSplit.begin(), q.fSplit.end());
+        t.bmin.insert(t.bmin.end(), q.bmin.begin(), q.bmin.end());
+        t.bmax.insert(t.bmax.end(), q.bmax.begin(), q.bmax.end());
+        t.mass.insert(t.mass.end(), q.mass.begin(), q.mass.end());
+        t.soft.insert(t.soft.end(), q.soft.begin(), q.soft.end());
+        t.com.insert(t.com.end(), q.com.begin(), q.com.end());
+        t.fOpen2.insert(t.fOpen2.end(), q.fOpen2.begin(), q.fOpen2.end());
+        t.mom.insert(t.mom.end(), q.mom.begin(), q.mom.end());
+        t.bmaxr.insert(t.bmaxr.end(), q.bmaxr.begin(), q.bmaxr.end());
+        nb += q.nNodes;
+        pb += q.n;
+    }
+    t.n = pb;
+    t.nNodes = nb;
+    t.iRoot = nSys ? roots[0] : -1;
+    return 0;
+}
+
 }  // namespace gg

@@ -28,4 +28,13 @@ int build_binary_host(HostGravTree &t, int n, const double *r,
                       const uint32_t *iActive, int nBucket, int iOpenType,
                       double dCrit, int iOrder);
 
+// Independent systems in one numbering: system s's particles are caller
+// indices [off[s], off[s+1]); one tree per system, built in parallel, then
+// concatenated (node and slot indices offset, each tree's threading ending in
+// -1).  roots[s] is system s's root; nodeRoot[c] the root of node c's tree.
+int build_forest_host(HostGravTree &t, std::vector<int> &roots, std::vector<int> &nodeRoot,
+                      int nSys, const int *off, const double *r, const double *m,
+                      const double *soft, const uint32_t *iActive, int nBucket,
+                      int iOpenType, double dCrit, int iOrder);
+
 }  // namespace gg

@@ -65,3 +65,78 @@ def evrard(n, seed=5, u=0.05):
     pos = rad[:, None] * _iso(rng, n)
     return dict(r=_f32(pos), v=np.zeros((n, 3)), m=np.full(n, 1.0 / n),
                 u=np.full(n, float(u)))
+
+
+def _king_profile(W0):
+    """King (1966) model of central potential W0 = Psi(0)/sigma^2: radius in
+    units of the King radius r0 = sqrt(9 sigma^2 / (4 pi G rho0)), W(r), and
+    the enclosed mass in units of 4 pi rho0 r0^3, out to the tidal radius."""
+    from scipy.integrate import solve_ivp
+    from scipy.special import erf
+
+    def rho(W):
+        W = np.maximum(W, 0.0)
+        return np.exp(W) * erf(np.sqrt(W)) - np.sqrt(4 * W / np.pi) * (1 + 2 * W / 3)
+
+    rho0 = rho(np.array(W0))
+
+    def rhs(r, y):
+        W, dW, m = y
+        d = rho(np.array(W)) / rho0
+        return [dW, -9.0 * d - (2.0 / r) * dW, d * r * r]
+
+    def tidal(r, y):
+        return y[0]
+    tidal.terminal = True
+    r0 = 1e-6
+    y0 = [W0 - 1.5 * r0 * r0, -3.0 * r0, r0 ** 3 / 3]   # series: W ~ W0 - 3/2 r^2
+    sol = solve_ivp(rhs, (r0, 1e4), y0, events=tidal, max_step=0.05, rtol=1e-9, atol=1e-12,
+                    dense_output=True)
+    rt = sol.t_events[0][0]
+    rr = np.concatenate([[0.0], np.linspace(r0, rt, 4000)])
+    Y = sol.sol(np.clip(rr, r0, rt))
+    W = np.maximum(Y[0], 0.0)
+    W[0] = W0
+    m = np.maximum(Y[2], 0.0)
+    m[0] = 0.0
+    return rr, W, m
+
+
+def king(n, W0, seed=0):
+    """One King model: n equal-mass particles, total mass M = 1, G = 1,
+    lengths in King radii (r0 = 1).  Radii by the inverse cumulative mass,
+    isotropic directions, speeds by rejection from v^2 (exp(W - v^2/2) - 1)
+    in units of sigma = sqrt(1 / (9 mu)), mu the dimensionless total mass."""
+    rng = np.random.default_rng(seed)
+    rr, W, m = _king_profile(float(W0))
+    mu = m[-1]
+    rad = np.interp(rng.uniform(0, mu, n), m, rr)
+    Wr = np.interp(rad, rr, W)
+    v = np.empty(n)
+    todo = np.arange(n)
+    while len(todo):      # rejection, vectorised over the particles still open
+        Wt = Wr[todo]
+        vmax = np.sqrt(2 * Wt)
+        fmax = vmax * vmax * (np.exp(Wt) - 1) + 1e-300
+        x = rng.uniform(0, 1, len(todo)) * vmax
+        ok = rng.uniform(0, 1, len(todo)) * fmax <= x * x * (np.exp(Wt - 0.5 * x * x) - 1)
+        v[todo[ok]] = x[ok]
+        todo = todo[~ok]
+    sigma = np.sqrt(1.0 / (9.0 * mu))
+    pos = rad[:, None] * _iso(rng, n)
+    vel = (v * sigma)[:, None] * _iso(rng, n)
+    return dict(r=_f32(pos), v=_f32(vel), m=np.full(n, 1.0 / n))
+
+
+def king_ensemble(nsys, seed=0, nmin=1024, nmax=65536, W0=(3.0, 9.0)):
+    """nsys King models with W0 uniform in [W0[0], W0[1]] and N log-uniform in
+    [nmin, nmax], stored one after the other; returns the concatenated
+    particles and the offsets of the systems."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys)).astype(int)
+    ws = rng.uniform(W0[0], W0[1], nsys)
+    parts = [king(int(ns[k]), ws[k], seed=seed * 1000 + k) for k in range(nsys)]
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int32)
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("r", "v", "m")}
+    cat["off"], cat["W0"] = off, ws
+    return cat

@@ -43,6 +43,7 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
+       "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_init_accel", "gg_kick", "gg_drift",
        "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
        "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
@@ -116,6 +117,11 @@ def lib():
     L.gg_smooth_lists.argtypes = [_vp, C.c_int, _ip]
     L.gg_timings.argtypes = [_vp, _dp, C.c_int]
     L.gg_smooth_stats.argtypes = [_vp, _i64p, C.c_int]
+    L.gg_build_forest.argtypes = [_vp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, C.c_int]
+    L.gg_grav_graph.argtypes = [_vp, C.c_int]
+    L.gg_grav_replay.argtypes = [_vp, _ip, _dp, _dp, _dp, _dp]
+    L.gg_tune_bucket.argtypes = [_vp, C.c_int, _ip, C.c_int, C.c_double, C.c_int, _ip, C.c_int,
+                                 _dp, _ip]
     L.gg_set_seed_slack.argtypes = [_vp, C.c_double]
     L.gg_init_accel.argtypes = [_vp]
     L.gg_kick.argtypes = [_vp] + [C.c_double] * 6 + [C.c_int]
@@ -204,6 +210,37 @@ class Pkd:
                                  C.byref(fl)))
         return dict(nActive=na.value, dPartSum=ps.value, dCellSum=cs.value,
                     dSoftSum=ss.value, dFlop=fl.value)
+
+    def build_forest(self, sysOff, nBucket=16, iOpenType=1, dCrit=0.7, iOrder=2):
+        off = np.ascontiguousarray(sysOff, np.int32)
+        _check(lib().gg_build_forest(self.h, len(off) - 1, _p(off, _ip), int(nBucket),
+                                     int(iOpenType), float(dCrit), int(iOrder)))
+
+    def grav_graph(self, iOrder=2):
+        _check(lib().gg_grav_graph(self.h, int(iOrder)))
+
+    def grav_replay(self):
+        na = C.c_int()
+        ps, cs, ss, fl = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        _check(lib().gg_grav_replay(self.h, C.byref(na), C.byref(ps), C.byref(cs),
+                                    C.byref(ss), C.byref(fl)))
+        return dict(nActive=na.value, dPartSum=ps.value, dCellSum=cs.value,
+                    dSoftSum=ss.value, dFlop=fl.value)
+
+    def tune_bucket(self, candidates=(8, 12, 16, 24, 32), sysOff=None, iOpenType=1, dCrit=0.7,
+                    iOrder=2):
+        cand = np.ascontiguousarray(candidates, np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int()
+        if sysOff is None:
+            off, ns = None, 0
+        else:
+            off = np.ascontiguousarray(sysOff, np.int32)
+            ns = len(off) - 1
+        _check(lib().gg_tune_bucket(self.h, ns, None if off is None else _p(off, _ip),
+                                    int(iOpenType), float(dCrit), int(iOrder), _p(cand, _ip),
+                                    len(cand), _p(ms), C.byref(best)))
+        return best.value, dict(zip(cand.tolist(), ms.tolist()))
 
     def grav_lists(self):
         nb = C.c_int()

@@ -229,6 +229,30 @@ int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr);
  * particles whose distance was computed.  n <= 10. */
 int gg_smooth_stats(GGPKD pkd, int64_t *s, int n);
 
+/* --------------------------------------------- ensembles, graphs, tuning */
+
+/* Many independent systems in one store: system s is caller indices
+ * [sysOff[s], sysOff[s+1]), sysOff[0] = 0, sysOff[nSys] = the particle count.
+ * One tree per system, as gg_build_binary would build it alone (no system
+ * sees another), walked and evaluated by gg_grav_all in one launch.  No
+ * counterpart in the reference, which runs such systems as separate jobs. */
+int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenType,
+                    double dCrit, int iOrder);
+/* Capture the device work of one gg_grav_all for the current tree in a HIP
+ * graph (kernel launches only), and replay it: gg_grav_replay adds to a,
+ * fPot and dtGrav exactly as gg_grav_all does and returns the same counters.
+ * A rebuild invalidates the graph.  GG_EUNSUP if the tree needs the listed
+ * walk (gg_grav_all handles it). */
+int gg_grav_graph(GGPKD pkd, int iOrder);
+int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double *pdCellSum,
+                   double *pdSoftSum, double *pdFlop);
+/* Bucket-size autotuner: builds the tree (sysOff = NULL) or the forest with
+ * each of the nCand candidate bucket sizes, times the device work of
+ * gg_grav_all, and leaves the handle built with the fastest (*pBest);
+ * ms[k] receives candidate k's time.  a, fPot and dtGrav are preserved. */
+int gg_tune_bucket(GGPKD pkd, int nSys, const int *sysOff, int iOpenType, double dCrit,
+                   int iOrder, const int *cand, int nCand, double *ms, int *pBest);
+
 /* ------------------------------------------- kick, drift, accel reset */
 
 /* pkdInitAccel (pkd.c:5443): a, fPot, dtGrav = 0 for TYPE_ACTIVE particles. */
