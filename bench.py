@@ -104,6 +104,39 @@ def time_grav(gg, torch, n, prec, steps, warmup):
     return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter))
 
 
+def time_ensemble(gg, torch, e, steps, prec):
+    """Batched ensemble: nsys King models (W0 in [3, 9], N log-uniform in
+    [1024, 65536]), eps = 0.005 King radii, theta 0.6, quadrupole; one tree
+    per system in one forest, bucket size by the autotuner, the gravity step
+    replayed from a HIP graph.  Returns times per step (eager and graph)."""
+    n = len(e["m"])
+    nsys = len(e["off"]) - 1
+    pk = gg.Pkd(n, prec=prec)
+    pk.put("r", e["r"])
+    pk.put("fMass", e["m"])
+    pk.put("fSoft", np.full(n, 0.005))
+    best, tune = pk.tune_bucket((8, 12, 16, 24, 32), sysOff=e["off"], dCrit=0.6)
+    st = pk.grav_all(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = pk.grav_all(2)
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / steps
+    pk.grav_graph(2)
+    pk.grav_replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = pk.grav_replay()
+    torch.cuda.synchronize()
+    graph = (time.perf_counter() - t0) / steps
+    pk.close()
+    nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
+    return dict(n=n, nsys=nsys, nBucket=best, tune_ms=tune, ms_eager=eager * 1e3,
+                ms_graph=graph * 1e3, interactions=nint, interactions_per_s=nint / graph)
+
+
 def cpu_baseline_sph(n=1 << 20):
     """The oracle's SPH step (kNN + DenDVDX + EOS + SphPressureTermsSym) on a
     periodic box of n particles, pair-evals/s on the host cores."""
@@ -143,6 +176,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--ngrav", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nsys", type=int, default=256, help="ensemble systems (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run five device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE) and exit")
@@ -171,6 +205,12 @@ def main():
     g32 = time_grav(gg, torch, args.ngrav, gg.PREC_FP32, args.steps, args.warmup)
     g64 = time_grav(gg, torch, args.ngrav, gg.PREC_FP64, max(2, args.steps // 2), 1)
 
+    ens32 = ens64 = None
+    if args.nsys > 0:
+        from gasoline_amd import ics
+        e = ics.king_ensemble(args.nsys, seed=7)
+        ens32 = time_ensemble(gg, torch, e, args.steps, gg.PREC_FP32)
+        ens64 = time_ensemble(gg, torch, e, max(2, args.steps // 2), gg.PREC_FP64)
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream; the dominant one is reported
     kern = {
@@ -234,6 +274,15 @@ def main():
             if "k_walk_interact<float,2>" in traffic else None,
         },
     }
+    if ens32:
+        line["ensemble"] = {
+            "workload": "%d King models (W0 in [3,9], N log-uniform in [1024,65536], %d "
+                        "particles), eps 0.005 r0, theta 0.6, quadrupole; one forest, one "
+                        "launch; bucket size autotuned; HIP-graph replay" % (ens32["nsys"],
+                                                                           ens32["n"]),
+            "fp32": ens32, "fp64": ens64,
+            "speedup_vs_fp64": ens64["ms_graph"] / ens32["ms_graph"],
+        }
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v_sph, el_sph = cpu_baseline_sph()
