@@ -166,11 +166,13 @@ int exclusive_scan_i64(hipStream_t st, const int *in, int64_t *out, int n, DevBu
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, st, nb, bsum, tot);
         hipLaunchKernelGGL(k_scan_add, dim3(nb + 1), dim3(SCAN_B), 0, st, n, bsum, out, tot);
         GG_HIP(hipGetLastError());
-        GG_HIP(hipMemcpyAsync(total_host, tot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
+        if (total_host) {
+            GG_HIP(hipMemcpyAsync(total_host, tot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            GG_HIP(hipStreamSynchronize(st));
+        }
     } else {
         GG_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), st));
-        *total_host = 0;
+        if (total_host) *total_host = 0;
     }
     return GG_OK;
 }
@@ -234,10 +236,7 @@ extern "C" int gg_finish(GGPKD pkd)
     (void)hipStreamSynchronize(pkd->stream);
     for (auto &b : pkd->f) b.release();
     GravDev &G = pkd->grav;
-    DevBuf *gb[] = {&G.px, &G.py, &G.pz, &G.pm, &G.ph, &G.pact, &G.perm, &G.wnode, &G.cell,
-                    &G.momOH, &G.bucket, &G.bbox, &G.cnt, &G.off, &G.ent, &G.acc, &G.stats,
-                    &G.scan, &G.werr, &G.broot};
-    for (DevBuf *b : gb) b->release();
+    G.release_all();
     if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
     if (G.graph) (void)hipGraphDestroy(G.graph);
     sph_release(pkd);

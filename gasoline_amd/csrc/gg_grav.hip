@@ -11,16 +11,6 @@
 
 namespace gg {
 
-struct WalkNode {        // 64 bytes: what the walk reads per visited cell
-    double cx, cy, cz, fOpen2, fSoft;
-    int pLower, pUpper, iLower, iNext;
-    int iUpper, pad1;       // second child (-1 for a bucket)
-};
-
-// Per node FP64 record read when a cell is on an interaction list.
-constexpr int CR_M = 0, CR_X = 1, CR_H = 4, CR_QR = 5, CR_QT = 11, CR_N = 20;
-constexpr int OH_N = 25;  // octupole (10) + hexadecapole (15)
-
 // ------------------------------------------------------------------ walk
 // INTERSECTNP (walk.h:12-30): squared distance from point to box <= fBall2.
 __device__ __forceinline__ bool box_within(const double *bb, double fBall2,
@@ -879,121 +869,6 @@ __global__ void k_grav_scatter(int n, const int *__restrict__ perm,
 }
 
 // ------------------------------------------------------------- host side
-int grav_upload(GGPKD pkd)
-{
-    GravDev &G = pkd->grav;
-    HostGravTree &t = G.host;
-    const int n = t.n, nn = t.nNodes;
-    hipStream_t st = pkd->stream;
-    GG_TRY(G.px.reserve(sizeof(double) * std::max(n, 1)));
-    GG_TRY(G.py.reserve(sizeof(double) * std::max(n, 1)));
-    GG_TRY(G.pz.reserve(sizeof(double) * std::max(n, 1)));
-    GG_TRY(G.pm.reserve(sizeof(double) * std::max(n, 1)));
-    GG_TRY(G.ph.reserve(sizeof(double) * std::max(n, 1)));
-    GG_TRY(G.pact.reserve(sizeof(uint32_t) * std::max(n, 1)));
-    GG_TRY(G.perm.reserve(sizeof(int) * std::max(n, 1)));
-    GG_TRY(G.acc.reserve(sizeof(double) * 5 * std::max(n, 1)));
-    GG_TRY(G.wnode.reserve(sizeof(WalkNode) * nn));
-    GG_TRY(G.cell.reserve(sizeof(double) * CR_N * nn));
-    if (n > 0) {
-        GG_HIP(hipMemcpyAsync(G.px.p, t.x.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.py.p, t.y.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.pz.p, t.z.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.pm.p, t.m.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.ph.p, t.h.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.pact.p, t.act.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.perm.p, t.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    }
-    std::vector<WalkNode> wn(nn);
-    std::vector<double> cell((size_t)CR_N * nn, 0.0);
-    std::vector<double> oh;
-    if (t.iOrder >= 3) oh.assign((size_t)OH_N * nn, 0.0);
-    for (int c = 0; c < nn; ++c) {
-        WalkNode &w = wn[c];
-        w.cx = t.com[3 * c];
-        w.cy = t.com[3 * c + 1];
-        w.cz = t.com[3 * c + 2];
-        w.fOpen2 = t.fOpen2[c];
-        w.fSoft = t.soft[c];
-        w.pLower = t.pLower[c];
-        w.pUpper = t.pUpper[c];
-        w.iLower = t.iLower[c];
-        w.iNext = t.iNext[c];
-        w.iUpper = t.iLower[c] >= 0 ? t.iNext[t.iLower[c]] : -1;
-        w.pad1 = 0;
-        double *r = &cell[(size_t)CR_N * c];
-        r[CR_M] = t.mass[c];
-        for (int d = 0; d < 3; ++d) r[CR_X + d] = t.com[3 * c + d];
-        r[CR_H] = t.soft[c];
-        for (int k = 0; k < 6; ++k) {
-            r[CR_QR + k] = t.mom[(size_t)MOM_N * c + MOM_QR + k];
-            r[CR_QT + k] = t.mom[(size_t)MOM_N * c + MOM_QT + k];
-        }
-        if (t.iOrder >= 3)
-            for (int k = 0; k < 25; ++k) oh[(size_t)OH_N * c + k] = t.mom[(size_t)MOM_N * c + MOM_O + k];
-    }
-    GG_HIP(hipMemcpyAsync(G.wnode.p, wn.data(), sizeof(WalkNode) * nn, hipMemcpyHostToDevice, st));
-    GG_HIP(hipMemcpyAsync(G.cell.p, cell.data(), sizeof(double) * CR_N * nn, hipMemcpyHostToDevice, st));
-    if (t.iOrder >= 3) {
-        GG_TRY(G.momOH.reserve(sizeof(double) * OH_N * nn));
-        GG_HIP(hipMemcpyAsync(G.momOH.p, oh.data(), sizeof(double) * OH_N * nn, hipMemcpyHostToDevice, st));
-    }
-    // active buckets, their active-particle bounds and maximum softening
-    // (pkd.c:2916-2937, walk.c:319-324)
-    std::vector<int> bk, br;
-    std::vector<double> bb;
-    for (int c = 0; c < nn; ++c) {
-        if (t.iDim[c] != -1 || t.pLower[c] > t.pUpper[c]) continue;
-        double lo[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
-        double hi[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-        int na = 0;
-        double fsm = 0.0;
-        for (int i = t.pLower[c]; i <= t.pUpper[c]; ++i) {
-            if (t.h[i] > fsm) fsm = t.h[i];
-            if (!(t.act[i] & 1u)) continue;
-            ++na;
-            const double r[3] = {t.x[i], t.y[i], t.z[i]};
-            for (int d = 0; d < 3; ++d) {
-                if (r[d] < lo[d]) lo[d] = r[d];
-                if (r[d] > hi[d]) hi[d] = r[d];
-            }
-        }
-        if (na == 0) continue;
-        bk.push_back(c);
-        br.push_back(G.nodeRoot.empty() ? t.iRoot : G.nodeRoot[c]);
-        for (int d = 0; d < 3; ++d) bb.push_back(lo[d]);
-        for (int d = 0; d < 3; ++d) bb.push_back(hi[d]);
-        bb.push_back(fsm);
-        bb.push_back((double)na);
-    }
-    G.nb = (int)bk.size();
-    // the interaction kernel maps one bucket onto one 64-lane wavefront
-    for (int c : bk)
-        GG_CHECK(t.pUpper[c] - t.pLower[c] + 1 <= 64, GG_EUNSUP,
-                 "bucket %d holds %d particles (> 64): coincident positions?", c,
-                 t.pUpper[c] - t.pLower[c] + 1);
-    GG_TRY(G.bucket.reserve(sizeof(int) * std::max(G.nb, 1)));
-    GG_TRY(G.broot.reserve(sizeof(int) * std::max(G.nb, 1)));
-    GG_TRY(G.bbox.reserve(sizeof(double) * 8 * std::max(G.nb, 1)));
-    GG_TRY(G.cnt.reserve(sizeof(int) * 3 * std::max(G.nb, 1)));
-    GG_TRY(G.off.reserve(sizeof(int64_t) * (3 * (size_t)std::max(G.nb, 1) + 1)));
-    if (G.nb) {
-        GG_HIP(hipMemcpyAsync(G.bucket.p, bk.data(), sizeof(int) * G.nb, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.broot.p, br.data(), sizeof(int) * G.nb, hipMemcpyHostToDevice, st));
-        GG_HIP(hipMemcpyAsync(G.bbox.p, bb.data(), sizeof(double) * 8 * G.nb, hipMemcpyHostToDevice, st));
-    }
-    GG_HIP(hipStreamSynchronize(st));
-    G.n = n;
-    G.nNodes = nn;
-    G.iRoot = t.iRoot;
-    G.iOrder = t.iOrder;
-    G.nBucket = t.nBucket;
-    G.built = 1;
-    G.listsCurrent = 0;
-    ++G.treeId;
-    return GG_OK;
-}
-
 template <typename T>
 static int launch_interact(int order, const IArgs &A, hipStream_t st)
 {
@@ -1024,25 +899,11 @@ extern "C" int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCr
     GG_CHECK(bGravity == 1, GG_EUNSUP, "gg_build_binary builds gravity trees only");
     GG_CHECK(bTreeActiveOnly == 0, GG_EUNSUP, "bTreeActiveOnly is not supported");
     GG_CHECK(dCrit > 0, GG_EINVAL, "dCrit must be positive");
-    const int n = pkd->nLocal;
     GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
              "positions, masses and softenings must be put before gg_build_binary");
     GG_TRY(ensure_field(pkd, GG_IACTIVE));
-    std::vector<double> r(3 * (size_t)n), m(n), h(n);
-    std::vector<uint32_t> bits(n);
-    hipStream_t st = pkd->stream;
-    if (n > 0) {
-        GG_HIP(hipMemcpyAsync(r.data(), pkd->f[GG_R].p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(m.data(), pkd->f[GG_MASS].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(h.data(), pkd->f[GG_SOFT].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(bits.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
-    }
-    GG_TRY(build_binary_host(pkd->grav.host, n, r.data(), m.data(), h.data(), bits.data(),
-                             nBucket, iOpenType, dCrit, iOrder));
-    pkd->grav.roots.assign(1, pkd->grav.host.iRoot);
-    pkd->grav.nodeRoot.assign((size_t)pkd->grav.host.nNodes, pkd->grav.host.iRoot);
-    return grav_upload(pkd);
+    const int off[2] = {0, pkd->nLocal};
+    return gtree_build(pkd, 1, off, nBucket, iOpenType, dCrit, iOrder);
 }
 
 extern "C" int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBucket,
@@ -1063,20 +924,7 @@ extern "C" int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBuck
     GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
              "positions, masses and softenings must be put before gg_build_forest");
     GG_TRY(ensure_field(pkd, GG_IACTIVE));
-    std::vector<double> r(3 * (size_t)n), m(n), h(n);
-    std::vector<uint32_t> bits(n);
-    hipStream_t st = pkd->stream;
-    if (n > 0) {
-        GG_HIP(hipMemcpyAsync(r.data(), pkd->f[GG_R].p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(m.data(), pkd->f[GG_MASS].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(h.data(), pkd->f[GG_SOFT].p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipMemcpyAsync(bits.data(), pkd->f[GG_IACTIVE].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
-    }
-    GravDev &G = pkd->grav;
-    GG_TRY(build_forest_host(G.host, G.roots, G.nodeRoot, nSys, sysOff, r.data(), m.data(),
-                             h.data(), bits.data(), nBucket, iOpenType, dCrit, iOrder));
-    return grav_upload(pkd);
+    return gtree_build(pkd, nSys, sysOff, nBucket, iOpenType, dCrit, iOrder);
 }
 
 static IArgs make_iargs(GGPKD pkd, int iOrder)
@@ -1252,31 +1100,18 @@ extern "C" int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *
     GG_CHECK(pkd, GG_EINVAL, "null handle");
     GravDev &G = pkd->grav;
     GG_CHECK(G.built, GG_ESTATE, "no gravity tree");
-    const HostGravTree &t = G.host;
-    if (nNodes) *nNodes = t.nNodes;
-    if (iRoot) *iRoot = t.iRoot;
-    if (perm) std::memcpy(perm, t.perm.data(), sizeof(int) * t.n);
+    hipStream_t st = pkd->stream;
+    if (nNodes) *nNodes = G.nNodes;
+    if (iRoot) *iRoot = G.iRoot;
+    if (perm && G.n)
+        GG_HIP(hipMemcpyAsync(perm, G.perm.p, sizeof(int) * G.n, hipMemcpyDeviceToHost, st));
     if (ints)
-        for (int c = 0; c < t.nNodes; ++c) {
-            ints[6 * c] = t.iDim[c];
-            ints[6 * c + 1] = t.pLower[c];
-            ints[6 * c + 2] = t.pUpper[c];
-            ints[6 * c + 3] = t.iLower[c];
-            ints[6 * c + 4] = t.iNext[c];
-            ints[6 * c + 5] = 0;
-        }
+        GG_HIP(hipMemcpyAsync(ints, G.tint.p, sizeof(int) * 6 * G.nNodes, hipMemcpyDeviceToHost,
+                              st));
     if (dbls)
-        for (int c = 0; c < t.nNodes; ++c) {
-            double *o = dbls + 8 * (size_t)c;
-            o[0] = t.mass[c];
-            o[1] = t.soft[c];
-            o[2] = t.com[3 * c];
-            o[3] = t.com[3 * c + 1];
-            o[4] = t.com[3 * c + 2];
-            o[5] = t.fOpen2[c];
-            o[6] = t.bmaxr[c];
-            o[7] = t.fSplit[c];
-        }
+        GG_HIP(hipMemcpyAsync(dbls, G.tdbl.p, sizeof(double) * 8 * G.nNodes,
+                              hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
     return GG_OK;
 }
 

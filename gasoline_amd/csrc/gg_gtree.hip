@@ -1,0 +1,845 @@
+// Gravity tree build on the device: the tree of gasoline's pkdBuildBinary
+// (pkd.c:2627-2723), node for node.
+//
+// Level by level, every cell of a level is split at the midpoint of the
+// longest axis of its particles' bounding box (pkd.c:2493-2505) unless it
+// holds nBucket particles or fewer or has no extent (pkd.c:2489).  The
+// partition reproduces pkdUpperPart's Hoare scan (pkd.c:1109-1157): a Hoare
+// scan swaps the k-th particle of the lower part that belongs above with the
+// k-th particle, counted from the top, of the upper part that belongs below,
+// so each particle's destination follows from prefix counts and the cells
+// of a level partition in parallel.  pkdActiveTypeOrder's partitions
+// (pkd.c:1304-1355) run the same way over each system.  Cells are numbered
+// in preorder afterwards, from subtree sizes.  Mass, softening and centre of
+// mass are summed as the reference sums them (buckets over their particles
+// in storage order, cells from their children, lower then upper), and
+// bounds, Bmax and fOpen2 are exact, so every quantity the walk decides on
+// is the reference's to the bit.  The multipole moments (pkdCalcCell,
+// pkd.c:2018-2135) are direct sums over each cell's particles in a fixed
+// but different order; they enter only the force arithmetic.
+//
+// Several independent systems build at once: each is a root of its own.
+#include "gg_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace gg {
+namespace {
+
+int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+constexpr unsigned long long MSB = 0x8000000000000000ull;
+constexpr int MOM_CH = 2048;     // particles per moment work item
+constexpr int NM = 32;           // per item: raw Q (6), O (10), H (15), Bmax
+
+// Monotone map of doubles to unsigned integers (for exact atomic min/max).
+__device__ __forceinline__ unsigned long long ord_enc(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u & MSB) ? ~u : (u | MSB);
+}
+__device__ __forceinline__ double ord_dec(unsigned long long e)
+{
+    const unsigned long long u = (e & MSB) ? (e & ~MSB) : ~e;
+    return __longlong_as_double((long long)u);
+}
+
+// Per cell in creation (breadth-first) order.
+struct GNode {
+    int lo, hi, par, side;       // particle range, parent, 0 lower / 1 upper
+    int ch0, ch1, dim, root;     // children (-1), split axis (-1: bucket), root cell
+    int s, size, pre, pad;       // split position, subtree size, preorder id
+    double split;
+};
+
+__global__ void k_gt_init(int n, const double *__restrict__ r, double *__restrict__ x,
+                          double *__restrict__ y, double *__restrict__ z, int *__restrict__ src)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = r[3 * (size_t)i];
+    y[i] = r[3 * (size_t)i + 1];
+    z[i] = r[3 * (size_t)i + 2];
+    src[i] = i;
+}
+
+// Flag of each position of a partitioning cell: 1 = belongs to the lower part.
+// mode 0: coordinate below the cell's split; mode 1: type bits & mask set.
+__global__ void k_gt_flag(int n, int a, int b, int mode, uint32_t mask,
+                          const int *__restrict__ pnode, const GNode *__restrict__ nd,
+                          const double *__restrict__ x, const double *__restrict__ y,
+                          const double *__restrict__ z, const int *__restrict__ src,
+                          const uint32_t *__restrict__ bits, int *__restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = pnode[i];
+    int f = 0;
+    if (c >= a && c < b) {
+        const GNode &g = nd[c];
+        if (mode == 1) {
+            f = (bits[src[i]] & mask) != 0;
+        } else if (g.dim >= 0) {
+            const double v = g.dim == 0 ? x[i] : (g.dim == 1 ? y[i] : z[i]);
+            f = v < g.split;
+        }
+    }
+    flag[i] = f;
+}
+
+// Split position s = lo + number of lower-part particles, children counts.
+// A split that leaves one side empty (the reference asserts against it,
+// pkd.c:2512-2516) sets *err.
+__global__ void k_gt_count(int a, int b, int mode, const int64_t *__restrict__ F,
+                           GNode *__restrict__ nd, int *__restrict__ nch, int *__restrict__ err)
+{
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    GNode &g = nd[c];
+    const bool part = mode == 1 || g.dim >= 0;
+    g.s = part ? g.lo + (int)(F[g.hi + 1] - F[g.lo]) : g.lo;
+    if (mode == 0) {
+        const bool split = g.dim >= 0;
+        if (split && (g.s == g.lo || g.s > g.hi)) atomicOr(err, 1);
+        nch[c - a] = split ? 2 : 0;
+    }
+}
+
+// Positions of the upper-part positions that hold a lower particle, by
+// rank counted down from hi: posR[lo + k].
+__global__ void k_gt_rank(int n, int a, int b, int mode, const int *__restrict__ pnode,
+                          const GNode *__restrict__ nd, const int *__restrict__ flag,
+                          const int64_t *__restrict__ F, int *__restrict__ posR)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = pnode[i];
+    if (c < a || c >= b) return;
+    const GNode &g = nd[c];
+    if (mode == 0 && g.dim < 0) return;
+    if (i >= g.s && flag[i]) {
+        const int k = (int)(F[g.hi + 1] - F[i + 1]);
+        posR[g.lo + k] = i;
+    }
+}
+
+// The swaps of the Hoare scan: the lower-part position holding the k-th
+// upper particle (counted up from lo) <-> posR[lo + k], each pair swapped
+// by its lower member.
+__global__ void k_gt_swap(int n, int a, int b, int mode, const int *__restrict__ pnode,
+                          const GNode *__restrict__ nd, const int *__restrict__ flag,
+                          const int64_t *__restrict__ F, const int *__restrict__ posR,
+                          double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                          int *__restrict__ src)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = pnode[i];
+    if (c < a || c >= b) return;
+    const GNode &g = nd[c];
+    if (mode == 0 && g.dim < 0) return;
+    if (i < g.s && !flag[i]) {
+        const int k = (i - g.lo) - (int)(F[i] - F[g.lo]);
+        const int j = posR[g.lo + k];
+        double t = x[i]; x[i] = x[j]; x[j] = t;
+        t = y[i]; y[i] = y[j]; y[j] = t;
+        t = z[i]; z[i] = z[j]; z[j] = t;
+        const int u = src[i]; src[i] = src[j]; src[j] = u;
+    }
+}
+
+__global__ void k_gt_boxinit(int a, int b, unsigned long long *__restrict__ box)
+{
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    for (int d = 0; d < 3; ++d) {
+        box[6 * (size_t)c + d] = ~0ull;      // min
+        box[6 * (size_t)c + 3 + d] = 0ull;   // max
+    }
+}
+
+// Bounding box of each cell of the level: exact min/max by atomics on the
+// order-preserving encodings, reduced per wavefront first.
+__global__ void k_gt_box(int n, int a, int b, const int *__restrict__ pnode,
+                         const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ z, unsigned long long *__restrict__ box)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = i < n ? pnode[i] : -1;
+    const bool in = c >= a && c < b;
+    unsigned long long lo[3], hi[3];
+    if (in) {
+        lo[0] = hi[0] = ord_enc(x[i]);
+        lo[1] = hi[1] = ord_enc(y[i]);
+        lo[2] = hi[2] = ord_enc(z[i]);
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = ~0ull;
+            hi[d] = 0ull;
+        }
+    }
+    const int c0 = __shfl(c, 0);
+    const bool same = __all(c == c0);
+    if (same) {
+        for (int o = 32; o > 0; o >>= 1)
+            for (int d = 0; d < 3; ++d) {
+                const unsigned long long l = __shfl_xor(lo[d], o), h = __shfl_xor(hi[d], o);
+                lo[d] = l < lo[d] ? l : lo[d];
+                hi[d] = h > hi[d] ? h : hi[d];
+            }
+        if ((threadIdx.x & 63) == 0 && c0 >= a && c0 < b)
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&box[6 * (size_t)c0 + d], lo[d]);
+                atomicMax(&box[6 * (size_t)c0 + 3 + d], hi[d]);
+            }
+    } else if (in) {
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&box[6 * (size_t)c + d], lo[d]);
+            atomicMax(&box[6 * (size_t)c + 3 + d], hi[d]);
+        }
+    }
+}
+
+// Split decision of each cell of the level (pkd.c:2489-2505).
+__global__ void k_gt_decide(int a, int b, int nBucket, const unsigned long long *__restrict__ box,
+                            GNode *__restrict__ nd)
+{
+#pragma clang fp contract(off)
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    GNode &g = nd[c];
+    double bmin[3], bmax[3];
+    bool extent = false;
+    for (int d = 0; d < 3; ++d) {
+        bmin[d] = ord_dec(box[6 * (size_t)c + d]);
+        bmax[d] = ord_dec(box[6 * (size_t)c + 3 + d]);
+        extent |= bmax[d] > bmin[d];
+    }
+    g.dim = -1;
+    g.split = 0.0;
+    g.ch0 = g.ch1 = -1;
+    if (g.hi - g.lo + 1 > nBucket && extent) {
+        int d = 0;
+        for (int k = 1; k < 3; ++k)
+            if (bmax[k] - bmin[k] > bmax[d] - bmin[d]) d = k;
+        g.dim = d;
+        g.split = 0.5 * (bmin[d] + bmax[d]);
+    }
+}
+
+// Children of the level's split cells, at next + their offsets.
+__global__ void k_gt_children(int a, int b, int next, const int64_t *__restrict__ choff,
+                              GNode *__restrict__ nd)
+{
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    GNode &g = nd[c];
+    if (g.dim < 0) return;
+    int k = next + (int)choff[c - a];
+    if (g.s > g.lo) {   // lower part [lo, s-1]
+        GNode &q = nd[k];
+        q.lo = g.lo;
+        q.hi = g.s - 1;
+        q.par = c;
+        q.side = 0;
+        q.root = g.root;
+        g.ch0 = k++;
+    }
+    if (g.s <= g.hi) {  // upper part [s, hi]
+        GNode &q = nd[k];
+        q.lo = g.s;
+        q.hi = g.hi;
+        q.par = c;
+        q.side = 1;
+        q.root = g.root;
+        g.ch1 = k;
+    }
+}
+
+__global__ void k_gt_descend(int n, int a, int b, const GNode *__restrict__ nd,
+                             int *__restrict__ pnode)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = pnode[i];
+    if (c < a || c >= b) return;
+    const GNode &g = nd[c];
+    pnode[i] = g.dim < 0 ? -1 : (i < g.s ? g.ch0 : g.ch1);
+}
+
+__global__ void k_gt_size(int a, int b, GNode *__restrict__ nd)
+{
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    GNode &g = nd[c];
+    g.size = 1 + (g.ch0 >= 0 ? nd[g.ch0].size : 0) + (g.ch1 >= 0 ? nd[g.ch1].size : 0);
+}
+
+__global__ void k_gt_pre(int a, int b, GNode *__restrict__ nd)
+{
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    const GNode &g = nd[c];
+    if (g.ch0 >= 0) nd[g.ch0].pre = g.pre + 1;
+    if (g.ch1 >= 0) nd[g.ch1].pre = g.pre + 1 + (g.ch0 >= 0 ? nd[g.ch0].size : 0);
+}
+
+// Storage-order particle fields: masses, softenings, type bits by source.
+__global__ void k_gt_gather(int n, const int *__restrict__ src, const double *__restrict__ m,
+                            const double *__restrict__ h, const uint32_t *__restrict__ bits,
+                            double *__restrict__ pm, double *__restrict__ ph,
+                            uint32_t *__restrict__ pact)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = src[i];
+    pm[i] = m[s];
+    ph[i] = h[s];
+    pact[i] = bits[s];
+}
+
+// Mass, mass-weighted softening and centre of mass (pkd.c:2510-2576):
+// buckets over their particles in storage order, cells from their
+// children, lower then upper.  Deepest level first.
+__global__ void k_gt_mass(int a, int b, const GNode *__restrict__ nd,
+                          const double *__restrict__ x, const double *__restrict__ y,
+                          const double *__restrict__ z, const double *__restrict__ pm,
+                          const double *__restrict__ ph, double *__restrict__ cm)
+{
+#pragma clang fp contract(off)
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    const GNode &g = nd[c];
+    double fm, M = 0.0, S = 0.0, R[3] = {0.0, 0.0, 0.0};
+    if (g.dim >= 0) {
+        for (int k = 0; k < 2; ++k) {
+            const int ch = k ? g.ch1 : g.ch0;
+            if (ch == -1) continue;
+            const double *q = cm + 5 * (size_t)ch;
+            fm = q[0];
+            M += fm;
+            S += fm * q[1];
+            for (int d = 0; d < 3; ++d) R[d] += fm * q[2 + d];
+        }
+    } else {
+        for (int i = g.lo; i <= g.hi; ++i) {
+            fm = pm[i];
+            M += fm;
+            S += fm * ph[i];
+            R[0] += fm * x[i];
+            R[1] += fm * y[i];
+            R[2] += fm * z[i];
+        }
+    }
+    if (M > 0) {
+        S /= M;
+        for (int d = 0; d < 3; ++d) R[d] /= M;
+    }
+    double *o = cm + 5 * (size_t)c;
+    o[0] = M;
+    o[1] = S;
+    for (int d = 0; d < 3; ++d) o[2 + d] = R[d];
+}
+
+// Work items of the moment sums: item k covers MOM_CH particles of a cell.
+__global__ void k_gt_items(int nn, const GNode *__restrict__ nd, int *__restrict__ nit)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn) return;
+    const int cnt = nd[c].hi - nd[c].lo + 1;
+    nit[c] = cnt > 0 ? (cnt + MOM_CH - 1) / MOM_CH : 0;
+}
+
+__global__ void k_gt_itemcell(int nn, const int64_t *__restrict__ ioff, int *__restrict__ icell)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn) return;
+    for (int64_t k = ioff[c]; k < ioff[c + 1]; ++k) icell[k] = c;
+}
+
+// Partial moments of one item (one wavefront): raw Q (6), reduced O (10)
+// and H (15) about the cell's centre of mass, and the largest distance.
+// Per-lane sums, then a fixed butterfly: the result does not depend on
+// scheduling.
+__global__ __launch_bounds__(256) void k_gt_moments(int nitem, int iOrder,
+                                                    const int *__restrict__ icell,
+                                                    const int64_t *__restrict__ ioff,
+                                                    const GNode *__restrict__ nd,
+                                                    const double *__restrict__ cm,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ y,
+                                                    const double *__restrict__ z,
+                                                    const double *__restrict__ pm,
+                                                    double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (it >= nitem) return;
+    const int c = icell[it];
+    const GNode &g = nd[c];
+    const int first = g.lo + (int)(it - ioff[c]) * MOM_CH;
+    const int last = min(g.hi, first + MOM_CH - 1);
+    const double cx = cm[5 * (size_t)c + 2], cy = cm[5 * (size_t)c + 3], cz = cm[5 * (size_t)c + 4];
+    const bool wantO = iOrder >= 3, wantH = iOrder >= 4;
+    double Q[6] = {0, 0, 0, 0, 0, 0}, O[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double H[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double Bmax = 0.0;
+    for (int i = first + lane; i <= last; i += 64) {
+        const double mm = pm[i];
+        const double dx = x[i] - cx, dy = y[i] - cy, dz = z[i] - cz;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double d1 = sqrt(d2);
+        if (d1 > Bmax) Bmax = d1;
+        if (wantH) {
+            H[0] += mm * (dx * dx * dx * dx - 6.0 / 7.0 * d2 * (dx * dx - 0.1 * d2));
+            H[1] += mm * (dx * dy * dy * dy - 3.0 / 7.0 * d2 * dx * dy);
+            H[2] += mm * (dx * dx * dx * dy - 3.0 / 7.0 * d2 * dx * dy);
+            H[3] += mm * (dy * dy * dy * dy - 6.0 / 7.0 * d2 * (dy * dy - 0.1 * d2));
+            H[4] += mm * (dx * dx * dx * dz - 3.0 / 7.0 * d2 * dx * dz);
+            H[5] += mm * (dy * dy * dy * dz - 3.0 / 7.0 * d2 * dy * dz);
+            H[6] += mm * (dx * dx * dy * dy - 1.0 / 7.0 * d2 * (dx * dx + dy * dy - 0.2 * d2));
+            H[7] += mm * (dx * dx * dy * dz - 1.0 / 7.0 * d2 * dy * dz);
+            H[8] += mm * (dx * dy * dy * dz - 1.0 / 7.0 * d2 * dx * dz);
+            H[9] += mm * (dx * dx * dz * dz - 1.0 / 7.0 * d2 * (dx * dx + dz * dz - 0.2 * d2));
+            H[10] += mm * (dx * dy * dz * dz - 1.0 / 7.0 * d2 * dx * dy);
+            H[11] += mm * (dx * dz * dz * dz - 3.0 / 7.0 * d2 * dx * dz);
+            H[12] += mm * (dy * dy * dz * dz - 1.0 / 7.0 * d2 * (dy * dy + dz * dz - 0.2 * d2));
+            H[13] += mm * (dy * dz * dz * dz - 3.0 / 7.0 * d2 * dy * dz);
+            H[14] += mm * (dz * dz * dz * dz - 6.0 / 7.0 * d2 * (dz * dz - 0.1 * d2));
+        }
+        if (wantO) {
+            O[0] += mm * (dx * dx * dx - 0.6 * d2 * dx);
+            O[1] += mm * (dx * dy * dy - 0.2 * d2 * dx);
+            O[2] += mm * (dx * dx * dy - 0.2 * d2 * dy);
+            O[3] += mm * (dy * dy * dy - 0.6 * d2 * dy);
+            O[4] += mm * (dx * dx * dz - 0.2 * d2 * dz);
+            O[5] += mm * (dy * dy * dz - 0.2 * d2 * dz);
+            O[6] += mm * dx * dy * dz;
+            O[7] += mm * (dx * dz * dz - 0.2 * d2 * dx);
+            O[8] += mm * (dy * dz * dz - 0.2 * d2 * dy);
+            O[9] += mm * (dz * dz * dz - 0.6 * d2 * dz);
+        }
+        Q[0] += mm * dx * dx;
+        Q[1] += mm * dy * dy;
+        Q[2] += mm * dz * dz;
+        Q[3] += mm * dx * dy;
+        Q[4] += mm * dx * dz;
+        Q[5] += mm * dy * dz;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 6; ++k) Q[k] += __shfl_xor(Q[k], o);
+        if (wantO)
+            for (int k = 0; k < 10; ++k) O[k] += __shfl_xor(O[k], o);
+        if (wantH)
+            for (int k = 0; k < 15; ++k) H[k] += __shfl_xor(H[k], o);
+        Bmax = fmax(Bmax, __shfl_xor(Bmax, o));
+    }
+    if (lane == 0) {
+        double *p = part + (size_t)NM * it;
+        for (int k = 0; k < 6; ++k) p[k] = Q[k];
+        for (int k = 0; k < 10; ++k) p[6 + k] = O[k];
+        for (int k = 0; k < 15; ++k) p[16 + k] = H[k];
+        p[31] = Bmax;
+    }
+}
+
+// Per cell in preorder: the parity record (iDim, pLower, pUpper, iLower,
+// iNext, 0; mass, softening, c.o.m., fOpen2, Bmax, fSplit), the walk record,
+// the cell record and O/H: moments summed over the cell's items in order,
+// traceless Q as the walk stores it (walk.c:41-48), fOpen2 by pkdCalcOpen
+// (pkd.c:2228-2264).
+__global__ void k_gt_emit(int nn, int iOrder, int iOpenType, double dCrit,
+                          const GNode *__restrict__ nd, const double *__restrict__ cm,
+                          const int64_t *__restrict__ ioff, const double *__restrict__ part,
+                          int *__restrict__ tint, double *__restrict__ tdbl,
+                          WalkNode *__restrict__ wn, double *__restrict__ cell,
+                          double *__restrict__ momOH)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nn) return;
+    const GNode &g = nd[b];
+    const int c = g.pre;
+    const double *q = cm + 5 * (size_t)b;
+    double M[31];
+    for (int k = 0; k < 31; ++k) M[k] = 0.0;
+    double Bmax = 0.0;
+    for (int64_t it = ioff[b]; it < ioff[b + 1]; ++it) {
+        const double *p = part + (size_t)NM * it;
+        for (int k = 0; k < 31; ++k) M[k] += p[k];
+        Bmax = fmax(Bmax, p[31]);
+    }
+    double dOpen;
+    if (iOpenType == 1) {
+        dOpen = 2 / sqrt(3.0) * Bmax / dCrit;
+        if (dOpen < Bmax) dOpen = Bmax;
+    } else {
+        dOpen = Bmax;
+    }
+    const double fOpen2 = g.hi >= g.lo ? dOpen * dOpen : 0.0;
+    const GNode &rt = nd[g.root];
+    const int end = rt.pre + rt.size;
+    const int first = g.ch0 >= 0 ? nd[g.ch0].pre : -1;
+    const int second = g.ch1 >= 0 ? nd[g.ch1].pre : -1;
+    const int next = c + g.size < end ? c + g.size : -1;
+    int *ti = tint + 6 * (size_t)c;   // iLower: the first child is c + 1 in preorder
+    ti[0] = g.dim;
+    ti[1] = g.lo;
+    ti[2] = g.hi;
+    ti[3] = g.dim >= 0 ? c + 1 : -1;
+    ti[4] = next;
+    ti[5] = 0;
+    double *td = tdbl + 8 * (size_t)c;
+    for (int k = 0; k < 5; ++k) td[k] = q[k];
+    td[5] = fOpen2;
+    td[6] = Bmax;
+    td[7] = g.split;
+    WalkNode w;
+    w.cx = q[2];
+    w.cy = q[3];
+    w.cz = q[4];
+    w.fOpen2 = fOpen2;
+    w.fSoft = q[1];
+    w.pLower = g.lo;
+    w.pUpper = g.hi;
+    w.iLower = first;
+    w.iNext = next;
+    w.iUpper = second;
+    w.pad1 = 0;
+    wn[c] = w;
+    double *r = cell + (size_t)CR_N * c;
+    r[CR_M] = q[0];
+    for (int d = 0; d < 3; ++d) r[CR_X + d] = q[2 + d];
+    r[CR_H] = q[1];
+    for (int k = 0; k < 6; ++k) r[CR_QR + k] = M[k];
+    const double tr = M[0] + M[1] + M[2];
+    r[CR_QT] = M[0] - tr / 3.0;
+    r[CR_QT + 1] = M[1] - tr / 3.0;
+    r[CR_QT + 2] = M[2] - tr / 3.0;
+    r[CR_QT + 3] = M[3];
+    r[CR_QT + 4] = M[4];
+    r[CR_QT + 5] = M[5];
+    if (iOrder >= 3)
+        for (int k = 0; k < 25; ++k) momOH[(size_t)OH_N * c + k] = M[6 + k];
+}
+
+// Buckets with an active particle: flag by preorder id; bounds of the
+// active particles, largest softening, active count (pkd.c:2916-2937,
+// walk.c:319-324); the largest bucket (the interaction maps a bucket onto
+// one wavefront).
+__global__ void k_gt_bucket(int nn, const GNode *__restrict__ nd, const double *__restrict__ x,
+                            const double *__restrict__ y, const double *__restrict__ z,
+                            const double *__restrict__ ph, const uint32_t *__restrict__ pact,
+                            int *__restrict__ flag, double *__restrict__ bb8,
+                            int *__restrict__ maxb)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nn) return;
+    const GNode &g = nd[b];
+    const int c = g.pre;
+    if (g.dim >= 0 || g.lo > g.hi) {
+        flag[c] = 0;
+        return;
+    }
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    int na = 0;
+    double fsm = 0.0;
+    for (int i = g.lo; i <= g.hi; ++i) {
+        if (ph[i] > fsm) fsm = ph[i];
+        if (!(pact[i] & 1u)) continue;
+        ++na;
+        const double r[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; ++d) {
+            if (r[d] < lo[d]) lo[d] = r[d];
+            if (r[d] > hi[d]) hi[d] = r[d];
+        }
+    }
+    flag[c] = na > 0;
+    if (na > 0) atomicMax(maxb, g.hi - g.lo + 1);
+    double *o = bb8 + 8 * (size_t)c;
+    for (int d = 0; d < 3; ++d) {
+        o[d] = lo[d];
+        o[3 + d] = hi[d];
+    }
+    o[6] = fsm;
+    o[7] = (double)na;
+}
+
+__global__ void k_gt_bucket_list(int nn, const int *__restrict__ flag,
+                                 const int64_t *__restrict__ pos, const double *__restrict__ bb8,
+                                 const int *__restrict__ rootOf, int *__restrict__ bucket,
+                                 double *__restrict__ bbox, int *__restrict__ broot)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn || !flag[c]) return;
+    const int k = (int)pos[c];
+    bucket[k] = c;
+    for (int d = 0; d < 8; ++d) bbox[8 * (size_t)k + d] = bb8[8 * (size_t)c + d];
+    broot[k] = rootOf[c];
+}
+
+__global__ void k_gt_rootof(int nn, const GNode *__restrict__ nd, int *__restrict__ rootOf)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nn) return;
+    rootOf[nd[b].pre] = nd[nd[b].root].pre;
+}
+
+__global__ void k_gt_pos2root(int n, int nr, const int *__restrict__ rlo,
+                              int *__restrict__ pos2root)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nr - 1;   // last root whose first position <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rlo[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    pos2root[i] = lo;
+}
+
+__global__ void k_gt_roots(int nr, const int *__restrict__ rlo, const int *__restrict__ rhi,
+                           GNode *__restrict__ nd)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    GNode &g = nd[k];
+    g.lo = rlo[k];
+    g.hi = rhi[k];
+    g.par = -1;
+    g.side = -1;
+    g.root = k;
+    g.ch0 = g.ch1 = -1;
+    g.dim = 0;      // partitioned by type first
+}
+
+}  // namespace
+
+// One partition pass over the cells [a, b) (mode 0: their splits; mode 1:
+// type bits & mask, every cell).
+static int gt_partition(GGPKD pkd, int a, int b, int mode, uint32_t mask)
+{
+    GravDev &G = pkd->grav;
+    hipStream_t st = pkd->stream;
+    const int n = pkd->nLocal;
+    GNode *nd = G.gnode.as<GNode>();
+    hipLaunchKernelGGL(k_gt_flag, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode, mask,
+                       G.gpn.as<int>(), nd, G.px.as<double>(), G.py.as<double>(),
+                       G.pz.as<double>(), G.perm.as<int>(),
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), G.gflag.as<int>());
+    GG_TRY(exclusive_scan_i64(st, G.gflag.as<int>(), G.gF.as<int64_t>(), n, G.scan, nullptr));
+    hipLaunchKernelGGL(k_gt_count, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, mode,
+                       G.gF.as<int64_t>(), nd, G.gnch.as<int>(), G.gerr.as<int>());
+    hipLaunchKernelGGL(k_gt_rank, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode,
+                       G.gpn.as<int>(), nd, G.gflag.as<int>(), G.gF.as<int64_t>(),
+                       G.gposR.as<int>());
+    hipLaunchKernelGGL(k_gt_swap, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode,
+                       G.gpn.as<int>(), nd, G.gflag.as<int>(), G.gF.as<int64_t>(),
+                       G.gposR.as<int>(), G.px.as<double>(), G.py.as<double>(),
+                       G.pz.as<double>(), G.perm.as<int>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenType, double dCrit,
+                int iOrder)
+{
+    GravDev &G = pkd->grav;
+    hipStream_t st = pkd->stream;
+    const int n = pkd->nLocal;
+    G.built = 0;
+    // roots: the non-empty systems
+    std::vector<int> rlo;
+    for (int s = 0; s < nSys; ++s)
+        if (sysOff[s + 1] > sysOff[s]) rlo.push_back(sysOff[s]);
+    const int nr = (int)rlo.size();
+    for (int k = 0; k < nr; ++k) rlo.push_back(k + 1 < nr ? rlo[k + 1] - 1 : n - 1);
+    const size_t n1 = (size_t)std::max(n, 1);
+    const size_t cap = 2 * n1 + 2;   // cells at most: every split has two children
+    GG_TRY(G.px.reserve(sizeof(double) * n1));
+    GG_TRY(G.py.reserve(sizeof(double) * n1));
+    GG_TRY(G.pz.reserve(sizeof(double) * n1));
+    GG_TRY(G.pm.reserve(sizeof(double) * n1));
+    GG_TRY(G.ph.reserve(sizeof(double) * n1));
+    GG_TRY(G.pact.reserve(sizeof(uint32_t) * n1));
+    GG_TRY(G.perm.reserve(sizeof(int) * n1));
+    GG_TRY(G.acc.reserve(sizeof(double) * 5 * n1));
+    GG_TRY(G.gpn.reserve(sizeof(int) * n1));
+    GG_TRY(G.gflag.reserve(sizeof(int) * n1));
+    GG_TRY(G.gF.reserve(sizeof(int64_t) * (n1 + 1)));
+    GG_TRY(G.gposR.reserve(sizeof(int) * n1));
+    GG_TRY(G.gnode.reserve(sizeof(GNode) * cap));
+    GG_TRY(G.gbox.reserve(sizeof(unsigned long long) * 6 * cap));
+    GG_TRY(G.gnch.reserve(sizeof(int) * cap));
+    GG_TRY(G.gchoff.reserve(sizeof(int64_t) * (cap + 1)));
+    GG_TRY(G.gerr.reserve(sizeof(int) * 2));
+    GG_TRY(G.groot.reserve(sizeof(int) * 2 * std::max(nr, 1)));
+    GG_HIP(hipMemsetAsync(G.gerr.p, 0, sizeof(int) * 2, st));
+    GNode *nd = G.gnode.as<GNode>();
+    int nn = 0;
+    std::vector<std::pair<int, int>> levels;
+    if (nr > 0) {
+        const int *rl = G.groot.as<int>(), *rh = rl + nr;
+        GG_HIP(hipMemcpyAsync(G.groot.p, rlo.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice,
+                              st));
+        hipLaunchKernelGGL(k_gt_roots, dim3(nblk(nr, 256)), dim3(256), 0, st, nr, rl, rh, nd);
+        // every position starts in its system's root (the offsets cover the store)
+        hipLaunchKernelGGL(k_gt_pos2root, dim3(nblk(n, 256)), dim3(256), 0, st, n, nr, rl,
+                           G.gpn.as<int>());
+        hipLaunchKernelGGL(k_gt_init, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           pkd->f[GG_R].as<double>(), G.px.as<double>(), G.py.as<double>(),
+                           G.pz.as<double>(), G.perm.as<int>());
+        GG_HIP(hipGetLastError());
+        // msrBuildTree's active-type orders (master.c:4262), then the tree
+        GG_TRY(gt_partition(pkd, 0, nr, 1, GG_TYPE_ACTIVE | GG_TYPE_TREEACTIVE));
+        GG_TRY(gt_partition(pkd, 0, nr, 1, GG_TYPE_TREEACTIVE));
+        int a = 0, b = nr;
+        while (b > a) {
+            levels.push_back({a, b});
+            hipLaunchKernelGGL(k_gt_boxinit, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b,
+                               G.gbox.as<unsigned long long>());
+            hipLaunchKernelGGL(k_gt_box, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b,
+                               G.gpn.as<int>(), G.px.as<double>(), G.py.as<double>(),
+                               G.pz.as<double>(), G.gbox.as<unsigned long long>());
+            hipLaunchKernelGGL(k_gt_decide, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b,
+                               nBucket, G.gbox.as<unsigned long long>(), nd);
+            GG_TRY(gt_partition(pkd, a, b, 0, 0));
+            int64_t nch = 0;
+            GG_TRY(exclusive_scan_i64(st, G.gnch.as<int>(), G.gchoff.as<int64_t>(), b - a,
+                                      G.scan, &nch));
+            int err = 0;
+            GG_HIP(hipMemcpyAsync(&err, G.gerr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            GG_HIP(hipStreamSynchronize(st));
+            GG_CHECK(!err, GG_EUNSUP,
+                     "a cell split leaves one side empty (positions differ in the last bit only)");
+            GG_CHECK((size_t)b + (size_t)nch <= cap, GG_EHIP, "tree cell capacity exceeded");
+            hipLaunchKernelGGL(k_gt_children, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, b,
+                               G.gchoff.as<int64_t>(), nd);
+            hipLaunchKernelGGL(k_gt_descend, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, nd,
+                               G.gpn.as<int>());
+            GG_HIP(hipGetLastError());
+            a = b;
+            b += (int)nch;
+        }
+        nn = b;
+        // preorder numbering from subtree sizes
+        for (int L = (int)levels.size() - 1; L >= 0; --L)
+            hipLaunchKernelGGL(k_gt_size, dim3(nblk(levels[L].second - levels[L].first, 256)),
+                               dim3(256), 0, st, levels[L].first, levels[L].second, nd);
+        std::vector<GNode> roots((size_t)nr);
+        GG_HIP(hipMemcpyAsync(roots.data(), nd, sizeof(GNode) * nr, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        int pre = 0;
+        for (int k = 0; k < nr; ++k) {
+            roots[k].pre = pre;
+            pre += roots[k].size;
+        }
+        GG_HIP(hipMemcpyAsync(nd, roots.data(), sizeof(GNode) * nr, hipMemcpyHostToDevice, st));
+        for (size_t L = 0; L < levels.size(); ++L)
+            hipLaunchKernelGGL(k_gt_pre, dim3(nblk(levels[L].second - levels[L].first, 256)),
+                               dim3(256), 0, st, levels[L].first, levels[L].second, nd);
+        GG_HIP(hipGetLastError());
+        GG_HIP(hipStreamSynchronize(st));
+    }
+    const bool empty = nn == 0;
+    const int nNodes = empty ? 1 : nn;
+    // storage-order particle fields
+    if (n > 0)
+        hipLaunchKernelGGL(k_gt_gather, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           G.perm.as<int>(), pkd->f[GG_MASS].as<double>(),
+                           pkd->f[GG_SOFT].as<double>(), pkd->f[GG_IACTIVE].as<uint32_t>(),
+                           G.pm.as<double>(), G.ph.as<double>(), G.pact.as<uint32_t>());
+    GG_TRY(G.wnode.reserve(sizeof(WalkNode) * nNodes));
+    GG_TRY(G.cell.reserve(sizeof(double) * CR_N * nNodes));
+    GG_TRY(G.momOH.reserve(sizeof(double) * OH_N * (iOrder >= 3 ? nNodes : 1)));
+    GG_TRY(G.tint.reserve(sizeof(int) * 6 * nNodes));
+    GG_TRY(G.tdbl.reserve(sizeof(double) * 8 * nNodes));
+    GG_TRY(G.gcm.reserve(sizeof(double) * 5 * nNodes));
+    int nb = 0;
+    if (empty) {   // an empty store: one empty bucket (pkd.c:2684-2712)
+        const int ti[6] = {-1, 0, -1, -1, -1, 0};
+        const double td[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        WalkNode w{};
+        w.pLower = 0;
+        w.pUpper = -1;
+        w.iLower = w.iNext = w.iUpper = -1;
+        GG_HIP(hipMemcpyAsync(G.tint.p, ti, sizeof ti, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.tdbl.p, td, sizeof td, hipMemcpyHostToDevice, st));
+        GG_HIP(hipMemcpyAsync(G.wnode.p, &w, sizeof w, hipMemcpyHostToDevice, st));
+        GG_HIP(hipStreamSynchronize(st));
+    } else {
+        // mass, softening, c.o.m.: deepest level first
+        for (int L = (int)levels.size() - 1; L >= 0; --L)
+            hipLaunchKernelGGL(k_gt_mass, dim3(nblk(levels[L].second - levels[L].first, 256)),
+                               dim3(256), 0, st, levels[L].first, levels[L].second, nd,
+                               G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
+                               G.pm.as<double>(), G.ph.as<double>(), G.gcm.as<double>());
+        // moments: work items of MOM_CH particles per cell
+        GG_TRY(G.gnit.reserve(sizeof(int) * nn));
+        GG_TRY(G.gioff.reserve(sizeof(int64_t) * ((size_t)nn + 1)));
+        hipLaunchKernelGGL(k_gt_items, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, nd,
+                           G.gnit.as<int>());
+        int64_t nitem = 0;
+        GG_TRY(exclusive_scan_i64(st, G.gnit.as<int>(), G.gioff.as<int64_t>(), nn, G.scan, &nitem));
+        GG_TRY(G.gicell.reserve(sizeof(int) * std::max<int64_t>(nitem, 1)));
+        GG_TRY(G.gpart.reserve(sizeof(double) * NM * std::max<int64_t>(nitem, 1)));
+        hipLaunchKernelGGL(k_gt_itemcell, dim3(nblk(nn, 256)), dim3(256), 0, st, nn,
+                           G.gioff.as<int64_t>(), G.gicell.as<int>());
+        if (nitem > 0)
+            hipLaunchKernelGGL(k_gt_moments, dim3(nblk(nitem, 4)), dim3(256), 0, st, (int)nitem,
+                               iOrder, G.gicell.as<int>(), G.gioff.as<int64_t>(), nd,
+                               G.gcm.as<double>(), G.px.as<double>(), G.py.as<double>(),
+                               G.pz.as<double>(), G.pm.as<double>(), G.gpart.as<double>());
+        hipLaunchKernelGGL(k_gt_emit, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, iOrder,
+                           iOpenType, dCrit, nd, G.gcm.as<double>(), G.gioff.as<int64_t>(),
+                           G.gpart.as<double>(), G.tint.as<int>(), G.tdbl.as<double>(),
+                           G.wnode.as<WalkNode>(), G.cell.as<double>(), G.momOH.as<double>());
+        // active buckets in preorder
+        GG_TRY(G.gbflag.reserve(sizeof(int) * nn));
+        GG_TRY(G.gbb8.reserve(sizeof(double) * 8 * nn));
+        GG_TRY(G.grootof.reserve(sizeof(int) * nn));
+        hipLaunchKernelGGL(k_gt_bucket, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, nd,
+                           G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
+                           G.ph.as<double>(), G.pact.as<uint32_t>(), G.gbflag.as<int>(),
+                           G.gbb8.as<double>(), G.gerr.as<int>() + 1);
+        hipLaunchKernelGGL(k_gt_rootof, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, nd,
+                           G.grootof.as<int>());
+        int64_t nb64 = 0;
+        GG_TRY(exclusive_scan_i64(st, G.gbflag.as<int>(), G.gchoff.as<int64_t>(), nn, G.scan,
+                                  &nb64));
+        nb = (int)nb64;
+        int maxb = 0;
+        GG_HIP(hipMemcpyAsync(&maxb, G.gerr.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost,
+                              st));
+        GG_HIP(hipStreamSynchronize(st));
+        GG_CHECK(maxb <= 64, GG_EUNSUP,
+                 "a bucket holds %d particles (> 64): coincident positions?", maxb);
+        GG_TRY(G.bucket.reserve(sizeof(int) * std::max(nb, 1)));
+        GG_TRY(G.broot.reserve(sizeof(int) * std::max(nb, 1)));
+        GG_TRY(G.bbox.reserve(sizeof(double) * 8 * std::max(nb, 1)));
+        hipLaunchKernelGGL(k_gt_bucket_list, dim3(nblk(nn, 256)), dim3(256), 0, st, nn,
+                           G.gbflag.as<int>(), G.gchoff.as<int64_t>(), G.gbb8.as<double>(),
+                           G.grootof.as<int>(), G.bucket.as<int>(), G.bbox.as<double>(),
+                           G.broot.as<int>());
+        GG_HIP(hipGetLastError());
+    }
+    GG_TRY(G.cnt.reserve(sizeof(int) * 3 * std::max(nb, 1)));
+    GG_TRY(G.off.reserve(sizeof(int64_t) * (3 * (size_t)std::max(nb, 1) + 1)));
+    GG_HIP(hipStreamSynchronize(st));
+    G.nb = nb;
+    G.n = n;
+    G.nNodes = nNodes;
+    G.iRoot = 0;
+    G.iOrder = iOrder;
+    G.nBucket = nBucket;
+    G.built = 1;
+    G.listsCurrent = 0;
+    ++G.treeId;
+    return GG_OK;
+}
+
+}  // namespace gg

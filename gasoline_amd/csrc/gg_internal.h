@@ -8,7 +8,6 @@
 #include <vector>
 
 #include "../../include/gasoline_amd.h"
-#include "gg_tree.h"
 
 namespace gg {
 
@@ -47,6 +46,18 @@ struct DevBuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Per node, what the walk reads for a visited cell (64 bytes).
+struct WalkNode {
+    double cx, cy, cz, fOpen2, fSoft;
+    int pLower, pUpper, iLower, iNext;
+    int iUpper, pad1;       // second child (-1 for a bucket)
+};
+
+// Per node FP64 record read when a cell is on an interaction list: mass,
+// centre of mass, softening, raw and traceless quadrupole.
+constexpr int CR_M = 0, CR_X = 1, CR_H = 4, CR_QR = 5, CR_QT = 11, CR_N = 20;
+constexpr int OH_N = 25;  // octupole (10) + hexadecapole (15)
+
 // Gravity state on the device.
 struct GravDev {
     int built = 0, n = 0, nNodes = 0, iRoot = -1, iOrder = 2, nBucket = 0;
@@ -65,7 +76,10 @@ struct GravDev {
     DevBuf scan;           // scan scratch
     DevBuf werr;           // walk frontier overflow flag
     DevBuf broot;          // per active bucket: the root of its tree
-    std::vector<int> roots, nodeRoot;   // per system / per node (host)
+    DevBuf tint, tdbl;     // per node: 6 ints, 8 doubles (gg_tree_info's layout)
+    // tree build scratch
+    DevBuf gpn, gflag, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
+    DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof;
     hipGraph_t graph = nullptr;         // captured step (gg_grav_graph)
     hipGraphExec_t gexec = nullptr;
     int graphOrder = 0;
@@ -73,7 +87,14 @@ struct GravDev {
     int serialWalk = 0;    // the last listed walk fell back to one thread per bucket
     int listsCurrent = 0;  // cnt/off/ent hold the lists of the current tree
     int fusedWalk = 0;     // the last gg_grav_all walked inside the interaction kernel
-    HostGravTree host;
+    void release_all()
+    {
+        DevBuf *all[] = {&px, &py, &pz, &pm, &ph, &pact, &perm, &wnode, &cell, &momOH, &bucket,
+                         &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
+                         &tdbl, &gpn, &gflag, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
+                         &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof};
+        for (DevBuf *b : all) b->release();
+    }
 };
 
 // SPH state on the device (search structure, lists and per-call scratch).
@@ -153,10 +174,15 @@ struct gg_pkd_s {
 namespace gg {
 int field_width(int field);                 // doubles per particle (0: uint32)
 int ensure_field(GGPKD pkd, int field);     // allocate (zeroed) if missing
-// out[0..n]: exclusive prefix sums, out[n] = total (out holds n+1 entries)
+// out[0..n]: exclusive prefix sums, out[n] = total (out holds n+1 entries);
+// the total is copied to *total_host (after a stream synchronisation) unless
+// total_host is null.
 int exclusive_scan_i64(hipStream_t s, const int *in, int64_t *out, int n,
                        DevBuf &scratch, int64_t *total_host);
-int grav_upload(GGPKD pkd);
+// Gravity tree of the systems [sysOff[s], sysOff[s+1]) of the store, one
+// root per non-empty system (gg_build_binary, gg_build_forest).
+int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenType, double dCrit,
+                int iOrder);
 void sph_release(GGPKD pkd);
 void timer_mark(GGPKD pkd, int slot);
 void timer_collect(GGPKD pkd, int first_slot, int nphases, const int *phase);
