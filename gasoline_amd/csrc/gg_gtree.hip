@@ -31,21 +31,9 @@ namespace {
 
 int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
-constexpr unsigned long long MSB = 0x8000000000000000ull;
 constexpr int MOM_CH = 2048;     // particles per moment work item
 constexpr int NM = 32;           // per item: raw Q (6), O (10), H (15), Bmax
-
-// Monotone map of doubles to unsigned integers (for exact atomic min/max).
-__device__ __forceinline__ unsigned long long ord_enc(double v)
-{
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    return (u & MSB) ? ~u : (u | MSB);
-}
-__device__ __forceinline__ double ord_dec(unsigned long long e)
-{
-    const unsigned long long u = (e & MSB) ? (e & ~MSB) : ~e;
-    return __longlong_as_double((long long)u);
-}
+constexpr int BOX_CH = 1024;     // particles per bounds work item
 
 // Per cell in creation (breadth-first) order.
 struct GNode {
@@ -151,73 +139,108 @@ __global__ void k_gt_swap(int n, int a, int b, int mode, const int *__restrict__
     }
 }
 
-__global__ void k_gt_boxinit(int a, int b, unsigned long long *__restrict__ box)
+// Work items of the cells [a, b): ceil(count / ch) each for cells of more
+// than minc particles, none for the others.
+__global__ void k_gt_items(int a, int b, int ch, int minc, const GNode *__restrict__ nd,
+                           int *__restrict__ nit)
 {
     const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b) return;
-    for (int d = 0; d < 3; ++d) {
-        box[6 * (size_t)c + d] = ~0ull;      // min
-        box[6 * (size_t)c + 3 + d] = 0ull;   // max
-    }
+    const int cnt = nd[c].hi - nd[c].lo + 1;
+    nit[c - a] = cnt > minc ? (cnt + ch - 1) / ch : 0;
 }
 
-// Bounding box of each cell of the level: exact min/max by atomics on the
-// order-preserving encodings, reduced per wavefront first.
-__global__ void k_gt_box(int n, int a, int b, const int *__restrict__ pnode,
-                         const double *__restrict__ x, const double *__restrict__ y,
-                         const double *__restrict__ z, unsigned long long *__restrict__ box)
+// The cell (relative index) of work item k: the last r with ioff[r] <= k
+// (cells without items share their offset with the next cell).
+__device__ __forceinline__ int item_cell(const int64_t *__restrict__ ioff, int ncell, int64_t k)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = i < n ? pnode[i] : -1;
-    const bool in = c >= a && c < b;
-    unsigned long long lo[3], hi[3];
-    if (in) {
-        lo[0] = hi[0] = ord_enc(x[i]);
-        lo[1] = hi[1] = ord_enc(y[i]);
-        lo[2] = hi[2] = ord_enc(z[i]);
-    } else {
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = ~0ull;
-            hi[d] = 0ull;
-        }
+    int lo = 0, hi = ncell - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ioff[mid] <= k) lo = mid;
+        else hi = mid - 1;
     }
-    const int c0 = __shfl(c, 0);
-    const bool same = __all(c == c0);
-    if (same) {
-        for (int o = 32; o > 0; o >>= 1)
-            for (int d = 0; d < 3; ++d) {
-                const unsigned long long l = __shfl_xor(lo[d], o), h = __shfl_xor(hi[d], o);
-                lo[d] = l < lo[d] ? l : lo[d];
-                hi[d] = h > hi[d] ? h : hi[d];
-            }
-        if ((threadIdx.x & 63) == 0 && c0 >= a && c0 < b)
-            for (int d = 0; d < 3; ++d) {
-                atomicMin(&box[6 * (size_t)c0 + d], lo[d]);
-                atomicMax(&box[6 * (size_t)c0 + 3 + d], hi[d]);
-            }
-    } else if (in) {
-        for (int d = 0; d < 3; ++d) {
-            atomicMin(&box[6 * (size_t)c + d], lo[d]);
-            atomicMax(&box[6 * (size_t)c + 3 + d], hi[d]);
-        }
-    }
+    return lo;
 }
 
-// Split decision of each cell of the level (pkd.c:2489-2505).
-__global__ void k_gt_decide(int a, int b, int nBucket, const unsigned long long *__restrict__ box,
-                            GNode *__restrict__ nd)
+// Bounds of BOX_CH particles of a cell of more than BOX_CH particles (one
+// wavefront per work item): part[6k..] = min xyz, max xyz.
+__global__ __launch_bounds__(256) void k_gt_box(int a, int b, const int64_t *__restrict__ ioff,
+                                                const GNode *__restrict__ nd,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ z,
+                                                double *__restrict__ part)
+{
+    const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, ncell = b - a;
+    if (k >= ioff[ncell]) return;
+    const int r = item_cell(ioff, ncell, k);
+    const GNode &g = nd[a + r];
+    const int first = g.lo + (int)(k - ioff[r]) * BOX_CH;
+    const int last = min(g.hi, first + BOX_CH - 1);
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int i = first + lane; i <= last; i += 64) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = v[d] < lo[d] ? v[d] : lo[d];
+            hi[d] = v[d] > hi[d] ? v[d] : hi[d];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            const double l = __shfl_xor(lo[d], o), h = __shfl_xor(hi[d], o);
+            lo[d] = l < lo[d] ? l : lo[d];
+            hi[d] = h > hi[d] ? h : hi[d];
+        }
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) {
+            part[6 * k + d] = lo[d];
+            part[6 * k + 3 + d] = hi[d];
+        }
+}
+
+// Split decision of each cell of the level (pkd.c:2489-2505), one wavefront
+// per cell: its bounds from its particles, or from its items' bounds for a
+// cell of more than BOX_CH particles.
+__global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
+                                                   const int64_t *__restrict__ ioff,
+                                                   const double *__restrict__ part,
+                                                   const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ z,
+                                                   GNode *__restrict__ nd)
 {
 #pragma clang fp contract(off)
-    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= b) return;
-    GNode &g = nd[c];
-    double bmin[3], bmax[3];
-    bool extent = false;
-    for (int d = 0; d < 3; ++d) {
-        bmin[d] = ord_dec(box[6 * (size_t)c + d]);
-        bmax[d] = ord_dec(box[6 * (size_t)c + 3 + d]);
-        extent |= bmax[d] > bmin[d];
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= b - a) return;
+    GNode &g = nd[a + r];
+    double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (g.hi - g.lo + 1 > BOX_CH) {
+        for (int64_t it = ioff[r] + lane; it < ioff[r + 1]; it += 64)
+            for (int d = 0; d < 3; ++d) {
+                const double l = part[6 * it + d], h = part[6 * it + 3 + d];
+                bmin[d] = l < bmin[d] ? l : bmin[d];
+                bmax[d] = h > bmax[d] ? h : bmax[d];
+            }
+    } else {
+        for (int i = g.lo + lane; i <= g.hi; i += 64) {
+            const double v[3] = {x[i], y[i], z[i]};
+            for (int d = 0; d < 3; ++d) {
+                bmin[d] = v[d] < bmin[d] ? v[d] : bmin[d];
+                bmax[d] = v[d] > bmax[d] ? v[d] : bmax[d];
+            }
+        }
     }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            const double l = __shfl_xor(bmin[d], o), h = __shfl_xor(bmax[d], o);
+            bmin[d] = l < bmin[d] ? l : bmin[d];
+            bmax[d] = h > bmax[d] ? h : bmax[d];
+        }
+    if (lane) return;
+    bool extent = false;
+    for (int d = 0; d < 3; ++d) extent |= bmax[d] > bmin[d];
     g.dim = -1;
     g.split = 0.0;
     g.ch0 = g.ch1 = -1;
@@ -344,15 +367,7 @@ __global__ void k_gt_mass(int a, int b, const GNode *__restrict__ nd,
     for (int d = 0; d < 3; ++d) o[2 + d] = R[d];
 }
 
-// Work items of the moment sums: item k covers MOM_CH particles of a cell.
-__global__ void k_gt_items(int nn, const GNode *__restrict__ nd, int *__restrict__ nit)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nn) return;
-    const int cnt = nd[c].hi - nd[c].lo + 1;
-    nit[c] = cnt > 0 ? (cnt + MOM_CH - 1) / MOM_CH : 0;
-}
-
+// Cell of each moment item (a thread per cell writes its items).
 __global__ void k_gt_itemcell(int nn, const int64_t *__restrict__ ioff, int *__restrict__ icell)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,8 +379,8 @@ __global__ void k_gt_itemcell(int nn, const int64_t *__restrict__ ioff, int *__r
 // and H (15) about the cell's centre of mass, and the largest distance.
 // Per-lane sums, then a fixed butterfly: the result does not depend on
 // scheduling.
-__global__ __launch_bounds__(256) void k_gt_moments(int nitem, int iOrder,
-                                                    const int *__restrict__ icell,
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_gt_moments(int nitem, const int *__restrict__ icell,
                                                     const int64_t *__restrict__ ioff,
                                                     const GNode *__restrict__ nd,
                                                     const double *__restrict__ cm,
@@ -383,7 +398,7 @@ __global__ __launch_bounds__(256) void k_gt_moments(int nitem, int iOrder,
     const int first = g.lo + (int)(it - ioff[c]) * MOM_CH;
     const int last = min(g.hi, first + MOM_CH - 1);
     const double cx = cm[5 * (size_t)c + 2], cy = cm[5 * (size_t)c + 3], cz = cm[5 * (size_t)c + 4];
-    const bool wantO = iOrder >= 3, wantH = iOrder >= 4;
+    constexpr bool wantO = ORDER >= 3, wantH = ORDER >= 4;
     double Q[6] = {0, 0, 0, 0, 0, 0}, O[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double H[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double Bmax = 0.0;
@@ -451,6 +466,35 @@ __global__ __launch_bounds__(256) void k_gt_moments(int nitem, int iOrder,
 // the cell record and O/H: moments summed over the cell's items in order,
 // traceless Q as the walk stores it (walk.c:41-48), fOpen2 by pkdCalcOpen
 // (pkd.c:2228-2264).
+// Cells of more than one moment item (one wavefront per cell): lane j sums
+// the items j, j + 64, ..., a fixed butterfly combines the lanes, and the
+// total replaces the cell's first item.
+__global__ __launch_bounds__(256) void k_gt_combine(int nn, int nv,
+                                                    const int64_t *__restrict__ ioff,
+                                                    double *__restrict__ part)
+{
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (b >= nn) return;
+    const int64_t i0 = ioff[b], i1 = ioff[b + 1];
+    if (i1 - i0 < 2) return;
+    double M[31];
+    for (int k = 0; k < 31; ++k) M[k] = 0.0;
+    double Bmax = 0.0;
+    for (int64_t it = i0 + lane; it < i1; it += 64) {
+        const double *p = part + (size_t)NM * it;
+        for (int k = 0; k < nv; ++k) M[k] += p[k];
+        Bmax = fmax(Bmax, p[31]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < nv; ++k) M[k] += __shfl_xor(M[k], o);
+        Bmax = fmax(Bmax, __shfl_xor(Bmax, o));
+    }
+    if (lane) return;
+    double *p = part + (size_t)NM * i0;
+    for (int k = 0; k < nv; ++k) p[k] = M[k];
+    p[31] = Bmax;
+}
+
 __global__ void k_gt_emit(int nn, int iOrder, int iOpenType, double dCrit,
                           const GNode *__restrict__ nd, const double *__restrict__ cm,
                           const int64_t *__restrict__ ioff, const double *__restrict__ part,
@@ -464,14 +508,8 @@ __global__ void k_gt_emit(int nn, int iOrder, int iOpenType, double dCrit,
     const GNode &g = nd[b];
     const int c = g.pre;
     const double *q = cm + 5 * (size_t)b;
-    double M[31];
-    for (int k = 0; k < 31; ++k) M[k] = 0.0;
-    double Bmax = 0.0;
-    for (int64_t it = ioff[b]; it < ioff[b + 1]; ++it) {
-        const double *p = part + (size_t)NM * it;
-        for (int k = 0; k < 31; ++k) M[k] += p[k];
-        Bmax = fmax(Bmax, p[31]);
-    }
+    const double *M = part + (size_t)NM * ioff[b];
+    const double Bmax = M[31];
     double dOpen;
     if (iOpenType == 1) {
         dOpen = 2 / sqrt(3.0) * Bmax / dCrit;
@@ -673,7 +711,9 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
     GG_TRY(G.gF.reserve(sizeof(int64_t) * (n1 + 1)));
     GG_TRY(G.gposR.reserve(sizeof(int) * n1));
     GG_TRY(G.gnode.reserve(sizeof(GNode) * cap));
-    GG_TRY(G.gbox.reserve(sizeof(unsigned long long) * 6 * cap));
+    GG_TRY(G.gbox.reserve(sizeof(double) * 6 * (2 * (n1 / BOX_CH) + 2)));
+    GG_TRY(G.gnit.reserve(sizeof(int) * cap));
+    GG_TRY(G.gioff.reserve(sizeof(int64_t) * (cap + 1)));
     GG_TRY(G.gnch.reserve(sizeof(int) * cap));
     GG_TRY(G.gchoff.reserve(sizeof(int64_t) * (cap + 1)));
     GG_TRY(G.gerr.reserve(sizeof(int) * 2));
@@ -700,13 +740,18 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
         int a = 0, b = nr;
         while (b > a) {
             levels.push_back({a, b});
-            hipLaunchKernelGGL(k_gt_boxinit, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b,
-                               G.gbox.as<unsigned long long>());
-            hipLaunchKernelGGL(k_gt_box, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b,
-                               G.gpn.as<int>(), G.px.as<double>(), G.py.as<double>(),
-                               G.pz.as<double>(), G.gbox.as<unsigned long long>());
-            hipLaunchKernelGGL(k_gt_decide, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b,
-                               nBucket, G.gbox.as<unsigned long long>(), nd);
+            // bounds by work items of BOX_CH particles (at most one per cell
+            // plus n / BOX_CH), then the split decisions
+            hipLaunchKernelGGL(k_gt_items, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, BOX_CH,
+                               BOX_CH, nd, G.gnit.as<int>());
+            GG_TRY(exclusive_scan_i64(st, G.gnit.as<int>(), G.gioff.as<int64_t>(), b - a, G.scan,
+                                      nullptr));
+            hipLaunchKernelGGL(k_gt_box, dim3(nblk(2 * (int64_t)(n / BOX_CH) + 1, 4)), dim3(256),
+                               0, st, a, b, G.gioff.as<int64_t>(), nd, G.px.as<double>(),
+                               G.py.as<double>(), G.pz.as<double>(), G.gbox.as<double>());
+            hipLaunchKernelGGL(k_gt_decide, dim3(nblk(b - a, 4)), dim3(256), 0, st, a, b, nBucket,
+                               G.gioff.as<int64_t>(), G.gbox.as<double>(), G.px.as<double>(),
+                               G.py.as<double>(), G.pz.as<double>(), nd);
             GG_TRY(gt_partition(pkd, a, b, 0, 0));
             int64_t nch = 0;
             GG_TRY(exclusive_scan_i64(st, G.gnch.as<int>(), G.gchoff.as<int64_t>(), b - a,
@@ -779,21 +824,25 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
                                G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
                                G.pm.as<double>(), G.ph.as<double>(), G.gcm.as<double>());
         // moments: work items of MOM_CH particles per cell
-        GG_TRY(G.gnit.reserve(sizeof(int) * nn));
-        GG_TRY(G.gioff.reserve(sizeof(int64_t) * ((size_t)nn + 1)));
-        hipLaunchKernelGGL(k_gt_items, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, nd,
-                           G.gnit.as<int>());
+        hipLaunchKernelGGL(k_gt_items, dim3(nblk(nn, 256)), dim3(256), 0, st, 0, nn, MOM_CH, 0,
+                           nd, G.gnit.as<int>());
         int64_t nitem = 0;
         GG_TRY(exclusive_scan_i64(st, G.gnit.as<int>(), G.gioff.as<int64_t>(), nn, G.scan, &nitem));
-        GG_TRY(G.gicell.reserve(sizeof(int) * std::max<int64_t>(nitem, 1)));
         GG_TRY(G.gpart.reserve(sizeof(double) * NM * std::max<int64_t>(nitem, 1)));
+        GG_TRY(G.gicell.reserve(sizeof(int) * std::max<int64_t>(nitem, 1)));
         hipLaunchKernelGGL(k_gt_itemcell, dim3(nblk(nn, 256)), dim3(256), 0, st, nn,
                            G.gioff.as<int64_t>(), G.gicell.as<int>());
-        if (nitem > 0)
-            hipLaunchKernelGGL(k_gt_moments, dim3(nblk(nitem, 4)), dim3(256), 0, st, (int)nitem,
-                               iOrder, G.gicell.as<int>(), G.gioff.as<int64_t>(), nd,
-                               G.gcm.as<double>(), G.px.as<double>(), G.py.as<double>(),
-                               G.pz.as<double>(), G.pm.as<double>(), G.gpart.as<double>());
+        if (nitem > 0) {
+            auto mom = iOrder >= 4 ? k_gt_moments<4> : (iOrder >= 3 ? k_gt_moments<3>
+                                                                      : k_gt_moments<2>);
+            hipLaunchKernelGGL(mom, dim3(nblk(nitem, 4)), dim3(256), 0, st, (int)nitem,
+                               G.gicell.as<int>(), G.gioff.as<int64_t>(), nd, G.gcm.as<double>(),
+                               G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
+                               G.pm.as<double>(), G.gpart.as<double>());
+        }
+        const int nv = iOrder >= 4 ? 31 : (iOrder >= 3 ? 16 : 6);
+        hipLaunchKernelGGL(k_gt_combine, dim3(nblk(nn, 4)), dim3(256), 0, st, nn, nv,
+                           G.gioff.as<int64_t>(), G.gpart.as<double>());
         hipLaunchKernelGGL(k_gt_emit, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, iOrder,
                            iOpenType, dCrit, nd, G.gcm.as<double>(), G.gioff.as<int64_t>(),
                            G.gpart.as<double>(), G.tint.as<int>(), G.tdbl.as<double>(),

@@ -54,6 +54,23 @@ def test_forest_equals_separate_runs(gpu, ens, prec):
         assert st[k] == tot[k], k
 
 
+def test_forest_with_empty_systems(gpu, ens):
+    # empty systems have no tree; the others are evaluated as alone
+    off = ens["off"]
+    n = len(ens["m"])
+    off2 = np.r_[0, 0, off[1:3], off[3], off[3:], n].astype(np.int32)
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        load(pk, ens, 0.005)
+        pk.build_forest(off, nBucket=16, dCrit=0.6)
+        pk.grav_all(2)
+        a = pk.get("a")
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        load(pk, ens, 0.005)
+        pk.build_forest(off2, nBucket=16, dCrit=0.6)
+        pk.grav_all(2)
+        assert np.array_equal(pk.get("a"), a)
+
+
 def test_forest_system_vs_oracle(gpu, ens):
     off = ens["off"]
     s = 2

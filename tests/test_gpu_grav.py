@@ -55,6 +55,25 @@ def test_tree_bitexact(gpu, plummer32k):
     assert np.array_equal(dbls[:, 6], od[:, 44])
 
 
+def test_tree_bitexact_lattice_ties(gpu):
+    # a 21^3 lattice: midpoint splits fall on lattice planes, so particles
+    # lie exactly on the split and the partition's tie rule decides
+    g = np.arange(21) * 0.05
+    r = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    m = np.full(len(r), 1.0 / len(r))
+    soft = np.full(len(r), 0.01)
+    act = np.where(np.arange(len(r)) % 3 == 0, 3, 2).astype(np.uint32)
+    T = O.Tree(r, m, soft, iactive=act, nBucket=8, dTheta=0.7, iOrder=4)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64, iactive=act) as pk:
+        pk.build_binary(8, 1, 0.7, 4)
+        root, perm, ints, dbls = pk.tree_info()
+    oi, od = T.nodes()
+    assert np.array_equal(perm, T.perm())
+    assert np.array_equal(ints[:, :5], oi[:, :5])
+    assert np.array_equal(dbls[:, :6], od[:, [7, 8, 9, 10, 11, 12]])
+    assert np.array_equal(dbls[:, 6], od[:, 44])
+
+
 def test_interaction_lists_bitexact(gpu, plummer32k):
     r, m, soft, T, (ao, po, dto, sto) = plummer32k
     with setup_pkd(gpu, r, m, soft, gpu.PREC_FP32) as pk:
