@@ -99,9 +99,13 @@ def time_grav(gg, torch, n, prec, steps, warmup):
         inter.append(t["interact"])
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(3):
+        pk.build_binary(16, 1, 0.7, 2)
+    tb = (time.perf_counter() - t0) / 3
     pk.close()
     nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
-    return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter))
+    return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter)), tb
 
 
 def time_ensemble(gg, torch, e, steps, prec):
@@ -135,6 +139,65 @@ def time_ensemble(gg, torch, e, steps, prec):
     nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
     return dict(n=n, nsys=nsys, nBucket=best, tune_ms=tune, ms_eager=eager * 1e3,
                 ms_graph=graph * 1e3, interactions=nint, interactions_per_s=nint / graph)
+
+
+def force_errors(gg, d, soft, bits, theta, nsample=10000, seed=1, nBucket=16):
+    """rms and max relative gravity-force error of the FP32 and FP64 tree
+    forces against the FP64 direct sum (gg_direct_grav) on nsample random
+    targets, and of FP32 against FP64 on the same tree."""
+    from gasoline_amd import evolve
+    n = len(d["m"])
+    idx = np.sort(np.random.default_rng(seed).choice(n, min(nsample, n), replace=False))
+    acc = {}
+    for name, prec in (("fp32", gg.PREC_FP32), ("fp64", gg.PREC_FP64)):
+        with gg.Pkd(n, prec=prec) as pk:
+            evolve.start(pk, d, soft, bits)
+            pk.build_binary(nBucket, 1, theta, 2)
+            pk.init_accel()
+            pk.grav_all(2)
+            acc[name] = pk.get("a")[idx]
+            if name == "fp64":
+                acc["direct"], _ = pk.direct_grav(idx)
+
+    def err(x, ref):
+        e = np.linalg.norm(x - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        return {"rms": float(np.sqrt(np.mean(e * e))), "max": float(e.max())}
+    return {"targets": int(len(idx)), "fp32_vs_direct": err(acc["fp32"], acc["direct"]),
+            "fp64_vs_direct": err(acc["fp64"], acc["direct"]),
+            "fp32_vs_fp64": err(acc["fp32"], acc["fp64"])}
+
+
+def time_kdk(gg, torch, d, soft, bits, prec, mode, K, dt, theta, steps, warmup, visc=1):
+    """Kick-drift-kick steps with gravity + SPH forces (evolve.KDK): the tree
+    build, gravity walk and interactions, SPH tree, kNN + DenDVDX, EOS and
+    SphPressureTermsSym each step.  mode: "mlp" (h-proposal network seeds
+    the search, trained online), "prev" (previous fBall2 seeds it) or "cold"
+    (no seed: the plain search)."""
+    from gasoline_amd import evolve
+    n = len(d["m"])
+    pk = gg.Pkd(n, prec=prec)
+    evolve.start(pk, d, soft, bits)
+    hm = None
+    if mode == "mlp":
+        pk.hmlp_init(1)
+        hm = dict(train_iters=50, batch=8192, lr=3e-3)
+    pk.set_seed_slack({"cold": 0.0, "mlp": 1.08, "prev": 1.15}[mode])
+    run = evolve.KDK(pk, gg.Smf.default(iViscosityLimiter=visc), nSmooth=K, theta=theta,
+                     hmlp=hm)
+    run.forces(dt)
+    for _ in range(warmup):
+        run.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step(dt)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    st, sm = run.grav_stats, run.stats[-1]
+    pk.close()
+    return {"ms_per_step": el * 1e3, "mode": mode,
+            "grav_interactions": st["dPartSum"] + st["dCellSum"] + st["dSoftSum"],
+            "knn_candidates": int(sm["candidates"]), "knn_extra_passes": int(sm["extra_passes"])}
 
 
 def cpu_baseline_sph(n=1 << 20):
@@ -177,6 +240,9 @@ def main():
     ap.add_argument("--ngrav", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nsys", type=int, default=256, help="ensemble systems (0: skip)")
+    ap.add_argument("--nevrard", type=int, default=1 << 24,
+                    help="Evrard collapse particles of the full-step leg (0: skip)")
+    ap.add_argument("--disk", type=int, default=1, help="disk-galaxy full-step leg (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run five device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE) and exit")
@@ -211,6 +277,38 @@ def main():
         e = ics.king_ensemble(args.nsys, seed=7)
         ens32 = time_ensemble(gg, torch, e, args.steps, gg.PREC_FP32)
         ens64 = time_ensemble(gg, torch, e, max(2, args.steps // 2), gg.PREC_FP64)
+    evr = None
+    if args.nevrard > 0:
+        from gasoline_amd import ics
+        d = ics.evrard(args.nevrard, seed=5)
+        soft = np.full(args.nevrard, 0.004)
+        ks = max(2, args.steps // 2)
+        fast = time_kdk(gg, torch, d, soft, None, gg.PREC_FP32, "mlp", 64, 1e-3, 0.7, ks, 3)
+        plain = time_kdk(gg, torch, d, soft, None, gg.PREC_FP64, "cold", 64, 1e-3, 0.7, ks, 1)
+        evr = {"workload": "Evrard collapse N=%d (rho ~ 1/r, M=R=G=1, u=0.05, gamma 5/3), "
+                           "eps 0.004, theta 0.7, quadrupole, bucket 16, nSmooth 64, dt 0.001 "
+                           "(about the central Courant limit); "
+                           "KDK steps (gravity tree build + walk/interactions, SPH tree, kNN, "
+                           "DenDVDX, EOS, SphPressureTermsSym)" % args.nevrard,
+               "fp32_mlp": fast, "fp64_plain": plain,
+               "speedup_vs_fp64_plain": plain["ms_per_step"] / fast["ms_per_step"],
+               "force_error": force_errors(gg, d, soft, None, 0.7)}
+        del d
+    disk = None
+    if args.disk:
+        from gasoline_amd import ics
+        d = ics.disk_galaxy(seed=11)
+        bits = np.where(d["gas"], 3 | gg.TYPE_GAS, 3).astype(np.uint32)
+        ks = max(2, args.steps // 2)
+        f32 = time_kdk(gg, torch, d, d["soft"], bits, gg.PREC_FP32, "prev", 32, 5e-4, 0.7, ks, 1)
+        f64 = time_kdk(gg, torch, d, d["soft"], bits, gg.PREC_FP64, "prev", 32, 5e-4, 0.7, ks, 1)
+        disk = {"workload": "isolated disk galaxy: Hernquist halo 4194304 DM (1e12 Msun, a 30 "
+                            "kpc, cut at 10a) + exponential gas disk 1048576 (R_d 3 kpc, sech^2 "
+                            "z0 0.3 kpc, min Toomre Q 1.5, T 1e4 K, mu 1.22), eps 0.1/0.05 kpc, "
+                            "theta 0.7, quadrupole, nSmooth 32, dt 5e-4 (%s)" % d["units"],
+                "fp32": f32, "fp64": f64, "speedup_vs_fp64": f64["ms_per_step"] / f32["ms_per_step"],
+                "force_error": force_errors(gg, d, d["soft"], bits, 0.7)}
+        del d
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream; the dominant one is reported
     kern = {
@@ -262,6 +360,7 @@ def main():
         "grav_interactions_per_s_fp64": g64[1] / g64[0],
         "grav_speedup_vs_fp64": g64[0] / g32[0],
         "grav_ms_walk": g32[3], "grav_ms_interact_fp32": g32[4], "grav_ms_interact_fp64": g64[4],
+        "grav_ms_tree_build": g32[5] * 1e3,
         "roofline": roof(dom),
         "rooflines_other": [roof(k) for k in kern if k != dom],
         "roofline_gravity": {
@@ -283,6 +382,10 @@ def main():
             "fp32": ens32, "fp64": ens64,
             "speedup_vs_fp64": ens64["ms_graph"] / ens32["ms_graph"],
         }
+    if evr:
+        line["evrard"] = evr
+    if disk:
+        line["disk_galaxy"] = disk
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v_sph, el_sph = cpu_baseline_sph()

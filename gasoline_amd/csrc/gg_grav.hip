@@ -1094,6 +1094,98 @@ extern "C" int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucke
     return GG_OK;
 }
 
+namespace gg {
+// FP64 direct sum for sampled targets (the accuracy check of the tree
+// forces): every other particle of the store with the spline softening of
+// the particle-particle interaction, twoh = h_i + h_j (grav.c:81-108), no
+// tree.  One thread per target; sources staged through LDS in tiles of 256;
+// Kahan-compensated sums.
+__global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__restrict__ tgt,
+                                                const double *__restrict__ r,
+                                                const double *__restrict__ m,
+                                                const double *__restrict__ h,
+                                                double *__restrict__ out)
+{
+    __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int i = k < nt ? tgt[k] : -1;
+    const double xi = i >= 0 ? r[3 * (size_t)i] : 0.0, yi = i >= 0 ? r[3 * (size_t)i + 1] : 0.0;
+    const double zi = i >= 0 ? r[3 * (size_t)i + 2] : 0.0, hi = i >= 0 ? h[i] : 0.0;
+    double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+    for (int j0 = 0; j0 < n; j0 += 256) {
+        const int j = j0 + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = r[3 * (size_t)j];
+            sy[threadIdx.x] = r[3 * (size_t)j + 1];
+            sz[threadIdx.x] = r[3 * (size_t)j + 2];
+            sm[threadIdx.x] = m[j];
+            sh[threadIdx.x] = h[j];
+        }
+        __syncthreads();
+        const int nj = min(256, n - j0);
+        if (i >= 0)
+            for (int e = 0; e < nj; ++e) {
+                if (j0 + e == i) continue;
+                const double dx = sx[e] - xi, dy = sy[e] - yi, dz = sz[e] - zi;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                double a, b;
+                spline_ab<double>(r2, 1.0 / sqrt(r2), hi + sh[e], a, b);
+                const double v[4] = {dx * b * sm[e], dy * b * sm[e], dz * b * sm[e], -a * sm[e]};
+                for (int q = 0; q < 4; ++q) {   // Kahan
+                    const double y = v[q] - c[q], t = s[q] + y;
+                    c[q] = (t - s[q]) - y;
+                    s[q] = t;
+                }
+            }
+        __syncthreads();
+    }
+    if (i >= 0)
+        for (int q = 0; q < 4; ++q) out[4 * (size_t)k + q] = s[q];
+}
+}  // namespace gg
+
+extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot)
+{
+    GG_CHECK(pkd && (nTarget == 0 || (target && a && pot)), GG_EINVAL, "null argument");
+    GG_CHECK(nTarget >= 0, GG_EINVAL, "nTarget must be >= 0");
+    GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
+             "positions, masses and softenings must be put before gg_direct_grav");
+    const int n = pkd->nLocal;
+    for (int k = 0; k < nTarget; ++k)
+        GG_CHECK(target[k] >= 0 && target[k] < n, GG_EINVAL, "target %d outside 0..%d",
+                 target[k], n - 1);
+    if (nTarget == 0) return GG_OK;
+    hipStream_t st = pkd->stream;
+    DevBuf t, o;
+    GG_TRY(t.reserve(sizeof(int) * nTarget));
+    if (o.reserve(sizeof(double) * 4 * nTarget) != GG_OK) {
+        t.release();
+        return GG_EHIP;
+    }
+    int rc = GG_OK;
+    std::vector<double> res(4 * (size_t)nTarget);
+    if (hipMemcpyAsync(t.p, target, sizeof(int) * nTarget, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = GG_EHIP;
+    if (rc == GG_OK) {
+        hipLaunchKernelGGL(k_direct, dim3((nTarget + 255) / 256), dim3(256), 0, st, n, nTarget,
+                           t.as<int>(), pkd->f[GG_R].as<double>(), pkd->f[GG_MASS].as<double>(),
+                           pkd->f[GG_SOFT].as<double>(), o.as<double>());
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(res.data(), o.p, sizeof(double) * 4 * nTarget, hipMemcpyDeviceToHost,
+                           st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = GG_EHIP;
+    }
+    t.release();
+    o.release();
+    GG_CHECK(rc == GG_OK, GG_EHIP, "gg_direct_grav: HIP error");
+    for (int k = 0; k < nTarget; ++k) {
+        for (int d = 0; d < 3; ++d) a[3 * (size_t)k + d] = res[4 * (size_t)k + d];
+        pot[k] = res[4 * (size_t)k + 3];
+    }
+    return GG_OK;
+}
+
 extern "C" int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
                             double *dbls)
 {

@@ -584,6 +584,15 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
     } else {
         R2 = cold_r2(A.bden[A.bof[q]], K);
     }
+    // kNN: lo2 < R2 < hi2 brackets the radii that hold K..CAP particles (lo2:
+    // the largest tried that held fewer than K; hi2: the smallest that held
+    // more than CAP or met too many buckets); once both are known the next
+    // radius bisects them, so a thin shell of equidistant neighbours (an
+    // isolated particle far from the rest) is still found
+    double lo2 = 0.0, hi2 = DBL_MAX;
+    auto bisect = [&]() {
+        return hi2 > 4.0 * lo2 ? sqrt(lo2) * sqrt(hi2) : 0.5 * (lo2 + hi2);
+    };
     for (int attempt = 0; attempt < MAXTRY; ++attempt) {
         const double R2b = R2 * (1.0 + 1e-12);
         int np;
@@ -592,7 +601,8 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
             bfst, bcnt, QB, lane, &np);
         if (nb < 0) {
             if (A.mode != 0) break;   // a ball gather cannot change its radius
-            R2 *= 0.25;
+            hi2 = fmin(hi2, R2);
+            R2 = lo2 > 0.0 ? bisect() : R2 * 0.25;
             continue;
         }
         for (int k = lane; k < nb; k += 64) bfst[k] = A.bfirst[bfst[k]];
@@ -625,13 +635,15 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
         wave_sync();
         if (A.mode == 0) {
             if (total < K) {
+                lo2 = fmax(lo2, R2);
                 const double f = total > 0 ? cbrt((2.0 * K / total) * (2.0 * K / total)) : 4.0;
-                R2 *= fmin(fmax(f, 1.6), 16.0);
+                R2 = hi2 < DBL_MAX ? bisect() : R2 * fmin(fmax(f, 1.6), 16.0);
                 continue;
             }
             if (total > CAP) {
+                hi2 = fmin(hi2, R2);
                 const double f = cbrt((1.5 * K / total) * (1.5 * K / total));
-                R2 *= fmin(fmax(f, 0.05), 0.9);
+                R2 = lo2 > 0.0 ? bisect() : R2 * fmin(fmax(f, 0.05), 0.9);
                 continue;
             }
         } else if (total > CAP) {
@@ -2086,13 +2098,14 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(S.fb2s.reserve(sizeof(double) * n));
     GG_TRY(S.err.reserve(sizeof(unsigned) * 12));
     GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 12, st));
-    GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
+    const bool hint = S.seedSlack > 0.0;
+    if (hint) GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
     GG_TRY(S.retry.reserve(sizeof(int) * n));
     GG_TRY(S.nretry.reserve(sizeof(unsigned)));
     GG_HIP(hipMemsetAsync(S.nretry.p, 0, sizeof(unsigned), st));
     KLArgs KL;
     KL.Q = query_args(pkd, 0, K);
-    KL.Q.guess = S.guess.as<double>();
+    KL.Q.guess = hint ? S.guess.as<double>() : nullptr;
     KL.Q.nbr = S.nbr.as<int>();
     KL.nb = S.nb;
     KL.bfirst = S.bfirst.as<int>();
@@ -2119,7 +2132,7 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     S.retried = nretry;
     if (nretry > 0) {   // queries the leaf groups could not settle
         QArgs Q = query_args(pkd, 0, K);
-        Q.guess = S.guess.as<double>();
+        Q.guess = hint ? S.guess.as<double>() : nullptr;
         Q.nbr = S.nbr.as<int>();
         Q.qlist = S.retry.as<int>();
         hipLaunchKernelGGL(k_query, dim3(nretry), dim3(64), 0, st, Q);
@@ -2199,10 +2212,10 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     uint64_t cand;
     std::memcpy(&cand, &e[8], sizeof(cand));
     S.cand = (int64_t)cand;
+    const uint32_t *qf = S.qf.as<uint32_t>();
+    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, qf));   // -1 marks a failed search
     GG_CHECK(e[0] == 0, GG_EUNSUP,
              "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
-    const uint32_t *qf = S.qf.as<uint32_t>();
-    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, qf));
     GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, qf));
     if (iSmoothType == GG_SMX_DENDVDX) {
         GG_TRY(scatter_field(pkd, S.o_divv, GG_DIVV, 1, qf));
@@ -2440,7 +2453,8 @@ extern "C" int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr)
 extern "C" int gg_set_seed_slack(GGPKD pkd, double slack)
 {
     GG_CHECK(pkd, GG_EINVAL, "null handle");
-    GG_CHECK(slack >= 1.0 && slack <= 2.0, GG_EINVAL, "slack %g outside [1, 2]", slack);
+    GG_CHECK(slack == 0.0 || (slack >= 1.0 && slack <= 2.0), GG_EINVAL,
+             "slack %g: 0 (no hint) or in [1, 2]", slack);
     pkd->sph.seedSlack = slack;
     return GG_OK;
 }

@@ -25,6 +25,7 @@ class KDK:
         self.hmlp = hmlp            # dict(train_iters, batch, lr) or None
         self.dt_prev = None
         self.stats = []             # kNN counters of each density pass
+        self.grav_stats = None      # counters of the last gravity evaluation
 
     def seed_search(self):
         """Before a density pass: the network's proposal, if trained."""
@@ -36,7 +37,7 @@ class KDK:
         pk.init_accel()
         if self.gravity:
             pk.build_binary(self.nBucket, 1, self.theta, self.iOrder)
-            pk.grav_all(self.iOrder)
+            self.grav_stats = pk.grav_all(self.iOrder)
         pk.build_local(period=self.period)
         self.seed_search()
         pk.smooth(self.smf, self.K, self.period is not None)

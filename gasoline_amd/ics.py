@@ -140,3 +140,82 @@ def king_ensemble(nsys, seed=0, nmin=1024, nmax=65536, W0=(3.0, 9.0)):
     cat = {k: np.concatenate([p[k] for p in parts]) for k in ("r", "v", "m")}
     cat["off"], cat["W0"] = off, ws
     return cat
+
+
+# Galactic units of the disk-galaxy workload: lengths in kpc, velocities in
+# km/s, G = 1; the mass unit is then 1 / G[kpc (km/s)^2 / Msun].
+G_KPC_KMS2_MSUN = 4.30091e-6
+MASS_UNIT_MSUN = 1.0 / G_KPC_KMS2_MSUN
+K_BOLTZ, M_PROTON = 1.380649e-23, 1.67262192e-27
+
+
+def _hernquist_sigma2(r, M, a):
+    """Isotropic radial velocity dispersion of a Hernquist sphere (Hernquist
+    1990, eq. 10), G = 1."""
+    x = r / a
+    t = 12.0 * x * (1 + x) ** 3 * np.log((1 + x) / x) \
+        - x / (1 + x) * (25 + 52 * x + 42 * x * x + 12 * x ** 3)
+    return np.maximum(M / (12.0 * a) * t, 0.0)
+
+
+def disk_galaxy(n_dm=1 << 22, n_gas=1 << 20, seed=11, M_halo=1e12, a_halo=30.0, r_cut=10.0,
+                R_d=3.0, z_0=0.3, Q=1.5, T=1e4, mu=1.22, gamma=5.0 / 3.0,
+                eps_dm=0.1, eps_gas=0.05):
+    """Isolated disk galaxy in galactic units (kpc, km/s, G = 1):
+
+    * Hernquist halo, M = M_halo Msun, scale a_halo kpc, truncated at
+      r_cut * a_halo (its particles carry the truncated mass); radii from the
+      inverse cumulative mass, isotropic Gaussian velocities with the Jeans
+      dispersion of the untruncated profile.
+    * Exponential gas disk, scale length R_d, sech^2(z / z_0) layers, cut at
+      10 R_d; cold circular rotation in the halo + disk potential (spherical
+      approximation of the disk's enclosed mass); temperature T with mean
+      molecular weight mu, so u = kT / ((gamma - 1) mu m_p); the disk mass is
+      set so that the minimum over R of the gas Toomre parameter
+      Q = c_s kappa / (pi G Sigma) (halo epicyclic frequency) equals Q.
+
+    Returns dict(r, v, m, u, soft, gas) with gas a boolean mask (the first
+    n_dm particles are the halo) and the derived disk mass in M_disk (Msun).
+    """
+    rng = np.random.default_rng(seed)
+    Mh = M_halo / MASS_UNIT_MSUN
+    f_cut = (r_cut / (1.0 + r_cut)) ** 2
+    X = rng.uniform(0.0, f_cut, n_dm)
+    sx = np.sqrt(X)
+    rad = a_halo * sx / (1.0 - sx)
+    r_dm = rad[:, None] * _iso(rng, n_dm)
+    sig = np.sqrt(_hernquist_sigma2(np.maximum(rad, 1e-6 * a_halo), Mh, a_halo))
+    v_dm = sig[:, None] * rng.normal(size=(n_dm, 3))
+    # gas: u and sound speed from T (km/s)^2
+    kT_mu = K_BOLTZ * T / (mu * M_PROTON) / 1e6
+    u = kT_mu / (gamma - 1.0)
+    cs = np.sqrt(gamma * kT_mu)
+    # disk mass from min Q over R (halo epicyclic frequency)
+    R = np.linspace(0.05 * R_d, 10 * R_d, 4000)
+    vc2h = Mh * R / (R + a_halo) ** 2
+    om2 = vc2h / R ** 2
+    kap = np.sqrt(np.maximum(R * np.gradient(om2, R) + 4 * om2, 0.0))
+    # Q = cs kap / (pi Sigma), Sigma = Md / (2 pi R_d^2) e^{-R/R_d}
+    Md = np.min(cs * kap * 2.0 * R_d ** 2 * np.exp(R / R_d) / Q)
+    Rg = np.empty(n_gas)
+    todo = np.arange(n_gas)
+    while todo.size:
+        rr = -R_d * np.log(rng.uniform(size=todo.size) * rng.uniform(size=todo.size))
+        ok = rr < 10 * R_d
+        Rg[todo[ok]] = rr[ok]
+        todo = todo[~ok]
+    ph = rng.uniform(0, 2 * np.pi, n_gas)
+    z = z_0 * np.arctanh(np.clip(rng.uniform(-1, 1, n_gas), -1 + 1e-12, 1 - 1e-12))
+    r_g = np.stack([Rg * np.cos(ph), Rg * np.sin(ph), z], axis=1)
+    m_enc = Mh * Rg ** 2 / (Rg + a_halo) ** 2 + Md * (1 - (1 + Rg / R_d) * np.exp(-Rg / R_d))
+    vc = np.sqrt(m_enc / np.maximum(Rg, 1e-6))
+    v_g = np.stack([-vc * np.sin(ph), vc * np.cos(ph), np.zeros(n_gas)], axis=1)
+    r = np.concatenate([r_dm, r_g])
+    v = np.concatenate([v_dm, v_g])
+    m = np.concatenate([np.full(n_dm, Mh * f_cut / n_dm), np.full(n_gas, Md / n_gas)])
+    gas = np.zeros(n_dm + n_gas, bool)
+    gas[n_dm:] = True
+    soft = np.where(gas, eps_gas, eps_dm)
+    return dict(r=_f32(r), v=_f32(v), m=m, u=np.where(gas, u, 0.0), soft=soft, gas=gas,
+                M_disk=Md * MASS_UNIT_MSUN, cs=cs, units="kpc, km/s, G=1, mass unit %.6g Msun"
+                % MASS_UNIT_MSUN)

@@ -41,7 +41,7 @@ TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
 # exported entry points (include/gasoline_amd.h)
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
        "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
-       "gg_grav_lists", "gg_tree_info", "gg_build_local", "gg_smooth", "gg_resmooth",
+       "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_init_accel", "gg_kick", "gg_drift",
@@ -110,6 +110,7 @@ def lib():
                               _ip, _dp, _dp, _dp, _dp]
     L.gg_grav_lists.argtypes = [_vp, _ip, _i64p, _ip, _i64p, _ip, _ip]
     L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
+    L.gg_direct_grav.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
     L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
     L.gg_smooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int, _ip]
     L.gg_resmooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int]
@@ -254,6 +255,15 @@ class Pkd:
                                    _p(off, _i64p), _p(cnt, _ip), _p(ent, _ip)))
         n = nb.value
         return bk[:n], off[:3 * n].reshape(n, 3), cnt[:3 * n].reshape(n, 3), ent[:ne.value]
+
+    def direct_grav(self, targets):
+        """FP64 direct-sum (a, pot) of the given caller indices (G = 1, spline
+        softening h_i + h_j)."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        a = np.empty((len(t), 3))
+        pot = np.empty(len(t))
+        _check(lib().gg_direct_grav(self.h, len(t), _p(t, _ip), _p(a), _p(pot)))
+        return a, pot
 
     def tree_info(self):
         nn, root = C.c_int(), C.c_int()

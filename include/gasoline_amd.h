@@ -154,6 +154,14 @@ int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucket_node,
 int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
                  double *dbls);
 
+/* FP64 direct-sum accelerations and potentials of the particles target[k]
+ * (caller indices) from every other particle, with the particle-particle
+ * spline softening twoh = h_i + h_j (grav.c:81-108) and G = 1: the
+ * reference force against which the tree's force errors are stated.  No
+ * counterpart in the reference (it has no direct-sum path); a[3*k..],
+ * pot[k] are host arrays. */
+int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
+
 /* ----------------------------------------------------------------- SPH */
 
 /* Smoothing parameters: the subset of gasoline's SMF (smoothfcn.h:22-130)
@@ -207,8 +215,10 @@ int gg_gas_pressure(GGPKD pkd, const gg_smf *smf);
  * whose stored fBall2 is positive (a hint: the previous step's value, or a
  * proposal of the h-proposal network), widening it if it holds fewer than
  * nSmooth particles; the result does not depend on it.  Default 1.15; an
- * accurate hint does with less.  No counterpart in the reference, whose
- * search starts from the bucket's own particles (smooth.c:1204-1305). */
+ * accurate hint does with less.  slack = 0 ignores the stored fBall2: every
+ * search starts cold, from the density of the particle's tree cell.  No
+ * counterpart in the reference, whose search starts from the bucket's own
+ * particles (smooth.c:1204-1305). */
 int gg_set_seed_slack(GGPKD pkd, double slack);
 
 /* Neighbour lists of the last gg_smooth, for parity checks: nbr[i*nSmooth+k]

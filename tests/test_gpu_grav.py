@@ -194,6 +194,21 @@ def test_small_and_degenerate_inputs(gpu):
             pk.build_binary(16, 1, 0.7, 2)
 
 
+def test_direct_sum_vs_oracle(gpu):
+    # the FP64 direct sum used to state force errors at large N: the oracle's
+    # O(N^2) sum on sampled targets, coincident pairs included
+    r, v, m = ics.plummer(6000, seed=12)
+    r[17] = r[5]
+    soft = np.where(np.arange(len(m)) % 2 == 0, 0.01, 0.03)
+    idx = np.random.default_rng(2).choice(len(m), 700, replace=False)
+    idx[0] = 5
+    ao, po = O.direct_sum(r, m, soft, idx)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64) as pk:
+        a, pot = pk.direct_grav(idx)
+    assert rel_err(a, ao).max() < 1e-12
+    assert np.max(np.abs(pot / po - 1)) < 1e-12
+
+
 def test_c_host_driver(gpu):
     # the C host sequences (gasoline_amd/host) drive a gravity and an SPH
     # step through the C ABI and check conservation

@@ -92,6 +92,76 @@ def test_knn_lattice_ties(gpu):
     assert np.array_equal(np.sort(nbr, axis=1), np.sort(nbr_o, axis=1))
 
 
+def test_gas_subset_of_a_mixed_store(gpu):
+    # a disk galaxy's store holds halo and gas particles: the SPH path works on
+    # the TYPE_GAS ones only, exactly as on a store of the gas alone
+    d = ics.disk_galaxy(1 << 14, 1 << 13, seed=5)
+    g = d["gas"]
+    smf = gpu.Smf.default(iViscosityLimiter=1)
+    res = []
+    for sel, bits in ((np.ones(len(g), bool), np.where(g, 3 | gpu.TYPE_GAS, 3)),
+                      (g, np.full(g.sum(), 3 | gpu.TYPE_GAS))):
+        with gpu.Pkd(int(sel.sum())) as pk:
+            for f, k in (("r", "r"), ("v", "v"), ("vPred", "v"), ("fMass", "m"), ("u", "u"),
+                         ("uPred", "u")):
+                pk.put(f, d[k][sel])
+            pk.put("iActive", bits.astype(np.uint32))
+            pk.build_local()
+            pk.smooth(smf, 32, False)
+            pk.gas_pressure(smf)
+            pk.resmooth(smf, 32, False)
+            gi = np.nonzero(g[sel])[0]
+            res.append({f: pk.get(f)[gi] for f in ("fBall2", "fDensity", "divv", "a", "uDotPdV",
+                                                    "uDotAV")})
+            if sel.all():
+                halo = ~g
+                assert np.all(pk.get("fDensity")[halo] == 0) and np.all(pk.get("a")[halo] == 0)
+    for f in res[0]:
+        assert np.array_equal(res[0][f], res[1][f]), f
+
+
+def test_knn_isolated_particles(gpu):
+    # particles far outside a uniform ball: their K nearest lie in a thin
+    # shell on the ball's near side (the 64th and 256th squared distances
+    # differ by about 1.4%), which the search must still bracket
+    rng = np.random.default_rng(8)
+    n = 20000
+    v = rng.normal(size=(n, 3))
+    r = (rng.uniform(size=n) ** (1 / 3))[:, None] * v / np.linalg.norm(v, axis=1)[:, None]
+    far = np.array([[9.0, 0, 0], [0, -12.0, 0.5], [3.0, 4.0, -7.0]])
+    r = ics._f32(np.concatenate([r, far]))
+    nbr_o, fb2_o, _ = O.knn(r, 64)
+    with gpu.Pkd(len(r)) as pk:
+        pk.put("r", r)
+        pk.put("fMass", np.full(len(r), 1.0 / len(r)))
+        for slack in (0.0, 1.15):
+            pk.set_seed_slack(slack)
+            pk.build_local()
+            pk.smooth(gpu.Smf.default(), 64, False)
+            assert np.array_equal(pk.get("fBall2"), fb2_o)
+            assert np.array_equal(np.sort(pk.smooth_lists(64), 1), np.sort(nbr_o, 1))
+
+
+def test_knn_cold_switch(gpu):
+    # slack 0 ignores a stored fBall2: the search, its work counters and its
+    # result equal those of a store with no hint at all
+    d = ics.evrard(20000, seed=6)
+    out = []
+    for fb2, slack in ((np.zeros(20000), 1.15), (np.full(20000, 1e-4), 0.0)):
+        with gpu.Pkd(20000) as pk:
+            pk.put("r", d["r"])
+            pk.put("fMass", d["m"])
+            pk.put("fBall2", fb2)
+            pk.set_seed_slack(slack)
+            pk.build_local()
+            pk.smooth(gpu.Smf.default(), 32, False)
+            st = pk.smooth_stats()
+            out.append((pk.get("fBall2"), np.sort(pk.smooth_lists(32), 1), st["candidates"],
+                        st["extra_passes"]))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2:] == out[1][2:]
+
+
 @pytest.mark.parametrize("prec", ["FP64", "FP32"])
 @pytest.mark.parametrize("case", ["box64k", "evrard32k"])
 def test_density_and_hydro_vs_oracle(gpu, case, prec, request):
