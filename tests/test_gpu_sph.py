@@ -113,6 +113,9 @@ def test_gas_subset_of_a_mixed_store(gpu):
             gi = np.nonzero(g[sel])[0]
             res.append({f: pk.get(f)[gi] for f in ("fBall2", "fDensity", "divv", "a", "uDotPdV",
                                                     "uDotAV")})
+            st = pk.smooth_stats()
+            res[-1]["work"] = np.array([st["queries"], st["candidates"]])
+            assert st["queries"] == g.sum() and st["candidates"] > 32 * g.sum()
             if sel.all():
                 halo = ~g
                 assert np.all(pk.get("fDensity")[halo] == 0) and np.all(pk.get("a")[halo] == 0)
