@@ -46,6 +46,8 @@ def sph_setup(gg, n, prec, seed=2):
     pk.put("fMass", d["m"])
     pk.put("vPred", d["v"])
     pk.put("uPred", d["u"])
+    pk.build_local(period=[1.0, 1.0, 1.0])
+    pk.reorder()                         # the store in the structure's (Morton) order
     return pk
 
 
@@ -172,7 +174,8 @@ def time_kdk(gg, torch, d, soft, bits, prec, mode, K, dt, theta, steps, warmup, 
     build, gravity walk and interactions, SPH tree, kNN + DenDVDX, EOS and
     SphPressureTermsSym each step.  mode: "mlp" (h-proposal network seeds
     the search, trained online), "prev" (previous fBall2 seeds it) or "cold"
-    (no seed: the plain search)."""
+    (no seed: the plain search).  The store is kept in the SPH structure's
+    order (gg_reorder after each SPH build)."""
     from gasoline_amd import evolve
     n = len(d["m"])
     pk = gg.Pkd(n, prec=prec)
@@ -183,7 +186,7 @@ def time_kdk(gg, torch, d, soft, bits, prec, mode, K, dt, theta, steps, warmup, 
         hm = dict(train_iters=50, batch=8192, lr=3e-3)
     pk.set_seed_slack({"cold": 0.0, "mlp": 1.08, "prev": 1.15}[mode])
     run = evolve.KDK(pk, gg.Smf.default(iViscosityLimiter=visc), nSmooth=K, theta=theta,
-                     hmlp=hm)
+                     hmlp=hm, reorder=True)
     run.forces(dt)
     for _ in range(warmup):
         run.step(dt)
@@ -298,7 +301,8 @@ def main():
     if args.disk:
         from gasoline_amd import ics
         d = ics.disk_galaxy(seed=11)
-        bits = np.where(d["gas"], 3 | gg.TYPE_GAS, 3).astype(np.uint32)
+        gas_bits = gg.TYPE_ACTIVE | gg.TYPE_TREEACTIVE | gg.TYPE_SMOOTHACTIVE | gg.TYPE_GAS
+        bits = np.where(d["gas"], gas_bits, 3).astype(np.uint32)
         ks = max(2, args.steps // 2)
         f32 = time_kdk(gg, torch, d, d["soft"], bits, gg.PREC_FP32, "prev", 32, 5e-4, 0.7, ks, 1)
         f64 = time_kdk(gg, torch, d, d["soft"], bits, gg.PREC_FP64, "prev", 32, 5e-4, 0.7, ks, 1)

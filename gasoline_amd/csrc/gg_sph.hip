@@ -15,6 +15,8 @@
 // not depend on this structure.
 #include "gg_internal.h"
 
+#include <utility>
+
 #include <cstring>
 #include <algorithm>
 #include <cmath>
@@ -2447,6 +2449,127 @@ extern "C" int gg_smooth_lists(GGPKD pkd, int nSmooth, int *nbr)
         if (!qf[s]) continue;
         for (int k = 0; k < K; ++k) nbr[(size_t)ord[s] * K + k] = ord[lst[(size_t)s * K + k]];
     }
+    return GG_OK;
+}
+
+namespace gg {
+// ---------------------------------------------------------- store reorder
+__global__ void k_fill_i32(int n, int v, int *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Slots of the SPH structure: their caller indices are not "other".
+__global__ void k_mark_slots(int ns, const int *__restrict__ order, int *__restrict__ flag)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) flag[order[s]] = 0;
+}
+
+__global__ void k_perm_build(int n, int ns, const int *__restrict__ order,
+                             const int *__restrict__ flag, const int64_t *__restrict__ pos,
+                             int *__restrict__ perm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i < ns) perm[i] = order[i];
+    if (flag[i]) perm[ns + (int)pos[i]] = i;   // the others, in their previous order
+}
+
+// w interleaved components per particle, in np planes of n particles each
+template <typename T>
+__global__ void k_permute(int n, int w, int np, const int *__restrict__ perm,
+                          const T *__restrict__ src, T *__restrict__ dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t o = (size_t)w * perm[k];
+    for (int p = 0; p < np; ++p)
+        for (int c = 0; c < w; ++c)
+            dst[(size_t)p * w * n + (size_t)w * k + c] = src[(size_t)p * w * n + o + c];
+}
+
+__global__ void k_iota(int n, int *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// The permuted copy goes to a spare buffer of the same capacity, which then
+// takes the old one's place: spares are kept per capacity, so after the first
+// call no allocation happens.
+static DevBuf &spare(GGPKD pkd, size_t bytes)
+{
+    for (DevBuf &b : pkd->sph.spares)
+        if (b.bytes == bytes) return b;
+    pkd->sph.spares.emplace_back();
+    return pkd->sph.spares.back();
+}
+
+template <typename T>
+static int permute_buf(GGPKD pkd, DevBuf &buf, int w, int np, const int *perm)
+{
+    const int n = pkd->nLocal;
+    DevBuf &tmp = spare(pkd, buf.bytes);
+    GG_TRY(tmp.reserve(buf.bytes));
+    hipLaunchKernelGGL((k_permute<T>), dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, w, np,
+                       perm, buf.as<T>(), tmp.as<T>());
+    GG_HIP(hipGetLastError());
+    std::swap(buf, tmp);
+    return GG_OK;
+}
+}  // namespace gg
+
+// Permutes the particle store into the SPH structure's slot order (the gas
+// in Morton order first, then the other particles in their previous order),
+// as pkdBuildLocal reorders pStore while it builds (pkd.c:2732-2848), so that
+// the gathers and scatters between slot and store order read consecutive
+// memory.  perm (optional, host): the previous index of the particle now at
+// index k.
+extern "C" int gg_reorder(GGPKD pkd, int *perm)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    SphDev &S = pkd->sph;
+    GG_CHECK(S.built, GG_ESTATE, "gg_reorder before gg_build_local");
+    const int n = pkd->nLocal;
+    GG_CHECK(S.n <= n, GG_ESTATE, "SPH structure larger than the store");
+    hipStream_t st = pkd->stream;
+    if (n == 0) return GG_OK;
+    DevBuf &P = S.sel;          // scratch: the permutation
+    DevBuf &F = S.diffcnt;      // scratch: flags, then their prefix sums in S.pos
+    GG_TRY(P.reserve(sizeof(int) * n));
+    GG_TRY(F.reserve(sizeof(int) * n));
+    GG_TRY(S.pos.reserve(sizeof(int64_t) * ((size_t)n + 1)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, st, n, 1, F.as<int>());
+    hipLaunchKernelGGL(k_mark_slots, dim3(nblk(S.n, 256)), dim3(256), 0, st, S.n,
+                       S.order.as<int>(), F.as<int>());
+    GG_TRY(exclusive_scan_i64(st, F.as<int>(), S.pos.as<int64_t>(), n, S.sortTmp, nullptr));
+    hipLaunchKernelGGL(k_perm_build, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.n,
+                       S.order.as<int>(), F.as<int>(), S.pos.as<int64_t>(), P.as<int>());
+    GG_HIP(hipGetLastError());
+    const int *pp = P.as<int>();
+    int rc = GG_OK;
+    for (int f = 0; f < GG_NFIELDS && rc == GG_OK; ++f) {
+        if (!pkd->f[f].p) continue;
+        rc = f == GG_IACTIVE ? permute_buf<uint32_t>(pkd, pkd->f[f], 1, 1, pp)
+                             : permute_buf<double>(pkd, pkd->f[f], field_width(f), 1, pp);
+    }
+    HmlpDev &M = pkd->mlp;
+    if (rc == GG_OK && M.nHist == n) {   // features: 4 planes of n floats
+        rc = permute_buf<float>(pkd, M.hx, 1, 4, pp);
+        if (rc == GG_OK) rc = permute_buf<int>(pkd, M.hv, 1, 1, pp);
+    }
+    if (rc == GG_OK) {
+        hipLaunchKernelGGL(k_iota, dim3(nblk(S.n, 256)), dim3(256), 0, st, S.n, S.order.as<int>());
+        if (perm && hipMemcpyAsync(perm, P.p, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = GG_EHIP;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && rc == GG_OK) rc = GG_EHIP;
+    GG_CHECK(rc == GG_OK, rc, "gg_reorder: %s", rc == GG_EHIP ? "HIP error" : "failed");
+    // the gravity tree and its lists refer to the old indices
+    pkd->grav.built = 0;
+    pkd->grav.listsCurrent = 0;
     return GG_OK;
 }
 

@@ -6,7 +6,9 @@ force path (gravity tree + SPH) and the device kick/drift through the C ABI.
           -> drift (r by dt) -> forces at the new positions -> close kick.
 
 Forces: gg_init_accel, then gravity (gg_build_binary + gg_grav_all) if
-enabled, then SPH (gg_build_local, gg_smooth(DenDVDX), gg_gas_pressure,
+enabled, then SPH (gg_build_local, optionally gg_reorder, which permutes the
+store into the SPH structure's order as pkdBuildLocal does, gg_smooth(DenDVDX),
+gg_gas_pressure,
 gg_resmooth(SphPressureTermsSym), which adds the hydro acceleration and the
 uDot terms).  The h-proposal network, when given, seeds the density pass's
 search radius from the recorded history and is trained after it.
@@ -18,7 +20,7 @@ from . import pkd as P
 
 class KDK:
     def __init__(self, pk, smf, nSmooth=64, gravity=True, theta=0.7, iOrder=2, nBucket=16,
-                 period=None, hmlp=None):
+                 period=None, hmlp=None, reorder=False, track_order=False):
         self.pk, self.smf, self.K = pk, smf, nSmooth
         self.gravity, self.theta, self.iOrder, self.nBucket = gravity, theta, iOrder, nBucket
         self.period = period
@@ -26,6 +28,9 @@ class KDK:
         self.dt_prev = None
         self.stats = []             # kNN counters of each density pass
         self.grav_stats = None      # counters of the last gravity evaluation
+        self.reorder = reorder      # keep the store in the SPH structure's order
+        self.track_order = track_order
+        self.iord = None            # with track_order: original index of each stored particle
 
     def seed_search(self):
         """Before a density pass: the network's proposal, if trained."""
@@ -39,6 +44,10 @@ class KDK:
             pk.build_binary(self.nBucket, 1, self.theta, self.iOrder)
             self.grav_stats = pk.grav_all(self.iOrder)
         pk.build_local(period=self.period)
+        if self.reorder:
+            perm = pk.reorder(want_perm=self.track_order)
+            if self.track_order:
+                self.iord = perm if self.iord is None else self.iord[perm]
         self.seed_search()
         pk.smooth(self.smf, self.K, self.period is not None)
         self.stats.append(pk.smooth_stats())

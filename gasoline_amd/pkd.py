@@ -44,7 +44,7 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
-       "gg_set_seed_slack", "gg_init_accel", "gg_kick", "gg_drift",
+       "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
        "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
        "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
 
@@ -111,6 +111,7 @@ def lib():
     L.gg_grav_lists.argtypes = [_vp, _ip, _i64p, _ip, _i64p, _ip, _ip]
     L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
     L.gg_direct_grav.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
+    L.gg_reorder.argtypes = [_vp, _ip]
     L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
     L.gg_smooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int, _ip]
     L.gg_resmooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int]
@@ -307,6 +308,14 @@ class Pkd:
         return dict(retried=int(s[0]), ties=int(s[1]), extras=int(s[2]), queries=int(s[3]),
                     groups_overflowed=int(s[4]), short_at_RL=int(s[5]), unsettled=int(s[6]),
                     max_buckets=int(s[7]), extra_passes=int(s[8]), candidates=int(s[9]))
+
+    def reorder(self, want_perm=True):
+        """Permute the store into the SPH structure's order (gg_reorder);
+        returns perm with new[k] = old[perm[k]] (None if not wanted: the
+        copy to the host is then skipped)."""
+        perm = np.empty(self.n, np.int32) if want_perm else None
+        _check(lib().gg_reorder(self.h, _p(perm, _ip) if want_perm else None))
+        return perm
 
     def set_seed_slack(self, slack):
         _check(lib().gg_set_seed_slack(self.h, float(slack)))

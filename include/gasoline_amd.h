@@ -221,6 +221,18 @@ int gg_gas_pressure(GGPKD pkd, const gg_smf *smf);
  * particles (smooth.c:1204-1305). */
 int gg_set_seed_slack(GGPKD pkd, double slack);
 
+/* Permute the particle store into the order of the SPH structure built by
+ * gg_build_local: its particles in Morton order first, then the others in
+ * their previous order; every stored field and the h-proposal history move
+ * with their particles.  pkdBuildLocal reorders pStore in place the same way
+ * while it builds (pkd.c:2732-2848); here it is a separate call so that a
+ * caller keeping its own indices can skip it.  Afterwards the gathers and
+ * scatters between the structure and the store read consecutive memory.
+ * perm (host, nLocal ints, may be null) receives for each new index k the
+ * previous index of the particle now there.  The gravity tree is dropped
+ * (build it again before gg_grav_all). */
+int gg_reorder(GGPKD pkd, int *perm);
+
 /* Neighbour lists of the last gg_smooth, for parity checks: nbr[i*nSmooth+k]
  * in caller order.  The order within a list is the search's own (the
  * reference's is its priority-queue order); the set is exact. */

@@ -10,7 +10,7 @@ second-derivative quantities dvds and for single values near zero)."""
 import numpy as np
 import pytest
 
-from gasoline_amd import ics
+from gasoline_amd import evolve, ics
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -99,8 +99,9 @@ def test_gas_subset_of_a_mixed_store(gpu):
     g = d["gas"]
     smf = gpu.Smf.default(iViscosityLimiter=1)
     res = []
-    for sel, bits in ((np.ones(len(g), bool), np.where(g, 3 | gpu.TYPE_GAS, 3)),
-                      (g, np.full(g.sum(), 3 | gpu.TYPE_GAS))):
+    gas_bits = gpu.TYPE_ACTIVE | gpu.TYPE_TREEACTIVE | gpu.TYPE_SMOOTHACTIVE | gpu.TYPE_GAS
+    for sel, bits in ((np.ones(len(g), bool), np.where(g, gas_bits, 3)),
+                      (g, np.full(g.sum(), gas_bits))):
         with gpu.Pkd(int(sel.sum())) as pk:
             for f, k in (("r", "r"), ("v", "v"), ("vPred", "v"), ("fMass", "m"), ("u", "u"),
                          ("uPred", "u")):
@@ -143,6 +144,49 @@ def test_knn_isolated_particles(gpu):
             pk.smooth(gpu.Smf.default(), 64, False)
             assert np.array_equal(pk.get("fBall2"), fb2_o)
             assert np.array_equal(np.sort(pk.smooth_lists(64), 1), np.sort(nbr_o, 1))
+
+
+def test_reorder_moves_particles_and_keeps_results(gpu):
+    # gg_reorder permutes the store into the SPH structure's order: every
+    # field and the h-proposal history move with their particles, the SPH
+    # results are the same particle by particle, and the gravity tree must be
+    # built again
+    d = ics.evrard(20000, seed=9)
+    d["v"] = -0.3 * d["r"]
+    n = len(d["m"])
+    smf = gpu.Smf.default(iViscosityLimiter=1)
+    out = []
+    for reorder in (False, True):
+        with gpu.Pkd(n) as pk:
+            evolve.start(pk, d, 0.01)
+            pk.build_local()
+            pk.smooth(smf, 32, False)
+            pk.hmlp_init(2)
+            pk.hmlp_record(0.05)
+            pk.build_binary(16, 1, 0.7, 2)
+            perm = np.arange(n)
+            if reorder:
+                pk.build_local()
+                perm = pk.reorder()
+                assert np.array_equal(np.sort(perm), np.arange(n)) and not np.all(perm == np.arange(n))
+                with pytest.raises(gpu.GGError):
+                    pk.grav_all(2)
+                assert np.array_equal(pk.get("r"), d["r"][perm])
+            pk.hmlp_predict()
+            res = {"hint": pk.get("fBall2")}
+            pk.build_local()
+            pk.smooth(smf, 32, False)
+            pk.gas_pressure(smf)
+            pk.resmooth(smf, 32, False)
+            for f in ("fBall2", "fDensity", "divv", "curlv", "a", "uDotPdV", "uDotAV", "fMass"):
+                res[f] = pk.get(f)
+            pk.build_binary(16, 1, 0.7, 2)
+            pk.init_accel()
+            pk.grav_all(2)
+            out.append((perm, res))
+    perm = out[1][0]
+    for f in out[0][1]:
+        assert np.array_equal(out[1][1][f], out[0][1][f][perm]), f
 
 
 def test_knn_cold_switch(gpu):
