@@ -1,6 +1,7 @@
 // Handle management, particle-store I/O, device scan and timing for the
 // C ABI of libgasoline_amd (include/gasoline_amd.h).
 #include "gg_internal.h"
+#include "gg_scan.h"
 
 #include <cstdarg>
 #include <cstdio>
@@ -98,83 +99,10 @@ int ensure_field(GGPKD pkd, int field)
 }
 
 // ------------------------------------------------------------ scan
-constexpr int SCAN_B = 1024;
-
-__global__ void k_scan_block(int n, const int *__restrict__ in, int64_t *__restrict__ out,
-                             int64_t *__restrict__ bsum)
-{
-    __shared__ int64_t s[SCAN_B];
-    const int i = blockIdx.x * SCAN_B + threadIdx.x;
-    const int64_t v = i < n ? in[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < SCAN_B; o <<= 1) {
-        const int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    if (i < n) out[i] = s[threadIdx.x] - v;          // exclusive within block
-    if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = s[threadIdx.x];
-}
-
-// Exclusive scan of the block sums by one workgroup, carrying across chunks.
-__global__ void k_scan_sums(int nb, int64_t *__restrict__ bsum, int64_t *__restrict__ total)
-{
-    __shared__ int64_t s[SCAN_B];
-    __shared__ int64_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += SCAN_B) {
-        const int i = base + threadIdx.x;
-        const int64_t v = i < nb ? bsum[i] : 0;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < SCAN_B; o <<= 1) {
-            const int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == SCAN_B - 1) carry += s[threadIdx.x];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
-__global__ void k_scan_add(int n, const int64_t *__restrict__ bsum, int64_t *__restrict__ out,
-                           const int64_t *__restrict__ total)
-{
-    const int i = blockIdx.x * SCAN_B + threadIdx.x;
-    if (i < n) out[i] += bsum[blockIdx.x];
-    if (i == n) out[n] = *total;
-}
-
-// out[0..n] = exclusive prefix sums of in[0..n-1]; out[n] = total, which is
-// also returned to the host.  out must hold n+1 entries.
 int exclusive_scan_i64(hipStream_t st, const int *in, int64_t *out, int n, DevBuf &scratch,
                        int64_t *total_host)
 {
-    const int nb = (n + SCAN_B - 1) / SCAN_B;
-    GG_TRY(scratch.reserve(sizeof(int64_t) * ((size_t)nb + 2)));
-    int64_t *bsum = scratch.as<int64_t>();
-    int64_t *tot = bsum + nb + 1;
-    if (nb > 0) {
-        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(SCAN_B), 0, st, n, in, out, bsum);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, st, nb, bsum, tot);
-        hipLaunchKernelGGL(k_scan_add, dim3(nb + 1), dim3(SCAN_B), 0, st, n, bsum, out, tot);
-        GG_HIP(hipGetLastError());
-        if (total_host) {
-            GG_HIP(hipMemcpyAsync(total_host, tot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-            GG_HIP(hipStreamSynchronize(st));
-        }
-    } else {
-        GG_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), st));
-        if (total_host) *total_host = 0;
-    }
-    return GG_OK;
+    return exclusive_scan(st, ScanArray{in}, out, n, scratch, total_host);
 }
 
 // ----------------------------------------------------------- timing

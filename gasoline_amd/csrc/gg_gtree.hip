@@ -20,6 +20,7 @@
 //
 // Several independent systems build at once: each is a root of its own.
 #include "gg_internal.h"
+#include "gg_scan.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -54,61 +55,55 @@ __global__ void k_gt_init(int n, const double *__restrict__ r, double *__restric
     src[i] = i;
 }
 
-// Flag of each position of a partitioning cell: 1 = belongs to the lower part.
+// Flag of position i of a partitioning cell: 1 = belongs to the lower part.
 // mode 0: coordinate below the cell's split; mode 1: type bits & mask set.
-__global__ void k_gt_flag(int n, int a, int b, int mode, uint32_t mask,
-                          const int *__restrict__ pnode, const GNode *__restrict__ nd,
-                          const double *__restrict__ x, const double *__restrict__ y,
-                          const double *__restrict__ z, const int *__restrict__ src,
-                          const uint32_t *__restrict__ bits, int *__restrict__ flag)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = pnode[i];
-    int f = 0;
-    if (c >= a && c < b) {
+// Evaluated inside the prefix-sum pass and again by the rank and swap
+// kernels, each time before the position's particle can have moved.
+struct FlagFn {
+    const int *pnode;
+    const GNode *nd;
+    const double *x, *y, *z;
+    const int *src;
+    const uint32_t *bits;
+    int a, b, mode;
+    uint32_t mask;
+    __device__ __forceinline__ int operator()(int64_t i) const
+    {
+        const int c = pnode[i];
+        if (c < a || c >= b) return 0;
         const GNode &g = nd[c];
-        if (mode == 1) {
-            f = (bits[src[i]] & mask) != 0;
-        } else if (g.dim >= 0) {
-            const double v = g.dim == 0 ? x[i] : (g.dim == 1 ? y[i] : z[i]);
-            f = v < g.split;
-        }
+        if (mode == 1) return (bits[src[i]] & mask) != 0;
+        if (g.dim < 0) return 0;
+        const double v = g.dim == 0 ? x[i] : (g.dim == 1 ? y[i] : z[i]);
+        return v < g.split;
     }
-    flag[i] = f;
-}
+};
 
-// Split position s = lo + number of lower-part particles, children counts.
-// A split that leaves one side empty (the reference asserts against it,
-// pkd.c:2512-2516) sets *err.
+// Split position s = lo + number of lower-part particles.  A split that
+// leaves one side empty (the reference asserts against it, pkd.c:2512-2516)
+// sets *err.
 __global__ void k_gt_count(int a, int b, int mode, const int64_t *__restrict__ F,
-                           GNode *__restrict__ nd, int *__restrict__ nch, int *__restrict__ err)
+                           GNode *__restrict__ nd, int *__restrict__ err)
 {
     const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b) return;
     GNode &g = nd[c];
     const bool part = mode == 1 || g.dim >= 0;
     g.s = part ? g.lo + (int)(F[g.hi + 1] - F[g.lo]) : g.lo;
-    if (mode == 0) {
-        const bool split = g.dim >= 0;
-        if (split && (g.s == g.lo || g.s > g.hi)) atomicOr(err, 1);
-        nch[c - a] = split ? 2 : 0;
-    }
+    if (mode == 0 && part && (g.s == g.lo || g.s > g.hi)) atomicOr(err, 1);
 }
 
 // Positions of the upper-part positions that hold a lower particle, by
 // rank counted down from hi: posR[lo + k].
-__global__ void k_gt_rank(int n, int a, int b, int mode, const int *__restrict__ pnode,
-                          const GNode *__restrict__ nd, const int *__restrict__ flag,
-                          const int64_t *__restrict__ F, int *__restrict__ posR)
+__global__ void k_gt_rank(int n, FlagFn fl, const int64_t *__restrict__ F, int *__restrict__ posR)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = pnode[i];
-    if (c < a || c >= b) return;
-    const GNode &g = nd[c];
-    if (mode == 0 && g.dim < 0) return;
-    if (i >= g.s && flag[i]) {
+    const int c = fl.pnode[i];
+    if (c < fl.a || c >= fl.b) return;
+    const GNode &g = fl.nd[c];
+    if (fl.mode == 0 && g.dim < 0) return;
+    if (i >= g.s && fl(i)) {
         const int k = (int)(F[g.hi + 1] - F[i + 1]);
         posR[g.lo + k] = i;
     }
@@ -116,20 +111,23 @@ __global__ void k_gt_rank(int n, int a, int b, int mode, const int *__restrict__
 
 // The swaps of the Hoare scan: the lower-part position holding the k-th
 // upper particle (counted up from lo) <-> posR[lo + k], each pair swapped
-// by its lower member.
-__global__ void k_gt_swap(int n, int a, int b, int mode, const int *__restrict__ pnode,
-                          const GNode *__restrict__ nd, const int *__restrict__ flag,
-                          const int64_t *__restrict__ F, const int *__restrict__ posR,
-                          double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
-                          int *__restrict__ src)
+// by its lower member.  With descend set, every position of the level also
+// moves to its cell's child (lower part: ch0, upper: ch1; -1 in a bucket).
+__global__ void k_gt_swap(int n, FlagFn fl, int descend, const int64_t *__restrict__ F,
+                          const int *__restrict__ posR, double *__restrict__ x,
+                          double *__restrict__ y, double *__restrict__ z, int *__restrict__ src,
+                          int *pnode)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = pnode[i];
-    if (c < a || c >= b) return;
-    const GNode &g = nd[c];
-    if (mode == 0 && g.dim < 0) return;
-    if (i < g.s && !flag[i]) {
+    const int c = fl.pnode[i];
+    if (c < fl.a || c >= fl.b) return;
+    const GNode &g = fl.nd[c];
+    if (fl.mode == 0 && g.dim < 0) {
+        if (descend) pnode[i] = -1;
+        return;
+    }
+    if (i < g.s && !fl(i)) {
         const int k = (i - g.lo) - (int)(F[i] - F[g.lo]);
         const int j = posR[g.lo + k];
         double t = x[i]; x[i] = x[j]; x[j] = t;
@@ -137,6 +135,7 @@ __global__ void k_gt_swap(int n, int a, int b, int mode, const int *__restrict__
         t = z[i]; z[i] = z[j]; z[j] = t;
         const int u = src[i]; src[i] = src[j]; src[j] = u;
     }
+    if (descend) pnode[i] = i < g.s ? g.ch0 : g.ch1;
 }
 
 // Work items of the cells [a, b): ceil(count / ch) each for cells of more
@@ -209,7 +208,7 @@ __global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
                                                    const double *__restrict__ x,
                                                    const double *__restrict__ y,
                                                    const double *__restrict__ z,
-                                                   GNode *__restrict__ nd)
+                                                   GNode *__restrict__ nd, int *__restrict__ nch)
 {
 #pragma clang fp contract(off)
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -240,6 +239,7 @@ __global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
         }
     if (lane) return;
     bool extent = false;
+    nch[r] = 0;
     for (int d = 0; d < 3; ++d) extent |= bmax[d] > bmin[d];
     g.dim = -1;
     g.split = 0.0;
@@ -250,47 +250,38 @@ __global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
             if (bmax[k] - bmin[k] > bmax[d] - bmin[d]) d = k;
         g.dim = d;
         g.split = 0.5 * (bmin[d] + bmax[d]);
+        nch[r] = 2;
     }
 }
 
-// Children of the level's split cells, at next + their offsets.
-__global__ void k_gt_children(int a, int b, int next, const int64_t *__restrict__ choff,
-                              GNode *__restrict__ nd)
+// Child ids of the level's split cells: next + their offsets.
+__global__ void k_gt_kids(int a, int b, int next, const int64_t *__restrict__ choff,
+                          GNode *__restrict__ nd)
 {
     const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= b) return;
     GNode &g = nd[c];
     if (g.dim < 0) return;
-    int k = next + (int)choff[c - a];
-    if (g.s > g.lo) {   // lower part [lo, s-1]
-        GNode &q = nd[k];
-        q.lo = g.lo;
-        q.hi = g.s - 1;
-        q.par = c;
-        q.side = 0;
-        q.root = g.root;
-        g.ch0 = k++;
-    }
-    if (g.s <= g.hi) {  // upper part [s, hi]
-        GNode &q = nd[k];
-        q.lo = g.s;
-        q.hi = g.hi;
-        q.par = c;
-        q.side = 1;
-        q.root = g.root;
-        g.ch1 = k;
-    }
+    g.ch0 = next + (int)choff[c - a];
+    g.ch1 = g.ch0 + 1;
 }
 
-__global__ void k_gt_descend(int n, int a, int b, const GNode *__restrict__ nd,
-                             int *__restrict__ pnode)
+// The children's records once the split positions are known.
+__global__ void k_gt_children(int a, int b, GNode *__restrict__ nd)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = pnode[i];
-    if (c < a || c >= b) return;
+    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
     const GNode &g = nd[c];
-    pnode[i] = g.dim < 0 ? -1 : (i < g.s ? g.ch0 : g.ch1);
+    if (g.dim < 0) return;
+    GNode &l = nd[g.ch0], &u = nd[g.ch1];
+    l.lo = g.lo;
+    l.hi = g.s - 1;
+    u.lo = g.s;
+    u.hi = g.hi;
+    l.par = u.par = c;
+    l.side = 0;
+    u.side = 1;
+    l.root = u.root = g.root;
 }
 
 __global__ void k_gt_size(int a, int b, GNode *__restrict__ nd)
@@ -657,28 +648,34 @@ __global__ void k_gt_roots(int nr, const int *__restrict__ rlo, const int *__res
 
 }  // namespace
 
-// One partition pass over the cells [a, b) (mode 0: their splits; mode 1:
-// type bits & mask, every cell).
+// One partition pass over the cells [a, b) (mode 0: their splits, and every
+// position moves to its child; mode 1: type bits & mask, every cell).
 static int gt_partition(GGPKD pkd, int a, int b, int mode, uint32_t mask)
 {
     GravDev &G = pkd->grav;
     hipStream_t st = pkd->stream;
     const int n = pkd->nLocal;
     GNode *nd = G.gnode.as<GNode>();
-    hipLaunchKernelGGL(k_gt_flag, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode, mask,
-                       G.gpn.as<int>(), nd, G.px.as<double>(), G.py.as<double>(),
-                       G.pz.as<double>(), G.perm.as<int>(),
-                       pkd->f[GG_IACTIVE].as<uint32_t>(), G.gflag.as<int>());
-    GG_TRY(exclusive_scan_i64(st, G.gflag.as<int>(), G.gF.as<int64_t>(), n, G.scan, nullptr));
+    FlagFn fl;
+    fl.pnode = G.gpn.as<int>();
+    fl.nd = nd;
+    fl.x = G.px.as<double>();
+    fl.y = G.py.as<double>();
+    fl.z = G.pz.as<double>();
+    fl.src = G.perm.as<int>();
+    fl.bits = pkd->f[GG_IACTIVE].as<uint32_t>();
+    fl.a = a;
+    fl.b = b;
+    fl.mode = mode;
+    fl.mask = mask;
+    GG_TRY(exclusive_scan(st, fl, G.gF.as<int64_t>(), n, G.scan, nullptr));
     hipLaunchKernelGGL(k_gt_count, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, mode,
-                       G.gF.as<int64_t>(), nd, G.gnch.as<int>(), G.gerr.as<int>());
-    hipLaunchKernelGGL(k_gt_rank, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode,
-                       G.gpn.as<int>(), nd, G.gflag.as<int>(), G.gF.as<int64_t>(),
-                       G.gposR.as<int>());
-    hipLaunchKernelGGL(k_gt_swap, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, mode,
-                       G.gpn.as<int>(), nd, G.gflag.as<int>(), G.gF.as<int64_t>(),
-                       G.gposR.as<int>(), G.px.as<double>(), G.py.as<double>(),
-                       G.pz.as<double>(), G.perm.as<int>());
+                       G.gF.as<int64_t>(), nd, G.gerr.as<int>());
+    hipLaunchKernelGGL(k_gt_rank, dim3(nblk(n, 256)), dim3(256), 0, st, n, fl,
+                       G.gF.as<int64_t>(), G.gposR.as<int>());
+    hipLaunchKernelGGL(k_gt_swap, dim3(nblk(n, 256)), dim3(256), 0, st, n, fl, mode == 0 ? 1 : 0,
+                       G.gF.as<int64_t>(), G.gposR.as<int>(), G.px.as<double>(),
+                       G.py.as<double>(), G.pz.as<double>(), G.perm.as<int>(), G.gpn.as<int>());
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -707,7 +704,6 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
     GG_TRY(G.perm.reserve(sizeof(int) * n1));
     GG_TRY(G.acc.reserve(sizeof(double) * 5 * n1));
     GG_TRY(G.gpn.reserve(sizeof(int) * n1));
-    GG_TRY(G.gflag.reserve(sizeof(int) * n1));
     GG_TRY(G.gF.reserve(sizeof(int64_t) * (n1 + 1)));
     GG_TRY(G.gposR.reserve(sizeof(int) * n1));
     GG_TRY(G.gnode.reserve(sizeof(GNode) * cap));
@@ -751,8 +747,9 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
                                G.py.as<double>(), G.pz.as<double>(), G.gbox.as<double>());
             hipLaunchKernelGGL(k_gt_decide, dim3(nblk(b - a, 4)), dim3(256), 0, st, a, b, nBucket,
                                G.gioff.as<int64_t>(), G.gbox.as<double>(), G.px.as<double>(),
-                               G.py.as<double>(), G.pz.as<double>(), nd);
-            GG_TRY(gt_partition(pkd, a, b, 0, 0));
+                               G.py.as<double>(), G.pz.as<double>(), nd, G.gnch.as<int>());
+            // child ids (two per split), then the partition, which also moves
+            // every position to its child
             int64_t nch = 0;
             GG_TRY(exclusive_scan_i64(st, G.gnch.as<int>(), G.gchoff.as<int64_t>(), b - a,
                                       G.scan, &nch));
@@ -762,10 +759,13 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
             GG_CHECK(!err, GG_EUNSUP,
                      "a cell split leaves one side empty (positions differ in the last bit only)");
             GG_CHECK((size_t)b + (size_t)nch <= cap, GG_EHIP, "tree cell capacity exceeded");
-            hipLaunchKernelGGL(k_gt_children, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, b,
-                               G.gchoff.as<int64_t>(), nd);
-            hipLaunchKernelGGL(k_gt_descend, dim3(nblk(n, 256)), dim3(256), 0, st, n, a, b, nd,
-                               G.gpn.as<int>());
+            if (nch > 0) {
+                hipLaunchKernelGGL(k_gt_kids, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b, b,
+                                   G.gchoff.as<int64_t>(), nd);
+                GG_TRY(gt_partition(pkd, a, b, 0, 0));
+                hipLaunchKernelGGL(k_gt_children, dim3(nblk(b - a, 256)), dim3(256), 0, st, a, b,
+                                   nd);
+            }
             GG_HIP(hipGetLastError());
             a = b;
             b += (int)nch;

@@ -78,7 +78,7 @@ struct GravDev {
     DevBuf broot;          // per active bucket: the root of its tree
     DevBuf tint, tdbl;     // per node: 6 ints, 8 doubles (gg_tree_info's layout)
     // tree build scratch
-    DevBuf gpn, gflag, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
+    DevBuf gpn, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
     DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof;
     hipGraph_t graph = nullptr;         // captured step (gg_grav_graph)
     hipGraphExec_t gexec = nullptr;
@@ -91,7 +91,7 @@ struct GravDev {
     {
         DevBuf *all[] = {&px, &py, &pz, &pm, &ph, &pact, &perm, &wnode, &cell, &momOH, &bucket,
                          &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
-                         &tdbl, &gpn, &gflag, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
+                         &tdbl, &gpn, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
                          &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof};
         for (DevBuf *b : all) b->release();
     }
