@@ -35,12 +35,13 @@ int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 constexpr int MOM_CH = 2048;     // particles per moment work item
 constexpr int NM = 32;           // per item: raw Q (6), O (10), H (15), Bmax
 constexpr int BOX_CH = 1024;     // particles per bounds work item
+constexpr int LMAX = 512;        // cells of at most LMAX particles finish in one workgroup
 
 // Per cell in creation (breadth-first) order.
 struct GNode {
     int lo, hi, par, side;       // particle range, parent, 0 lower / 1 upper
     int ch0, ch1, dim, root;     // children (-1), split axis (-1: bucket), root cell
-    int s, size, pre, pad;       // split position, subtree size, preorder id
+    int s, size, pre, depth;     // split position, subtree size, preorder id, level
     double split;
 };
 
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_gt_box(int a, int b, const int64_t *__r
 // Split decision of each cell of the level (pkd.c:2489-2505), one wavefront
 // per cell: its bounds from its particles, or from its items' bounds for a
 // cell of more than BOX_CH particles.
-__global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
+__global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket, int depth,
                                                    const int64_t *__restrict__ ioff,
                                                    const double *__restrict__ part,
                                                    const double *__restrict__ x,
@@ -244,7 +245,12 @@ __global__ __launch_bounds__(256) void k_gt_decide(int a, int b, int nBucket,
     g.dim = -1;
     g.split = 0.0;
     g.ch0 = g.ch1 = -1;
+    g.depth = depth;
     if (g.hi - g.lo + 1 > nBucket && extent) {
+        if (g.hi - g.lo + 1 <= LMAX) {   // split by k_gt_local
+            g.dim = -2;
+            return;
+        }
         int d = 0;
         for (int k = 1; k < 3; ++k)
             if (bmax[k] - bmin[k] > bmax[d] - bmin[d]) d = k;
@@ -284,21 +290,315 @@ __global__ void k_gt_children(int a, int b, GNode *__restrict__ nd)
     l.root = u.root = g.root;
 }
 
-__global__ void k_gt_size(int a, int b, GNode *__restrict__ nd)
+// Subtree sizes and preorder ids, over the nodes of one depth (list).
+__global__ void k_gt_size(int m, const int *__restrict__ list, GNode *__restrict__ nd)
 {
-    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= b) return;
-    GNode &g = nd[c];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    GNode &g = nd[list[k]];
     g.size = 1 + (g.ch0 >= 0 ? nd[g.ch0].size : 0) + (g.ch1 >= 0 ? nd[g.ch1].size : 0);
 }
 
-__global__ void k_gt_pre(int a, int b, GNode *__restrict__ nd)
+__global__ void k_gt_pre(int m, const int *__restrict__ list, GNode *__restrict__ nd)
 {
-    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= b) return;
-    const GNode &g = nd[c];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const GNode &g = nd[list[k]];
     if (g.ch0 >= 0) nd[g.ch0].pre = g.pre + 1;
     if (g.ch1 >= 0) nd[g.ch1].pre = g.pre + 1 + (g.ch0 >= 0 ? nd[g.ch0].size : 0);
+}
+
+// Nodes by depth: a count per depth, then each node to its depth's list.
+// Each workgroup counts in LDS
+// first, so the global counters see one atomic per workgroup and depth.
+constexpr int DMAX = 2048;       // depths a tree may have (level loop + local builds)
+
+__global__ __launch_bounds__(256) void k_gt_depth_count(int nn, int ndep,
+                                                        const GNode *__restrict__ nd,
+                                                        int *__restrict__ cnt)
+{
+    __shared__ int h[DMAX];
+    for (int d = threadIdx.x; d < ndep; d += 256) h[d] = 0;
+    __syncthreads();
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c < nn) atomicAdd(&h[nd[c].depth], 1);
+    __syncthreads();
+    for (int d = threadIdx.x; d < ndep; d += 256)
+        if (h[d]) atomicAdd(&cnt[d], h[d]);
+}
+
+__global__ __launch_bounds__(256) void k_gt_depth_list(int nn, int ndep,
+                                                       const GNode *__restrict__ nd,
+                                                       int *__restrict__ cur,
+                                                       int *__restrict__ list)
+{
+    __shared__ int h[DMAX], b0[DMAX];
+    for (int d = threadIdx.x; d < ndep; d += 256) h[d] = 0;
+    __syncthreads();
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const bool ok = c < nn;
+    const int dep = ok ? nd[c].depth : 0;
+    const int r = ok ? atomicAdd(&h[dep], 1) : 0;
+    __syncthreads();
+    for (int d = threadIdx.x; d < ndep; d += 256)
+        if (h[d]) b0[d] = atomicAdd(&cur[d], h[d]);
+    __syncthreads();
+    if (ok) list[b0[dep] + r] = c;
+}
+
+// Cells marked for the local build, and the scratch node slots each
+// reserves for its descendants (2 count - 2 at most).
+__global__ void k_gt_lmark(int nn, const GNode *__restrict__ nd, int *__restrict__ flag,
+                           int *__restrict__ res)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn) return;
+    const bool l = nd[c].dim == -2;
+    flag[c] = l;
+    res[c] = l ? 2 * (nd[c].hi - nd[c].lo + 1) - 2 : 0;
+}
+
+__global__ void k_gt_lcompact(int nn, const int *__restrict__ flag, const int64_t *__restrict__ pos,
+                              int *__restrict__ list)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nn && flag[c]) list[pos[c]] = c;
+}
+
+// The subtree of one marked cell (at most LMAX particles), built in LDS by
+// one workgroup with the rules of the level loop: per level, each cell's
+// bounds and split decision (a wavefront per cell), child ids in order,
+// then each split cell's partition by the Hoare pairing of the k-th
+// misplaced particles from either end (a wavefront per cell).  The cell is
+// local node 0; descendant k >= 1 (in creation order) goes to scratch slot
+// lres[root] + k - 1 with its links as local ids, and *used receives the
+// node count; k_gt_lcopy then moves them into place.  The particles are
+// written back in their new order.
+__global__ __launch_bounds__(256) void k_gt_local(const int *__restrict__ lroot,
+                                                  const int64_t *__restrict__ lres, int nBucket,
+                                                  GNode *__restrict__ nd, GNode *__restrict__ scr,
+                                                  int *__restrict__ used,
+                                                  double *__restrict__ px, double *__restrict__ py,
+                                                  double *__restrict__ pz, int *__restrict__ perm,
+                                                  int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+    __shared__ double lx[LMAX], ly[LMAX], lz[LMAX];
+    __shared__ int ls[LMAX];
+    __shared__ short posR[LMAX];
+    __shared__ short qa[2][LMAX];              // cells of the current / next level
+    __shared__ short nlo[2 * LMAX], nhi[2 * LMAX];   // cell ranges (local positions)
+    __shared__ short coff[LMAX], cs[LMAX];     // child offset (-1: none), split position
+    __shared__ signed char cdim[LMAX];
+    __shared__ double csplit[LMAX];
+    __shared__ int wsum[4];
+    __shared__ int nsplit_sh;
+    const int rootId = lroot[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int glo = nd[rootId].lo, cnt = nd[rootId].hi - glo + 1;
+    const int troot = nd[rootId].root, rdepth = nd[rootId].depth;
+    GNode *const sb = scr + lres[rootId] - 1;   // local node k >= 1 at sb[k]
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        lx[i] = px[glo + i];
+        ly[i] = py[glo + i];
+        lz[i] = pz[glo + i];
+        ls[i] = perm[glo + i];
+    }
+    if (threadIdx.x == 0) {
+        nlo[0] = 0;
+        nhi[0] = (short)(cnt - 1);
+        qa[0][0] = 0;
+    }
+    __syncthreads();
+    int nq = 1, cur = 0, nloc = 1, depth = 0;
+    while (nq > 0) {
+        // bounds, decision and lower-part count, a wavefront per cell
+        for (int t = w; t < nq; t += 4) {
+            const int k = qa[cur][t];
+            const int lo = nlo[k], hi = nhi[k];
+            double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+            double bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+            for (int i = lo + lane; i <= hi; i += 64) {
+                const double v[3] = {lx[i], ly[i], lz[i]};
+                for (int d = 0; d < 3; ++d) {
+                    bmin[d] = v[d] < bmin[d] ? v[d] : bmin[d];
+                    bmax[d] = v[d] > bmax[d] ? v[d] : bmax[d];
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1)
+                for (int d = 0; d < 3; ++d) {
+                    const double l = __shfl_xor(bmin[d], o), h = __shfl_xor(bmax[d], o);
+                    bmin[d] = l < bmin[d] ? l : bmin[d];
+                    bmax[d] = h > bmax[d] ? h : bmax[d];
+                }
+            bool extent = false;
+            for (int d = 0; d < 3; ++d) extent |= bmax[d] > bmin[d];
+            int dim = -1;
+            double split = 0.0;
+            if (hi - lo + 1 > nBucket && extent) {
+                int d = 0;
+                for (int q = 1; q < 3; ++q)
+                    if (bmax[q] - bmin[q] > bmax[d] - bmin[d]) d = q;
+                dim = d;
+                split = 0.5 * (bmin[d] + bmax[d]);
+            }
+            int s = lo;
+            if (dim >= 0) {
+                const double *cc = dim == 0 ? lx : (dim == 1 ? ly : lz);
+                int nl = 0;
+                for (int c0 = lo; c0 <= hi; c0 += 64) {
+                    const int i = c0 + lane;
+                    nl += __popcll(__ballot(i <= hi && cc[i] < split));
+                }
+                s = lo + nl;
+                if (s == lo || s > hi) {   // one side empty: reported, left a bucket
+                    if (lane == 0) atomicOr(err, 1);
+                    dim = -1;
+                    s = lo;
+                }
+            }
+            if (lane == 0) {
+                GNode &g = k == 0 ? nd[rootId] : sb[k];
+                g.lo = glo + lo;
+                g.hi = glo + hi;
+                g.dim = dim;
+                g.split = dim >= 0 ? split : 0.0;
+                g.ch0 = g.ch1 = -1;
+                g.s = glo + s;
+                g.root = troot;
+                g.depth = rdepth + depth;
+                coff[t] = dim >= 0 ? 1 : 0;
+                cdim[t] = (signed char)dim;
+                csplit[t] = split;
+                cs[t] = (short)s;
+            }
+        }
+        __syncthreads();
+        {   // child offsets: exclusive sum of the split flags
+            int v[4], sum = 0;
+            for (int q = 0; q < 4; ++q) {
+                const int t = threadIdx.x * 4 + q;
+                v[q] = t < nq ? coff[t] : 0;
+                sum += v[q];
+            }
+            int inc = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(inc, o);
+                if (lane >= o) inc += u;
+            }
+            if (lane == 63) wsum[w] = inc;
+            __syncthreads();
+            int ex = inc - sum, tot = 0;
+            for (int q = 0; q < 4; ++q) {
+                if (q < w) ex += wsum[q];
+                tot += wsum[q];
+            }
+            for (int q = 0; q < 4; ++q) {
+                const int t = threadIdx.x * 4 + q;
+                if (t < nq) coff[t] = v[q] ? (short)ex : (short)-1;
+                ex += v[q];
+            }
+            if (threadIdx.x == 0) nsplit_sh = tot;
+        }
+        __syncthreads();
+        const int nsplit = nsplit_sh;
+        // partitions and children, a wavefront per split cell
+        for (int t = w; t < nq; t += 4) {
+            const int co = coff[t];
+            if (co < 0) continue;
+            const int k = qa[cur][t];
+            const int lo = nlo[k], hi = nhi[k], s = cs[t];
+            const double split = csplit[t];
+            const double *cc = cdim[t] == 0 ? lx : (cdim[t] == 1 ? ly : lz);
+            const int nl = s - lo;
+            int run = 0;
+            for (int c0 = lo; c0 <= hi; c0 += 64) {   // upper positions holding a lower particle
+                const int i = c0 + lane;
+                const bool f = i <= hi && cc[i] < split;
+                const unsigned long long m = __ballot(f);
+                const int incl = run + __popcll(m & (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1)));
+                if (f && i >= s) posR[lo + (nl - incl)] = (short)i;
+                run += __popcll(m);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            run = 0;
+            for (int c0 = lo; c0 < s; c0 += 64) {     // lower positions holding an upper one
+                const int i = c0 + lane;
+                const bool in = i < s;
+                const bool f = in && cc[i] < split;
+                const unsigned long long m = __ballot(f);
+                const int excl = run + __popcll(m & ((1ull << lane) - 1));
+                if (in && !f) {
+                    const int j = posR[lo + (i - lo) - excl];
+                    double d = lx[i]; lx[i] = lx[j]; lx[j] = d;
+                    d = ly[i]; ly[i] = ly[j]; ly[j] = d;
+                    d = lz[i]; lz[i] = lz[j]; lz[j] = d;
+                    const int u = ls[i]; ls[i] = ls[j]; ls[j] = u;
+                }
+                run += __popcll(m);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                const int a0 = nloc + 2 * co, a1 = a0 + 1;
+                nlo[a0] = (short)lo;
+                nhi[a0] = (short)(s - 1);
+                nlo[a1] = (short)s;
+                nhi[a1] = (short)hi;
+                qa[cur ^ 1][2 * co] = (short)a0;
+                qa[cur ^ 1][2 * co + 1] = (short)a1;
+                GNode &g = k == 0 ? nd[rootId] : sb[k];
+                g.ch0 = a0;
+                g.ch1 = a1;
+                sb[a0].par = k;
+                sb[a0].side = 0;
+                sb[a1].par = k;
+                sb[a1].side = 1;
+            }
+        }
+        __syncthreads();
+        nloc += 2 * nsplit;
+        nq = 2 * nsplit;
+        cur ^= 1;
+        ++depth;
+    }
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        px[glo + i] = lx[i];
+        py[glo + i] = ly[i];
+        pz[glo + i] = lz[i];
+        perm[glo + i] = ls[i];
+    }
+    if (threadIdx.x == 0) used[blockIdx.x] = nloc - 1;
+}
+
+// The descendants of local build r from scratch to nodes nng + dbase[r] ...,
+// links turned from local ids (0: the marked cell) to node ids.
+__global__ __launch_bounds__(256) void k_gt_lcopy(const int *__restrict__ lroot,
+                                                  const int64_t *__restrict__ lres,
+                                                  const int *__restrict__ used,
+                                                  const int64_t *__restrict__ dbase, int nng,
+                                                  GNode *__restrict__ nd,
+                                                  const GNode *__restrict__ scr)
+{
+    const int r = blockIdx.x, rootId = lroot[r], u = used[r];
+    const int db = nng + (int)dbase[r];
+    const GNode *const sb = scr + lres[rootId] - 1;
+    auto fin = [&](int k) { return k < 0 ? -1 : (k == 0 ? rootId : db + k - 1); };
+    for (int k = 1 + threadIdx.x; k <= u; k += 256) {
+        GNode g = sb[k];
+        g.ch0 = fin(g.ch0);
+        g.ch1 = fin(g.ch1);
+        g.par = fin(g.par);
+        nd[db + k - 1] = g;
+    }
+    if (threadIdx.x == 0) {
+        GNode &g = nd[rootId];
+        g.ch0 = fin(g.ch0);
+        g.ch1 = fin(g.ch1);
+    }
 }
 
 // Storage-order particle fields: masses, softenings, type bits by source.
@@ -317,15 +617,16 @@ __global__ void k_gt_gather(int n, const int *__restrict__ src, const double *__
 
 // Mass, mass-weighted softening and centre of mass (pkd.c:2510-2576):
 // buckets over their particles in storage order, cells from their
-// children, lower then upper.  Deepest level first.
-__global__ void k_gt_mass(int a, int b, const GNode *__restrict__ nd,
+// children, lower then upper.  Deepest level first, one depth per launch.
+__global__ void k_gt_mass(int m, const int *__restrict__ list, const GNode *__restrict__ nd,
                           const double *__restrict__ x, const double *__restrict__ y,
                           const double *__restrict__ z, const double *__restrict__ pm,
                           const double *__restrict__ ph, double *__restrict__ cm)
 {
 #pragma clang fp contract(off)
-    const int c = a + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= b) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int c = list[k];
     const GNode &g = nd[c];
     double fm, M = 0.0, S = 0.0, R[3] = {0.0, 0.0, 0.0};
     if (g.dim >= 0) {
@@ -717,7 +1018,7 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
     GG_HIP(hipMemsetAsync(G.gerr.p, 0, sizeof(int) * 2, st));
     GNode *nd = G.gnode.as<GNode>();
     int nn = 0;
-    std::vector<std::pair<int, int>> levels;
+    std::vector<std::pair<int, int>> levels, depths;
     if (nr > 0) {
         const int *rl = G.groot.as<int>(), *rh = rl + nr;
         GG_HIP(hipMemcpyAsync(G.groot.p, rlo.data(), sizeof(int) * 2 * nr, hipMemcpyHostToDevice,
@@ -746,7 +1047,8 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
                                0, st, a, b, G.gioff.as<int64_t>(), nd, G.px.as<double>(),
                                G.py.as<double>(), G.pz.as<double>(), G.gbox.as<double>());
             hipLaunchKernelGGL(k_gt_decide, dim3(nblk(b - a, 4)), dim3(256), 0, st, a, b, nBucket,
-                               G.gioff.as<int64_t>(), G.gbox.as<double>(), G.px.as<double>(),
+                               (int)levels.size() - 1, G.gioff.as<int64_t>(), G.gbox.as<double>(),
+                               G.px.as<double>(),
                                G.py.as<double>(), G.pz.as<double>(), nd, G.gnch.as<int>());
             // child ids (two per split), then the partition, which also moves
             // every position to its child
@@ -770,11 +1072,74 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
             a = b;
             b += (int)nch;
         }
-        nn = b;
+        const int nng = b;
+        // cells of at most LMAX particles: their subtrees in one launch, each
+        // in node slots reserved after the level loop's cells
+        GG_TRY(G.gbflag.reserve(sizeof(int) * (size_t)nng));
+        hipLaunchKernelGGL(k_gt_lmark, dim3(nblk(nng, 256)), dim3(256), 0, st, nng, nd,
+                           G.gbflag.as<int>(), G.gnit.as<int>());
+        int64_t nloc = 0, nres = 0;
+        GG_TRY(exclusive_scan_i64(st, G.gbflag.as<int>(), G.gchoff.as<int64_t>(), nng, G.scan,
+                                  &nloc));
+        GG_TRY(exclusive_scan_i64(st, G.gnit.as<int>(), G.gioff.as<int64_t>(), nng, G.scan, &nres));
+        GG_CHECK((size_t)nng + (size_t)nres <= cap, GG_EHIP, "tree cell capacity exceeded");
+        nn = nng;
+        if (nloc > 0) {
+            GG_TRY(G.glroot.reserve(sizeof(int) * (size_t)nloc));
+            GG_TRY(G.glused.reserve(sizeof(int) * (size_t)nloc));
+            GG_TRY(G.glbase.reserve(sizeof(int64_t) * ((size_t)nloc + 1)));
+            GG_TRY(G.gscr.reserve(sizeof(GNode) * (size_t)std::max<int64_t>(nres, 1)));
+            hipLaunchKernelGGL(k_gt_lcompact, dim3(nblk(nng, 256)), dim3(256), 0, st, nng,
+                               G.gbflag.as<int>(), G.gchoff.as<int64_t>(), G.glroot.as<int>());
+            hipLaunchKernelGGL(k_gt_local, dim3((unsigned)nloc), dim3(256), 0, st,
+                               G.glroot.as<int>(), G.gioff.as<int64_t>(), nBucket, nd,
+                               G.gscr.as<GNode>(), G.glused.as<int>(), G.px.as<double>(),
+                               G.py.as<double>(), G.pz.as<double>(), G.perm.as<int>(),
+                               G.gerr.as<int>());
+            GG_HIP(hipGetLastError());
+            int64_t nused = 0;
+            GG_TRY(exclusive_scan_i64(st, G.glused.as<int>(), G.glbase.as<int64_t>(), (int)nloc,
+                                      G.scan, &nused));
+            int err = 0;
+            GG_HIP(hipMemcpyAsync(&err, G.gerr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            GG_HIP(hipStreamSynchronize(st));
+            GG_CHECK(!err, GG_EUNSUP,
+                     "a cell split leaves one side empty (positions differ in the last bit only)");
+            nn = nng + (int)nused;
+            hipLaunchKernelGGL(k_gt_lcopy, dim3((unsigned)nloc), dim3(256), 0, st,
+                               G.glroot.as<int>(), G.gioff.as<int64_t>(), G.glused.as<int>(),
+                               G.glbase.as<int64_t>(), nng, nd, G.gscr.as<GNode>());
+            GG_HIP(hipGetLastError());
+        }
+        // nodes by depth (the level loop's levels, then up to LMAX local ones)
+        const int ndep = (int)levels.size() + LMAX + 1;
+        GG_CHECK(ndep <= DMAX, GG_EUNSUP, "tree deeper than %d levels", DMAX);
+        GG_TRY(G.gdcnt.reserve(sizeof(int) * 2 * (size_t)ndep));
+        GG_TRY(G.gdlist.reserve(sizeof(int) * (size_t)nn));
+        GG_HIP(hipMemsetAsync(G.gdcnt.p, 0, sizeof(int) * ndep, st));
+        hipLaunchKernelGGL(k_gt_depth_count, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, ndep, nd,
+                           G.gdcnt.as<int>());
+        std::vector<int> dcnt((size_t)ndep), doff((size_t)ndep + 1, 0);
+        GG_HIP(hipMemcpyAsync(dcnt.data(), G.gdcnt.p, sizeof(int) * ndep, hipMemcpyDeviceToHost,
+                              st));
+        GG_HIP(hipStreamSynchronize(st));
+        int nd_used = 0;
+        for (int d = 0; d < ndep; ++d) {
+            doff[d + 1] = doff[d] + dcnt[d];
+            if (dcnt[d]) nd_used = d + 1;
+        }
+        GG_HIP(hipMemcpyAsync(G.gdcnt.as<int>() + ndep, doff.data(), sizeof(int) * ndep,
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_gt_depth_list, dim3(nblk(nn, 256)), dim3(256), 0, st, nn, ndep, nd,
+                           G.gdcnt.as<int>() + ndep, G.gdlist.as<int>());
+        depths.clear();
+        for (int d = 0; d < nd_used; ++d) depths.push_back({doff[d], dcnt[d]});
+        const int *dl = G.gdlist.as<int>();
         // preorder numbering from subtree sizes
-        for (int L = (int)levels.size() - 1; L >= 0; --L)
-            hipLaunchKernelGGL(k_gt_size, dim3(nblk(levels[L].second - levels[L].first, 256)),
-                               dim3(256), 0, st, levels[L].first, levels[L].second, nd);
+        for (int d = (int)depths.size() - 1; d >= 0; --d)
+            if (depths[d].second)
+                hipLaunchKernelGGL(k_gt_size, dim3(nblk(depths[d].second, 256)), dim3(256), 0, st,
+                                   depths[d].second, dl + depths[d].first, nd);
         std::vector<GNode> roots((size_t)nr);
         GG_HIP(hipMemcpyAsync(roots.data(), nd, sizeof(GNode) * nr, hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
@@ -784,9 +1149,10 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
             pre += roots[k].size;
         }
         GG_HIP(hipMemcpyAsync(nd, roots.data(), sizeof(GNode) * nr, hipMemcpyHostToDevice, st));
-        for (size_t L = 0; L < levels.size(); ++L)
-            hipLaunchKernelGGL(k_gt_pre, dim3(nblk(levels[L].second - levels[L].first, 256)),
-                               dim3(256), 0, st, levels[L].first, levels[L].second, nd);
+        for (size_t d = 0; d < depths.size(); ++d)
+            if (depths[d].second)
+                hipLaunchKernelGGL(k_gt_pre, dim3(nblk(depths[d].second, 256)), dim3(256), 0, st,
+                                   depths[d].second, dl + depths[d].first, nd);
         GG_HIP(hipGetLastError());
         GG_HIP(hipStreamSynchronize(st));
     }
@@ -818,11 +1184,12 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
         GG_HIP(hipStreamSynchronize(st));
     } else {
         // mass, softening, c.o.m.: deepest level first
-        for (int L = (int)levels.size() - 1; L >= 0; --L)
-            hipLaunchKernelGGL(k_gt_mass, dim3(nblk(levels[L].second - levels[L].first, 256)),
-                               dim3(256), 0, st, levels[L].first, levels[L].second, nd,
-                               G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
-                               G.pm.as<double>(), G.ph.as<double>(), G.gcm.as<double>());
+        for (int d = (int)depths.size() - 1; d >= 0; --d)
+            if (depths[d].second)
+                hipLaunchKernelGGL(k_gt_mass, dim3(nblk(depths[d].second, 256)), dim3(256), 0, st,
+                                   depths[d].second, G.gdlist.as<int>() + depths[d].first, nd,
+                                   G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
+                                   G.pm.as<double>(), G.ph.as<double>(), G.gcm.as<double>());
         // moments: work items of MOM_CH particles per cell
         hipLaunchKernelGGL(k_gt_items, dim3(nblk(nn, 256)), dim3(256), 0, st, 0, nn, MOM_CH, 0,
                            nd, G.gnit.as<int>());

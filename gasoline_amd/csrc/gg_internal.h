@@ -79,7 +79,7 @@ struct GravDev {
     DevBuf tint, tdbl;     // per node: 6 ints, 8 doubles (gg_tree_info's layout)
     // tree build scratch
     DevBuf gpn, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
-    DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof;
+    DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof, glroot, gdcnt, gdlist, glused, glbase, gscr;
     hipGraph_t graph = nullptr;         // captured step (gg_grav_graph)
     hipGraphExec_t gexec = nullptr;
     int graphOrder = 0;
@@ -92,7 +92,7 @@ struct GravDev {
         DevBuf *all[] = {&px, &py, &pz, &pm, &ph, &pact, &perm, &wnode, &cell, &momOH, &bucket,
                          &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
                          &tdbl, &gpn, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
-                         &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof};
+                         &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof, &glroot, &gdcnt, &gdlist, &glused, &glbase, &gscr};
         for (DevBuf *b : all) b->release();
     }
 };
