@@ -32,7 +32,12 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA)
 FP32_NONPACKED_TFLOPS = 78.6      # without packed FP32, which this build does not emit
 # Algorithmic bytes per unit of work (the data the operator must touch once):
-KNN_BYTES_PER_QUERY = 24 + 8     # + 4 K: position read, fBall2 and K list entries written
+# kNN per query: its own position and fBall2, and per neighbour the j-side
+# position a distance needs (24 B, as the density pass counts its j record)
+# plus the list entry written (4 B); the strict output-only model (32 B +
+# 4 K per query) is reported beside it
+KNN_BYTES_PER_QUERY = 24 + 8     # + 28 K
+KNN_BYTES_PER_NEIGHBOUR = 24 + 4
 DENS_BYTES_PER_PAIR = 64         # j side, slot-ordered FP64 arrays: r 24, m 8, vPred 24, uPred 8
 HYDRO_BYTES_PER_PAIR = 64        # j side, packed FP32 force record (16 floats)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
@@ -316,7 +321,7 @@ def main():
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream; the dominant one is reported
     kern = {
-        "k_knn_bucket": (ph32["knn"], n * (KNN_BYTES_PER_QUERY + 4 * K)),
+        "k_knn_bucket": (ph32["knn"], n * (KNN_BYTES_PER_QUERY + KNN_BYTES_PER_NEIGHBOUR * K)),
         "k_dendvdx<float>": (ph32["density"], n * K * DENS_BYTES_PER_PAIR),
         "k_hydro2<float>": (ph32["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
     }
@@ -334,6 +339,7 @@ def main():
                 "traffic": (t["fetch_bytes"] + t["write_bytes"]) if t else None}
 
     dom = max(kern, key=lambda k: kern[k][0])
+    knn_strict_gbs = n * (KNN_BYTES_PER_QUERY + 4 * K) / (ph32["knn"] * 1e-3) / 1e9
     g_inter_s = g32[1] / (g32[0])
     line = {
         "metric": METRIC,
@@ -367,6 +373,7 @@ def main():
         "grav_ms_tree_build": g32[5] * 1e3,
         "roofline": roof(dom),
         "rooflines_other": [roof(k) for k in kern if k != dom],
+        "knn_output_only_gbs": knn_strict_gbs,
         "roofline_gravity": {
             "bound": "valu", "kernel": "k_walk_interact<float,2> (walk included)",
             "achieved": g32[2] / (g32[4] * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,

@@ -1098,13 +1098,16 @@ namespace gg {
 // FP64 direct sum for sampled targets (the accuracy check of the tree
 // forces): every other particle of the store with the spline softening of
 // the particle-particle interaction, twoh = h_i + h_j (grav.c:81-108), no
-// tree.  One thread per target; sources staged through LDS in tiles of 256;
-// Kahan-compensated sums.
+// tree.  One thread per target and DSPLIT workgroups per 256 targets, each
+// over its share of the sources, staged through LDS in tiles of 256, with
+// Kahan-compensated sums; the shares are added in order afterwards.
+constexpr int DSPLIT = 64;
+
 __global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__restrict__ tgt,
                                                 const double *__restrict__ r,
                                                 const double *__restrict__ m,
                                                 const double *__restrict__ h,
-                                                double *__restrict__ out)
+                                                double *__restrict__ part)
 {
     __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256];
     const int k = blockIdx.x * 256 + threadIdx.x;
@@ -1112,9 +1115,11 @@ __global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__rest
     const double xi = i >= 0 ? r[3 * (size_t)i] : 0.0, yi = i >= 0 ? r[3 * (size_t)i + 1] : 0.0;
     const double zi = i >= 0 ? r[3 * (size_t)i + 2] : 0.0, hi = i >= 0 ? h[i] : 0.0;
     double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
-    for (int j0 = 0; j0 < n; j0 += 256) {
-        const int j = j0 + threadIdx.x;
-        if (j < n) {
+    const int per = (n + DSPLIT - 1) / DSPLIT;
+    const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
+    for (int t0 = j0; t0 < j1; t0 += 256) {
+        const int j = t0 + threadIdx.x;
+        if (j < j1) {
             sx[threadIdx.x] = r[3 * (size_t)j];
             sy[threadIdx.x] = r[3 * (size_t)j + 1];
             sz[threadIdx.x] = r[3 * (size_t)j + 2];
@@ -1122,10 +1127,10 @@ __global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__rest
             sh[threadIdx.x] = h[j];
         }
         __syncthreads();
-        const int nj = min(256, n - j0);
+        const int nj = min(256, j1 - t0);
         if (i >= 0)
             for (int e = 0; e < nj; ++e) {
-                if (j0 + e == i) continue;
+                if (t0 + e == i) continue;
                 const double dx = sx[e] - xi, dy = sy[e] - yi, dz = sz[e] - zi;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 double a, b;
@@ -1140,7 +1145,18 @@ __global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__rest
         __syncthreads();
     }
     if (i >= 0)
-        for (int q = 0; q < 4; ++q) out[4 * (size_t)k + q] = s[q];
+        for (int q = 0; q < 4; ++q) part[((size_t)blockIdx.y * nt + k) * 4 + q] = s[q] - c[q];
+}
+
+__global__ void k_direct_sum(int nt, const double *__restrict__ part, double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    for (int q = 0; q < 4; ++q) {
+        double s = 0.0;
+        for (int p = 0; p < DSPLIT; ++p) s += part[((size_t)p * nt + k) * 4 + q];
+        out[4 * (size_t)k + q] = s;
+    }
 }
 }  // namespace gg
 
@@ -1158,8 +1174,11 @@ extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double 
     hipStream_t st = pkd->stream;
     DevBuf t, o;
     GG_TRY(t.reserve(sizeof(int) * nTarget));
-    if (o.reserve(sizeof(double) * 4 * nTarget) != GG_OK) {
+    DevBuf pt;
+    if (o.reserve(sizeof(double) * 4 * nTarget) != GG_OK ||
+        pt.reserve(sizeof(double) * 4 * (size_t)DSPLIT * nTarget) != GG_OK) {
         t.release();
+        o.release();
         return GG_EHIP;
     }
     int rc = GG_OK;
@@ -1167,9 +1186,12 @@ extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double 
     if (hipMemcpyAsync(t.p, target, sizeof(int) * nTarget, hipMemcpyHostToDevice, st) != hipSuccess)
         rc = GG_EHIP;
     if (rc == GG_OK) {
-        hipLaunchKernelGGL(k_direct, dim3((nTarget + 255) / 256), dim3(256), 0, st, n, nTarget,
-                           t.as<int>(), pkd->f[GG_R].as<double>(), pkd->f[GG_MASS].as<double>(),
-                           pkd->f[GG_SOFT].as<double>(), o.as<double>());
+        hipLaunchKernelGGL(k_direct, dim3((nTarget + 255) / 256, DSPLIT), dim3(256), 0, st, n,
+                           nTarget, t.as<int>(), pkd->f[GG_R].as<double>(),
+                           pkd->f[GG_MASS].as<double>(), pkd->f[GG_SOFT].as<double>(),
+                           pt.as<double>());
+        hipLaunchKernelGGL(k_direct_sum, dim3((nTarget + 255) / 256), dim3(256), 0, st, nTarget,
+                           pt.as<double>(), o.as<double>());
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(res.data(), o.p, sizeof(double) * 4 * nTarget, hipMemcpyDeviceToHost,
                            st) != hipSuccess ||
@@ -1178,6 +1200,7 @@ extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double 
     }
     t.release();
     o.release();
+    pt.release();
     GG_CHECK(rc == GG_OK, GG_EHIP, "gg_direct_grav: HIP error");
     for (int k = 0; k < nTarget; ++k) {
         for (int d = 0; d < 3; ++d) a[3 * (size_t)k + d] = res[4 * (size_t)k + d];
