@@ -71,6 +71,29 @@ def test_forest_with_empty_systems(gpu, ens):
         assert np.array_equal(pk.get("a"), a)
 
 
+def test_forest_of_tiny_and_uneven_systems(gpu):
+    # systems of 1, 2, 17, 600 and 5000 particles (the small ones become
+    # per-workgroup subtrees at the root level): each equals its own run
+    sizes = [1, 2, 17, 600, 5000]
+    parts = [ics.king(k, 5.0, seed=30 + k) if k > 1 else
+             dict(r=np.zeros((1, 3)), v=np.zeros((1, 3)), m=np.ones(1)) for k in sizes]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("r", "m")}
+    n = off[-1]
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        load(pk, cat, 0.01)
+        pk.build_forest(off, nBucket=16, dCrit=0.6)
+        pk.grav_all(2)
+        a = pk.get("a")
+    for s in range(len(sizes)):
+        sl = slice(off[s], off[s + 1])
+        with gpu.Pkd(sizes[s], prec=gpu.PREC_FP64) as pk:
+            load(pk, {k: cat[k][sl] for k in ("r", "m")}, 0.01)
+            pk.build_binary(16, 1, 0.6, 2)
+            pk.grav_all(2)
+            assert np.array_equal(pk.get("a"), a[sl]), s
+
+
 def test_forest_system_vs_oracle(gpu, ens):
     off = ens["off"]
     s = 2

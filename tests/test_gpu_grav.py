@@ -74,6 +74,48 @@ def test_tree_bitexact_lattice_ties(gpu):
     assert np.array_equal(dbls[:, 6], od[:, 44])
 
 
+def _tree_vs_oracle(gpu, r, m, soft, nb, order=2, act=None):
+    T = O.Tree(r, m, soft, iactive=act, nBucket=nb, dTheta=0.7, iOrder=order)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64, iactive=act) as pk:
+        pk.build_binary(nb, 1, 0.7, order)
+        root, perm, ints, dbls = pk.tree_info()
+        st = pk.grav_all(order)
+        a = pk.get("a")
+    oi, od = T.nodes()
+    assert np.array_equal(perm, T.perm())
+    assert np.array_equal(ints[:, :5], oi[:, :5])
+    assert np.array_equal(dbls[:, :6], od[:, [7, 8, 9, 10, 11, 12]])
+    assert np.array_equal(dbls[:, 6], od[:, 44])
+    ao, _, _, sto = T.grav_all()
+    assert st["dPartSum"] == sto["dPartSum"] and st["dCellSum"] == sto["dCellSum"]
+    e = rel_err(a, ao)        # the FP64 bar of test_forces_vs_oracle
+    assert np.sqrt(np.mean(e * e)) < 1e-8 and e.max() < 1e-7
+
+
+@pytest.mark.parametrize("case", ["bucket1", "clump", "flat", "tiny"])
+def test_tree_bitexact_shapes(gpu, case):
+    # trees the device build finishes mostly in its per-workgroup subtrees:
+    # one particle per bucket, a dense clump many levels deep inside a sphere,
+    # a flat disk (no extent in z), and stores smaller than a bucket
+    rng = np.random.default_rng(17)
+    if case == "bucket1":
+        r, v, m = ics.plummer(3000, seed=3)
+        nb = 1
+    elif case == "clump":
+        r, v, m = ics.plummer(6000, seed=4)
+        r[:2000] = ics._f32(rng.normal(0, 1e-4, (2000, 3)) + 0.3)
+        nb = 8
+    elif case == "flat":
+        r, v, m = ics.plummer(8000, seed=5)
+        r[:, 2] = 0.0
+        nb = 16
+    else:
+        r, v, m = ics.plummer(7, seed=6)
+        nb = 16
+    soft = np.full(len(m), 0.01)
+    _tree_vs_oracle(gpu, r, m, soft, nb)
+
+
 def test_interaction_lists_bitexact(gpu, plummer32k):
     r, m, soft, T, (ao, po, dto, sto) = plummer32k
     with setup_pkd(gpu, r, m, soft, gpu.PREC_FP32) as pk:
