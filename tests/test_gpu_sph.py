@@ -120,6 +120,13 @@ def test_gas_subset_of_a_mixed_store(gpu):
             if sel.all():
                 halo = ~g
                 assert np.all(pk.get("fDensity")[halo] == 0) and np.all(pk.get("a")[halo] == 0)
+                # reordering puts the gas first, in the structure's order, and
+                # the halo after it in its previous order
+                perm = pk.reorder()
+                ng = int(g.sum())
+                assert np.all(g[perm[:ng]]) and np.array_equal(perm[ng:], np.nonzero(halo)[0])
+                assert np.array_equal(pk.get("fDensity")[:ng], res[-1]["fDensity"][
+                    np.searchsorted(gi, perm[:ng])])
     for f in res[0]:
         assert np.array_equal(res[0][f], res[1][f]), f
 
