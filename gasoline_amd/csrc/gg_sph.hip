@@ -917,13 +917,13 @@ struct KLGroup {
     int nb;
 };
 
-// Settles one query over the nsel <= KL_SB buckets listed in sel (two
-// buckets per 64 lanes): finds T <= Tsel with K <= |{d2 <= T}| <= CAP and
+// Settles one query over the nsel <= KL_SB buckets listed in sel (first
+// slot, count; two buckets per 64 lanes): finds T <= Tsel with K <= |{d2 <= T}| <= CAP and
 // compacts that ball into (bd, bs).  Returns 0 when settled (*cnt = ball
 // size), 1 when fewer than K lie within Tsel (*cnt = their number), 2 when
 // no such T was found.
 template <bool IMG>
-__device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char *sel, int nsel,
+__device__ int knn_settle(const QArgs &A, const KLGroup &G, const int2 *sel, int nsel,
                           double x, double y, double z, double Tsel, double *bd, int *bs,
                           int lane, int *cnt_out, unsigned &ncand)
 {
@@ -935,8 +935,8 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char 
         ci[k] = -1;
         const int e = 2 * k + (lane >> 5), o = lane & 31;
         if (k < ns && e < nsel) {
-            const int b = sel[e];
-            if (o < G.lcnt[b]) ci[k] = G.lfst[b] + o;
+            const int2 fc = sel[e];
+            if (o < fc.y) ci[k] = fc.x + o;
         }
     }
     double dd[KL_NS];
@@ -997,7 +997,7 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const unsigned char 
 // narrow it while the ball meets more than KL_SB buckets.  Returns false if
 // the query must be retried by k_query.
 template <bool IMG>
-__device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, int q, double x,
+__device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, double x,
                           double y, double z, double Tsel, double RL2, double *bd, int *bs,
                           int lane, unsigned &npass, unsigned &ncand)
 {
@@ -1017,7 +1017,7 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, unsigned char *sel, 
             }
             const unsigned long long m = __ballot(in);
             const int p = nsel + __popcll(m & ((1ull << lane) - 1));
-            if (in && p < KL_SB) sel[p] = (unsigned char)k;
+            if (in && p < KL_SB) sel[p] = make_int2(G.lfst[k], G.lcnt[k]);
             nsel += __popcll(m);
         }
         wave_sync();
@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(64) void k_knn_bucket(KLArgs KA)
     const QArgs &A = KA.Q;
     __shared__ float lbx[KL_MB][6];
     __shared__ int lfst[KL_MB], lcnt[KL_MB];
-    __shared__ unsigned char sel[KL_SB];
+    __shared__ int2 sel[KL_SB];                 // per query: first slot, count of its buckets
     __shared__ double bd[CAP];
     __shared__ int bs[CAP];
     const int lane = threadIdx.x;
