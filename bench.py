@@ -38,8 +38,12 @@ FP32_NONPACKED_TFLOPS = 78.6      # without packed FP32, which this build does n
 # 4 K per query) is reported beside it
 KNN_BYTES_PER_QUERY = 24 + 8     # + 28 K
 KNN_BYTES_PER_NEIGHBOUR = 24 + 4
-DENS_BYTES_PER_PAIR = 64         # j side, slot-ordered FP64 arrays: r 24, m 8, vPred 24, uPred 8
-HYDRO_BYTES_PER_PAIR = 64        # j side, packed FP32 force record (16 floats)
+# SPH passes per pair: the j-side record as an FP32 SoA record plus flags,
+# and the materialised neighbour index: density r 12, m 4, vPred 12, uPred 4,
+# flags 4 (+4); force r 12, m 4, vPred 12, rho 4, P/rho^2 4, c 4, Balsara 4,
+# dt 4, flags 4 (+4)
+DENS_BYTES_PER_PAIR = 36 + 4
+HYDRO_BYTES_PER_PAIR = 52 + 4
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 

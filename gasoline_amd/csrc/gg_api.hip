@@ -86,6 +86,7 @@ int ensure_field(GGPKD pkd, int field)
     const size_t cnt = field == GG_IACTIVE ? (size_t)n : (size_t)field_width(field) * n;
     const unsigned grid = (unsigned)((cnt + 255) / 256);
     if (field == GG_IACTIVE) {
+        pkd->sph.gasMaskValid = 0;
         hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, pkd->stream, cnt,
                            GG_TYPE_ACTIVE | GG_TYPE_TREEACTIVE | GG_TYPE_SMOOTHACTIVE,
                            b.as<uint32_t>());
@@ -184,6 +185,7 @@ extern "C" int gg_set_local(GGPKD pkd, int nLocal)
     pkd->nLocal = nLocal;
     pkd->grav.built = 0;
     pkd->sph.built = 0;
+    pkd->sph.gasMaskValid = 0;
     return GG_OK;
 }
 
@@ -208,6 +210,7 @@ extern "C" int gg_put(GGPKD pkd, int field, const void *src, int bDevice)
                               bDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               pkd->stream));
     GG_HIP(hipStreamSynchronize(pkd->stream));
+    if (field == GG_IACTIVE) pkd->sph.gasMaskValid = 0;
     if (field == GG_R || field == GG_MASS || field == GG_SOFT || field == GG_IACTIVE) {
         pkd->grav.built = 0;
         if (field != GG_MASS && field != GG_SOFT) pkd->sph.built = 0;

@@ -105,6 +105,8 @@ struct SphDev {
     unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0}, passes = 0;
     int64_t cand = 0;                          // kNN candidates examined
     double seedSlack = 1.15;                   // first trial ball / fBall2 hint
+    int gasMaskValid = 0;                      // gasMask holds whether any particle is TYPE_GAS
+    uint32_t gasMask = 0;
     int64_t nExtra = 0;
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
     DevBuf sx, sy, sz;                         // slot order positions
@@ -119,6 +121,7 @@ struct SphDev {
     DevBuf retry, nretry, hrec;                // kNN retries, force records
     DevBuf qf, act, nba, err, guess, diffcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
+    DevBuf jv, ju;                             // FP32 density record (vPred, fMass; uPred)
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
     std::vector<DevBuf> spares;                // gg_reorder's spare buffers, by capacity
     void release_all()
@@ -130,7 +133,7 @@ struct SphDev {
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
-                         &bal, &dif, &dt, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
+                         &bal, &dif, &dt, &jv, &ju, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();
     }

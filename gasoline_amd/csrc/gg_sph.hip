@@ -1178,7 +1178,9 @@ struct DArgs {
     const int64_t *off;                      // CSR (slot order)
     const int *idx;
     const uint32_t *qflag;
-    const double *m, *vx, *vy, *vz, *u;      // slot order, FP64
+    const double *m, *vx, *vy, *vz, *u;      // slot order, FP64 (pair arithmetic in FP64)
+    const float4 *jv;                        // slot order, FP32 form: vPred xyz, fMass
+    const float *ju;                         //   and uPred
     const uint32_t *act;                     // slot order: TYPE_ACTIVE
     double H, a, gamma;
     int bCannonical, iViscosityLimiter, bConstantDiffusion;
@@ -1203,7 +1205,18 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     const double ih2d = 4.0 / fb2;
     const T vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
     const double x = A.sx[i], y = A.sy[i], z = A.sz[i];
-    const T vxi = (T)A.vx[i], vyi = (T)A.vy[i], vzi = (T)A.vz[i];
+    constexpr bool F32 = sizeof(T) == 4;
+    T vxi, vyi, vzi;
+    if constexpr (F32) {
+        const float4 r = A.jv[i];
+        vxi = r.x;
+        vyi = r.y;
+        vzi = r.z;
+    } else {
+        vxi = A.vx[i];
+        vyi = A.vy[i];
+        vzi = A.vz[i];
+    }
     const bool pact = A.act[i] & 1u;
     double rho = 0, xx = 0, xy = 0, xz = 0, yx = 0, yy = 0, yz = 0, zx = 0, zy = 0, zz = 0;
     double dn = 0, gx = 0, gy = 0, gz = 0;
@@ -1216,7 +1229,19 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
         const double dzd = img_sep(z, A.sz[q], A.per, A.L[2]);
         const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
         const T r2 = (T)(sq_dist(dxd, dyd, dzd) * ih2d);
-        const T mq = (T)A.m[q];
+        T mq, vxq, vyq, vzq;
+        if constexpr (F32) {
+            const float4 r = A.jv[q];
+            vxq = r.x;
+            vyq = r.y;
+            vzq = r.z;
+            mq = r.w;
+        } else {
+            mq = A.m[q];
+            vxq = A.vx[q];
+            vyq = A.vy[q];
+            vzq = A.vz[q];
+        }
         if (pact) A.nbract[q] = 32u;   // every writer stores the same flag
         qact |= A.act[q];
         {   // extras of the force pass: q's own list misses i when i lies
@@ -1239,10 +1264,12 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
         rho += (double)(m4_w<T>(r2) * mq);
         if (A.op == GG_SMX_DENDVDX) {
             const T rs1 = m4_dw<T>(r2) * mq;
-            const T dvx = ((T)A.vx[q] - vxi) * vFac;
-            const T dvy = ((T)A.vy[q] - vyi) * vFac;
-            const T dvz = ((T)A.vz[q] - vzi) * vFac;
-            const T uq = (T)A.u[q];
+            const T dvx = (vxq - vxi) * vFac;
+            const T dvy = (vyq - vyi) * vFac;
+            const T dvz = (vzq - vzi) * vFac;
+            T uq;
+            if constexpr (F32) uq = A.ju[q];
+            else uq = A.u[q];
             xx += (double)(dvx * dx * rs1); xy += (double)(dvx * dy * rs1); xz += (double)(dvx * dz * rs1);
             yx += (double)(dvy * dx * rs1); yy += (double)(dvy * dy * rs1); yz += (double)(dvy * dz * rs1);
             zx += (double)(dvz * dx * rs1); zy += (double)(dvz * dy * rs1); zz += (double)(dvz * dz * rs1);
@@ -1843,6 +1870,20 @@ __global__ void k_flags_out(int n, const int *__restrict__ order, const uint32_t
 }
 
 
+// The FP32 form's j-side record of the density pass, slot order, gathered
+// from the store: vPred, fMass (one 16-byte load per pair) and uPred.
+__global__ void k_dpack(int n, const int *__restrict__ order, const double *__restrict__ m,
+                        const double *__restrict__ vp, const double *__restrict__ u,
+                        float4 *__restrict__ jv, float *__restrict__ ju)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    jv[s] = make_float4((float)vp[3 * (size_t)i], (float)vp[3 * (size_t)i + 1],
+                        (float)vp[3 * (size_t)i + 2], (float)m[i]);
+    ju[s] = (float)u[i];
+}
+
 // ------------------------------------------------------------ host side
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
@@ -1872,10 +1913,16 @@ static int scatter_field(GGPKD pkd, const DevBuf &src, int field, int width,
 
 // The SPH structure holds TYPE_GAS particles, or every particle when no
 // particle carries TYPE_GAS.
+// The answer is kept until the type bits are put again or the store changes
+// size: no kernel here sets or clears TYPE_GAS.
 static int gas_mask(GGPKD pkd, uint32_t *mask)
 {
     const int n = pkd->nLocal;
     SphDev &S = pkd->sph;
+    if (S.gasMaskValid) {
+        *mask = S.gasMask;
+        return GG_OK;
+    }
     GG_TRY(S.diffcnt.reserve(sizeof(unsigned)));
     GG_HIP(hipMemsetAsync(S.diffcnt.p, 0, sizeof(unsigned), pkd->stream));
     hipLaunchKernelGGL(k_any_bits, dim3(std::max(1, std::min(nblk(n, 256), 1024))), dim3(256), 0,
@@ -1885,6 +1932,8 @@ static int gas_mask(GGPKD pkd, uint32_t *mask)
     GG_HIP(hipMemcpyAsync(&any, S.diffcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, pkd->stream));
     GG_HIP(hipStreamSynchronize(pkd->stream));
     *mask = any ? GG_TYPE_GAS : 0;
+    S.gasMask = *mask;
+    S.gasMaskValid = 1;
     return GG_OK;
 }
 
@@ -2144,11 +2193,21 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(S.off.reserve(sizeof(int64_t) * (n + 1)));
     hipLaunchKernelGGL(k_fixed_off, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, K,
                        S.off.as<int64_t>());
-    GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
-    GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
-    GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
+    const bool f32 = pkd->prec == GG_PREC_FP32;
+    if (f32) {
+        for (int f : {GG_MASS, GG_VPRED, GG_UPRED}) GG_TRY(ensure_field(pkd, f));
+        GG_TRY(S.jv.reserve(sizeof(float4) * (size_t)n));
+        GG_TRY(S.ju.reserve(sizeof(float) * (size_t)n));
+        hipLaunchKernelGGL(k_dpack, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                           pkd->f[GG_MASS].as<double>(), pkd->f[GG_VPRED].as<double>(),
+                           pkd->f[GG_UPRED].as<double>(), S.jv.as<float4>(), S.ju.as<float>());
+    } else {
+        GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
+        GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
+    }
     GG_TRY(S.o_rho.reserve(sizeof(double) * n));
     GG_TRY(S.o_divv.reserve(sizeof(double) * n));
     GG_TRY(S.o_curl.reserve(sizeof(double) * 3 * n));
@@ -2173,6 +2232,8 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     D.vy = S.vy.as<double>();
     D.vz = S.vz.as<double>();
     D.u = S.u.as<double>();
+    D.jv = S.jv.as<float4>();
+    D.ju = S.ju.as<float>();
     D.act = S.act.as<uint32_t>();
     D.H = smf->H;
     D.a = smf->a;
