@@ -379,7 +379,7 @@ struct IArgs {
 };
 
 // One 64-lane workgroup per active bucket.  Lane = (target t, slice s):
-// TPW = next power of two >= bucket size targets, S = 64/TPW slices that
+// TPW = bucket size targets, S = floor(64/TPW) slices that
 // split each staged chunk; slice partials are reduced at the end.
 // One target of an interaction workgroup, and its running sums.
 template <typename T> struct Tgt {
@@ -599,8 +599,8 @@ __device__ __forceinline__ void eval_newt(const Stage<T, ORDER> &L, int m, const
 }
 
 // Target set-up of a bucket's workgroup and its intra-bucket pairs.
-// Lane = (target t, slice s): TPW = next power of two >= bucket size
-// targets, S = 64/TPW slices that split each staged chunk.
+// Lane = (target t, slice s): TPW = bucket size targets, S = floor(64/TPW)
+// slices that split each staged chunk (at most TPW - 1 lanes idle).
 template <typename T, int ORDER>
 __device__ __forceinline__ void bucket_targets(const IArgs &A, const WalkNode &bn,
                                                Stage<T, ORDER> &L, Tgt<T> &g, Acc<T> &R,
@@ -608,12 +608,11 @@ __device__ __forceinline__ void bucket_targets(const IArgs &A, const WalkNode &b
 {
     const int lane = threadIdx.x;
     const int pL = bn.pLower, nT = bn.pUpper - bn.pLower + 1;
-    TPW = 1;
-    while (TPW < nT) TPW <<= 1;
+    TPW = nT;
     g.S = 64 / TPW;
-    const int t = lane & (TPW - 1);
+    const int t = lane % TPW;
     g.s = lane / TPW;
-    const bool valid = t < nT;
+    const bool valid = g.s < g.S;
     it = pL + (valid ? t : 0);
     g.act = valid && (A.pact[it] & 1u);
     const double cx = bn.cx, cy = bn.cy, cz = bn.cz;     // bucket centre of mass
@@ -633,14 +632,17 @@ template <typename T>
 __device__ __forceinline__ void bucket_store(const IArgs &A, const Tgt<T> &g, const Acc<T> &R,
                                              int it, int TPW)
 {
-    double ax = R.SX.value(), ay = R.SY.value(), az = R.SZ.value(), pot = R.SP.value();
-    double dg = (double)R.dtg;
-    for (int o = TPW; o < 64; o <<= 1) {
-        ax += __shfl_xor(ax, o);
-        ay += __shfl_xor(ay, o);
-        az += __shfl_xor(az, o);
-        pot += __shfl_xor(pot, o);
-        const double od = __shfl_xor(dg, o);
+    const double vx = R.SX.value(), vy = R.SY.value(), vz = R.SZ.value(), vp = R.SP.value();
+    const double vd = (double)R.dtg;
+    double ax = vx, ay = vy, az = vz, pot = vp, dg = vd;
+    const int lane = threadIdx.x;
+    for (int k = 1; k < g.S; ++k) {      // slice k of this lane's target, in order
+        const int src = lane + k * TPW < 64 ? lane + k * TPW : lane;
+        ax += __shfl(vx, src);
+        ay += __shfl(vy, src);
+        az += __shfl(vz, src);
+        pot += __shfl(vp, src);
+        const double od = __shfl(vd, src);
         dg = od > dg ? od : dg;
     }
     if (g.act && g.s == 0) {
