@@ -1326,6 +1326,66 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     A.diff[i] = A.bConstantDiffusion ? 1 : 0.25 * fb2 * sqrt(2 * S2);
 }
 
+struct DSArgs {
+    int n, sym, per;
+    double L[3];
+    const double *sx, *sy, *sz, *fb2;        // slot order
+    const int64_t *off, *roff;               // own lists; lists of the owners holding i
+    const int *idx, *ridx;
+    const uint32_t *qflag;
+    const double *m;                         // slot order
+    const double *rho0;                      // slot order: what unsmoothed particles hold
+    double *rho;                             // slot order
+};
+
+// Density (smoothfcn.c:239-256) when sym == 0: the gather over i's own list,
+// bit for bit the density DenDVDX computes.  DensitySym (smoothfcn.c:258-274)
+// when sym == 1, in gather form: i sums the half-normalised kernel of its own
+// list with h_i and m_q, then over the owners q whose lists hold i (sorted by
+// slot, so the order is fixed) the half-normalised kernel with h_q and m_q,
+// on top of 0 (smooth-active i, initDensity smoothfcn.c:229-232) or of the
+// density i held.  16 lanes per particle; pair arithmetic in T, sums in FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_density(DSArgs A)
+{
+    const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (i >= A.n) return;
+    const bool own = A.qflag[i] != 0;
+    if (!A.sym && !own) return;
+    const double x = A.sx[i], y = A.sy[i], z = A.sz[i];
+    double so = 0, sr = 0, ih2 = 0;
+    if (own) {
+        ih2 = 4.0 / A.fb2[i];
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16) {
+            const int q = A.idx[e];
+            const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
+                                      img_sep(y, A.sy[q], A.per, A.L[1]),
+                                      img_sep(z, A.sz[q], A.per, A.L[2]));
+            so += (double)(m4_w<T>((T)(d2 * ih2)) * (T)A.m[q]);
+        }
+    }
+    if (A.sym) {
+        for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 16) {
+            const int q = A.ridx[e];
+            const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
+                                      img_sep(y, A.sy[q], A.per, A.L[1]),
+                                      img_sep(z, A.sz[q], A.per, A.L[2]));
+            const double ih2q = 4.0 / A.fb2[q];
+            const double fq = 0.5 * M_1_PI * sqrt(ih2q) * ih2q;
+            sr += fq * (double)(m4_w<T>((T)(d2 * ih2q)) * (T)A.m[q]);
+        }
+    }
+    for (int w = 1; w < 16; w <<= 1) {
+        so += __shfl_xor(so, w);
+        sr += __shfl_xor(sr, w);
+    }
+    if (sub != 0) return;
+    const double fN = M_1_PI * ih2 * sqrt(ih2);
+    if (!A.sym) A.rho[i] = so * fN;
+    else A.rho[i] = (own ? 0.0 : A.rho0[i]) + 0.5 * fN * so + sr;
+}
+
 // Pressure of the adiabatic gas (pkdGasPressure, pkd.c:5750-5785), caller
 // order, particles with the selection mask only.
 __global__ void k_gas_pressure(int n, const uint32_t *__restrict__ bits, uint32_t mask,
@@ -1863,7 +1923,7 @@ __global__ void k_flags_out(int n, const int *__restrict__ order, const uint32_t
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int c = order[s];
-    uint32_t b = bits[c] & ~(uint32_t)GG_TYPE_NbrOfACTIVE;
+    uint32_t b = bits[c];   // the operators set flags, never clear them
     if (qf[s]) b |= GG_TYPE_SMOOTHDONE;
     if (nba && (nba[s] & 32u)) b |= GG_TYPE_NbrOfACTIVE;
     bits[c] = b;
@@ -2130,6 +2190,215 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     return GG_OK;
 }
 
+// The kNN lists' bookkeeping once the operator has run: the search radii
+// (-1 marks a failed search), the convergence check, what gg_resmooth may
+// reuse, and the number of particles smoothed.
+static int knn_finish(GGPKD pkd, const unsigned *e, int K, int *pnSmoothed)
+{
+    SphDev &S = pkd->sph;
+    hipStream_t st = pkd->stream;
+    const int n = S.n;
+    S.fails = e[0];
+    S.ties = e[1];
+    for (int k = 0; k < 4; ++k) S.diag[k] = e[2 + k];
+    S.passes = e[6];
+    uint64_t cand;
+    std::memcpy(&cand, &e[8], sizeof(cand));
+    S.cand = (int64_t)cand;
+    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, S.qf.as<uint32_t>()));
+    GG_CHECK(e[0] == 0, GG_EUNSUP,
+             "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
+    GG_TRY(S.fb2knn.reserve(sizeof(double) * n));
+    GG_TRY(S.qfknn.reserve(sizeof(uint32_t) * n));
+    GG_HIP(hipMemcpyAsync(S.fb2knn.p, S.fb2s.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    S.listK = K;
+    S.listValid = (S.ties == 0);
+    int64_t nq = 0;
+    GG_TRY(exclusive_scan_i64(st, (const int *)S.qf.as<uint32_t>(), S.pos.as<int64_t>(), n,
+                              S.sortTmp, &nq));
+    S.listQueries = (int)nq;
+    if (pnSmoothed) *pnSmoothed = (int)nq;
+    return GG_OK;
+}
+
+// SMX_DENSITY over the fixed-K lists gg_smooth has just found: Density, or
+// DensitySym (bSymmetric) through the reverse lists.  Neither marks
+// TYPE_NbrOfACTIVE nor records the force pass's extras.
+static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
+{
+    SphDev &S = pkd->sph;
+    hipStream_t st = pkd->stream;
+    const int n = S.n;
+    S.exValid = 0;
+    GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
+    GG_TRY(S.o_rho.reserve(sizeof(double) * n));
+    DSArgs D;
+    D.n = n;
+    D.sym = bSymmetric;
+    D.per = S.bPeriodic;
+    for (int d = 0; d < 3; ++d) D.L[d] = S.L[d];
+    D.sx = S.sx.as<double>();
+    D.sy = S.sy.as<double>();
+    D.sz = S.sz.as<double>();
+    D.fb2 = S.fb2s.as<double>();
+    D.off = S.off.as<int64_t>();
+    D.idx = S.nbr.as<int>();
+    D.roff = nullptr;
+    D.ridx = nullptr;
+    D.qflag = S.qf.as<uint32_t>();
+    D.m = S.m.as<double>();
+    D.rho0 = nullptr;
+    D.rho = S.o_rho.as<double>();
+    if (bSymmetric) {   // lists of the owners that hold each particle, sorted
+        GG_TRY(gather_field(pkd, GG_DENSITY, 1, 0, S.rho));
+        GG_TRY(S.rcnt.reserve(sizeof(int) * (n + 1)));
+        GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
+        GG_HIP(hipMemsetAsync(S.rcnt.p, 0, sizeof(int) * (n + 1), st));
+        hipLaunchKernelGGL(k_rev_count, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           S.qf.as<uint32_t>(), S.off.as<int64_t>(), S.nbr.as<int>(),
+                           S.rcnt.as<int>());
+        int64_t rtot = 0;
+        GG_TRY(exclusive_scan_i64(st, S.rcnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &rtot));
+        GG_TRY(S.ridx.reserve(sizeof(int) * std::max<int64_t>(rtot, 1)));
+        GG_TRY(S.rcur.reserve(sizeof(int) * (n + 1)));
+        GG_HIP(hipMemsetAsync(S.rcur.p, 0, sizeof(int) * (n + 1), st));
+        hipLaunchKernelGGL(k_rev_fill, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           S.qf.as<uint32_t>(), S.off.as<int64_t>(), S.nbr.as<int>(),
+                           S.roff.as<int64_t>(), S.rcur.as<int>(), S.ridx.as<int>());
+        hipLaunchKernelGGL(k_rev_sort, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                           S.roff.as<int64_t>(), S.ridx.as<int>());
+        GG_HIP(hipGetLastError());
+        D.roff = S.roff.as<int64_t>();
+        D.ridx = S.ridx.as<int>();
+        D.rho0 = S.rho.as<double>();
+    }
+    timer_mark(pkd, 4);
+    const int grid = nblk(n * 16, 256);
+    if (pkd->prec == GG_PREC_FP32)
+        hipLaunchKernelGGL(k_density<float>, dim3(grid), dim3(256), 0, st, D);
+    else
+        hipLaunchKernelGGL(k_density<double>, dim3(grid), dim3(256), 0, st, D);
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 5);
+    unsigned e[12];
+    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    GG_TRY(knn_finish(pkd, e, K, pnSmoothed));
+    // DensitySym writes every particle of the structure (the unsmoothed ones
+    // receive their neighbours' halves); Density the smoothed ones
+    GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, bSymmetric ? nullptr : S.qf.as<uint32_t>()));
+    hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                       S.qf.as<uint32_t>(), (const uint32_t *)nullptr,
+                       pkd->f[GG_IACTIVE].as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    timer_mark(pkd, 6);
+    GG_HIP(hipStreamSynchronize(st));
+    const int ph[6] = {9, 3, 10, bSymmetric ? 6 : 8, 4, 8};
+    timer_collect(pkd, 0, 6, ph);
+    return GG_OK;
+}
+
+// DenDVDX (smoothfcn.c:3390-3627) over the CSR lists (off, idx) of the
+// smooth-active particles: slot-order inputs, the force pass's extras reset,
+// then the kernel.  The caller reads the error words.
+static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, const int *idx)
+{
+    SphDev &S = pkd->sph;
+    hipStream_t st = pkd->stream;
+    const int n = S.n;
+    const bool f32 = pkd->prec == GG_PREC_FP32;
+    if (f32) {
+        for (int f : {GG_MASS, GG_VPRED, GG_UPRED}) GG_TRY(ensure_field(pkd, f));
+        GG_TRY(S.jv.reserve(sizeof(float4) * (size_t)n));
+        GG_TRY(S.ju.reserve(sizeof(float) * (size_t)n));
+        hipLaunchKernelGGL(k_dpack, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                           pkd->f[GG_MASS].as<double>(), pkd->f[GG_VPRED].as<double>(),
+                           pkd->f[GG_UPRED].as<double>(), S.jv.as<float4>(), S.ju.as<float>());
+    } else {
+        GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
+        GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
+        GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
+    }
+    GG_TRY(S.o_rho.reserve(sizeof(double) * n));
+    GG_TRY(S.o_divv.reserve(sizeof(double) * n));
+    GG_TRY(S.o_curl.reserve(sizeof(double) * 3 * n));
+    GG_TRY(S.o_dvds.reserve(sizeof(double) * n));
+    GG_TRY(S.o_divvt.reserve(sizeof(double) * n));
+    GG_TRY(S.o_bal.reserve(sizeof(double) * n));
+    GG_TRY(S.o_diff.reserve(sizeof(double) * n));
+    DArgs D;
+    D.n = n;
+    D.op = GG_SMX_DENDVDX;
+    D.per = S.bPeriodic;
+    for (int d = 0; d < 3; ++d) D.L[d] = S.L[d];
+    D.sx = S.sx.as<double>();
+    D.sy = S.sy.as<double>();
+    D.sz = S.sz.as<double>();
+    D.fb2 = S.fb2s.as<double>();
+    D.off = off;
+    D.idx = idx;
+    D.qflag = S.qf.as<uint32_t>();
+    D.m = S.m.as<double>();
+    D.vx = S.vx.as<double>();
+    D.vy = S.vy.as<double>();
+    D.vz = S.vz.as<double>();
+    D.u = S.u.as<double>();
+    D.jv = S.jv.as<float4>();
+    D.ju = S.ju.as<float>();
+    D.act = S.act.as<uint32_t>();
+    D.H = smf->H;
+    D.a = smf->a;
+    D.gamma = smf->gamma;
+    D.bCannonical = smf->bCannonical;
+    D.iViscosityLimiter = smf->iViscosityLimiter;
+    D.bConstantDiffusion = smf->bConstantDiffusion;
+    D.rho = S.o_rho.as<double>();
+    D.divv = S.o_divv.as<double>();
+    D.curl = S.o_curl.as<double>();
+    D.dvds = S.o_dvds.as<double>();
+    D.divvt = S.o_divvt.as<double>();
+    D.bal = S.o_bal.as<double>();
+    D.diff = S.o_diff.as<double>();
+    D.nbract = S.nba.as<uint32_t>();
+    GG_TRY(S.ecnt.reserve(sizeof(int) * (size_t)n));
+    GG_TRY(S.eidx.reserve(sizeof(int) * (size_t)n * ECAP));
+    GG_HIP(hipMemsetAsync(S.ecnt.p, 0, sizeof(int) * (size_t)n, st));
+    D.ecnt = S.ecnt.as<int>();
+    D.eidx = S.eidx.as<int>();
+    D.ecap = ECAP;
+    D.err = S.err.as<unsigned>();
+    const int grid = nblk(n * 16, 256);
+    if (pkd->prec == GG_PREC_FP32)
+        hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
+    else
+        hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// DenDVDX's outputs into the store (smoothed particles only) and the flags.
+static int dendvdx_store(GGPKD pkd)
+{
+    SphDev &S = pkd->sph;
+    hipStream_t st = pkd->stream;
+    const int n = S.n;
+    const uint32_t *qf = S.qf.as<uint32_t>();
+    GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_divv, GG_DIVV, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_curl, GG_CURLV, 3, qf));
+    GG_TRY(scatter_field(pkd, S.o_dvds, GG_DVDS, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_divvt, GG_FDIVV_T, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_bal, GG_BALSARA, 1, qf));
+    GG_TRY(scatter_field(pkd, S.o_diff, GG_DIFF, 1, qf));
+    hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
+                       S.qf.as<uint32_t>(), S.nba.as<uint32_t>(), pkd->f[GG_IACTIVE].as<uint32_t>());
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
 extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
                          int bSymmetric, int iSmoothType, int *pnSmoothed)
 {
@@ -2139,7 +2408,8 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_CHECK(bPeriodic == S.bPeriodic, GG_EINVAL, "bPeriodic differs from gg_build_local");
     GG_CHECK(iSmoothType == GG_SMX_DENSITY || iSmoothType == GG_SMX_DENDVDX, GG_EUNSUP,
              "gg_smooth implements SMX_DENSITY and SMX_DENDVDX");
-    GG_CHECK(bSymmetric == 0, GG_EUNSUP, "the density operators run in gather form (bSymmetric=0)");
+    GG_CHECK(bSymmetric == 0 || bSymmetric == 1, GG_EINVAL, "bSymmetric %d is not 0 or 1",
+             bSymmetric);
     GG_CHECK(nSmooth >= 1 && nSmooth <= 128, GG_EINVAL, "nSmooth %d outside 1..128", nSmooth);
     GG_CHECK(nSmooth <= S.n, GG_EINVAL, "nSmooth %d exceeds the %d particles", nSmooth, S.n);
     hipStream_t st = pkd->stream;
@@ -2193,116 +2463,17 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(S.off.reserve(sizeof(int64_t) * (n + 1)));
     hipLaunchKernelGGL(k_fixed_off, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, K,
                        S.off.as<int64_t>());
-    const bool f32 = pkd->prec == GG_PREC_FP32;
-    if (f32) {
-        for (int f : {GG_MASS, GG_VPRED, GG_UPRED}) GG_TRY(ensure_field(pkd, f));
-        GG_TRY(S.jv.reserve(sizeof(float4) * (size_t)n));
-        GG_TRY(S.ju.reserve(sizeof(float) * (size_t)n));
-        hipLaunchKernelGGL(k_dpack, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
-                           pkd->f[GG_MASS].as<double>(), pkd->f[GG_VPRED].as<double>(),
-                           pkd->f[GG_UPRED].as<double>(), S.jv.as<float4>(), S.ju.as<float>());
-    } else {
-        GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
-        GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
-    }
-    GG_TRY(S.o_rho.reserve(sizeof(double) * n));
-    GG_TRY(S.o_divv.reserve(sizeof(double) * n));
-    GG_TRY(S.o_curl.reserve(sizeof(double) * 3 * n));
-    GG_TRY(S.o_dvds.reserve(sizeof(double) * n));
-    GG_TRY(S.o_divvt.reserve(sizeof(double) * n));
-    GG_TRY(S.o_bal.reserve(sizeof(double) * n));
-    GG_TRY(S.o_diff.reserve(sizeof(double) * n));
-    DArgs D;
-    D.n = n;
-    D.op = iSmoothType;
-    D.per = S.bPeriodic;
-    for (int d = 0; d < 3; ++d) D.L[d] = S.L[d];
-    D.sx = S.sx.as<double>();
-    D.sy = S.sy.as<double>();
-    D.sz = S.sz.as<double>();
-    D.fb2 = S.fb2s.as<double>();
-    D.off = S.off.as<int64_t>();
-    D.idx = S.nbr.as<int>();
-    D.qflag = S.qf.as<uint32_t>();
-    D.m = S.m.as<double>();
-    D.vx = S.vx.as<double>();
-    D.vy = S.vy.as<double>();
-    D.vz = S.vz.as<double>();
-    D.u = S.u.as<double>();
-    D.jv = S.jv.as<float4>();
-    D.ju = S.ju.as<float>();
-    D.act = S.act.as<uint32_t>();
-    D.H = smf->H;
-    D.a = smf->a;
-    D.gamma = smf->gamma;
-    D.bCannonical = smf->bCannonical;
-    D.iViscosityLimiter = smf->iViscosityLimiter;
-    D.bConstantDiffusion = smf->bConstantDiffusion;
-    D.rho = S.o_rho.as<double>();
-    D.divv = S.o_divv.as<double>();
-    D.curl = S.o_curl.as<double>();
-    D.dvds = S.o_dvds.as<double>();
-    D.divvt = S.o_divvt.as<double>();
-    D.bal = S.o_bal.as<double>();
-    D.diff = S.o_diff.as<double>();
-    D.nbract = S.nba.as<uint32_t>();
-    GG_TRY(S.ecnt.reserve(sizeof(int) * (size_t)n));
-    GG_TRY(S.eidx.reserve(sizeof(int) * (size_t)n * ECAP));
-    GG_HIP(hipMemsetAsync(S.ecnt.p, 0, sizeof(int) * (size_t)n, st));
-    D.ecnt = S.ecnt.as<int>();
-    D.eidx = S.eidx.as<int>();
-    D.ecap = ECAP;
-    D.err = S.err.as<unsigned>();
-    const int grid = nblk(n * 16, 256);
+    if (iSmoothType == GG_SMX_DENSITY) return density_pass(pkd, K, bSymmetric, pnSmoothed);
     timer_mark(pkd, 4);
-    if (pkd->prec == GG_PREC_FP32)
-        hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
-    else
-        hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
-    GG_HIP(hipGetLastError());
+    GG_TRY(dendvdx_launch(pkd, smf, S.off.as<int64_t>(), S.nbr.as<int>()));
     timer_mark(pkd, 5);
     unsigned e[12];
     GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
-    S.fails = e[0];
-    S.ties = e[1];
-    for (int k = 0; k < 4; ++k) S.diag[k] = e[2 + k];
-    S.passes = e[6];
     S.exValid = (e[7] == 0);   // the extras lists are whole
-    uint64_t cand;
-    std::memcpy(&cand, &e[8], sizeof(cand));
-    S.cand = (int64_t)cand;
-    const uint32_t *qf = S.qf.as<uint32_t>();
-    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, qf));   // -1 marks a failed search
-    GG_CHECK(e[0] == 0, GG_EUNSUP,
-             "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
-    GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, qf));
-    if (iSmoothType == GG_SMX_DENDVDX) {
-        GG_TRY(scatter_field(pkd, S.o_divv, GG_DIVV, 1, qf));
-        GG_TRY(scatter_field(pkd, S.o_curl, GG_CURLV, 3, qf));
-        GG_TRY(scatter_field(pkd, S.o_dvds, GG_DVDS, 1, qf));
-        GG_TRY(scatter_field(pkd, S.o_divvt, GG_FDIVV_T, 1, qf));
-        GG_TRY(scatter_field(pkd, S.o_bal, GG_BALSARA, 1, qf));
-        GG_TRY(scatter_field(pkd, S.o_diff, GG_DIFF, 1, qf));
-    }
-    hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
-                       S.qf.as<uint32_t>(), S.nba.as<uint32_t>(), pkd->f[GG_IACTIVE].as<uint32_t>());
+    GG_TRY(knn_finish(pkd, e, K, pnSmoothed));
+    GG_TRY(dendvdx_store(pkd));
     GG_HIP(hipGetLastError());
-    // remember which lists are exact ball lists for gg_resmooth
-    GG_TRY(S.fb2knn.reserve(sizeof(double) * n));
-    GG_TRY(S.qfknn.reserve(sizeof(uint32_t) * n));
-    GG_HIP(hipMemcpyAsync(S.fb2knn.p, S.fb2s.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
-    S.listK = K;
-    S.listValid = (S.ties == 0);
-    int64_t nq = 0;
-    GG_TRY(exclusive_scan_i64(st, (const int *)S.qf.as<uint32_t>(), S.pos.as<int64_t>(), n,
-                              S.sortTmp, &nq));
-    S.listQueries = (int)nq;
-    if (pnSmoothed) *pnSmoothed = (int)nq;
     timer_mark(pkd, 6);
     GG_HIP(hipStreamSynchronize(st));
     const int ph[6] = {9, 3, 10, 8, 4, 8};
@@ -2339,17 +2510,18 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     SphDev &S = pkd->sph;
     GG_CHECK(S.built, GG_ESTATE, "gg_resmooth before gg_build_local");
     GG_CHECK(bPeriodic == S.bPeriodic, GG_EINVAL, "bPeriodic differs from gg_build_local");
-    GG_CHECK(iSmoothType == GG_SMX_SPHPRESSURETERMS, GG_EUNSUP,
-             "gg_resmooth implements SMX_SPHPRESSURETERMS");
-    GG_CHECK(bSymmetric == 1, GG_EUNSUP, "SphPressureTerms is the symmetric operator (bSymmetric=1)");
+    GG_CHECK(iSmoothType == GG_SMX_SPHPRESSURETERMS || iSmoothType == GG_SMX_DENDVDX, GG_EUNSUP,
+             "gg_resmooth implements SMX_SPHPRESSURETERMS and SMX_DENDVDX");
+    GG_CHECK(bSymmetric == 1 || (iSmoothType == GG_SMX_DENDVDX && bSymmetric == 0), GG_EUNSUP,
+             "SphPressureTerms is the symmetric operator (bSymmetric=1)");
     hipStream_t st = pkd->stream;
     const int n = S.n;
     timer_mark(pkd, 0);
     GG_TRY(make_flags(pkd, GG_TYPE_SMOOTHACTIVE));
     GG_TRY(S.fb2s.reserve(sizeof(double) * n));
     GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.fb2s));
-    GG_TRY(S.err.reserve(sizeof(unsigned) * 2));
-    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 2, st));
+    GG_TRY(S.err.reserve(sizeof(unsigned) * 12));
+    GG_HIP(hipMemsetAsync(S.err.p, 0, sizeof(unsigned) * 12, st));
     // reuse the kNN lists when they are exactly the ball lists of the
     // stored radii (same queries, same fBall2, no boundary ties)
     bool reuse = false;
@@ -2392,6 +2564,21 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         idx = S.gent.as<int>();
     }
     timer_mark(pkd, 1);
+    if (iSmoothType == GG_SMX_DENDVDX) {   // DenDVDX over the ball lists
+        GG_TRY(dendvdx_launch(pkd, smf, off, idx));
+        timer_mark(pkd, 2);
+        unsigned e[12];
+        GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
+        GG_HIP(hipStreamSynchronize(st));
+        // the extras belong to the kNN lists only when those were the lists
+        S.exValid = reuse && e[7] == 0;
+        GG_TRY(dendvdx_store(pkd));
+        timer_mark(pkd, 3);
+        GG_HIP(hipStreamSynchronize(st));
+        const int ph[3] = {5, 4, 8};
+        timer_collect(pkd, 0, 3, ph);
+        return GG_OK;
+    }
     const bool fixedExtras = reuse && S.exValid;
     if (fixedExtras) {   // the density pass found them: sort each list
         hipLaunchKernelGGL(k_rev_sort_fixed, dim3(nblk(n, 256)), dim3(256), 0, st, n, ECAP,

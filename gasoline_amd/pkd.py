@@ -284,10 +284,12 @@ class Pkd:
             L = np.ascontiguousarray(period, dtype=np.float64)
             _check(lib().gg_build_local(self.h, int(nBucket), 1, _p(L)))
 
-    def smooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_DENDVDX):
+    def smooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_DENDVDX, symmetric=False):
+        """gg_smooth; symmetric selects DensitySym for SMX_DENSITY (DenDVDX
+        is the same operator either way)."""
         ns = C.c_int()
-        _check(lib().gg_smooth(self.h, C.byref(smf), int(nSmooth), int(bool(periodic)), 0,
-                               int(iSmoothType), C.byref(ns)))
+        _check(lib().gg_smooth(self.h, C.byref(smf), int(nSmooth), int(bool(periodic)),
+                               int(bool(symmetric)), int(iSmoothType), C.byref(ns)))
         return ns.value
 
     def resmooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_SPHPRESSURETERMS):

@@ -193,16 +193,25 @@ int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const double *fPeriod)
  * squared distance, then the operator iSmoothType (GG_SMX_DENSITY or
  * GG_SMX_DENDVDX) over that list.  Replaces smInitialize + smSmooth + smFinish
  * (smooth.h:616-617,638; smooth.c:1439-1713) for those operators.
+ * bSymmetric selects the operator as smInitialize does (smooth.c:101-106,
+ * 272-279): SMX_DENSITY runs Density (0, smoothfcn.c:239-256) or DensitySym
+ * (1, smoothfcn.c:258-274: each pair adds half of the list owner's kernel to
+ * both ends; particles of the structure that are not smooth-active keep
+ * their fDensity plus what their neighbours add).  SMX_DENDVDX runs DenDVDX
+ * either way.  Flags are only set: TYPE_SMOOTHDONE on the smoothed
+ * particles, TYPE_NbrOfACTIVE by DenDVDX (smoothfcn.c:3424,3487); the caller
+ * resets them as msrResetType does (master.c:9279).
  * *pnSmoothed receives smf->nSmoothed. */
 int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
               int bSymmetric, int iSmoothType, int *pnSmoothed);
 
 /* Ball-gather re-smooth at each particle's stored fBall2 (d2 <= fBall2) and
- * the operator iSmoothType (GG_SMX_SPHPRESSURETERMS).  The symmetric
- * operator is evaluated in gather form over the neighbour and reverse-
- * neighbour lists, so no scatter or atomics are used.  Replaces smInitialize
- * + smReSmooth + smFinish (smooth.h:636; smooth.c:1716-1749) with
- * SphPressureTermsSym (smoothfcn.c:2276-2385). */
+ * the operator iSmoothType: GG_SMX_SPHPRESSURETERMS (bSymmetric=1,
+ * SphPressureTermsSym smoothfcn.c:2276-2385), evaluated in gather form over
+ * the neighbour and reverse-neighbour lists, so no scatter or atomics are
+ * used; or GG_SMX_DENDVDX (DenDVDX over the ball lists, as msrInitSph calls
+ * it, master.c:9044).  Replaces smInitialize + smReSmooth + smFinish
+ * (smooth.h:636; smooth.c:1716-1749) for those operators. */
 int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
                 int bSymmetric, int iSmoothType);
 

@@ -30,6 +30,10 @@
  *   orc_knn                brute-force full sort on small inputs.
  *   orc_dendvdx            uniform density, linear velocity field (div, curl
  *                          exact), M4 normalisation.
+ *   orc_density_sym        sum of m*rho equal to the gather density's (each pair
+ *                          counted once per end), gather equality on a lattice
+ *                          whose lists are symmetric, unsmoothed particles keep
+ *                          their value plus what their neighbours scatter.
  *   orc_sph_pressure_sym   momentum conservation, energy conservation, uniform
  *                          pressure => zero force, AV only for approaching pairs.
  */
@@ -134,6 +138,15 @@ void orc_dendvdx(int n, const double *m, const double *vpred,
                  const double *d2, const orc_smf *smf, double *rho,
                  double *divv, double *curlv, double *dvds, double *fDivv_t,
                  double *balsara, double *diff, unsigned char *nbrOfActive);
+
+/* DensitySym (smoothfcn.c:258-274) over CSR lists (the nSmooth nearest of
+ * each smooth-active particle, d2 per entry): rho of smooth-active
+ * particles starts at 0 (initDensity, smoothfcn.c:229-232), the others keep
+ * the rho passed in; each pair adds the half-normalised kernel of the list
+ * owner to both ends. */
+void orc_density_sym(int n, const double *m, const unsigned char *smoothActive,
+                     const double *fBall2, const long long *off, const int *idx,
+                     const double *d2, double *rho);
 
 /* pkdGasPressure (pkd.c:5750-5785), adiabatic, no cooling: PoverRho2, c and
  * the Courant update of dt (in/out). */

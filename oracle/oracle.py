@@ -75,6 +75,7 @@ def lib():
         L.orc_dendvdx.argtypes = [C.c_int, _dp, _dp, _dp, _up, _up, _dp, _lp, _ip,
                                   _dp, _dp, C.POINTER(Smf), _dp, _dp, _dp, _dp,
                                   _dp, _dp, _dp, _up]
+        L.orc_density_sym.argtypes = [C.c_int, _dp, _up, _dp, _lp, _ip, _dp, _dp]
         L.orc_gas_pressure.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Smf),
                                        _dp, _dp, _dp]
         L.orc_sph_pressure_sym.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -273,6 +274,19 @@ def dendvdx(m, vpred, upred, fBall2, lists, smf, active=None, smooth_active=None
                       P(out["fDivv_t"]), P(out["balsara"]), P(out["diff"]), P(nbract, _up))
     out["curlv"] = curlv
     out["nbrOfActive"] = nbract
+    return out
+
+
+def density_sym(m, fBall2, lists, smooth_active=None, rho=None):
+    """DensitySym over CSR lists (off, idx, dxyz, d2); rho (if given) is the
+    value the particles that are not smooth-active start from."""
+    off, idx, _, d2 = lists
+    n = len(m)
+    sact = np.ones(n, np.uint8) if smooth_active is None else np.ascontiguousarray(smooth_active, np.uint8)
+    out = np.zeros(n) if rho is None else _d(rho).copy()
+    lib().orc_density_sym(n, P(_d(m)), P(sact, _up), P(_d(fBall2)),
+                          P(np.ascontiguousarray(off, np.int64), _lp),
+                          P(np.ascontiguousarray(idx, np.int32), _ip), P(_d(d2)), P(out))
     return out
 
 

@@ -359,6 +359,32 @@ void orc_dendvdx(int n, const double *m, const double *vpred,
     }
 }
 
+/* ------------------------------------ DensitySym (smoothfcn.c:258-274) */
+void orc_density_sym(int n, const double *m, const unsigned char *smoothActive,
+                     const double *fBall2, const long long *off, const int *idx,
+                     const double *d2, double *rho)
+{
+    int i;
+    /* initDensity on the smooth-active particles (smooth.c:466-472,
+     * smoothfcn.c:229-232); the others keep what they held. */
+    for (i = 0; i < n; ++i)
+        if (smoothActive[i]) rho[i] = 0.0;
+    /* serial: every pair updates both ends */
+    for (i = 0; i < n; ++i) {
+        double ih2, fNorm;
+        long long k;
+        if (!smoothActive[i]) continue;
+        ih2 = 4.0 / fBall2[i];
+        fNorm = 0.5 * M_1_PI * sqrt(ih2) * ih2;
+        for (k = off[i]; k < off[i + 1]; ++k) {
+            int q = idx[k];
+            double rs = m4_w(d2[k] * ih2) * fNorm;
+            rho[i] += rs * m[q];
+            rho[q] += rs * m[i];
+        }
+    }
+}
+
 void orc_gas_pressure(int n, const double *upred, const double *rho,
                       const double *fBall2, const double *udot_hydro,
                       const orc_smf *smf, double *PoverRho2, double *c,

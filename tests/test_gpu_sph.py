@@ -249,6 +249,77 @@ def test_density_and_hydro_vs_oracle(gpu, case, prec, request):
     assert (h["uDotDiff"] == 0).all()
 
 
+@pytest.mark.parametrize("prec", ["FP64", "FP32"])
+@pytest.mark.parametrize("case", ["box64k", "evrard32k"])
+def test_density_operators(gpu, case, prec, request):
+    # Density equals DenDVDX's density bit for bit and marks no neighbour of
+    # an active particle; DensitySym against the oracle's serial scatter
+    d, ref = request.getfixturevalue(case)
+    periodic = case == "box64k"
+    per = [1, 1, 1] if periodic else None
+    K = 64 if periodic else 32
+    smf = gpu.Smf.default()
+    p = gpu.PREC_FP64 if prec == "FP64" else gpu.PREC_FP32
+    tol = 1e-12 if prec == "FP64" else 1e-5
+    n = len(d["m"])
+    assert ref["ties"] == 0
+    sym_ref = O.density_sym(d["m"], ref["fb2"], O.ball_gather(d["r"], ref["fb2"], period=per))
+    bits = np.full(n, 7, np.uint32)
+    with gpu.Pkd(n, prec=p) as pk:
+        put_gas(pk, d)
+        pk.put("iActive", bits)
+        pk.build_local(period=per)
+        pk.smooth(smf, K, periodic)
+        rho_dv = pk.get("fDensity")
+        pk.put("iActive", bits)               # drops the structure: build it again
+        pk.put("fDensity", np.zeros(n))
+        pk.put("fBall2", np.zeros(n))         # the same cold search: the same list order
+        pk.build_local(period=per)
+        assert pk.smooth(smf, K, periodic, iSmoothType=gpu.SMX_DENSITY) == n
+        rho = pk.get("fDensity")
+        flags = pk.get("iActive")
+        fb2 = pk.get("fBall2")
+        pk.put("fDensity", np.full(n, 5.0))      # overwritten: every particle is smoothed
+        pk.smooth(smf, K, periodic, iSmoothType=gpu.SMX_DENSITY, symmetric=True)
+        sym = pk.get("fDensity")
+    assert np.array_equal(rho, rho_dv)
+    assert np.array_equal(fb2, ref["fb2"])
+    assert (flags & 32 == 0).all() and (flags & 8).all()
+    assert scaled(sym, sym_ref) < tol
+    # each pair counted once per end: the mass-weighted sums agree
+    assert np.dot(d["m"], sym) == pytest.approx(np.dot(d["m"], rho), rel=10 * tol)
+
+
+def test_density_sym_partial_smooth_active(gpu):
+    # particles that are not smooth-active keep their density plus what
+    # their smooth-active neighbours add; their fBall2 is left alone
+    d = ics.sph_box(1 << 14, seed=33)
+    n = len(d["m"])
+    rng = np.random.default_rng(6)
+    sact = rng.uniform(size=n) < 0.05
+    m = d["m"] * rng.uniform(0.5, 1.5, n)
+    rho0 = rng.uniform(1, 2, n)
+    fb0 = rng.uniform(1e-3, 2e-3, n)
+    per = [1, 1, 1]
+    nbr, fb2, ties = O.knn(d["r"], 64, period=per)
+    assert ties == 0
+    ref = O.density_sym(m, fb2, O.ball_gather(d["r"], fb2, period=per),
+                        smooth_active=sact.astype(np.uint8), rho=rho0)
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        pk.put("r", d["r"])
+        pk.put("fMass", m)
+        pk.put("iActive", np.where(sact, 7, 3).astype(np.uint32))
+        pk.put("fDensity", rho0)
+        pk.put("fBall2", fb0)
+        pk.build_local(period=per)
+        ns = pk.smooth(gpu.Smf.default(), 64, True, iSmoothType=gpu.SMX_DENSITY, symmetric=True)
+        got, gfb2, flags = pk.get("fDensity"), pk.get("fBall2"), pk.get("iActive")
+    assert ns == sact.sum()
+    assert scaled(got, ref) < 1e-12
+    assert np.array_equal(gfb2[sact], fb2[sact]) and np.array_equal(gfb2[~sact], fb0[~sact])
+    assert np.array_equal((flags & 8) != 0, sact)
+
+
 def test_resmooth_ball_gather_path(gpu, box64k):
     # enlarged radii force the explicit ball gather (lists longer than nSmooth)
     d, ref = box64k
@@ -265,6 +336,31 @@ def test_resmooth_ball_gather_path(gpu, box64k):
         acc, pdv = pk.get("a"), pk.get("uDotPdV")
     assert scaled(acc, oref["hyd"]["acc"]) < 1e-12
     assert scaled(pdv, oref["hyd"]["uDotPdV"]) < 1e-12
+
+
+def test_resmooth_dendvdx(gpu, box64k):
+    # at the stored radii the kNN lists are the ball lists: the re-smooth
+    # repeats the smooth bit for bit; at enlarged radii it gathers the balls
+    d, ref = box64k
+    smf = gpu.Smf.default()
+    fb2 = ref["fb2"] * 1.21
+    oden = O.dendvdx(d["m"], d["v"], d["u"], fb2, O.ball_gather(d["r"], fb2, period=[1, 1, 1]),
+                     O.default_smf())
+    with gpu.Pkd(len(d["m"]), prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        pk.build_local(period=[1, 1, 1])
+        pk.smooth(smf, 64, True, symmetric=True)
+        first = {k: pk.get(v) for k, v in DENS.items()}
+        pk.resmooth(smf, 64, True, iSmoothType=gpu.SMX_DENDVDX)
+        again = {k: pk.get(v) for k, v in DENS.items()}
+        pk.put("fBall2", fb2)
+        pk.resmooth(smf, 64, True, iSmoothType=gpu.SMX_DENDVDX)
+        wide = {k: pk.get(v) for k, v in DENS.items()}
+        curl = pk.get("curlv")
+    for k in DENS:
+        assert np.array_equal(again[k], first[k]), k
+        assert scaled(wide[k], oden[k]) < 1e-12, k
+    assert scaled(curl, oden["curlv"]) < 1e-12
 
 
 def test_partial_activity(gpu):
@@ -290,7 +386,9 @@ def test_partial_activity(gpu):
         pk.gas_pressure(gpu.Smf.default())
         pk.resmooth(gpu.Smf.default(), 64, True)
         acc, mumax, dtnew = pk.get("a"), pk.get("mumax"), pk.get("dtNew")
+        flags2 = pk.get("iActive")
     assert np.array_equal((flags & 32) != 0, den["nbrOfActive"] != 0)
+    assert np.array_equal(flags2 & 32, flags & 32)   # SphPressureTerms leaves the mark
     assert (flags & 8).all()                     # TYPE_SMOOTHDONE set
     assert (acc[~active] == 0).all()
     assert scaled(acc[active], hyd["acc"][active]) < 1e-12

@@ -145,3 +145,56 @@ def test_viscosity_only_for_approaching_pairs(box):
         else:
             assert (h["uDotAV"][inner] > 0).all() and (h["mumax"][inner] > 0).all()
             assert (h["uDotPdV"][inner] > 0).all()
+
+
+def test_density_sym_weighted_sum_equals_gather(box):
+    # every pair (p, q in N(p)) adds m_p*m_q*W_p*fNorm_p/2 to both ends, so the
+    # mass-weighted sum of the symmetric density equals the gather density's
+    m = box["m"] * np.random.default_rng(3).uniform(0.5, 1.5, len(box["m"]))
+    out = O.dendvdx(m, np.zeros_like(box["r"]), box["u"], box["fb2"], box["lists"],
+                    O.default_smf())
+    sym = O.density_sym(m, box["fb2"], box["lists"])
+    assert np.dot(m, sym) == pytest.approx(np.dot(m, out["rho"]), rel=1e-12)
+    # unequal radii make it differ particle by particle
+    assert np.abs(sym / out["rho"] - 1).max() > 1e-3
+
+
+def test_density_sym_partial_smooth_active(box):
+    # particles that are not smooth-active keep the density they held plus
+    # what smooth-active neighbours scatter onto them
+    n = len(box["m"])
+    rng = np.random.default_rng(4)
+    sact = rng.uniform(size=n) < 0.01
+    m = box["m"] * rng.uniform(0.5, 1.5, n)
+    rho0 = rng.uniform(1, 2, n)
+    out = O.dendvdx(m, np.zeros_like(box["r"]), box["u"], box["fb2"], box["lists"],
+                    O.default_smf(), smooth_active=sact.astype(np.uint8))
+    sym = O.density_sym(m, box["fb2"], box["lists"], smooth_active=sact.astype(np.uint8),
+                        rho=rho0)
+    base = np.where(sact, 0.0, rho0)
+    assert np.dot(m, sym - base) == pytest.approx(np.dot(m[sact], out["rho"][sact]), rel=1e-12)
+    off, idx = box["lists"][0], box["lists"][1]
+    touched = np.zeros(n, bool)
+    for p in np.flatnonzero(sact):
+        touched[idx[off[p]:off[p + 1]]] = True
+    untouched = ~sact & ~touched
+    assert untouched.any() and np.array_equal(sym[untouched], rho0[untouched])
+    # (the farthest neighbour sits at u = 2, where W = 0)
+    assert (sym[~sact & touched] >= rho0[~sact & touched]).all()
+    assert np.mean(sym[~sact & touched] > rho0[~sact & touched]) > 0.9
+
+
+def test_density_sym_equals_gather_on_symmetric_lists():
+    # perfect periodic cubic lattice, 27 neighbours (shells 0, 1, sqrt2, sqrt3
+    # lattice spacings; the next shell is at 2): every list is symmetric and
+    # every radius equal, so the symmetric and gather forms agree
+    g = 8
+    r = (np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) / g
+    n = len(r)
+    nbr, fb2, ties = O.knn(r, 27, period=[1, 1, 1])
+    assert ties == 0 and np.all(fb2 == fb2[0])
+    lists = O.ball_gather(r, fb2, period=[1, 1, 1])
+    m = np.full(n, 1.0 / n)
+    out = O.dendvdx(m, np.zeros((n, 3)), np.ones(n), fb2, lists, O.default_smf())
+    sym = O.density_sym(m, fb2, lists)
+    assert np.allclose(sym, out["rho"], rtol=1e-14, atol=0)
