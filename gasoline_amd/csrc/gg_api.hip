@@ -5,7 +5,9 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace gg {
 
@@ -158,6 +160,8 @@ extern "C" int gg_initialize(GGPKD *ppkd, int device, int nStore)
     return GG_OK;
 }
 
+static void aos_release(GGPKD pkd);
+
 extern "C" int gg_finish(GGPKD pkd)
 {
     if (!pkd) return GG_OK;
@@ -170,6 +174,7 @@ extern "C" int gg_finish(GGPKD pkd)
     if (G.graph) (void)hipGraphDestroy(G.graph);
     sph_release(pkd);
     pkd->mlp.release_all();
+    aos_release(pkd);
     if (pkd->tm.created)
         for (int k = 0; k < 16; ++k) (void)hipEventDestroy(pkd->tm.ev[k]);
     (void)hipStreamDestroy(pkd->stream);
@@ -199,6 +204,17 @@ extern "C" int gg_set_precision(GGPKD pkd, int prec)
     return GG_OK;
 }
 
+// What a new value of a field invalidates: the trees hold positions, masses,
+// softenings and type bits; the SPH structure positions and type bits.
+static void put_done(GGPKD pkd, int field)
+{
+    if (field == GG_IACTIVE) pkd->sph.gasMaskValid = 0;
+    if (field == GG_R || field == GG_MASS || field == GG_SOFT || field == GG_IACTIVE) {
+        pkd->grav.built = 0;
+        if (field != GG_MASS && field != GG_SOFT) pkd->sph.built = 0;
+    }
+}
+
 extern "C" int gg_put(GGPKD pkd, int field, const void *src, int bDevice)
 {
     GG_CHECK(pkd && src, GG_EINVAL, "null argument");
@@ -210,11 +226,7 @@ extern "C" int gg_put(GGPKD pkd, int field, const void *src, int bDevice)
                               bDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               pkd->stream));
     GG_HIP(hipStreamSynchronize(pkd->stream));
-    if (field == GG_IACTIVE) pkd->sph.gasMaskValid = 0;
-    if (field == GG_R || field == GG_MASS || field == GG_SOFT || field == GG_IACTIVE) {
-        pkd->grav.built = 0;
-        if (field != GG_MASS && field != GG_SOFT) pkd->sph.built = 0;
-    }
+    put_done(pkd, field);
     return GG_OK;
 }
 
@@ -228,6 +240,171 @@ extern "C" int gg_get(GGPKD pkd, int field, void *dst, int bDevice)
         GG_HIP(hipMemcpyAsync(dst, pkd->f[field].p, nb,
                               bDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                               pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+// ------------------------------------------------ array-of-structs I/O
+static const size_t AOS_CHUNK = 32u << 20;   // staging bytes per buffer
+
+static int aos_stage(GGPKD pkd, size_t bytes)
+{
+    AosStage &A = pkd->aos;
+    if (!A.created) {
+        for (int k = 0; k < 2; ++k) GG_HIP(hipEventCreateWithFlags(&A.ev[k], hipEventDisableTiming));
+        A.created = 1;
+    }
+    if (A.bytes >= bytes) return GG_OK;
+    for (int k = 0; k < 2; ++k) {
+        if (A.buf[k]) (void)hipHostFree(A.buf[k]);
+        A.buf[k] = nullptr;
+    }
+    A.bytes = 0;
+    for (int k = 0; k < 2; ++k) {
+        hipError_t e = hipHostMalloc(&A.buf[k], bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            A.buf[k] = nullptr;
+            set_error("hipHostMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+            return GG_ENOMEM;
+        }
+    }
+    A.bytes = bytes;
+    return GG_OK;
+}
+
+static void aos_release(GGPKD pkd)
+{
+    AosStage &A = pkd->aos;
+    for (int k = 0; k < 2; ++k) {
+        if (A.buf[k]) (void)hipHostFree(A.buf[k]);
+        A.buf[k] = nullptr;
+    }
+    if (A.created)
+        for (int k = 0; k < 2; ++k) (void)hipEventDestroy(A.ev[k]);
+    A.created = 0;
+    A.bytes = 0;
+}
+
+static size_t aos_width(int field)
+{
+    return field == GG_IACTIVE ? sizeof(uint32_t) : sizeof(double) * field_width(field);
+}
+
+// Validates the member list; w[k] = bytes of member k, *row = their sum.
+static int aos_check(GGPKD pkd, const void *base, size_t stride, const gg_aos_field *f, int nf,
+                     std::vector<size_t> &w, size_t *row)
+{
+    GG_CHECK(pkd && f, GG_EINVAL, "null argument");
+    GG_CHECK(base || pkd->nLocal == 0, GG_EINVAL, "null particle array");
+    GG_CHECK(nf >= 0 && nf <= GG_NFIELDS, GG_EINVAL, "%d members", nf);
+    w.resize(nf);
+    *row = 0;
+    for (int k = 0; k < nf; ++k) {
+        GG_CHECK(f[k].field >= 0 && f[k].field < GG_NFIELDS, GG_EINVAL, "member %d: unknown field %d",
+                 k, f[k].field);
+        for (int j = 0; j < k; ++j)
+            GG_CHECK(f[j].field != f[k].field, GG_EINVAL, "field %d listed twice", f[k].field);
+        w[k] = aos_width(f[k].field);
+        GG_CHECK(f[k].offset + w[k] <= stride, GG_EINVAL,
+                 "member %d (field %d, offset %zu, %zu bytes) outside the %zu-byte particle", k,
+                 f[k].field, f[k].offset, w[k], stride);
+        *row += w[k];
+    }
+    return GG_OK;
+}
+
+// Rows [r0, r1) between the particle array and the staging buffer, which
+// holds one segment per member, seg[k] bytes in; split over host threads.
+static void aos_move(char *aos, size_t stride, const gg_aos_field *f, int nf, const size_t *w,
+                     const size_t *seg, int r0, int r1, char *stage, bool pack)
+{
+    const int rows = r1 - r0;
+    int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 8u);
+    nt = std::max(1, std::min(nt, rows / 16384));
+    auto work = [&](int a, int b) {
+        for (int r = a; r < b; ++r) {
+            char *p = aos + (size_t)r * stride;
+            const size_t i = (size_t)(r - r0);
+            for (int k = 0; k < nf; ++k) {
+                char *s = stage + seg[k] + i * w[k];
+                if (pack) std::memcpy(s, p + f[k].offset, w[k]);
+                else std::memcpy(p + f[k].offset, s, w[k]);
+            }
+        }
+    };
+    if (nt == 1) {
+        work(r0, r1);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back(work, r0 + (int)((int64_t)rows * t / nt), r0 + (int)((int64_t)rows * (t + 1) / nt));
+    for (auto &x : th) x.join();
+}
+
+extern "C" int gg_put_aos(GGPKD pkd, const void *base, size_t stride, const gg_aos_field *f, int nf)
+{
+    std::vector<size_t> w;
+    size_t row = 0;
+    GG_TRY(aos_check(pkd, base, stride, f, nf, w, &row));
+    const int n = pkd->nLocal;
+    if (n == 0 || nf == 0) return GG_OK;
+    for (int k = 0; k < nf; ++k) GG_TRY(ensure_field(pkd, f[k].field));
+    const int C = (int)std::max<size_t>(1, std::min<size_t>(AOS_CHUNK / row, (size_t)n));
+    GG_TRY(aos_stage(pkd, (size_t)C * row));
+    AosStage &A = pkd->aos;
+    std::vector<size_t> seg(nf);
+    for (int k = 0, o = 0; k < nf; o += (int)(w[k] * C), ++k) seg[k] = (size_t)o;
+    for (int c = 0, r0 = 0; r0 < n; ++c, r0 += C) {
+        const int b = c & 1, r1 = std::min(n, r0 + C);
+        if (A.pending[b]) GG_HIP(hipEventSynchronize(A.ev[b]));   // its copies have read it
+        A.pending[b] = 0;
+        char *stage = (char *)A.buf[b];
+        aos_move((char *)base, stride, f, nf, w.data(), seg.data(), r0, r1, stage, true);
+        for (int k = 0; k < nf; ++k)
+            GG_HIP(hipMemcpyAsync((char *)pkd->f[f[k].field].p + (size_t)r0 * w[k], stage + seg[k],
+                                  (size_t)(r1 - r0) * w[k], hipMemcpyHostToDevice, pkd->stream));
+        GG_HIP(hipEventRecord(A.ev[b], pkd->stream));
+        A.pending[b] = 1;
+    }
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    A.pending[0] = A.pending[1] = 0;
+    for (int k = 0; k < nf; ++k) put_done(pkd, f[k].field);
+    return GG_OK;
+}
+
+extern "C" int gg_get_aos(GGPKD pkd, void *base, size_t stride, const gg_aos_field *f, int nf)
+{
+    std::vector<size_t> w;
+    size_t row = 0;
+    GG_TRY(aos_check(pkd, base, stride, f, nf, w, &row));
+    const int n = pkd->nLocal;
+    if (n == 0 || nf == 0) return GG_OK;
+    for (int k = 0; k < nf; ++k) GG_TRY(ensure_field(pkd, f[k].field));
+    const int C = (int)std::max<size_t>(1, std::min<size_t>(AOS_CHUNK / row, (size_t)n));
+    GG_TRY(aos_stage(pkd, (size_t)C * row));
+    AosStage &A = pkd->aos;
+    std::vector<size_t> seg(nf);
+    for (int k = 0, o = 0; k < nf; o += (int)(w[k] * C), ++k) seg[k] = (size_t)o;
+    const int nc = (n + C - 1) / C;
+    auto issue = [&](int c) -> int {
+        const int b = c & 1, r0 = c * C, r1 = std::min(n, r0 + C);
+        char *stage = (char *)A.buf[b];
+        for (int k = 0; k < nf; ++k)
+            GG_HIP(hipMemcpyAsync(stage + seg[k], (char *)pkd->f[f[k].field].p + (size_t)r0 * w[k],
+                                  (size_t)(r1 - r0) * w[k], hipMemcpyDeviceToHost, pkd->stream));
+        GG_HIP(hipEventRecord(A.ev[b], pkd->stream));
+        return GG_OK;
+    };
+    GG_TRY(issue(0));
+    for (int c = 0; c < nc; ++c) {
+        // chunk c+1 lands in the other buffer, whose chunk c-1 is unpacked
+        if (c + 1 < nc) GG_TRY(issue(c + 1));
+        GG_HIP(hipEventSynchronize(A.ev[c & 1]));
+        const int r0 = c * C, r1 = std::min(n, r0 + C);
+        aos_move((char *)base, stride, f, nf, w.data(), seg.data(), r0, r1, (char *)A.buf[c & 1],
+                 false);
+    }
     GG_HIP(hipStreamSynchronize(pkd->stream));
     return GG_OK;
 }

@@ -157,6 +157,16 @@ struct HmlpDev {
     }
 };
 
+// Pinned host staging of gg_put_aos / gg_get_aos: two buffers, one being
+// packed or unpacked while the other's copies run.
+struct AosStage {
+    void *buf[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    hipEvent_t ev[2];
+    int created = 0;
+    int pending[2] = {0, 0};
+};
+
 struct Timers {
     hipEvent_t ev[16];
     int created = 0;
@@ -175,6 +185,7 @@ struct gg_pkd_s {
     gg::SphDev sph;
     gg::HmlpDev mlp;
     gg::Timers tm;
+    gg::AosStage aos;
 };
 
 namespace gg {

@@ -40,7 +40,7 @@ TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
 
 # exported entry points (include/gasoline_amd.h)
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
-       "gg_put", "gg_get", "gg_set_precision", "gg_build_binary", "gg_grav_all",
+       "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
@@ -53,6 +53,11 @@ HMLP_NW, HMLP_NP = 1249, 1257
 
 class GGError(RuntimeError):
     pass
+
+
+class AosField(C.Structure):
+    """gg_aos_field: one member of the caller's particle struct."""
+    _fields_ = [("field", C.c_int), ("pad", C.c_int), ("offset", C.c_size_t)]
 
 
 class Smf(C.Structure):
@@ -103,6 +108,8 @@ def lib():
     L.gg_local.argtypes = [_vp]
     L.gg_put.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_get.argtypes = [_vp, C.c_int, _vp, C.c_int]
+    L.gg_put_aos.argtypes = [_vp, _vp, C.c_size_t, C.POINTER(AosField), C.c_int]
+    L.gg_get_aos.argtypes = [_vp, _vp, C.c_size_t, C.POINTER(AosField), C.c_int]
     L.gg_set_precision.argtypes = [_vp, C.c_int]
     L.gg_build_binary.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
     L.gg_grav_all.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -198,6 +205,34 @@ class Pkd:
         a = np.empty(self.n * w, dtype=dt)
         _check(lib().gg_get(self.h, fid, a.ctypes.data_as(_vp), 0))
         return a.reshape(self.n, w) if w > 1 else a
+
+    @staticmethod
+    def _aos_members(arr, members):
+        """members: {field name: member name of arr's structured dtype}."""
+        if arr.dtype.fields is None or not arr.flags.c_contiguous:
+            raise ValueError("a C-contiguous structured array is required")
+        tab = (AosField * max(len(members), 1))()
+        for k, (name, mem) in enumerate(members.items()):
+            tab[k].field = FIELDS[name][0]
+            tab[k].offset = arr.dtype.fields[mem][1]
+        return tab
+
+    def put_aos(self, arr, members):
+        """gg_put_aos: fields in from a structured array of nLocal records
+        (gasoline's PARTICLE layout or any other)."""
+        if len(arr) != self.n:
+            raise ValueError(f"expected {self.n} records, got {len(arr)}")
+        tab = self._aos_members(arr, members)
+        _check(lib().gg_put_aos(self.h, arr.ctypes.data_as(_vp), arr.dtype.itemsize, tab,
+                                len(members)))
+
+    def get_aos(self, arr, members):
+        """gg_get_aos: the listed members of arr overwritten in place."""
+        if len(arr) != self.n:
+            raise ValueError(f"expected {self.n} records, got {len(arr)}")
+        tab = self._aos_members(arr, members)
+        _check(lib().gg_get_aos(self.h, arr.ctypes.data_as(_vp), arr.dtype.itemsize, tab,
+                                len(members)))
 
     # --------------------------------------------------------- gravity
     def build_binary(self, nBucket=16, iOpenType=1, dCrit=0.7, iOrder=2):

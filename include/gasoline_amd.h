@@ -26,6 +26,7 @@
 #ifndef GASOLINE_AMD_H
 #define GASOLINE_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -109,6 +110,26 @@ int gg_local(GGPKD pkd);
  * (pkdReadTipsy pkd.c:297, pkdOutVector outtype.c:806). */
 int gg_put(GGPKD pkd, int field, const void *src, int bDevice);
 int gg_get(GGPKD pkd, int field, void *dst, int bDevice);
+
+/* One member of the caller's particle struct: the field it maps to and its
+ * byte offset (offsetof(PARTICLE, r) ...).  FP64 members (FLOAT is double,
+ * floattype.h:19), iActive as unsigned int. */
+typedef struct gg_aos_field {
+    int field;          /* GG_R ... GG_IACTIVE */
+    int pad;
+    size_t offset;      /* bytes from the start of a particle */
+} gg_aos_field;
+
+/* Copy fields in from (gg_put_aos) or out to (gg_get_aos) an array of
+ * structs of nLocal particles, `stride` bytes apart (sizeof(PARTICLE)):
+ * gasoline's pStore, whose PARTICLE layout depends on compile flags
+ * (pkd.h:90-225), used as it is.  One pass over the particles per chunk,
+ * staged through pinned host memory, the copies of one chunk overlapping
+ * the packing of the next.  gg_get_aos writes only the listed members.
+ * The pack/unpack loops a caller of pkdGravAll/smSmooth would otherwise
+ * write around each call. */
+int gg_put_aos(GGPKD pkd, const void *base, size_t stride, const gg_aos_field *f, int nf);
+int gg_get_aos(GGPKD pkd, void *base, size_t stride, const gg_aos_field *f, int nf);
 
 /* Select FP32 or FP64 interaction arithmetic (default GG_PREC_FP32). */
 int gg_set_precision(GGPKD pkd, int prec);
