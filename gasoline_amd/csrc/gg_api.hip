@@ -58,9 +58,11 @@ int field_width(int field)
     }
 }
 
+int field_is32(int field) { return field == GG_IACTIVE || field == GG_IRUNG; }
+
 static size_t field_bytes(int field, int n)
 {
-    if (field == GG_IACTIVE) return sizeof(uint32_t) * (size_t)n;
+    if (field_is32(field)) return sizeof(uint32_t) * (size_t)n;
     return sizeof(double) * (size_t)field_width(field) * (size_t)n;
 }
 
@@ -77,7 +79,8 @@ __global__ void k_fill_f64(size_t n, double v, double *__restrict__ p)
 }
 
 // Allocate a field on first use.  Defaults: iActive = ACTIVE | TREEACTIVE |
-// SMOOTHACTIVE, dt and dtNew = FLT_MAX (floattype.h:9), everything else 0.
+// SMOOTHACTIVE, dt and dtNew = FLT_MAX (floattype.h:9), iRung and everything
+// else 0.
 int ensure_field(GGPKD pkd, int field)
 {
     DevBuf &b = pkd->f[field];
@@ -85,12 +88,15 @@ int ensure_field(GGPKD pkd, int field)
     const int n = pkd->nStore > 0 ? pkd->nStore : 1;
     const size_t nb = field_bytes(field, n);
     GG_TRY(b.reserve(nb));
-    const size_t cnt = field == GG_IACTIVE ? (size_t)n : (size_t)field_width(field) * n;
+    const size_t cnt = field_is32(field) ? (size_t)n : (size_t)field_width(field) * n;
     const unsigned grid = (unsigned)((cnt + 255) / 256);
     if (field == GG_IACTIVE) {
         pkd->sph.gasMaskValid = 0;
         hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, pkd->stream, cnt,
                            GG_TYPE_ACTIVE | GG_TYPE_TREEACTIVE | GG_TYPE_SMOOTHACTIVE,
+                           b.as<uint32_t>());
+    } else if (field == GG_IRUNG) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, pkd->stream, cnt, 0u,
                            b.as<uint32_t>());
     } else {
         const double v = (field == GG_DT || field == GG_DTNEW) ? (double)3.402823466E+38F : 0.0;
@@ -168,6 +174,7 @@ extern "C" int gg_finish(GGPKD pkd)
     (void)hipSetDevice(pkd->device);
     (void)hipStreamSynchronize(pkd->stream);
     for (auto &b : pkd->f) b.release();
+    pkd->red.release();
     GravDev &G = pkd->grav;
     G.release_all();
     if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
@@ -287,7 +294,7 @@ static void aos_release(GGPKD pkd)
 
 static size_t aos_width(int field)
 {
-    return field == GG_IACTIVE ? sizeof(uint32_t) : sizeof(double) * field_width(field);
+    return field_is32(field) ? sizeof(uint32_t) : sizeof(double) * field_width(field);
 }
 
 // Validates the member list; w[k] = bytes of member k, *row = their sum.

@@ -186,10 +186,12 @@ struct gg_pkd_s {
     gg::HmlpDev mlp;
     gg::Timers tm;
     gg::AosStage aos;
+    gg::DevBuf red;                // reductions of the elementwise updates
 };
 
 namespace gg {
-int field_width(int field);                 // doubles per particle (0: uint32)
+int field_width(int field);                 // values per particle
+int field_is32(int field);                  // 32-bit integer field (iActive, iRung)
 int ensure_field(GGPKD pkd, int field);     // allocate (zeroed) if missing
 // out[0..n]: exclusive prefix sums, out[n] = total (out holds n+1 entries);
 // the total is copied to *total_host (after a stream synchronisation) unless

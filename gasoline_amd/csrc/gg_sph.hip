@@ -2800,7 +2800,7 @@ extern "C" int gg_reorder(GGPKD pkd, int *perm)
     int rc = GG_OK;
     for (int f = 0; f < GG_NFIELDS && rc == GG_OK; ++f) {
         if (!pkd->f[f].p) continue;
-        rc = f == GG_IACTIVE ? permute_buf<uint32_t>(pkd, pkd->f[f], 1, 1, pp)
+        rc = field_is32(f) ? permute_buf<uint32_t>(pkd, pkd->f[f], 1, 1, pp)
                              : permute_buf<double>(pkd, pkd->f[f], field_width(f), 1, pp);
     }
     HmlpDev &M = pkd->mlp;

@@ -1,8 +1,18 @@
 // The updates on either side of the force path, in FP64 on the device:
 // pkdInitAccel (pkd.c:5443-5456), pkdKick (pkd.c:3780-3967, the NOCOOLING
-// gas branch) and pkdDrift (pkd.c:3686-3778, without the sliding-patch and
-// inflow/outflow options).  Elementwise over the store, caller order.
+// gas branch), pkdKickVpred (pkd.c:6489-6552), pkdDrift (pkd.c:3686-3778,
+// without the sliding-patch and inflow/outflow options) and the timestep
+// criteria and rung assignment (pkd.c:4627-4826, 5893-5979).  Elementwise
+// over the store, caller order.  No contraction into FMA: every product and
+// sum rounds as in the reference's x86-64 build, so the results are the
+// reference's bit for bit.
 #include "gg_internal.h"
+
+#include <cfloat>
+#include <cstring>
+#include <initializer_list>
+
+#pragma clang fp contract(off)
 
 namespace gg {
 namespace {
@@ -70,10 +80,220 @@ __global__ void k_drift(int n, double dt, int per, double c0, double c1, double 
     }
 }
 
+// pkdKickVpred, NOCOOLING: gas particles only; bad counts uPred < 0
+// (the reference asserts uPred >= 0).
+__global__ void k_kick_vpred(int n, double vf1, double vf2, double du, int bGasU,
+                             const uint32_t *__restrict__ bits, const double *__restrict__ a,
+                             const double *__restrict__ udpdv, const double *__restrict__ udav,
+                             const double *__restrict__ uddif, double *__restrict__ vpred,
+                             double *__restrict__ upred, unsigned *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(bits[i] & GG_TYPE_GAS)) return;
+    for (int j = 0; j < 3; ++j) {
+        const size_t e = 3 * (size_t)i + j;
+        vpred[e] = vpred[e] * vf1 + a[e] * vf2;
+    }
+    if (bGasU) {
+        const double u = upred[i] + (udpdv[i] + udav[i] + uddif[i]) * du;
+        upred[i] = u;
+        if (!(u >= 0.0)) atomicAdd(bad, 1u);
+    }
+}
+
+__global__ void k_init_dt(int n, double dDelta, const uint32_t *__restrict__ bits,
+                          double *__restrict__ dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (bits[i] & GG_TYPE_ACTIVE)) dt[i] = dDelta;
+}
+
+// pkdGravStep: dt = min(dt, dEta/sqrt(dtGrav)); bad counts dtGrav <= 0.
+__global__ void k_grav_step(int n, double dEta, const uint32_t *__restrict__ bits,
+                            const double *__restrict__ dtg, double *__restrict__ dt,
+                            unsigned *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(bits[i] & GG_TYPE_ACTIVE)) return;
+    if (!(dtg[i] > 0.0)) {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    const double d = dEta / sqrt(dtg[i]);
+    if (d < dt[i]) dt[i] = d;
+}
+
+// pkdAccelStep: the acceleration criterion (softening, or h for gas) and
+// the sqrt(phi) criterion.
+__global__ void k_accel_step(int n, double dEta, double dAccFac, int bEpsAcc, int bSqrtPhi,
+                             const uint32_t *__restrict__ bits, const double *__restrict__ a,
+                             const double *__restrict__ soft, const double *__restrict__ fb2,
+                             const double *__restrict__ pot, double *__restrict__ dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = bits[i];
+    if (!(b & GG_TYPE_ACTIVE)) return;
+    double acc = 0;
+    for (int j = 0; j < 3; ++j) acc += a[3 * (size_t)i + j] * a[3 * (size_t)i + j];
+    acc = sqrt(acc) * dAccFac;
+    double dT = DBL_MAX;
+    if (bEpsAcc && acc > 0) {
+        if (b & GG_TYPE_GAS) dT = dEta * sqrt(sqrt(0.25 * fb2[i]) / acc);
+        else dT = dEta * sqrt(soft[i] / acc);
+    }
+    if (bSqrtPhi && acc > 0) {
+        const double dtemp = dEta * 3.5 * sqrt(dAccFac * fabs(pot[i])) / acc;
+        if (dtemp < dT) dT = dtemp;
+    }
+    if (dT < dt[i]) dt[i] = dT;
+}
+
+__global__ void k_density_step(int n, double dEta, double dRhoFac,
+                               const uint32_t *__restrict__ bits, const double *__restrict__ rho,
+                               double *__restrict__ dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(bits[i] & GG_TYPE_ACTIVE)) return;
+    const double dT = dEta / sqrt(rho[i] * dRhoFac);
+    if (dT < dt[i]) dt[i] = dT;
+}
+
+// Order-preserving map of a double onto uint64 (for atomic min/max).
+__device__ __forceinline__ unsigned long long dkey(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+struct SphStepArgs {
+    int n;
+    double dtFacCourant, dEtauDot, duMinDt, dDiffCoeff, dEtaDiffusion, dResolveJeans;
+    const uint32_t *bits;
+    const double *fb2, *c, *u, *udpdv, *udav, *uddif, *rho, *diff;
+    double *dtNew, *dt;
+    unsigned long long *minKey;      // smallest dt of the gas, as dkey
+};
+
+// pkdSphStep, NOCOOLING without THERMALCOND/TWOPHASE/UNONCOOL.
+__global__ void k_sph_step(SphStepArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long key = ~0ull;
+    if (i < A.n && (A.bits[i] & GG_TYPE_GAS)) {
+        if (A.bits[i] & GG_TYPE_ACTIVE) {
+            const double ph = sqrt(0.25 * A.fb2[i]);
+            double dT = A.dtNew[i];
+            A.dtNew[i] = (double)FLT_MAX;
+            const double uTotDot = A.udpdv[i] + A.udav[i] + A.uddif[i];
+            if (uTotDot > 0) {
+                // GAMMA_NONCOOL = 5/3 (pkd.h:228)
+                const double dtExtrap =
+                    A.dtFacCourant *
+                    sqrt(A.fb2[i] * 0.25 / (4 * (A.c[i] * A.c[i] + (5. / 3.) * uTotDot * A.dt[i])));
+                if (dtExtrap < dT) dT = dtExtrap;
+            }
+            if (A.dEtauDot > 0.0 && A.udpdv[i] < 0.0) {
+                // Jeans floor (pkd.c:5668-5688) over GAMMA_JEANS - 1 = 1, PONRHOFLOOR 0
+                const double floorJ = 0.25 * A.fb2[i] * A.dResolveJeans * A.rho[i];
+                double uEff = 0 + floorJ / (2.0 - 1) + A.u[i];
+                if (uEff < A.duMinDt) uEff = A.duMinDt;
+                const double dTu = A.dEtauDot * uEff / fabs(A.udpdv[i]);
+                if (dTu < dT) dT = dTu;
+            }
+            if (A.diff[i] > 0 && A.dDiffCoeff > 0) {
+                const double dTD = A.dEtaDiffusion * ph * ph / (A.dDiffCoeff * A.diff[i]);
+                if (dTD < dT) dT = dTD;
+            }
+            if (dT < A.dt[i]) A.dt[i] = dT;
+        }
+        key = dkey(A.dt[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long k2 = __shfl_xor(key, o);
+        key = k2 < key ? k2 : key;
+    }
+    if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(A.minKey, key);
+}
+
+// pkdOneParticleDtToRung (pkd.c:4704-4726)
+__device__ __forceinline__ int one_dt_to_rung(int iRung, double dDelta, double dt)
+{
+    const double dSteps = dDelta / dt;
+    int iSteps = (int)floor(dSteps);
+    if (fmod(dDelta, dt) == 0.0) iSteps--;   // an integer boundary goes to the lower rung
+    if (iSteps < 0) iSteps = 0;
+    int r = iRung;
+    while (iSteps) {
+        ++r;
+        iSteps >>= 1;
+    }
+    return r;
+}
+
+// pkdDtToRung (pkd.c:4729-4809) without the diagnostic print.  red[0]
+// counts broken assertions, red[1] = largest rung, red[2] = ideal largest
+// rung (k_rung_count adds red[3] = particles on red[1]).
+__global__ void k_dt_to_rung(int n, int iRung, double dDelta, int iMaxRung, int bAll,
+                             const uint32_t *__restrict__ bits, const double *__restrict__ soft,
+                             const double *__restrict__ dt, int *__restrict__ rung,
+                             double *__restrict__ dtOld, int *__restrict__ red)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int top = -1, ideal = 0;
+    if (i < n) {
+        int r = rung[i];
+        if (r >= iRung) {
+            if (!(bits[i] & GG_TYPE_ACTIVE)) {
+                atomicAdd(&red[0], 1);
+            } else if (bAll) {
+                const double d = dt[i];
+                if (!(soft[i] > 0.0) || !(d > 0.0) || !(dDelta / d < 2.1e9)) {
+                    atomicAdd(&red[0], 1);
+                } else {
+                    int t = one_dt_to_rung(iRung, dDelta, d);
+                    ideal = t + 1;
+                    if (t >= iMaxRung) t = iMaxRung - 1;
+                    r = t;
+                    dtOld[i] = d;
+                }
+            } else {
+                r = dDelta <= dt[i] ? iRung : iRung + 1;
+            }
+            rung[i] = r;
+        }
+        top = r;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        top = max(top, __shfl_xor(top, o));
+        ideal = max(ideal, __shfl_xor(ideal, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (top >= 0) atomicMax(&red[1], top);
+        if (ideal > 0) atomicMax(&red[2], ideal);
+    }
+}
+
+__global__ void k_rung_count(int n, const int *__restrict__ rung, int *__restrict__ red)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = i < n && rung[i] == red[1];
+    const int c = __popcll(__ballot(on));
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&red[3], c);
+}
+
 }  // namespace
 }  // namespace gg
 
 using namespace gg;
+
+namespace {
+int need_fields(GGPKD pkd, std::initializer_list<int> f)
+{
+    for (int k : f) GG_TRY(ensure_field(pkd, k));
+    return GG_OK;
+}
+}  // namespace
 
 extern "C" int gg_init_accel(GGPKD pkd)
 {
@@ -159,5 +379,172 @@ extern "C" int gg_drift(GGPKD pkd, double dDelta, const double *fCenter, int bPe
     pkd->grav.built = 0;
     // pkdDrift asserts that every particle ends inside the box (pkd.c:3760)
     GG_CHECK(nb == 0, GG_EINVAL, "%u coordinates left the periodic box in one drift", nb);
+    return GG_OK;
+}
+
+extern "C" int gg_kick_vpred(GGPKD pkd, double dvFacOne, double dvFacTwo, double duDelta,
+                             int bIsothermal)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_VPRED, GG_A, GG_UPRED, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF,
+                             GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(sizeof(unsigned)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, sizeof(unsigned), pkd->stream));
+    hipLaunchKernelGGL(k_kick_vpred, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, dvFacOne,
+                       dvFacTwo, duDelta, !bIsothermal, pkd->f[GG_IACTIVE].as<uint32_t>(),
+                       pkd->f[GG_A].as<double>(), pkd->f[GG_UDOTPDV].as<double>(),
+                       pkd->f[GG_UDOTAV].as<double>(), pkd->f[GG_UDOTDIFF].as<double>(),
+                       pkd->f[GG_VPRED].as<double>(), pkd->f[GG_UPRED].as<double>(),
+                       pkd->red.as<unsigned>());
+    GG_HIP(hipGetLastError());
+    unsigned bad = 0;
+    GG_HIP(hipMemcpyAsync(&bad, pkd->red.p, sizeof(unsigned), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    GG_CHECK(bad == 0, GG_EINVAL, "gg_kick_vpred: %u gas particles with uPred < 0", bad);
+    return GG_OK;
+}
+
+extern "C" int gg_init_dt(GGPKD pkd, double dDelta)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_DT, GG_IACTIVE}));
+    hipLaunchKernelGGL(k_init_dt, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, dDelta,
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), pkd->f[GG_DT].as<double>());
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_grav_step(GGPKD pkd, double dEta)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_DT, GG_DTGRAV, GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(sizeof(unsigned)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, sizeof(unsigned), pkd->stream));
+    hipLaunchKernelGGL(k_grav_step, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, dEta,
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), pkd->f[GG_DTGRAV].as<double>(),
+                       pkd->f[GG_DT].as<double>(), pkd->red.as<unsigned>());
+    GG_HIP(hipGetLastError());
+    unsigned bad = 0;
+    GG_HIP(hipMemcpyAsync(&bad, pkd->red.p, sizeof(unsigned), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    GG_CHECK(bad == 0, GG_EINVAL, "gg_grav_step: %u active particles with dtGrav <= 0", bad);
+    return GG_OK;
+}
+
+extern "C" int gg_accel_step(GGPKD pkd, double dEta, double dVelFac, double dAccFac,
+                             int bDoGravity, int bEpsAcc, int bSqrtPhi, double dhMinOverSoft)
+{
+    (void)dVelFac;        // the velocity enters no criterion (pkd.c:4657-4664)
+    (void)bDoGravity;
+    (void)dhMinOverSoft;
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_DT, GG_A, GG_SOFT, GG_FBALL2, GG_POT, GG_IACTIVE}));
+    hipLaunchKernelGGL(k_accel_step, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, dEta,
+                       dAccFac, bEpsAcc, bSqrtPhi, pkd->f[GG_IACTIVE].as<uint32_t>(),
+                       pkd->f[GG_A].as<double>(), pkd->f[GG_SOFT].as<double>(),
+                       pkd->f[GG_FBALL2].as<double>(), pkd->f[GG_POT].as<double>(),
+                       pkd->f[GG_DT].as<double>());
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_density_step(GGPKD pkd, double dEta, double dRhoFac)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_DT, GG_DENSITY, GG_IACTIVE}));
+    hipLaunchKernelGGL(k_density_step, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, dEta,
+                       dRhoFac, pkd->f[GG_IACTIVE].as<uint32_t>(),
+                       pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_DT].as<double>());
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_sph_step(GGPKD pkd, double dCosmoFac, double dEtaCourant, double dEtauDot,
+                           double duMinDt, double dDiffCoeff, double dEtaDiffusion,
+                           double dResolveJeans, int bViscosityLimitdt, double *pdtMinGas)
+{
+    (void)bViscosityLimitdt;   // read by no branch of the NOCOOLING build (pkd.c:5901-5979)
+    GG_CHECK(pkd && pdtMinGas, GG_EINVAL, "null argument");
+    const int n = pkd->nLocal;
+    *pdtMinGas = DBL_MAX;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_DT, GG_DTNEW, GG_FBALL2, GG_C, GG_U, GG_UDOTPDV, GG_UDOTAV,
+                             GG_UDOTDIFF, GG_DENSITY, GG_DIFF, GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(sizeof(unsigned long long)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0xff, sizeof(unsigned long long), pkd->stream));
+    SphStepArgs A;
+    A.n = n;
+    A.dtFacCourant = dEtaCourant * dCosmoFac * 2 / 1.6;   // pkdDtFacCourant, pkd.c:5893-5895
+    A.dEtauDot = dEtauDot;
+    A.duMinDt = duMinDt;
+    A.dDiffCoeff = dDiffCoeff;
+    A.dEtaDiffusion = dEtaDiffusion;
+    A.dResolveJeans = dResolveJeans;
+    A.bits = pkd->f[GG_IACTIVE].as<uint32_t>();
+    A.fb2 = pkd->f[GG_FBALL2].as<double>();
+    A.c = pkd->f[GG_C].as<double>();
+    A.u = pkd->f[GG_U].as<double>();
+    A.udpdv = pkd->f[GG_UDOTPDV].as<double>();
+    A.udav = pkd->f[GG_UDOTAV].as<double>();
+    A.uddif = pkd->f[GG_UDOTDIFF].as<double>();
+    A.rho = pkd->f[GG_DENSITY].as<double>();
+    A.diff = pkd->f[GG_DIFF].as<double>();
+    A.dtNew = pkd->f[GG_DTNEW].as<double>();
+    A.dt = pkd->f[GG_DT].as<double>();
+    A.minKey = pkd->red.as<unsigned long long>();
+    hipLaunchKernelGGL(k_sph_step, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, A);
+    GG_HIP(hipGetLastError());
+    unsigned long long key = ~0ull;
+    GG_HIP(hipMemcpyAsync(&key, pkd->red.p, sizeof(key), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    if (key != ~0ull) {   // undo the order-preserving map
+        const unsigned long long b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+        std::memcpy(pdtMinGas, &b, sizeof(double));
+    }
+    return GG_OK;
+}
+
+extern "C" int gg_dt_to_rung(GGPKD pkd, int iRung, double dDelta, int iMaxRung, int bAll,
+                             int bDiagExceed, int *pnMaxRung, int *piMaxRungIdeal,
+                             int *piMaxRungOut)
+{
+    (void)bDiagExceed;   // selects diagnostic prints only (pkd.c:4748-4789)
+    GG_CHECK(pkd && pnMaxRung && piMaxRungIdeal && piMaxRungOut, GG_EINVAL, "null argument");
+    const int n = pkd->nLocal;
+    *pnMaxRung = *piMaxRungIdeal = *piMaxRungOut = 0;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_IRUNG, GG_DT, GG_DTOLD, GG_SOFT, GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(4 * sizeof(int)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, 4 * sizeof(int), pkd->stream));
+    hipLaunchKernelGGL(k_dt_to_rung, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, iRung,
+                       dDelta, iMaxRung, bAll, pkd->f[GG_IACTIVE].as<uint32_t>(),
+                       pkd->f[GG_SOFT].as<double>(), pkd->f[GG_DT].as<double>(),
+                       pkd->f[GG_IRUNG].as<int>(), pkd->f[GG_DTOLD].as<double>(),
+                       pkd->red.as<int>());
+    hipLaunchKernelGGL(k_rung_count, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
+                       pkd->f[GG_IRUNG].as<int>(), pkd->red.as<int>());
+    GG_HIP(hipGetLastError());
+    int red[4];
+    GG_HIP(hipMemcpyAsync(red, pkd->red.p, sizeof(red), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    GG_CHECK(red[0] == 0, GG_EINVAL,
+             "gg_dt_to_rung: %d particles on rung >= %d are inactive or have fSoft <= 0, dt <= 0 "
+             "or dDelta/dt >= 2.1e9 (rungs are undefined)", red[0], iRung);
+    *piMaxRungOut = red[1];
+    *pnMaxRung = red[3];
+    *piMaxRungIdeal = red[2];
     return GG_OK;
 }

@@ -19,7 +19,7 @@ LIBPATH = os.path.join(_HERE, "libgasoline_amd.so")
 GG_R, GG_V, GG_VPRED, GG_A, GG_MASS, GG_SOFT, GG_POT, GG_DTGRAV, GG_U, GG_UPRED, \
     GG_FBALL2, GG_DENSITY, GG_DIVV, GG_CURLV, GG_DVDS, GG_FDIVV_T, GG_BALSARA, GG_DIFF, \
     GG_POVERRHO2, GG_C, GG_DT, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW, \
-    GG_IACTIVE = range(27)
+    GG_IACTIVE, GG_IRUNG, GG_DTOLD = range(29)
 
 FIELDS = {
     "r": (GG_R, 3), "v": (GG_V, 3), "vPred": (GG_VPRED, 3), "a": (GG_A, 3),
@@ -31,7 +31,9 @@ FIELDS = {
     "PoverRho2": (GG_POVERRHO2, 1), "c": (GG_C, 1), "dt": (GG_DT, 1),
     "uDotPdV": (GG_UDOTPDV, 1), "uDotAV": (GG_UDOTAV, 1), "uDotDiff": (GG_UDOTDIFF, 1),
     "mumax": (GG_MUMAX, 1), "dtNew": (GG_DTNEW, 1), "iActive": (GG_IACTIVE, 1),
+    "iRung": (GG_IRUNG, 1), "dtOld": (GG_DTOLD, 1),
 }
+_INT_FIELDS = {"iActive": np.uint32, "iRung": np.int32}
 
 PREC_FP64, PREC_FP32 = 0, 1
 SMX_DENSITY, SMX_SPHPRESSURETERMS, SMX_DENDVDX = 1, 21, 22
@@ -45,6 +47,8 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
+       "gg_kick_vpred", "gg_init_dt", "gg_grav_step", "gg_accel_step", "gg_density_step",
+       "gg_sph_step", "gg_dt_to_rung",
        "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
        "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
 
@@ -135,6 +139,15 @@ def lib():
     L.gg_init_accel.argtypes = [_vp]
     L.gg_kick.argtypes = [_vp] + [C.c_double] * 6 + [C.c_int]
     L.gg_drift.argtypes = [_vp, C.c_double, _dp, C.c_int, _dp]
+    L.gg_kick_vpred.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_int]
+    L.gg_init_dt.argtypes = [_vp, C.c_double]
+    L.gg_grav_step.argtypes = [_vp, C.c_double]
+    L.gg_accel_step.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.c_int, C.c_double]
+    L.gg_density_step.argtypes = [_vp, C.c_double, C.c_double]
+    L.gg_sph_step.argtypes = [_vp] + [C.c_double] * 7 + [C.c_int, _dp]
+    L.gg_dt_to_rung.argtypes = [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _ip, _ip,
+                                _ip]
     _fp = C.POINTER(C.c_float)
     L.gg_hmlp_init.argtypes = [_vp, C.c_uint]
     L.gg_hmlp_record.argtypes = [_vp, C.c_double]
@@ -193,7 +206,7 @@ class Pkd:
         if hasattr(arr, "data_ptr"):             # device tensor
             _check(lib().gg_put(self.h, fid, _vp(arr.data_ptr()), 1))
             return
-        dt = np.uint32 if name == "iActive" else np.float64
+        dt = _INT_FIELDS.get(name, np.float64)
         a = np.ascontiguousarray(arr, dtype=dt).reshape(-1)
         if a.size != self.n * w:
             raise ValueError(f"{name}: expected {self.n * w} values, got {a.size}")
@@ -201,7 +214,7 @@ class Pkd:
 
     def get(self, name):
         fid, w = FIELDS[name]
-        dt = np.uint32 if name == "iActive" else np.float64
+        dt = _INT_FIELDS.get(name, np.float64)
         a = np.empty(self.n * w, dtype=dt)
         _check(lib().gg_get(self.h, fid, a.ctypes.data_as(_vp), 0))
         return a.reshape(self.n, w) if w > 1 else a
@@ -373,6 +386,41 @@ class Pkd:
             c = np.ascontiguousarray(center, np.float64)
             L = np.ascontiguousarray(period, np.float64)
             _check(lib().gg_drift(self.h, dDelta, _p(c), 1, _p(L)))
+
+    def kick_vpred(self, dvFacOne, dvFacTwo, duDelta, bIsothermal=False):
+        _check(lib().gg_kick_vpred(self.h, dvFacOne, dvFacTwo, duDelta, int(bIsothermal)))
+
+    # --- timesteps and rungs (pkdInitDt, pkdGravStep, pkdAccelStep,
+    # pkdDensityStep, pkdSphStep, pkdDtToRung)
+    def init_dt(self, dDelta):
+        _check(lib().gg_init_dt(self.h, dDelta))
+
+    def grav_step(self, dEta):
+        _check(lib().gg_grav_step(self.h, dEta))
+
+    def accel_step(self, dEta, dVelFac=1.0, dAccFac=1.0, bDoGravity=True, bEpsAcc=True,
+                   bSqrtPhi=False, dhMinOverSoft=0.0):
+        _check(lib().gg_accel_step(self.h, dEta, dVelFac, dAccFac, int(bDoGravity), int(bEpsAcc),
+                                   int(bSqrtPhi), dhMinOverSoft))
+
+    def density_step(self, dEta, dRhoFac):
+        _check(lib().gg_density_step(self.h, dEta, dRhoFac))
+
+    def sph_step(self, dCosmoFac, dEtaCourant, dEtauDot=0.0, duMinDt=0.0, dDiffCoeff=0.0,
+                 dEtaDiffusion=0.1, dResolveJeans=0.0, bViscosityLimitdt=False):
+        """Returns dtMinGas."""
+        m = C.c_double()
+        _check(lib().gg_sph_step(self.h, dCosmoFac, dEtaCourant, dEtauDot, duMinDt, dDiffCoeff,
+                                 dEtaDiffusion, dResolveJeans, int(bViscosityLimitdt),
+                                 C.byref(m)))
+        return m.value
+
+    def dt_to_rung(self, iRung, dDelta, iMaxRung, bAll=True, bDiagExceed=False):
+        """Returns (iMaxRungOut, nMaxRung, iMaxRungIdeal)."""
+        nm, ideal, top = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().gg_dt_to_rung(self.h, int(iRung), dDelta, int(iMaxRung), int(bAll),
+                                   int(bDiagExceed), C.byref(nm), C.byref(ideal), C.byref(top)))
+        return top.value, nm.value, ideal.value
 
     # --- h-proposal MLP
     def hmlp_init(self, seed=1):

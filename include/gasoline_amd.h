@@ -73,6 +73,8 @@ enum gg_field {
     GG_MUMAX,        /* mumax */
     GG_DTNEW,        /* dtNew */
     GG_IACTIVE,      /* iActive type bits (uint32 per particle, pkd.h:288-329) */
+    GG_IRUNG,        /* iRung (int32 per particle) */
+    GG_DTOLD,        /* dtOld */
     GG_NFIELDS
 };
 
@@ -113,7 +115,7 @@ int gg_get(GGPKD pkd, int field, void *dst, int bDevice);
 
 /* One member of the caller's particle struct: the field it maps to and its
  * byte offset (offsetof(PARTICLE, r) ...).  FP64 members (FLOAT is double,
- * floattype.h:19), iActive as unsigned int. */
+ * floattype.h:19), iActive and iRung as 32-bit ints. */
 typedef struct gg_aos_field {
     int field;          /* GG_R ... GG_IACTIVE */
     int pad;
@@ -321,6 +323,47 @@ int gg_kick(GGPKD pkd, double dvFacOne, double dvFacTwo, double dvPredFacOne,
  * afterwards: build again before the next force call. */
 int gg_drift(GGPKD pkd, double dDelta, const double *fCenter, int bPeriodic,
              const double *fPeriod);
+/* pkdKickVpred (pkd.c:6489, NOCOOLING build): for TYPE_GAS particles
+ * vPred = vPred dvFacOne + a dvFacTwo and, unless bIsothermal,
+ * uPred += uDot duDelta; GG_EINVAL if a uPred turns negative (the
+ * reference asserts). */
+int gg_kick_vpred(GGPKD pkd, double dvFacOne, double dvFacTwo, double duDelta, int bIsothermal);
+
+/* ------------------------------------------- timesteps and rungs */
+/* These update dt (and iRung, dtOld) of the store like their reference
+ * functions, bit for bit: no contraction into FMA. */
+
+/* pkdInitDt (pkd.c:4817): dt = dDelta for TYPE_ACTIVE particles. */
+int gg_init_dt(GGPKD pkd, double dDelta);
+/* pkdGravStep (pkd.c:4627): dt = min(dt, dEta/sqrt(dtGrav)) for active
+ * particles; GG_EINVAL if one has dtGrav <= 0 (the reference asserts). */
+int gg_grav_step(GGPKD pkd, double dEta);
+/* pkdAccelStep (pkd.c:4643): the acceleration criterion (bEpsAcc:
+ * dEta sqrt(fSoft/|a|), or dEta sqrt(h/|a|) for gas with h = sqrt(fBall2)/2)
+ * and the potential criterion (bSqrtPhi: 3.5 dEta sqrt(dAccFac |fPot|)/|a|),
+ * |a| scaled by dAccFac.  dVelFac, bDoGravity and dhMinOverSoft enter no
+ * criterion, as in the reference. */
+int gg_accel_step(GGPKD pkd, double dEta, double dVelFac, double dAccFac, int bDoGravity,
+                  int bEpsAcc, int bSqrtPhi, double dhMinOverSoft);
+/* pkdDensityStep (pkd.c:4689): dt = min(dt, dEta/sqrt(fDensity dRhoFac)). */
+int gg_density_step(GGPKD pkd, double dEta, double dRhoFac);
+/* pkdSphStep (pkd.c:5901, NOCOOLING): for active gas the Courant estimate
+ * of the last force pass (dtNew, reset to FLT_MAX), the extrapolated
+ * Courant step of heating gas, the adiabatic-cooling limit (dEtauDot) and
+ * the diffusion limit; *pdtMinGas = the smallest dt of all gas particles
+ * (DBL_MAX if none). */
+int gg_sph_step(GGPKD pkd, double dCosmoFac, double dEtaCourant, double dEtauDot,
+                double duMinDt, double dDiffCoeff, double dEtaDiffusion, double dResolveJeans,
+                int bViscosityLimitdt, double *pdtMinGas);
+/* pkdDtToRung (pkd.c:4729): particles on rung >= iRung take the rung of
+ * their dt (bAll, capped at iMaxRung-1, dtOld = dt) or iRung / iRung+1;
+ * *piMaxRungOut = the largest rung (the reference's return value) with
+ * *pnMaxRung particles on it, *piMaxRungIdeal the rung the smallest dt
+ * wants.  GG_EINVAL where the reference asserts (an inactive particle on an
+ * updated rung, fSoft <= 0, dt <= 0, dDelta/dt >= 2.1e9); the rungs are
+ * then undefined.  bDiagExceed selects diagnostic prints only. */
+int gg_dt_to_rung(GGPKD pkd, int iRung, double dDelta, int iMaxRung, int bAll, int bDiagExceed,
+                  int *pnMaxRung, int *piMaxRungIdeal, int *piMaxRungOut);
 
 /* ------------------------------------------------- h-proposal MLP */
 

@@ -34,6 +34,14 @@
  *                          counted once per end), gather equality on a lattice
  *                          whose lists are symmetric, unsmoothed particles keep
  *                          their value plus what their neighbours scatter.
+ *   orc_kick, orc_drift    leapfrog of a uniform acceleration is exact
+ *                          (x0 + v0 t + a t^2/2, v0 + a t); a drift by one
+ *                          period returns every coordinate to itself.
+ *   orc_*_step             closed forms at special values (dtGrav = 4,
+ *                          rho = 1/dRhoFac ...), each limit alone.
+ *   orc_dt_to_rung         dt = dDelta/2^k lands on rung k (an integer
+ *                          boundary goes to the lower rung), a hair below
+ *                          on rung k+1, floor(log2) on random dt.
  *   orc_sph_pressure_sym   momentum conservation, energy conservation, uniform
  *                          pressure => zero force, AV only for approaching pairs.
  */
@@ -171,6 +179,35 @@ void orc_sph_pressure_sym(int n, const double *m, const double *vpred,
                           const orc_smf *smf, double *acc, double *uDotPdV,
                           double *uDotAV, double *uDotDiff, double *mumax,
                           double *dtNew);
+
+/* ------------------------------------------- updates either side (step_oracle.c) */
+void orc_init_accel(int n, const unsigned *bits, double *a, double *pot, double *dtg);
+void orc_kick(int n, const unsigned *bits, double dvFacOne, double dvFacTwo,
+              double dvPredFacOne, double dvPredFacTwo, double duDelta, double duPredDelta,
+              int bIsothermal, const double *a, const double *uDotPdV, const double *uDotAV,
+              const double *uDotDiff, double *v, double *vPred, double *u, double *uPred);
+int orc_kick_vpred(int n, const unsigned *bits, double dvFacOne, double dvFacTwo,
+                   double duDelta, int bIsothermal, const double *a, const double *uDotPdV,
+                   const double *uDotAV, const double *uDotDiff, double *vPred, double *uPred);
+int orc_drift(int n, double dDelta, const double *fCenter, int bPeriodic, const double *fPeriod,
+              const double *v, double *r);
+void orc_init_dt(int n, const unsigned *bits, double dDelta, double *dt);
+int orc_grav_step(int n, const unsigned *bits, double dEta, const double *dtGrav, double *dt);
+void orc_accel_step(int n, const unsigned *bits, double dEta, double dVelFac, double dAccFac,
+                    int bEpsAcc, int bSqrtPhi, const double *v, const double *a,
+                    const double *fSoft, const double *fBall2, const double *fPot, double *dt);
+void orc_density_step(int n, const unsigned *bits, double dEta, double dRhoFac,
+                      const double *rho, double *dt);
+void orc_sph_step(int n, const unsigned *bits, double dCosmoFac, double dEtaCourant,
+                  double dEtauDot, double duMinDt, double dDiffCoeff, double dEtaDiffusion,
+                  double dResolveJeans, const double *fBall2, const double *c,
+                  const double *u, const double *uDotPdV, const double *uDotAV,
+                  const double *uDotDiff, const double *rho, const double *diff, double *dtNew,
+                  double *dt, double *dtMinGas);
+int orc_one_dt_to_rung(int iRung, double dDelta, double dt);
+int orc_dt_to_rung(int n, const unsigned *bits, int iRung, double dDelta, int iMaxRung, int bAll,
+                   const double *fSoft, const double *dt, int *rung, double *dtOld,
+                   int *nMaxRung, int *iMaxRungIdeal);
 
 #ifdef __cplusplus
 }

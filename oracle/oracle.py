@@ -82,6 +82,26 @@ def lib():
                                            _dp, _dp, _dp, _up, _dp, _lp, _ip, _dp,
                                            _dp, C.POINTER(Smf), _dp, _dp, _dp, _dp,
                                            _dp, _dp]
+        _u = _uip
+        L.orc_init_accel.argtypes = [C.c_int, _u, _dp, _dp, _dp]
+        L.orc_kick.argtypes = [C.c_int, _u] + [C.c_double] * 6 + [C.c_int] + [_dp] * 8
+        L.orc_kick_vpred.restype = C.c_int
+        L.orc_kick_vpred.argtypes = [C.c_int, _u, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_drift.restype = C.c_int
+        L.orc_drift.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, _dp]
+        L.orc_init_dt.argtypes = [C.c_int, _u, C.c_double, _dp]
+        L.orc_grav_step.restype = C.c_int
+        L.orc_grav_step.argtypes = [C.c_int, _u, C.c_double, _dp, _dp]
+        L.orc_accel_step.argtypes = [C.c_int, _u, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_density_step.argtypes = [C.c_int, _u, C.c_double, C.c_double, _dp, _dp]
+        L.orc_sph_step.argtypes = [C.c_int, _u] + [C.c_double] * 7 + [_dp] * 11
+        L.orc_one_dt_to_rung.restype = C.c_int
+        L.orc_one_dt_to_rung.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_dt_to_rung.restype = C.c_int
+        L.orc_dt_to_rung.argtypes = [C.c_int, _u, C.c_int, C.c_double, C.c_int, C.c_int, _dp,
+                                     _dp, _ip, _dp, _ip, _ip]
         _LIB = L
     return _LIB
 
@@ -326,3 +346,108 @@ def sph_pressure_sym(m, vpred, upred, rho, PoverRho2, c, balsara, diff, dt, fBal
     outs["acc"] = acc
     outs["dtNew"] = dtNew
     return outs
+
+
+# ---------------------------------------------------- updates either side
+# Each takes a dict of SoA fields (names as the library's) and returns the
+# updated fields as new arrays; the inputs are not modified.
+def _bits(b):
+    return np.ascontiguousarray(b, np.uint32)
+
+
+def _cp(f, k):
+    return _d(f[k]).copy()
+
+
+def init_accel(f):
+    a, pot, dtg = _cp(f, "a"), _cp(f, "fPot"), _cp(f, "dtGrav")
+    b = _bits(f["iActive"])
+    lib().orc_init_accel(len(b), P(b, _uip), P(a), P(pot), P(dtg))
+    return dict(a=a, fPot=pot, dtGrav=dtg)
+
+
+def kick(f, dvFacOne, dvFacTwo, dvPredFacOne, dvPredFacTwo, duDelta, duPredDelta,
+         bIsothermal=False):
+    v, vp, u, up = _cp(f, "v"), _cp(f, "vPred"), _cp(f, "u"), _cp(f, "uPred")
+    b = _bits(f["iActive"])
+    lib().orc_kick(len(b), P(b, _uip), dvFacOne, dvFacTwo, dvPredFacOne, dvPredFacTwo, duDelta,
+                   duPredDelta, int(bIsothermal), P(_d(f["a"])), P(_d(f["uDotPdV"])),
+                   P(_d(f["uDotAV"])), P(_d(f["uDotDiff"])), P(v), P(vp), P(u), P(up))
+    return dict(v=v, vPred=vp, u=u, uPred=up)
+
+
+def kick_vpred(f, dvFacOne, dvFacTwo, duDelta, bIsothermal=False):
+    vp, up = _cp(f, "vPred"), _cp(f, "uPred")
+    b = _bits(f["iActive"])
+    bad = lib().orc_kick_vpred(len(b), P(b, _uip), dvFacOne, dvFacTwo, duDelta, int(bIsothermal),
+                               P(_d(f["a"])), P(_d(f["uDotPdV"])), P(_d(f["uDotAV"])),
+                               P(_d(f["uDotDiff"])), P(vp), P(up))
+    return dict(vPred=vp, uPred=up, bad=bad)
+
+
+def drift(f, dDelta, fCenter=None, fPeriod=None):
+    r = _cp(f, "r")
+    per = fPeriod is not None
+    c = _d(fCenter if per else [0, 0, 0])
+    L = _d(fPeriod if per else [0, 0, 0])
+    bad = lib().orc_drift(len(r), dDelta, P(c), int(per), P(L), P(_d(f["v"])), P(r))
+    return dict(r=r, bad=bad)
+
+
+def init_dt(f, dDelta):
+    dt = _cp(f, "dt")
+    b = _bits(f["iActive"])
+    lib().orc_init_dt(len(b), P(b, _uip), dDelta, P(dt))
+    return dict(dt=dt)
+
+
+def grav_step(f, dEta):
+    dt = _cp(f, "dt")
+    b = _bits(f["iActive"])
+    bad = lib().orc_grav_step(len(b), P(b, _uip), dEta, P(_d(f["dtGrav"])), P(dt))
+    return dict(dt=dt, bad=bad)
+
+
+def accel_step(f, dEta, dVelFac, dAccFac, bEpsAcc, bSqrtPhi):
+    dt = _cp(f, "dt")
+    b = _bits(f["iActive"])
+    lib().orc_accel_step(len(b), P(b, _uip), dEta, dVelFac, dAccFac, int(bEpsAcc), int(bSqrtPhi),
+                         P(_d(f["v"])), P(_d(f["a"])), P(_d(f["fSoft"])), P(_d(f["fBall2"])),
+                         P(_d(f["fPot"])), P(dt))
+    return dict(dt=dt)
+
+
+def density_step(f, dEta, dRhoFac):
+    dt = _cp(f, "dt")
+    b = _bits(f["iActive"])
+    lib().orc_density_step(len(b), P(b, _uip), dEta, dRhoFac, P(_d(f["fDensity"])), P(dt))
+    return dict(dt=dt)
+
+
+def sph_step(f, dCosmoFac, dEtaCourant, dEtauDot, duMinDt, dDiffCoeff, dEtaDiffusion,
+             dResolveJeans):
+    dt, dtn = _cp(f, "dt"), _cp(f, "dtNew")
+    b = _bits(f["iActive"])
+    m = C.c_double()
+    lib().orc_sph_step(len(b), P(b, _uip), dCosmoFac, dEtaCourant, dEtauDot, duMinDt, dDiffCoeff,
+                       dEtaDiffusion, dResolveJeans, P(_d(f["fBall2"])), P(_d(f["c"])),
+                       P(_d(f["u"])), P(_d(f["uDotPdV"])), P(_d(f["uDotAV"])),
+                       P(_d(f["uDotDiff"])), P(_d(f["fDensity"])), P(_d(f["diff"])), P(dtn),
+                       P(dt), C.byref(m))
+    return dict(dt=dt, dtNew=dtn, dtMinGas=m.value)
+
+
+def one_dt_to_rung(iRung, dDelta, dt):
+    return lib().orc_one_dt_to_rung(iRung, dDelta, dt)
+
+
+def dt_to_rung(f, iRung, dDelta, iMaxRung, bAll):
+    rung = np.ascontiguousarray(f["iRung"], np.int32).copy()
+    dto = _cp(f, "dtOld")
+    b = _bits(f["iActive"])
+    nmax, ideal = C.c_int(), C.c_int()
+    top = lib().orc_dt_to_rung(len(b), P(b, _uip), iRung, dDelta, iMaxRung, int(bAll),
+                               P(_d(f["fSoft"])), P(_d(f["dt"])), P(rung, _ip), P(dto),
+                               C.byref(nmax), C.byref(ideal))
+    return dict(iRung=rung, dtOld=dto, iMaxRung=top, nMaxRung=nmax.value,
+                iMaxRungIdeal=ideal.value)
