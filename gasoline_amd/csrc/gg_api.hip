@@ -203,6 +203,15 @@ extern "C" int gg_set_local(GGPKD pkd, int nLocal)
 
 extern "C" int gg_local(GGPKD pkd) { return pkd ? pkd->nLocal : -1; }
 
+extern "C" int gg_set_period(GGPKD pkd, const double *fPeriod)
+{
+    GG_CHECK(pkd && fPeriod, GG_EINVAL, "null argument");
+    for (int d = 0; d < 3; ++d)
+        GG_CHECK(fPeriod[d] > 0, GG_EINVAL, "fPeriod[%d] = %g is not positive", d, fPeriod[d]);
+    for (int d = 0; d < 3; ++d) pkd->fPeriod[d] = fPeriod[d];
+    return GG_OK;
+}
+
 extern "C" int gg_set_precision(GGPKD pkd, int prec)
 {
     GG_CHECK(pkd, GG_EINVAL, "null handle");

@@ -113,19 +113,31 @@ struct WalkDec {
     int cl, cu, pl, np;
 };
 
+// A replica (PER, bRep) sees every cell at its centre + (ox, oy, oz), its
+// own bucket included among the opened ones (walk.c:74-93).
+template <bool PER = false>
 __device__ __forceinline__ WalkDec walk_decide(const WalkNode &n, int c, int ib, const double *bb,
-                                               double fSoftMax)
+                                               double fSoftMax, double ox = 0.0, double oy = 0.0,
+                                               double oz = 0.0, bool bRep = false)
 {
 #pragma clang fp contract(off)
     WalkDec d{false, false, false, false, 0, 0, 0, 0};
-    bool open = box_within(bb, n.fOpen2, n.cx, n.cy, n.cz);
+    double x = n.cx, y = n.cy, z = n.cz;
+    if constexpr (PER) {
+        x += ox;
+        y += oy;
+        z += oz;
+    } else {
+        bRep = false;
+    }
+    bool open = box_within(bb, n.fOpen2, x, y, z);
     if (n.pUpper - n.pLower < 3) open = true;   // < 4 particles: open
     if (open) {
         if (n.iLower != -1) {
             d.kids = true;
             d.cl = n.iLower;
             d.cu = n.iUpper;
-        } else if (c != ib) {
+        } else if (c != ib || bRep) {
             d.pp = true;
             d.pl = n.pLower;
             d.np = n.pUpper - n.pLower + 1;
@@ -133,7 +145,7 @@ __device__ __forceinline__ WalkDec walk_decide(const WalkNode &n, int c, int ib,
     } else {
         double twoh2 = n.fSoft + fSoftMax;
         twoh2 *= twoh2;
-        d.soft = (twoh2 < n.fOpen2) ? false : box_within(bb, twoh2, n.cx, n.cy, n.cz);
+        d.soft = (twoh2 < n.fOpen2) ? false : box_within(bb, twoh2, x, y, z);
         d.newt = !d.soft;
     }
     return d;
@@ -367,6 +379,8 @@ template <typename T, int ORDER> struct Stage {
 
 struct IArgs {
     int nb, iOrder;
+    int nReps;                 // periodic replicas -nReps..nReps of each L[d] > 0
+    double L[3];
     const int *bucket;
     const WalkNode *wn;
     const double *cell, *momOH;
@@ -764,7 +778,7 @@ __device__ __forceinline__ void flush_pp(Stage<T, ORDER> &L, int *qpl, int *qpn,
     np -= m;
 }
 
-template <typename T, int ORDER>
+template <typename T, int ORDER, bool PER>
 __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
                                                       const int *__restrict__ broot,
                                                       int *__restrict__ cnt,
@@ -788,6 +802,24 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
     const unsigned long long below = (1ull << lane) - 1;
     int nS = 0, nN = 0, nR = 0, nRp = 0;          // pending (wave-uniform)
     int totP = 0, totS = 0, totN = 0;
+    const int rx = PER && A.L[0] > 0 ? A.nReps : 0, ry = PER && A.L[1] > 0 ? A.nReps : 0,
+              rz = PER && A.L[2] > 0 ? A.nReps : 0;
+    // replicas x outermost (walk.c:325-338); each walks the whole tree with
+    // its cells and particles seen at r + o, and its queues are evaluated
+    // before the next one starts
+    for (int ix = -rx; ix <= rx; ++ix)
+    for (int iy = -ry; iy <= ry; ++iy)
+    for (int iz = -rz; iz <= rz; ++iz) {
+    const double ox = PER ? ix * A.L[0] : 0.0, oy = PER ? iy * A.L[1] : 0.0,
+                 oz = PER ? iz * A.L[2] : 0.0;
+    const bool bRep = PER && (ix || iy || iz);
+    WalkNode sbo = bn;                 // staging reference: the bucket's c.o.m. less o
+    if constexpr (PER) {
+        sbo.cx = bn.cx - ox;
+        sbo.cy = bn.cy - oy;
+        sbo.cz = bn.cz - oz;
+    }
+    const WalkNode &sb = PER ? sbo : bn;
     int nf = 1, cur = 0;
     if (lane == 0) fr[0][0] = broot[b];
     __syncthreads();
@@ -799,7 +831,7 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
             WalkDec dc{false, false, false, false, 0, 0, 0, 0};
             if (k < nf) {
                 c = fr[cur][k];
-                dc = walk_decide(A.wn[c], c, ib, bb, fSoftMax);
+                dc = walk_decide<PER>(A.wn[c], c, ib, bb, fSoftMax, ox, oy, oz, bRep);
             }
             const unsigned long long mk = __ballot(dc.kids);
             if (dc.kids) {
@@ -829,9 +861,9 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
             totN += __popcll(mn);
             totP += add;
             __syncthreads();
-            while (nS >= 64) flush_cells(L, qs, nS, 64, 1, A, bn, g, R);
-            while (nN >= 64) flush_cells(L, qn, nN, 64, 2, A, bn, g, R);
-            while (nRp >= 64) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, bn, g, R);
+            while (nS >= 64) flush_cells(L, qs, nS, 64, 1, A, sb, g, R);
+            while (nN >= 64) flush_cells(L, qn, nN, 64, 2, A, sb, g, R);
+            while (nRp >= 64) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, sb, g, R);
         }
         __syncthreads();
         if (nn > WI_FC) {
@@ -841,9 +873,10 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
         nf = nn;
         cur ^= 1;
     }
-    if (nS > 0) flush_cells(L, qs, nS, nS, 1, A, bn, g, R);
-    if (nN > 0) flush_cells(L, qn, nN, nN, 2, A, bn, g, R);
-    while (nR > 0) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, bn, g, R);
+    if (nS > 0) flush_cells(L, qs, nS, nS, 1, A, sb, g, R);
+    if (nN > 0) flush_cells(L, qn, nN, nN, 2, A, sb, g, R);
+    while (nR > 0) flush_pp(L, qpl, qpn, pidx, nR, nRp, A, sb, g, R);
+    }
     if (lane == 0) {
         cnt[3 * (size_t)b] = totP;
         cnt[3 * (size_t)b + 1] = totS;
@@ -949,6 +982,8 @@ static IArgs make_iargs(GGPKD pkd, int iOrder)
     A.off = G.off.as<int64_t>();
     A.ent = G.ent.as<int>();
     A.acc = G.acc.as<double>();
+    A.nReps = 0;
+    A.L[0] = A.L[1] = A.L[2] = 0.0;
     return A;
 }
 
@@ -958,13 +993,21 @@ static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
     dim3 grid(A.nb), block(64);
     const double *bb = G.bbox.as<double>();
     int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
+    const int *br = G.broot.as<int>();
+    // replicas only when there are some: the isolated walk keeps its code
+#define GG_FUSED(O)                                                                              \
+    if (A.nReps > 0)                                                                             \
+        hipLaunchKernelGGL((k_walk_interact<T, O, true>), grid, block, 0, st, A, bb, br, cnt, err); \
+    else                                                                                         \
+        hipLaunchKernelGGL((k_walk_interact<T, O, false>), grid, block, 0, st, A, bb, br, cnt, err)
     switch (order) {
-    case 1: hipLaunchKernelGGL((k_walk_interact<T, 1>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
-    case 2: hipLaunchKernelGGL((k_walk_interact<T, 2>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
-    case 3: hipLaunchKernelGGL((k_walk_interact<T, 3>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
-    case 4: hipLaunchKernelGGL((k_walk_interact<T, 4>), grid, block, 0, st, A, bb, G.broot.as<int>(), cnt, err); break;
+    case 1: GG_FUSED(1); break;
+    case 2: GG_FUSED(2); break;
+    case 3: GG_FUSED(3); break;
+    case 4: GG_FUSED(4); break;
     default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
     }
+#undef GG_FUSED
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -1012,13 +1055,25 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
                            int *nActive, double *pdPartSum, double *pdCellSum,
                            double *pdSoftSum, double *pdFlop)
 {
-    (void)iEwOrder; (void)fEwCut; (void)fEwhCut; (void)dRhoFac; (void)dSunSoft; (void)aSun;
+    (void)dRhoFac; (void)dSunSoft; (void)aSun;
     GG_CHECK(pkd, GG_EINVAL, "null handle");
     GravDev &G = pkd->grav;
     GG_CHECK(G.built, GG_ESTATE, "gg_grav_all before gg_build_binary");
     GG_CHECK(G.n == pkd->nLocal, GG_ESTATE, "particle count changed since the tree was built");
-    GG_CHECK(nReps == 0 && !bPeriodic && !bEwald, GG_EUNSUP,
-             "periodic replicas and Ewald summation are outside this library");
+    const double *L = pkd->fPeriod;
+    if (bPeriodic) {
+        GG_CHECK(L[0] > 0 && L[1] > 0 && L[2] > 0, GG_ESTATE,
+                 "periodic gravity needs the box (gg_set_period)");
+        GG_CHECK(nReps >= 0 && nReps <= 8, GG_EINVAL, "nReps %d outside 0..8", nReps);
+        GG_CHECK(G.nSys == 1, GG_EUNSUP, "periodic gravity needs a single tree, not a forest");
+        GG_CHECK(!bEwald || (L[0] == L[1] && L[1] == L[2]), GG_EUNSUP,
+                 "Ewald summation needs a cubic box (ewald.c uses fPeriod[0])");
+        GG_CHECK(!bEwald || (iEwOrder >= 0 && iEwOrder <= 4), GG_EINVAL,
+                 "iEwOrder %d outside 0..4", iEwOrder);
+        GG_CHECK(!bEwald || (fEwCut > 0 && fEwCut <= 8 && fEwhCut > 0 && fEwhCut <= 8), GG_EINVAL,
+                 "Ewald cuts outside (0, 8]");
+    }
+    const bool per = bPeriodic != 0, ewald = per && bEwald;
     GG_CHECK(!bComove && !bDoSun, GG_EUNSUP, "comoving and solar-system terms are not supported");
     GG_CHECK(iOrder >= 1 && iOrder <= G.iOrder, GG_EINVAL,
              "iOrder %d exceeds the tree's moments (%d)", iOrder, G.iOrder);
@@ -1032,7 +1087,11 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
         timer_mark(pkd, 0);
         // walk and interactions fused, one workgroup per bucket; the listed
         // walk and k_interact if a breadth-first frontier outgrew its buffer
-        const IArgs A = make_iargs(pkd, iOrder);
+        IArgs A = make_iargs(pkd, iOrder);
+        if (per) {
+            A.nReps = nReps;
+            for (int d = 0; d < 3; ++d) A.L[d] = L[d];
+        }
         GG_TRY(G.werr.reserve(sizeof(int)));
         GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
         G.listsCurrent = 0;
@@ -1042,12 +1101,16 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
         GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
         G.fusedWalk = !werr;
+        GG_CHECK(!(werr && per), GG_EUNSUP,
+                 "a breadth-first frontier of the periodic walk outgrew its buffer");
         if (werr) {
             GG_TRY(walk_lists(pkd));
             const IArgs B = make_iargs(pkd, iOrder);
             if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_interact<float>(iOrder, B, st));
             else GG_TRY(launch_interact<double>(iOrder, B, st));
         }
+        double ewFlop = 0;
+        if (ewald) GG_TRY(ewald_apply(pkd, nReps, iEwOrder, L[0], fEwCut, fEwhCut, &ewFlop));
         timer_mark(pkd, 1);
         GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
         GG_HIP(hipMemsetAsync(G.stats.p, 0, sizeof(unsigned long long) * 5, st));
@@ -1066,6 +1129,9 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
         const int phase[2] = {1, 8};
         timer_collect(pkd, 0, 2, phase);
         pkd->tm.ms[0] = 0;     // the walk is inside the interaction kernel
+        if (ewald && iEwOrder)   // pkdBucketEwald's count (ewald.c:175-176)
+            ewFlop += (double)hs[0] * G.nEwh * 58;
+        hs[4] += (unsigned long long)ewFlop;
     }
     if (nActive) *nActive = (int)hs[0];
     if (pdPartSum) *pdPartSum = (double)hs[1];

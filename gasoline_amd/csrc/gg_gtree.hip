@@ -1250,6 +1250,7 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
     G.n = n;
     G.nNodes = nNodes;
     G.iRoot = 0;
+    G.nSys = nSys;
     G.iOrder = iOrder;
     G.nBucket = nBucket;
     G.built = 1;

@@ -80,6 +80,10 @@ struct GravDev {
     // tree build scratch
     DevBuf gpn, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
     DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof, glroot, gdcnt, gdlist, glused, glbase, gscr;
+    DevBuf ewt, emom, ewh;          // Ewald: moment partials, root moments, k-space table
+    int64_t ewLoops = 0;            // real-space terms of the last Ewald pass
+    int nEwh = 0;                   // its k-space terms
+    int nSys = 1;                   // systems (roots) of the tree
     hipGraph_t graph = nullptr;         // captured step (gg_grav_graph)
     hipGraphExec_t gexec = nullptr;
     int graphOrder = 0;
@@ -92,7 +96,8 @@ struct GravDev {
         DevBuf *all[] = {&px, &py, &pz, &pm, &ph, &pact, &perm, &wnode, &cell, &momOH, &bucket,
                          &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
                          &tdbl, &gpn, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
-                         &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof, &glroot, &gdcnt, &gdlist, &glused, &glbase, &gscr};
+                         &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof, &glroot, &gdcnt, &gdlist, &glused, &glbase, &gscr,
+                         &ewt, &emom, &ewh};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -179,6 +184,7 @@ struct gg_pkd_s {
     int device = 0;
     int nStore = 0, nLocal = 0;
     int prec = GG_PREC_FP32;
+    double fPeriod[3] = {0, 0, 0};  // periodic box (gg_set_period); 0: not set
     hipStream_t stream = nullptr;
     gg::DevBuf f[GG_NFIELDS];      // caller-order fields
     gg::GravDev grav;
@@ -203,6 +209,10 @@ int exclusive_scan_i64(hipStream_t s, const int *in, int64_t *out, int n,
 int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenType, double dCrit,
                 int iOrder);
 void sph_release(GGPKD pkd);
+// Ewald correction of the active particles added to the gravity slots
+// (gg_ewald.hip); *flop = its real-space flop count.
+int ewald_apply(GGPKD pkd, int nReps, int iOrder, double L, double fEwCut, double fEwhCut,
+                double *flop);
 void timer_mark(GGPKD pkd, int slot);
 void timer_collect(GGPKD pkd, int first_slot, int nphases, const int *phase);
 }  // namespace gg

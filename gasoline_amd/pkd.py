@@ -41,7 +41,7 @@ TYPE_ACTIVE, TYPE_TREEACTIVE, TYPE_SMOOTHACTIVE, TYPE_SMOOTHDONE = 1, 2, 4, 8
 TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
 
 # exported entry points (include/gasoline_amd.h)
-ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local",
+ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local", "gg_set_period",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
@@ -110,6 +110,7 @@ def lib():
     L.gg_finish.argtypes = [_vp]
     L.gg_set_local.argtypes = [_vp, C.c_int]
     L.gg_local.argtypes = [_vp]
+    L.gg_set_period.argtypes = [_vp, _dp]
     L.gg_put.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_get.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_put_aos.argtypes = [_vp, _vp, C.c_size_t, C.POINTER(AosField), C.c_int]
@@ -252,12 +253,19 @@ class Pkd:
         _check(lib().gg_build_binary(self.h, int(nBucket), int(iOpenType), float(dCrit),
                                      int(iOrder), 0, 1))
 
-    def grav_all(self, iOrder=2):
+    def set_period(self, period):
+        L = np.ascontiguousarray(period, np.float64)
+        _check(lib().gg_set_period(self.h, _p(L)))
+
+    def grav_all(self, iOrder=2, nReps=0, periodic=False, bEwald=False, iEwOrder=4,
+                 fEwCut=2.6, fEwhCut=2.8):
+        """gg_grav_all; periodic uses the box of set_period."""
         na = C.c_int()
         ps, cs, ss, fl = C.c_double(), C.c_double(), C.c_double(), C.c_double()
-        _check(lib().gg_grav_all(self.h, 0, 0, int(iOrder), 0, 4, 0.0, 0.0, 0, 0.0, 0, 0.0,
-                                 None, C.byref(na), C.byref(ps), C.byref(cs), C.byref(ss),
-                                 C.byref(fl)))
+        _check(lib().gg_grav_all(self.h, int(nReps), int(bool(periodic)), int(iOrder),
+                                 int(bool(bEwald)), int(iEwOrder), float(fEwCut), float(fEwhCut),
+                                 0, 0.0, 0, 0.0, None, C.byref(na), C.byref(ps), C.byref(cs),
+                                 C.byref(ss), C.byref(fl)))
         return dict(nActive=na.value, dPartSum=ps.value, dCellSum=cs.value,
                     dSoftSum=ss.value, dFlop=fl.value)
 

@@ -102,6 +102,10 @@ int gg_finish(GGPKD pkd);
 /* Message of the last error on this thread. */
 const char *gg_last_error(void);
 
+/* The periodic box (pkdInitialize's fPeriod, pkd.c:150): gg_grav_all with
+ * bPeriodic uses it.  Every side must be positive. */
+int gg_set_period(GGPKD pkd, const double *fPeriod);
+
 /* Number of particles currently in the store (pkdLocal). */
 int gg_set_local(GGPKD pkd, int nLocal);
 int gg_local(GGPKD pkd);
@@ -152,9 +156,14 @@ int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCrit,
  * (pkdBucketInteract grav.c:23-249).  a, fPot and dtGrav of active particles
  * are ACCUMULATED onto their stored values (gasoline zeroes them first in
  * msrInitAccel).  Replaces pkdGravAll (pkd.h:797-801, pkd.c:2869-3060) with
- * the same signature; nReps must be 0 and bEwald/bComove/bDoSun 0 (periodic
- * Ewald gravity is outside this library).  Counters follow pkd.c:2946-2949
- * and the flop model of grav.c:246-247. */
+ * the same signature.  bPeriodic: every bucket also walks the replicas
+ * -nReps..nReps of the box set by gg_set_period (walk.c:325-438; the own
+ * bucket enters a replica's PP list) and, with bEwald, the Ewald correction
+ * of the root's multipoles to order iEwOrder is added (pkdEwaldInit
+ * ewald.c:180, pkdBucketEwald ewald.c:15; real-space images within fEwCut
+ * boxes, k-space |h| <= fEwhCut; cubic box).  A periodic tree must be a
+ * single system.  bComove and bDoSun must be 0.  Counters follow
+ * pkd.c:2946-2949 and the flop models of grav.c:246-247 and ewald.c:175. */
 int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
                 int iEwOrder, double fEwCut, double fEwhCut, int bComove,
                 double dRhoFac, int bDoSun, double dSunSoft, double *aSun,

@@ -598,15 +598,13 @@ static const int qeval_flop[5] = {10, 10, 41, 120, 277};   /* qeval.h:66 */
 
 /* Reduced multipole evaluation (QEVAL, qeval.h:21-64), falling through from
  * iOrder down to the monopole. */
-static inline void qeval(int iOrder, const ONode *c, const double *tq,
-                         const double *gam, double dx, double dy, double dz,
+static inline void qeval(int iOrder, double m, const double *tq, const double *O,
+                         const double *H, const double *gam, double dx, double dy, double dz,
                          double *ax, double *ay, double *az, double *pot)
 {
     double Qmirx, Qmiry, Qmirz, Qmir, Qta = 0.0;
-    const double *O = c->O, *H = c->H;
     /* traceless quadrupole of SETILIST (walk.c:41-48): xx yy zz xy xz yz */
     double xx = tq[0], yy = tq[1], zz = tq[2], xy = tq[3], xz = tq[4], yz = tq[5];
-    double m = c->fMass;
     switch (iOrder) {
     case 4: {
         double xxxx = H[0], xyyy = H[1], xxxy = H[2], yyyy = H[3], xxxz = H[4],
@@ -685,6 +683,7 @@ static inline void traceless(const ONode *c, double *tq)
 typedef struct {
     int *part, *soft, *newt, cap;
     double *d2a, *sq;
+    double *po, *so, *no;    /* replica offset of each entry (3 per entry) */
 } OScratch;
 
 static int scratch_fit(OScratch *s, int need)
@@ -696,7 +695,10 @@ static int scratch_fit(OScratch *s, int need)
     s->newt = (int *)realloc(s->newt, (size_t)s->cap * sizeof(int));
     s->d2a = (double *)realloc(s->d2a, (size_t)s->cap * sizeof(double));
     s->sq = (double *)realloc(s->sq, (size_t)s->cap * sizeof(double));
-    return s->part && s->soft && s->newt && s->d2a && s->sq;
+    s->po = (double *)realloc(s->po, (size_t)s->cap * 3 * sizeof(double));
+    s->so = (double *)realloc(s->so, (size_t)s->cap * 3 * sizeof(double));
+    s->no = (double *)realloc(s->no, (size_t)s->cap * 3 * sizeof(double));
+    return s->part && s->soft && s->newt && s->d2a && s->sq && s->po && s->so && s->no;
 }
 
 /* a, pot, dtg are in storage order here (length t->n). */
@@ -715,15 +717,18 @@ static int bucket_interact(orc_tree *t, int ib, int iOrder, OScratch *s,
         x = p[i].r[0]; y = p[i].r[1]; z = p[i].r[2];
         h = p[i].soft;
         /* particle-particle (grav.c:81-108) */
+        /* an entry of replica o sits at r + o (ILP x = r + rOffset, walk.c:106-110) */
         for (j = 0; j < nPart; ++j) {
             const OPart *q = &p[s->part[j]];
-            dx = x - q->r[0]; dy = y - q->r[1]; dz = z - q->r[2];
+            const double *o = s->po + 3 * j;
+            dx = x - (q->r[0] + o[0]); dy = y - (q->r[1] + o[1]); dz = z - (q->r[2] + o[2]);
             s->d2a[j] = dx * dx + dy * dy + dz * dz;
         }
         if (nPart > 0) orc_vsqrt1(nPart, s->d2a, s->sq);
         for (j = 0; j < nPart; ++j) {
             const OPart *q = &p[s->part[j]];
-            dx = x - q->r[0]; dy = y - q->r[1]; dz = z - q->r[2];
+            const double *o = s->po + 3 * j;
+            dx = x - (q->r[0] + o[0]); dy = y - (q->r[1] + o[1]); dz = z - (q->r[2] + o[2]);
             twoh = h + q->soft;
             spline_m(s->sq[j], s->d2a[j], twoh, &aa, &bb);
             idt2 = (p[i].m + q->m) * bb;
@@ -738,15 +743,17 @@ static int bucket_interact(orc_tree *t, int ib, int iOrder, OScratch *s,
         /* softened cells, monopole + raw quadrupole (grav.c:117-150) */
         for (j = 0; j < nSoft; ++j) {
             const ONode *c = &t->c[s->soft[j]];
-            dx = x - c->r[0]; dy = y - c->r[1]; dz = z - c->r[2];
+            const double *o = s->so + 3 * j;
+            dx = x - (c->r[0] + o[0]); dy = y - (c->r[1] + o[1]); dz = z - (c->r[2] + o[2]);
             s->d2a[j] = dx * dx + dy * dy + dz * dz;
         }
         if (nSoft > 0) orc_vsqrt1(nSoft, s->d2a, s->sq);
         for (j = 0; j < nSoft; ++j) {
             const ONode *c = &t->c[s->soft[j]];
+            const double *o = s->so + 3 * j;
             double qirx, qiry, qirz, qir, tr, qir3;
             const double *Q = c->Q;   /* xx yy zz xy xz yz */
-            dx = x - c->r[0]; dy = y - c->r[1]; dz = z - c->r[2];
+            dx = x - (c->r[0] + o[0]); dy = y - (c->r[1] + o[1]); dz = z - (c->r[2] + o[2]);
             twoh = h + c->fSoft;
             orc_splineq(s->sq[j], s->d2a[j], twoh, &aa, &bb, &cc, &dd);
             qirx = Q[0] * dx + Q[3] * dy + Q[4] * dz;
@@ -765,14 +772,16 @@ static int bucket_interact(orc_tree *t, int ib, int iOrder, OScratch *s,
         /* Newtonian cells, reduced multipoles (grav.c:164-191) */
         for (j = 0; j < nNewt; ++j) {
             const ONode *c = &t->c[s->newt[j]];
-            dx = x - c->r[0]; dy = y - c->r[1]; dz = z - c->r[2];
+            const double *o = s->no + 3 * j;
+            dx = x - (c->r[0] + o[0]); dy = y - (c->r[1] + o[1]); dz = z - (c->r[2] + o[2]);
             s->d2a[j] = dx * dx + dy * dy + dz * dz;
         }
         if (nNewt > 0) orc_vsqrt1(nNewt, s->d2a, s->sq);
         for (j = 0; j < nNewt; ++j) {
             const ONode *c = &t->c[s->newt[j]];
+            const double *o = s->no + 3 * j;
             double gam[6], dir2, tq[6];
-            dx = x - c->r[0]; dy = y - c->r[1]; dz = z - c->r[2];
+            dx = x - (c->r[0] + o[0]); dy = y - (c->r[1] + o[1]); dz = z - (c->r[2] + o[2]);
             gam[0] = s->sq[j];
             dir2 = gam[0] * gam[0];
             gam[1] = gam[0] * dir2;
@@ -781,7 +790,7 @@ static int bucket_interact(orc_tree *t, int ib, int iOrder, OScratch *s,
             gam[4] = 7 * gam[3] * dir2;
             gam[5] = 9 * gam[4] * dir2;
             traceless(c, tq);
-            qeval(iOrder, c, tq, gam, dx, dy, dz, &ax, &ay, &az, &fPot);
+            qeval(iOrder, c->fMass, tq, c->O, c->H, gam, dx, dy, dz, &ax, &ay, &az, &fPot);
             idt2 = (p[i].m + c->fMass) * gam[1];
             if (idt2 > dtg[i]) dtg[i] = idt2;
         }
@@ -822,16 +831,385 @@ static int bucket_interact(orc_tree *t, int ib, int iOrder, OScratch *s,
     return nActive * ((nPart + n) * 38 + nSoft * 82 + nNewt * (35 + qeval_flop[iOrder]));
 }
 
-void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
-                  double *dtgrav, double *stats)
+/* One replica of pkdLocalWalk (walk.c:58-178) for bucket iBucket: cell
+ * centres and particles seen at r + ro; the bucket's own particles enter the
+ * PP list only in a replica (bRep).  Entries are appended from the counts
+ * in n[3] (stored while below cap, counted always). */
+static void walk_rep(const orc_tree *t, int iBucket, const OBnd *b, double fSoftMax,
+                     const double *ro, int bRep, OScratch *s, int n[3])
 {
-    int n = t->n, ic, i;
+    int ic = t->iRoot, k;
+    while (ic != -1) {
+        const ONode *c = &t->c[ic];
+        double x = c->r[0] + ro[0], y = c->r[1] + ro[1], z = c->r[2] + ro[2];
+        int open = intersect_np(b, c->fOpen2, x, y, z);
+        if (c->pUpper - c->pLower < 3) open = 1;
+        if (open) {
+            if (c->iLower != -1) { ic = c->iLower; continue; }
+            if (ic != iBucket || bRep) {
+                int pj;
+                for (pj = c->pLower; pj <= c->pUpper; ++pj, ++n[0])
+                    if (n[0] < s->cap) {
+                        s->part[n[0]] = pj;
+                        for (k = 0; k < 3; ++k) s->po[3 * n[0] + k] = ro[k];
+                    }
+            }
+            ic = c->iUpper;
+        } else {
+            double twoh2 = c->fSoft + fSoftMax;
+            int isSoft;
+            twoh2 *= twoh2;
+            if (twoh2 < c->fOpen2) isSoft = 0;
+            else isSoft = intersect_np(b, twoh2, x, y, z);
+            if (isSoft) {
+                if (n[1] < s->cap) {
+                    s->soft[n[1]] = ic;
+                    for (k = 0; k < 3; ++k) s->so[3 * n[1] + k] = ro[k];
+                }
+                ++n[1];
+            } else {
+                if (n[2] < s->cap) {
+                    s->newt[n[2]] = ic;
+                    for (k = 0; k < 3; ++k) s->no[3 * n[2] + k] = ro[k];
+                }
+                ++n[2];
+            }
+            ic = c->iUpper;
+        }
+    }
+}
+
+/* pkdBucketWalk (walk.c:306-439): the replicas -nReps..nReps of each
+ * periodic dimension (L[d] > 0), x outermost. */
+static void walk_all_reps(const orc_tree *t, int iBucket, const OBnd *b, double fSoftMax,
+                          int nReps, const double *L, OScratch *s, int n[3])
+{
+    int ix, iy, iz;
+    n[0] = n[1] = n[2] = 0;
+    for (ix = -nReps; ix <= nReps; ++ix) {
+        if (ix && !(L && L[0] > 0)) continue;
+        for (iy = -nReps; iy <= nReps; ++iy) {
+            if (iy && !(L && L[1] > 0)) continue;
+            for (iz = -nReps; iz <= nReps; ++iz) {
+                double ro[3];
+                if (iz && !(L && L[2] > 0)) continue;
+                ro[0] = L ? ix * L[0] : 0.0;
+                ro[1] = L ? iy * L[1] : 0.0;
+                ro[2] = L ? iz * L[2] : 0.0;
+                walk_rep(t, iBucket, b, fSoftMax, ro, ix || iy || iz, s, n);
+            }
+        }
+    }
+}
+
+/* ----------------------------------------------------------------- Ewald */
+/* The root's moments as pkdCalcRoot + pkdDistribRoot leave them in
+ * ilcnRoot (pkd.c:4395-4493): mass and c.o.m. of the root, its raw
+ * quadrupole (trace not removed), and the complete octupole and
+ * hexadecapole about the c.o.m. (d2 = 0 in pkd.c:4447-4469). */
+typedef struct {
+    double m, x, y, z;
+    double Q[6];    /* xx yy zz xy xz yz */
+    double O[10];   /* xxx xyy xxy yyy xxz yyz xyz xzz yzz zzz */
+    double H[15];   /* xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz xxzz xyzz xzzz yyzz yzzz zzzz */
+} OMom;
+
+static void calc_root(const orc_tree *t, OMom *mom)
+{
+    const ONode *r = &t->c[t->iRoot];
+    int pj, k;
+    memset(mom, 0, sizeof(*mom));
+    for (pj = r->pLower; pj <= r->pUpper; ++pj) {
+        double m = t->p[pj].m;
+        double dx = t->p[pj].r[0] - r->r[0], dy = t->p[pj].r[1] - r->r[1],
+               dz = t->p[pj].r[2] - r->r[2];
+        double *H = mom->H, *O = mom->O;
+        H[0] += m * (dx * dx * dx * dx);
+        H[1] += m * (dx * dy * dy * dy);
+        H[2] += m * (dx * dx * dx * dy);
+        H[3] += m * (dy * dy * dy * dy);
+        H[4] += m * (dx * dx * dx * dz);
+        H[5] += m * (dy * dy * dy * dz);
+        H[6] += m * (dx * dx * dy * dy);
+        H[7] += m * (dx * dx * dy * dz);
+        H[8] += m * (dx * dy * dy * dz);
+        H[9] += m * (dx * dx * dz * dz);
+        H[10] += m * (dx * dy * dz * dz);
+        H[11] += m * (dx * dz * dz * dz);
+        H[12] += m * (dy * dy * dz * dz);
+        H[13] += m * (dy * dz * dz * dz);
+        H[14] += m * (dz * dz * dz * dz);
+        O[0] += m * (dx * dx * dx);
+        O[1] += m * (dx * dy * dy);
+        O[2] += m * (dx * dx * dy);
+        O[3] += m * (dy * dy * dy);
+        O[4] += m * (dx * dx * dz);
+        O[5] += m * (dy * dy * dz);
+        O[6] += m * dx * dy * dz;
+        O[7] += m * (dx * dz * dz);
+        O[8] += m * (dy * dz * dz);
+        O[9] += m * (dz * dz * dz);
+    }
+    mom->m = r->fMass;
+    mom->x = r->r[0];
+    mom->y = r->r[1];
+    mom->z = r->r[2];
+    for (k = 0; k < 6; ++k) mom->Q[k] = r->Q[k];
+}
+
+/* MEVAL (meval.h:21-80): the complete multipole at (dx, dy, dz) from its
+ * centre, falling through from iOrder to the monopole. */
+static void meval(int iOrder, const OMom *mom, const double *gam, double dx, double dy,
+                  double dz, double *ax, double *ay, double *az, double *fPot)
+{
+    const double *Q = mom->Q, *O = mom->O, *H = mom->H;
+    const double xx = Q[0], yy = Q[1], zz = Q[2], xy = Q[3], xz = Q[4], yz = Q[5];
+    double Qmirx, Qmiry, Qmirz, Qmir, Qta = 0.0;
+    switch (iOrder) {
+    case 4: {
+        const double xxxx = H[0], xyyy = H[1], xxxy = H[2], yyyy = H[3], xxxz = H[4],
+                     yyyz = H[5], xxyy = H[6], xxyz = H[7], xyyz = H[8], xxzz = H[9],
+                     xyzz = H[10], xzzz = H[11], yyzz = H[12], yzzz = H[13], zzzz = H[14];
+        double Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, Qtr, Qhx, Qhy, Qhz, Qh;
+        Qmirx = (1.0 / 6.0) * (xzzz * dz * dz * dz + 3 * xyzz * dy * dz * dz + 3 * xyyz * dy * dy * dz
+                + xyyy * dy * dy * dy + 3 * xxzz * dx * dz * dz + 6 * xxyz * dx * dy * dz
+                + 3 * xxyy * dx * dy * dy + 3 * xxxz * dx * dx * dz + 3 * xxxy * dx * dx * dy
+                + xxxx * dx * dx * dx);
+        Qmiry = (1.0 / 6.0) * (yzzz * dz * dz * dz + 3 * xyzz * dx * dz * dz + 3 * xxyz * dx * dx * dz
+                + xxxy * dx * dx * dx + 3 * yyzz * dy * dz * dz + 6 * xyyz * dx * dy * dz
+                + 3 * xxyy * dx * dx * dy + 3 * yyyz * dy * dy * dz + 3 * xyyy * dx * dy * dy
+                + yyyy * dy * dy * dy);
+        Qmirz = (1.0 / 6.0) * (yyyz * dy * dy * dy + 3 * xyyz * dx * dy * dy + 3 * xxyz * dx * dx * dy
+                + xxxz * dx * dx * dx + 3 * yyzz * dy * dy * dz + 6 * xyzz * dx * dy * dz
+                + 3 * xxzz * dx * dx * dz + 3 * yzzz * dy * dz * dz + 3 * xzzz * dx * dz * dz
+                + zzzz * dz * dz * dz);
+        Qmir = (1.0 / 4.0) * (Qmirx * dx + Qmiry * dy + Qmirz * dz);
+        Qxx = xxxx + xxyy + xxzz;
+        Qxy = xxxy + xyyy + xyzz;
+        Qxz = xxxz + xyyz + xzzz;
+        Qyy = xxyy + yyyy + yyzz;
+        Qyz = xxyz + yyyz + yzzz;
+        Qzz = xxzz + yyzz + zzzz;
+        Qtr = (1.0 / 8.0) * (Qxx + Qyy + Qzz);
+        Qhx = 0.5 * (Qxx * dx + Qxy * dy + Qxz * dz);
+        Qhy = 0.5 * (Qxy * dx + Qyy * dy + Qyz * dz);
+        Qhz = 0.5 * (Qxz * dx + Qyz * dy + Qzz * dz);
+        Qh = 0.5 * (Qhx * dx + Qhy * dy + Qhz * dz);
+        *fPot -= gam[4] * Qmir - gam[3] * Qh + gam[2] * Qtr;
+        Qta += gam[5] * Qmir - gam[4] * Qh + gam[3] * Qtr;
+        *ax += gam[4] * Qmirx - gam[3] * Qhx;
+        *ay += gam[4] * Qmiry - gam[3] * Qhy;
+        *az += gam[4] * Qmirz - gam[3] * Qhz;
+    }
+    /* fall through */
+    case 3: {
+        const double xxx = O[0], xyy = O[1], xxy = O[2], yyy = O[3], xxz = O[4], yyz = O[5],
+                     xyz = O[6], xzz = O[7], yzz = O[8], zzz = O[9];
+        double Qx, Qy, Qz, Qtr;
+        Qmirx = (1.0 / 2.0) * (xzz * dz * dz + 2 * xyz * dy * dz + xyy * dy * dy
+                + 2 * xxz * dx * dz + 2 * xxy * dx * dy + xxx * dx * dx);
+        Qmiry = (1.0 / 2.0) * (yzz * dz * dz + 2 * xyz * dx * dz + xxy * dx * dx
+                + 2 * yyz * dy * dz + 2 * xyy * dx * dy + yyy * dy * dy);
+        Qmirz = (1.0 / 2.0) * (yyz * dy * dy + 2 * xyz * dx * dy + xxz * dx * dx
+                + 2 * yzz * dy * dz + 2 * xzz * dx * dz + zzz * dz * dz);
+        Qmir = (1.0 / 3.0) * (Qmirx * dx + Qmiry * dy + Qmirz * dz);
+        Qx = 0.5 * (xxx + xyy + xzz);
+        Qy = 0.5 * (xxy + yyy + yzz);
+        Qz = 0.5 * (xxz + yyz + zzz);
+        Qtr = Qx * dx + Qy * dy + Qz * dz;
+        *fPot -= gam[3] * Qmir - gam[2] * Qtr;
+        Qta += gam[4] * Qmir - gam[3] * Qtr;
+        *ax += gam[3] * Qmirx - gam[2] * Qx;
+        *ay += gam[3] * Qmiry - gam[2] * Qy;
+        *az += gam[3] * Qmirz - gam[2] * Qz;
+    }
+    /* fall through */
+    case 2: {
+        double Qtr;
+        Qmirx = (1.0 / 1.0) * (xz * dz + xy * dy + xx * dx);
+        Qmiry = (1.0 / 1.0) * (yz * dz + xy * dx + yy * dy);
+        Qmirz = (1.0 / 1.0) * (yz * dy + xz * dx + zz * dz);
+        Qmir = (1.0 / 2.0) * (Qmirx * dx + Qmiry * dy + Qmirz * dz);
+        Qtr = 0.5 * (xx + yy + zz);
+        *fPot -= gam[2] * Qmir - gam[1] * Qtr;
+        Qta += gam[3] * Qmir - gam[2] * Qtr;
+        *ax += gam[2] * Qmirx;
+        *ay += gam[2] * Qmiry;
+        *az += gam[2] * Qmirz;
+    }
+    /* fall through */
+    case 1:
+    default:
+        *fPot -= gam[0] * mom->m;
+        Qta += gam[1] * mom->m;
+        *ax -= dx * Qta;
+        *ay -= dy * Qta;
+        *az -= dz * Qta;
+    }
+}
+
+typedef struct { double hx, hy, hz, hCfac, hSfac; } OEwt;
+
+/* pkdEwaldInit (ewald.c:180-247): the h-loop table, |h| <= fhCut, h != 0. */
+static int ewald_init(const OMom *mom, double L, double fhCut, int iOrder, OEwt **pewt)
+{
+    int hReps = (int)ceil(fhCut), hx, hy, hz, i = 0, cap = 64;
+    double alpha = 2.0 / L, k4 = M_PI * M_PI / (alpha * alpha * L * L);
+    OEwt *ewt = (OEwt *)malloc(cap * sizeof(OEwt));
+    for (hx = -hReps; hx <= hReps; ++hx)
+        for (hy = -hReps; hy <= hReps; ++hy)
+            for (hz = -hReps; hz <= hReps; ++hz) {
+                int h2 = hx * hx + hy * hy + hz * hz, k;
+                double gam[6], ax, ay, az, mfacc, mfacs, tq[6];
+                if (h2 == 0) continue;
+                if (h2 > fhCut * fhCut) continue;
+                if (i == cap) {
+                    cap *= 2;
+                    ewt = (OEwt *)realloc(ewt, cap * sizeof(OEwt));
+                }
+                /* QEVAL (qeval.h) over the root moments as ilcnRoot holds them */
+                for (k = 0; k < 6; ++k) tq[k] = mom->Q[k];
+                gam[0] = exp(-k4 * h2) / (M_PI * h2 * L);
+                gam[1] = 2 * M_PI / L * gam[0];
+                gam[2] = -2 * M_PI / L * gam[1];
+                gam[3] = 2 * M_PI / L * gam[2];
+                gam[4] = -2 * M_PI / L * gam[3];
+                gam[5] = 2 * M_PI / L * gam[4];
+                gam[1] = 0.0;
+                gam[3] = 0.0;
+                gam[5] = 0.0;
+                ax = ay = az = 0.0;
+                mfacc = 0.0;
+                qeval(iOrder, mom->m, tq, mom->O, mom->H, gam, hx, hy, hz, &ax, &ay, &az, &mfacc);
+                gam[0] = exp(-k4 * h2) / (M_PI * h2 * L);
+                gam[1] = 2 * M_PI / L * gam[0];
+                gam[2] = -2 * M_PI / L * gam[1];
+                gam[3] = 2 * M_PI / L * gam[2];
+                gam[4] = -2 * M_PI / L * gam[3];
+                gam[5] = 2 * M_PI / L * gam[4];
+                gam[0] = 0.0;
+                gam[2] = 0.0;
+                gam[4] = 0.0;
+                ax = ay = az = 0.0;
+                mfacs = 0.0;
+                qeval(iOrder, mom->m, tq, mom->O, mom->H, gam, hx, hy, hz, &ax, &ay, &az, &mfacs);
+                ewt[i].hx = 2 * M_PI / L * hx;
+                ewt[i].hy = 2 * M_PI / L * hy;
+                ewt[i].hz = 2 * M_PI / L * hz;
+                ewt[i].hCfac = mfacc;
+                ewt[i].hSfac = mfacs;
+                ++i;
+            }
+    *pewt = ewt;
+    return i;
+}
+
+/* pkdBucketEwald (ewald.c:15-178) for the active particles of bucket ib;
+ * returns its flop count. */
+static double bucket_ewald(orc_tree *t, int ib, int nReps, double L, double fEwCut, int iOrder,
+                           const OMom *mom, const OEwt *ewt, int nEwh, double *a, double *pot)
+{
+    static const int nMultiFlop[5] = {10, 10, 48, 151, 343};   /* MEVAL_FLOP, meval.h:84 */
+    const ONode *bk = &t->c[ib];
+    int i, j, ix, iy, iz, nEwReps = (int)ceil(fEwCut), nLoop = 0, nActive = 0;
+    double alpha = 2.0 / L, alpha2 = alpha * alpha, k1 = M_PI / (alpha2 * L * L * L);
+    double ka = 2.0 * alpha / sqrt(M_PI), fEwCut2 = fEwCut * fEwCut * L * L;
+    if (!iOrder) return 0;
+    nEwReps = nEwReps > nReps ? nEwReps : nReps;
+    for (j = bk->pLower; j <= bk->pUpper; ++j) {
+        const OPart *p = &t->p[j];
+        double fPot, ax = 0, ay = 0, az = 0, dx, dy, dz;
+        if (!p->active) continue;
+        fPot = mom->m * k1;
+        dx = p->r[0] - mom->x;
+        dy = p->r[1] - mom->y;
+        dz = p->r[2] - mom->z;
+        for (ix = -nEwReps; ix <= nEwReps; ++ix) {
+            int bInHolex = (ix >= -nReps && ix <= nReps);
+            double dxo = dx + ix * L;
+            for (iy = -nEwReps; iy <= nEwReps; ++iy) {
+                int bInHolexy = (bInHolex && iy >= -nReps && iy <= nReps);
+                double dyo = dy + iy * L;
+                for (iz = -nEwReps; iz <= nEwReps; ++iz) {
+                    int bInHole = (bInHolexy && iz >= -nReps && iz <= nReps);
+                    double dzo = dz + iz * L, r2, gam[6], alphan;
+                    r2 = dxo * dxo + dyo * dyo + dzo * dzo;
+                    if (r2 > fEwCut2 && !bInHole) continue;
+                    if (r2 < 3.0e-3 * L * L) {   /* series about the origin */
+                        alphan = ka;
+                        r2 *= alpha2;
+                        gam[0] = alphan * (r2 / 3 - 1);
+                        alphan *= 2 * alpha2;
+                        gam[1] = alphan * (r2 / 5 - 1.0 / 3.0);
+                        alphan *= 2 * alpha2;
+                        gam[2] = alphan * (r2 / 7 - 1.0 / 5.0);
+                        alphan *= 2 * alpha2;
+                        gam[3] = alphan * (r2 / 9 - 1.0 / 7.0);
+                        alphan *= 2 * alpha2;
+                        gam[4] = alphan * (r2 / 11 - 1.0 / 9.0);
+                        alphan *= 2 * alpha2;
+                        gam[5] = alphan * (r2 / 13 - 1.0 / 11.0);
+                    } else {
+                        double r = sqrt(r2), dir = 1 / r, dir2 = dir * dir, ea;
+                        ea = exp(-r2 * alpha2);
+                        ea *= ka * dir2;
+                        if (bInHole) gam[0] = -erf(alpha * r);
+                        else gam[0] = erfc(alpha * r);
+                        gam[0] *= dir;
+                        gam[1] = gam[0] * dir2 + ea;
+                        alphan = 2 * alpha2;
+                        gam[2] = 3 * gam[1] * dir2 + alphan * ea;
+                        alphan *= 2 * alpha2;
+                        gam[3] = 5 * gam[2] * dir2 + alphan * ea;
+                        alphan *= 2 * alpha2;
+                        gam[4] = 7 * gam[3] * dir2 + alphan * ea;
+                        alphan *= 2 * alpha2;
+                        gam[5] = 9 * gam[4] * dir2 + alphan * ea;
+                    }
+                    meval(iOrder, mom, gam, dxo, dyo, dzo, &ax, &ay, &az, &fPot);
+                    ++nLoop;
+                }
+            }
+        }
+        for (i = 0; i < nEwh; ++i) {
+            double hdotx = ewt[i].hx * dx + ewt[i].hy * dy + ewt[i].hz * dz;
+            double c = cos(hdotx), sn = sin(hdotx);
+            fPot += ewt[i].hCfac * c + ewt[i].hSfac * sn;
+            ax += ewt[i].hx * (ewt[i].hCfac * sn - ewt[i].hSfac * c);
+            ay += ewt[i].hy * (ewt[i].hCfac * sn - ewt[i].hSfac * c);
+            az += ewt[i].hz * (ewt[i].hCfac * sn - ewt[i].hSfac * c);
+        }
+        pot[j] += fPot;
+        a[3 * j] += ax;
+        a[3 * j + 1] += ay;
+        a[3 * j + 2] += az;
+        ++nActive;
+    }
+    return (double)nLoop * (104 + nMultiFlop[iOrder]) + (double)nActive * nEwh * 58;
+}
+
+/* pkdGravAll (pkd.c:2869-3060): every bucket with active particles walks
+ * the replicas -nReps..nReps (periodic dimensions L[d] > 0; L = NULL for an
+ * isolated system) and, when bEwald, adds the Ewald correction of the
+ * root's multipoles (pkdEwaldInit + pkdBucketEwald, cubic box L[0]). */
+void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
+                           int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
+                           double *dtgrav, double *stats)
+{
+    int n = t->n, ic, i, nEwh = 0;
     double *sa = (double *)calloc((size_t)(n > 0 ? n : 1) * 3, sizeof(double));
     double *sp = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     double *sd = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     OScratch s = {0};
+    OMom mom;
+    OEwt *ewt = NULL;
     double nActive = 0, dPart = 0, dCell = 0, dSoft = 0, dFlop = 0;
+    const int ewald = L && bEwald;
     scratch_fit(&s, 1024);
+    if (ewald) {
+        calc_root(t, &mom);
+        nEwh = ewald_init(&mom, L[0], fEwhCut, iEwOrder, &ewt);
+    }
     ic = t->iRoot;
     while (ic != -1) {
         const ONode *c = &t->c[ic];
@@ -841,9 +1219,8 @@ void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
         if (c->iLower != -1) { ic = c->iLower; continue; }
         na = bucket_setup(t, ic, &b, &fsm);
         if (na > 0) {
-            int nb = c->pUpper - c->pLower + 1;
             for (;;) {
-                orc_walk_bucket(t, ic, iOrder, s.cap, s.part, s.soft, s.newt, counts);
+                walk_all_reps(t, ic, &b, fsm, L ? nReps : 0, L, &s, counts);
                 if (counts[0] <= s.cap && counts[1] <= s.cap && counts[2] <= s.cap) break;
                 scratch_fit(&s, counts[0] > counts[1] ? (counts[0] > counts[2] ? counts[0] : counts[2])
                                                       : (counts[1] > counts[2] ? counts[1] : counts[2]));
@@ -855,7 +1232,8 @@ void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
             dSoft += na * counts[1];
             dFlop += bucket_interact(t, ic, iOrder, &s, counts[0], counts[1], counts[2],
                                      sa, sp, sd);
-            (void)nb;
+            if (ewald)
+                dFlop += bucket_ewald(t, ic, nReps, L[0], fEwCut, iEwOrder, &mom, ewt, nEwh, sa, sp);
         }
         ic = c->iUpper;
     }
@@ -868,8 +1246,15 @@ void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
         dtgrav[src] = sd[i];
     }
     stats[0] = nActive; stats[1] = dPart; stats[2] = dCell; stats[3] = dSoft; stats[4] = dFlop;
-    free(sa); free(sp); free(sd);
+    free(sa); free(sp); free(sd); free(ewt);
     free(s.part); free(s.soft); free(s.newt); free(s.d2a); free(s.sq);
+    free(s.po); free(s.so); free(s.no);
+}
+
+void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
+                  double *dtgrav, double *stats)
+{
+    orc_grav_all_periodic(t, iOrder, 0, NULL, 0, 0, 0.0, 0.0, a, pot, dtgrav, stats);
 }
 
 /* ======================================================================

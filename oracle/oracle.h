@@ -27,6 +27,11 @@
  *                          closed forms of SPLINE at r=0 and r>=2h, single-
  *                          particle cells (Q=0) vs point masses.
  *   orc_direct_sum         Newton's law / Plummer analytic field.
+ *   orc_grav_all_periodic  a test mass near a point mass in the periodic box:
+ *                          zero force at the symmetry points of the lattice,
+ *                          odd in the separation, Newton plus the uniform
+ *                          background's (4 pi/3) m r / L^3 at small r; with
+ *                          the box large, the isolated result.
  *   orc_knn                brute-force full sort on small inputs.
  *   orc_dendvdx            uniform density, linear velocity field (div, curl
  *                          exact), M4 normalisation.
@@ -95,6 +100,15 @@ void orc_walk_bucket(orc_tree *t, int iBucket, int iOrder, int cap,
  * dCellSum, dSoftSum, dFlop. */
 void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
                   double *dtgrav, double *stats);
+
+/* The same with the replicas -nReps..nReps of the periodic box L[3]
+ * (pkdBucketWalk walk.c:306-439; L = NULL: isolated) and, when bEwald, the
+ * Ewald correction of the root's multipoles to order iEwOrder (pkdCalcRoot
+ * pkd.c:4395, pkdDistribRoot :4472, pkdEwaldInit ewald.c:180,
+ * pkdBucketEwald ewald.c:15; cubic box L[0]). */
+void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
+                           int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
+                           double *dtgrav, double *stats);
 
 /* FP64 O(N^2) direct sum with gasoline's SPLINE softening (grav.h:74-97),
  * twoh = h_i + h_j.  Targets are the nt indices in idx (or all if idx is

@@ -65,6 +65,8 @@ def lib():
         L.orc_walk_bucket.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
                                       _ip, _ip, _ip, _ip]
         L.orc_grav_all.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_grav_all_periodic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
+                                            C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_direct_sum.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp, _dp]
         L.orc_spline.argtypes = [C.c_double, C.c_double, _dp, _dp]
         L.orc_splineq.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
@@ -207,12 +209,19 @@ class Tree:
                 return part[:cnt[0]].copy(), soft[:cnt[1]].copy(), newt[:cnt[2]].copy()
             cap = int(cnt.max()) * 2
 
-    def grav_all(self):
+    def grav_all(self, nReps=0, period=None, bEwald=False, iEwOrder=4, fEwCut=2.6,
+                 fEwhCut=2.8):
         a = np.zeros((self.n, 3))
         pot = np.zeros(self.n)
         dtg = np.zeros(self.n)
         st = np.zeros(5)
-        lib().orc_grav_all(self.h, self.iOrder, P(a), P(pot), P(dtg), P(st))
+        if period is None:
+            lib().orc_grav_all(self.h, self.iOrder, P(a), P(pot), P(dtg), P(st))
+        else:
+            L = _d(period)
+            lib().orc_grav_all_periodic(self.h, self.iOrder, int(nReps), P(L), int(bEwald),
+                                        int(iEwOrder), fEwCut, fEwhCut, P(a), P(pot), P(dtg),
+                                        P(st))
         return a, pot, dtg, dict(nActive=st[0], dPartSum=st[1], dCellSum=st[2],
                                  dSoftSum=st[3], dFlop=st[4])
 

@@ -262,3 +262,58 @@ def test_c_host_driver(gpu):
     out = subprocess.run([exe, "32768", "32768"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "Gravity Calculated" in out.stdout and "ggforce ok" in out.stdout
+
+
+# ------------------------------------------------ periodic gravity, Ewald
+@pytest.fixture(scope="module")
+def periodic_box():
+    # a clustered periodic unit box: half uniform, half in three clumps,
+    # inside [-1/2, 1/2) as the reference's drift keeps it
+    rng = np.random.default_rng(21)
+    n = 8000
+    u = rng.uniform(-0.5, 0.5, (n // 2, 3))
+    c = rng.uniform(-0.5, 0.5, (3, 3))
+    k = rng.integers(0, 3, n - n // 2)
+    cl = c[k] + rng.normal(0, 0.03, (n - n // 2, 3))
+    r = ics._f32((np.concatenate([u, cl]) + 0.5) % 1.0 - 0.5)
+    m = np.full(n, 1.0 / n)
+    soft = np.full(n, 0.01)
+    return r, m, soft
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_periodic_ewald_vs_oracle(gpu, periodic_box, order):
+    r, m, soft = periodic_box
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=order)
+    ao, po, do, so = T.grav_all(nReps=1, period=[1.0] * 3, bEwald=True, iEwOrder=order)
+    for prec, rms, mx in ((gpu.PREC_FP64, 1e-8, 1e-7), (gpu.PREC_FP32, 1e-5, 1e-4)):
+        with setup_pkd(gpu, r, m, soft, prec) as pk:
+            pk.set_period([1.0] * 3)
+            pk.build_binary(16, 1, 0.7, order)
+            st = pk.grav_all(order, nReps=1, periodic=True, bEwald=True, iEwOrder=order)
+            a, pot = pk.get("a"), pk.get("fPot")
+        for k in ("nActive", "dPartSum", "dCellSum", "dSoftSum", "dFlop"):
+            assert st[k] == so[k], (prec, k)
+        e = rel_err(a, ao)
+        assert np.sqrt(np.mean(e * e)) < rms and e.max() < mx, prec
+        ep = np.abs(pot - po) / np.abs(po).max()
+        assert ep.max() < mx, prec
+
+
+def test_periodic_without_ewald_and_errors(gpu, periodic_box):
+    r, m, soft = periodic_box
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=2)
+    ao, _, _, so = T.grav_all(nReps=1, period=[1.0] * 3, bEwald=False)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64) as pk:
+        pk.build_binary(16, 1, 0.7, 2)
+        with pytest.raises(gpu.GGError, match="gg_set_period"):
+            pk.grav_all(2, nReps=1, periodic=True)
+        pk.set_period([1.0, 1.0, 2.0])
+        with pytest.raises(gpu.GGError, match="cubic"):
+            pk.grav_all(2, nReps=1, periodic=True, bEwald=True)
+        pk.set_period([1.0] * 3)
+        st = pk.grav_all(2, nReps=1, periodic=True, bEwald=False)   # replicas only
+        a = pk.get("a")
+    assert st["dPartSum"] == so["dPartSum"] and st["dCellSum"] == so["dCellSum"]
+    e = rel_err(a, ao)
+    assert np.sqrt(np.mean(e * e)) < 1e-8 and e.max() < 1e-7

@@ -220,3 +220,50 @@ def test_counters_follow_reference_formula(plummer):
         flop += nb * ((len(p) + nb) * 38 + len(s) * 82 + len(c) * (35 + 41))
     assert st["dPartSum"] == part and st["dCellSum"] == cell and st["dSoftSum"] == soft_n
     assert st["dFlop"] == flop
+
+
+# ------------------------------------------------ periodic gravity, Ewald
+def _test_mass_force(rel, L=1.0, order=4):
+    # a unit mass at the origin and a test mass of 1e-10 at rel: the root's
+    # expansion is the point mass itself, so the result is the periodic
+    # Green's function (with the uniform background the Ewald sum implies)
+    r = np.array([[0.0, 0.0, 0.0], rel])
+    T = O.Tree(r, np.array([1.0, 1e-10]), np.array([1e-7, 1e-7]), nBucket=16, dTheta=0.7,
+               iOrder=order)
+    a, pot, _, st = T.grav_all(nReps=1, period=[L, L, L], bEwald=True, iEwOrder=order)
+    return a[1], st
+
+
+@pytest.mark.parametrize("rel", [[0.5, 0, 0], [0.5, 0.5, 0], [0.5, 0.5, 0.5], [0, 0.5, 0.5]])
+def test_periodic_force_vanishes_at_lattice_symmetry_points(rel):
+    a, _ = _test_mass_force(np.array(rel, float))
+    assert np.abs(a).max() < 1e-8             # |Newton| alone would be 1.3 to 4
+
+
+def test_periodic_force_is_odd_and_newton_plus_background_at_small_r():
+    rng = np.random.default_rng(12)
+    for _ in range(6):
+        rel = rng.uniform(-0.05, 0.05, 3)
+        a, _ = _test_mass_force(rel)
+        b, _ = _test_mass_force(-rel)
+        assert np.allclose(a, -b, rtol=1e-12, atol=0)
+        r = np.linalg.norm(rel)
+        newton = -rel / r ** 3
+        # the images' tidal field vanishes at the origin by cubic symmetry;
+        # what is left is the background's (4 pi / 3) m r / L^3, up to O(r^3)
+        lin = 4 * np.pi / 3 * rel
+        assert np.linalg.norm(a - newton - lin) < 0.02 * np.linalg.norm(lin)
+
+
+def test_periodic_far_box_equals_isolated(plummer):
+    # a Plummer sphere in a box 10^4 times its size: replicas and the Ewald
+    # sum change the forces by ~1e-12 of themselves
+    r, m, soft = plummer
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=4)
+    a0, p0, _, s0 = T.grav_all()
+    a1, p1, _, s1 = T.grav_all(nReps=1, period=[1e4] * 3, bEwald=True)
+    assert np.abs(a1 - a0).max() < 1e-9 * np.abs(a0).max()
+    # every replica is accepted as the root cell: 26 more Newtonian cells per
+    # active particle, the same particle pairs
+    assert s1["dPartSum"] == s0["dPartSum"] and s1["dSoftSum"] == s0["dSoftSum"]
+    assert s1["dCellSum"] == s0["dCellSum"] + 26 * s0["nActive"]
