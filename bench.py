@@ -119,6 +119,46 @@ def time_grav(gg, torch, n, prec, steps, warmup):
     return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter)), tb
 
 
+def periodic_box(n, seed=21):
+    """Half uniform, half in three Gaussian clumps (sigma 0.03), wrapped into
+    the periodic unit box [-1/2, 1/2)^3; equal masses 1/n."""
+    from gasoline_amd import ics
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.5, 0.5, (n // 2, 3))
+    c = rng.uniform(-0.5, 0.5, (3, 3))
+    k = rng.integers(0, 3, n - n // 2)
+    cl = c[k] + rng.normal(0, 0.03, (n - n // 2, 3))
+    return ics._f32((np.concatenate([u, cl]) + 0.5) % 1.0 - 0.5), np.full(n, 1.0 / n)
+
+
+def time_periodic(gg, torch, n, prec, steps, warmup):
+    """Periodic gravity with the reference's defaults for it: nReps 1,
+    hexadecapole walk and Ewald (iOrder = iEwOrder = 4), dEwCut 2.6, dEwhCut
+    2.8, theta 0.7; the same step without Ewald gives the Ewald pass's time."""
+    r, m = periodic_box(n)
+    pk = gg.Pkd(n, prec=prec)
+    pk.put("r", r)
+    pk.put("fMass", m)
+    pk.put("fSoft", np.full(n, 0.2 / n ** (1 / 3)))
+    pk.set_period([1.0, 1.0, 1.0])
+    pk.build_binary(16, 1, 0.7, 4)
+    out = {}
+    for ew in (True, False):
+        for _ in range(warmup):
+            st = pk.grav_all(4, nReps=1, periodic=True, bEwald=ew)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = pk.grav_all(4, nReps=1, periodic=True, bEwald=ew)
+        torch.cuda.synchronize()
+        out[ew] = ((time.perf_counter() - t0) / steps, st)
+    pk.close()
+    dt, st = out[True]
+    nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
+    return {"ms": dt * 1e3, "ms_without_ewald": out[False][0] * 1e3,
+            "interactions": nint, "interactions_per_s": nint / dt, "flop": st["dFlop"]}
+
+
 def time_ensemble(gg, torch, e, steps, prec):
     """Batched ensemble: nsys King models (W0 in [3, 9], N log-uniform in
     [1024, 65536]), eps = 0.005 King radii, theta 0.6, quadrupole; one tree
@@ -255,6 +295,8 @@ def main():
     ap.add_argument("--nevrard", type=int, default=1 << 24,
                     help="Evrard collapse particles of the full-step leg (0: skip)")
     ap.add_argument("--disk", type=int, default=1, help="disk-galaxy full-step leg (0: skip)")
+    ap.add_argument("--periodic", type=int, default=1 << 20,
+                    help="particles of the periodic-gravity (Ewald) leg (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run five device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprof FETCH_SIZE/WRITE_SIZE) and exit")
@@ -283,6 +325,15 @@ def main():
     g32 = time_grav(gg, torch, args.ngrav, gg.PREC_FP32, args.steps, args.warmup)
     g64 = time_grav(gg, torch, args.ngrav, gg.PREC_FP64, max(2, args.steps // 2), 1)
 
+    per = None
+    if args.periodic > 0:
+        p32 = time_periodic(gg, torch, args.periodic, gg.PREC_FP32, args.steps, args.warmup)
+        p64 = time_periodic(gg, torch, args.periodic, gg.PREC_FP64, max(2, args.steps // 2), 1)
+        per = {"workload": "periodic unit box N=%d (half uniform, half in three clumps of "
+                           "sigma 0.03), eps 0.2 n^-1/3, theta 0.7, hexadecapole walk, nReps 1, "
+                           "Ewald iEwOrder 4, dEwCut 2.6, dEwhCut 2.8; gg_grav_all (tree build "
+                           "excluded)" % args.periodic,
+               "fp32": p32, "fp64": p64, "speedup_vs_fp64": p64["ms"] / p32["ms"]}
     ens32 = ens64 = None
     if args.nsys > 0:
         from gasoline_amd import ics
@@ -397,6 +448,8 @@ def main():
             "fp32": ens32, "fp64": ens64,
             "speedup_vs_fp64": ens64["ms_graph"] / ens32["ms_graph"],
         }
+    if per:
+        line["periodic_gravity"] = per
     if evr:
         line["evrard"] = evr
     if disk:
