@@ -7,9 +7,9 @@ no CPU fallback: importing the binding without the built library raises.
 from .pkd import (GGError, Pkd, Smf, lib, FIELDS, PREC_FP32, PREC_FP64,
                   SMX_DENSITY, SMX_DENDVDX, SMX_SPHPRESSURETERMS,
                   TYPE_ACTIVE, TYPE_TREEACTIVE, TYPE_SMOOTHACTIVE, TYPE_SMOOTHDONE,
-                  TYPE_NbrOfACTIVE, TYPE_GAS)
+                  TYPE_NbrOfACTIVE, TYPE_GAS, TYPE_DARK, TYPE_STAR, tipsy_header)
 
 __all__ = ["GGError", "Pkd", "Smf", "lib", "FIELDS", "PREC_FP32", "PREC_FP64",
            "SMX_DENSITY", "SMX_DENDVDX", "SMX_SPHPRESSURETERMS", "TYPE_ACTIVE",
            "TYPE_TREEACTIVE", "TYPE_SMOOTHACTIVE", "TYPE_SMOOTHDONE",
-           "TYPE_NbrOfACTIVE", "TYPE_GAS"]
+           "TYPE_NbrOfACTIVE", "TYPE_GAS", "TYPE_DARK", "TYPE_STAR", "tipsy_header"]

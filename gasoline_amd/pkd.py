@@ -38,10 +38,20 @@ _INT_FIELDS = {"iActive": np.uint32, "iRung": np.int32}
 PREC_FP64, PREC_FP32 = 0, 1
 SMX_DENSITY, SMX_SPHPRESSURETERMS, SMX_DENDVDX = 1, 21, 22
 TYPE_ACTIVE, TYPE_TREEACTIVE, TYPE_SMOOTHACTIVE, TYPE_SMOOTHDONE = 1, 2, 4, 8
-TYPE_NbrOfACTIVE, TYPE_GAS = 32, 256
+TYPE_NbrOfACTIVE, TYPE_GAS, TYPE_DARK, TYPE_STAR = 32, 256, 512, 1024
+
+
+def tipsy_header(path, standard=True):
+    """(time, nbodies, nsph, ndark, nstar) of a Tipsy file (gg_tipsy_header)."""
+    t = C.c_double()
+    nb, ng, nd, ns = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _check(lib().gg_tipsy_header(os.fsencode(path), int(bool(standard)), C.byref(t), C.byref(nb),
+                                 C.byref(ng), C.byref(nd), C.byref(ns)))
+    return t.value, nb.value, ng.value, nd.value, ns.value
 
 # exported entry points (include/gasoline_amd.h)
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local", "gg_set_period",
+       "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
@@ -111,6 +121,9 @@ def lib():
     L.gg_set_local.argtypes = [_vp, C.c_int]
     L.gg_local.argtypes = [_vp]
     L.gg_set_period.argtypes = [_vp, _dp]
+    L.gg_tipsy_header.argtypes = [C.c_char_p, C.c_int, _dp, _ip, _ip, _ip, _ip]
+    L.gg_read_tipsy.argtypes = [_vp, C.c_char_p, C.c_int, C.c_double, C.c_double, _dp]
+    L.gg_write_tipsy.argtypes = [_vp, C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double]
     L.gg_put.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_get.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_put_aos.argtypes = [_vp, _vp, C.c_size_t, C.POINTER(AosField), C.c_int]
@@ -252,6 +265,18 @@ class Pkd:
     def build_binary(self, nBucket=16, iOpenType=1, dCrit=0.7, iOrder=2):
         _check(lib().gg_build_binary(self.h, int(nBucket), int(iOpenType), float(dCrit),
                                      int(iOrder), 0, 1))
+
+    def read_tipsy(self, path, standard=True, dvFac=1.0, dTuFac=1.0):
+        """gg_read_tipsy; returns the file's time."""
+        t = C.c_double()
+        _check(lib().gg_read_tipsy(self.h, os.fsencode(path), int(bool(standard)), dvFac, dTuFac,
+                                   C.byref(t)))
+        self.n = lib().gg_local(self.h)
+        return t.value
+
+    def write_tipsy(self, path, time, standard=True, dvFac=1.0, duTFac=1.0):
+        _check(lib().gg_write_tipsy(self.h, os.fsencode(path), int(bool(standard)), time, dvFac,
+                                    duTFac))
 
     def set_period(self, period):
         L = np.ascontiguousarray(period, np.float64)

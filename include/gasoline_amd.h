@@ -85,6 +85,8 @@ enum gg_field {
 #define GG_TYPE_SMOOTHDONE    (1u << 3)
 #define GG_TYPE_NbrOfACTIVE   (1u << 5)
 #define GG_TYPE_GAS           (1u << 8)
+#define GG_TYPE_DARK          (1u << 9)
+#define GG_TYPE_STAR          (1u << 10)
 
 /* Arithmetic of the interaction kernels. */
 #define GG_PREC_FP64  0   /* plain FP64 throughout */
@@ -136,6 +138,30 @@ typedef struct gg_aos_field {
  * write around each call. */
 int gg_put_aos(GGPKD pkd, const void *base, size_t stride, const gg_aos_field *f, int nf);
 int gg_get_aos(GGPKD pkd, void *base, size_t stride, const gg_aos_field *f, int nf);
+
+/* ------------------------------------------------------- Tipsy files */
+/* The header of a Tipsy file (struct dump, tipsydefs.h:38-45): bStandard
+ * selects XDR (big-endian, 32 bytes with a pad word) over the host's struct.
+ * Needs no device. */
+int gg_tipsy_header(const char *path, int bStandard, double *dTime, int *nBodies, int *nGas,
+                    int *nDark, int *nStar);
+/* pkdReadTipsy (pkd.c:297-640) of a whole file into the store (nLocal =
+ * nbodies <= nStore): gas, dark and star records in that order set r,
+ * v = dvFac vel (and vPred for gas), fMass, fSoft = eps, fPot = phi; gas
+ * also fDensity = rho and u = uPred = dTuFac temp.  iActive = the type bit
+ * only (TYPE_GAS, TYPE_DARK, TYPE_STAR), iRung 0, dt = dtNew = FLT_MAX,
+ * fBall2 0, as pkdReadTipsy leaves them (the caller then sets activity as
+ * msrActiveType does).  Metals and star formation times are not carried.
+ * *pdTime receives the header's time. */
+int gg_read_tipsy(GGPKD pkd, const char *path, int bStandard, double dvFac, double dTuFac,
+                  double *pdTime);
+/* pkdWriteTipsy (pkd.c:1628-1800) with the header msrWriteTipsy writes:
+ * gas, dark and star records (a particle's type by its bits, dark first as
+ * the reference tests them), each type in store order; vel = dvFac v, gas
+ * temp = duTFac u (the non-cooling gas models), eps = fSoft; metals and
+ * formation times are written as 0. */
+int gg_write_tipsy(GGPKD pkd, const char *path, int bStandard, double dTime, double dvFac,
+                   double duTFac);
 
 /* Select FP32 or FP64 interaction arithmetic (default GG_PREC_FP32). */
 int gg_set_precision(GGPKD pkd, int prec);
