@@ -119,6 +119,11 @@ def time_grav(gg, torch, n, prec, steps, warmup):
     return dt, nint, st["dFlop"], float(np.mean(walk)), float(np.mean(inter)), tb
 
 
+def log(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def periodic_box(n, seed=21):
     """Half uniform, half in three Gaussian clumps (sigma 0.03), wrapped into
     the periodic unit box [-1/2, 1/2)^3; equal masses 1/n."""
@@ -322,8 +327,10 @@ def main():
     pairs = 2 * K * n                      # density pass + force pass
     t32, ph32 = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup)
     t64, ph64 = time_sph(gg, torch, n, gg.PREC_FP64, max(2, args.steps // 2), 1)
+    log("SPH box done")
     g32 = time_grav(gg, torch, args.ngrav, gg.PREC_FP32, args.steps, args.warmup)
     g64 = time_grav(gg, torch, args.ngrav, gg.PREC_FP64, max(2, args.steps // 2), 1)
+    log("gravity done")
 
     per = None
     if args.periodic > 0:
@@ -334,12 +341,14 @@ def main():
                            "Ewald iEwOrder 4, dEwCut 2.6, dEwhCut 2.8; gg_grav_all (tree build "
                            "excluded)" % args.periodic,
                "fp32": p32, "fp64": p64, "speedup_vs_fp64": p64["ms"] / p32["ms"]}
+        log("periodic gravity done")
     ens32 = ens64 = None
     if args.nsys > 0:
         from gasoline_amd import ics
         e = ics.king_ensemble(args.nsys, seed=7)
         ens32 = time_ensemble(gg, torch, e, args.steps, gg.PREC_FP32)
         ens64 = time_ensemble(gg, torch, e, max(2, args.steps // 2), gg.PREC_FP64)
+        log("ensemble done")
     evr = None
     if args.nevrard > 0:
         from gasoline_amd import ics
@@ -347,7 +356,9 @@ def main():
         soft = np.full(args.nevrard, 0.004)
         ks = max(2, args.steps // 2)
         fast = time_kdk(gg, torch, d, soft, None, gg.PREC_FP32, "mlp", 64, 1e-3, 0.7, ks, 3)
+        log("Evrard FP32 done")
         plain = time_kdk(gg, torch, d, soft, None, gg.PREC_FP64, "cold", 64, 1e-3, 0.7, ks, 1)
+        log("Evrard FP64 done")
         evr = {"workload": "Evrard collapse N=%d (rho ~ 1/r, M=R=G=1, u=0.05, gamma 5/3), "
                            "eps 0.004, theta 0.7, quadrupole, bucket 16, nSmooth 64, dt 0.001 "
                            "(about the central Courant limit); "
@@ -373,6 +384,7 @@ def main():
                 "fp32": f32, "fp64": f64, "speedup_vs_fp64": f64["ms_per_step"] / f32["ms_per_step"],
                 "force_error": force_errors(gg, d, d["soft"], bits, 0.7)}
         del d
+        log("disk galaxy done")
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream; the dominant one is reported
     kern = {
@@ -457,7 +469,9 @@ def main():
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v_sph, el_sph = cpu_baseline_sph()
+        log("CPU baseline (SPH) done")
         v_gr, el_gr = cpu_baseline_grav()
+        log("CPU baseline (gravity) done")
         line["cpu_baseline"] = {
             "value": v_sph, "unit": "SPH pair-evals/s", "cores": cores, "kind": "port",
             "sample": "oracle SPH step on the benchmark's box, 1048576 particles (%.1f s): "
