@@ -1,10 +1,12 @@
 // Tipsy files in and out of the store: the header (tipsydefs.h:38-45 read
 // and written by msrReadTipsy/msrWriteTipsy), pkdReadTipsy (pkd.c:297-640)
 // and pkdWriteTipsy (pkd.c:1628-1800) for one process holding every
-// particle.  Standard files are XDR (big-endian; a 32-byte header with a pad
-// word; gas 48, dark 36, star 44 bytes per particle), native ones the
-// host's structs (sizeof(struct dump) = 32 on x86-64).  Host code: the file
-// is read and written in chunks; the fields move with gg_put / gg_get.
+// particle; and the ASCII output arrays of msrOutArray/msrOutVector
+// (master.c:5420-5545, pkdOutVector outtype.c:806-1000, iBinaryOutput 0).
+// Standard files are XDR (big-endian; a 32-byte header with a pad word; gas
+// 48, dark 36, star 44 bytes per particle), native ones the host's structs
+// (sizeof(struct dump) = 32 on x86-64).  Host code: the file is read and
+// written in chunks; the fields move with gg_put / gg_get.
 #include "gg_internal.h"
 
 #include <cfloat>
@@ -285,6 +287,25 @@ extern "C" int gg_write_tipsy(GGPKD pkd, const char *path, int bStandard, double
         if (xdr) decode_floats(rec.data(), rec.size(), true);
         ok = ok && fwrite(rec.data(), sizeof(float), rec.size(), fp) == rec.size();
     }
+    ok = (fclose(fp) == 0) && ok;
+    GG_CHECK(ok, GG_EINVAL, "writing %s failed", path);
+    return GG_OK;
+}
+
+extern "C" int gg_out_array(GGPKD pkd, const char *path, int field)
+{
+    GG_CHECK(pkd && path, GG_EINVAL, "null argument");
+    GG_CHECK(field >= 0 && field < GG_NFIELDS && !field_is32(field), GG_EINVAL,
+             "field %d is not an FP64 field", field);
+    const int n = pkd->nLocal, w = field_width(field);
+    std::vector<double> v((size_t)w * std::max(n, 1));
+    GG_TRY(gg_get(pkd, field, v.data(), 0));
+    FILE *fp = fopen(path, "w");
+    GG_CHECK(fp, GG_EINVAL, "cannot create %s", path);
+    bool ok = fprintf(fp, "%d\n", n) > 0;
+    // a vector is written a component at a time (iDim 0, 1, 2)
+    for (int d = 0; d < w && ok; ++d)
+        for (int i = 0; i < n && ok; ++i) ok = fprintf(fp, "%.14g\n", v[(size_t)w * i + d]) > 0;
     ok = (fclose(fp) == 0) && ok;
     GG_CHECK(ok, GG_EINVAL, "writing %s failed", path);
     return GG_OK;

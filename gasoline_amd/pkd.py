@@ -51,7 +51,7 @@ def tipsy_header(path, standard=True):
 
 # exported entry points (include/gasoline_amd.h)
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local", "gg_set_period",
-       "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy",
+       "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy", "gg_out_array",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
@@ -124,6 +124,7 @@ def lib():
     L.gg_tipsy_header.argtypes = [C.c_char_p, C.c_int, _dp, _ip, _ip, _ip, _ip]
     L.gg_read_tipsy.argtypes = [_vp, C.c_char_p, C.c_int, C.c_double, C.c_double, _dp]
     L.gg_write_tipsy.argtypes = [_vp, C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.gg_out_array.argtypes = [_vp, C.c_char_p, C.c_int]
     L.gg_put.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_get.argtypes = [_vp, C.c_int, _vp, C.c_int]
     L.gg_put_aos.argtypes = [_vp, _vp, C.c_size_t, C.POINTER(AosField), C.c_int]
@@ -277,6 +278,10 @@ class Pkd:
     def write_tipsy(self, path, time, standard=True, dvFac=1.0, duTFac=1.0):
         _check(lib().gg_write_tipsy(self.h, os.fsencode(path), int(bool(standard)), time, dvFac,
                                     duTFac))
+
+    def out_array(self, path, name):
+        """gg_out_array: the field as the reference's ASCII output array."""
+        _check(lib().gg_out_array(self.h, os.fsencode(path), FIELDS[name][0]))
 
     def set_period(self, period):
         L = np.ascontiguousarray(period, np.float64)

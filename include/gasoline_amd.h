@@ -163,6 +163,14 @@ int gg_read_tipsy(GGPKD pkd, const char *path, int bStandard, double dvFac, doub
 int gg_write_tipsy(GGPKD pkd, const char *path, int bStandard, double dTime, double dvFac,
                    double duTFac);
 
+/* An output array as msrOutArray / msrOutVector write it in ASCII
+ * (iBinaryOutput 0; master.c:5420-5545, pkdOutVector outtype.c:946-960):
+ * the particle count on the first line, then one value per line with
+ * "%.14g", a vector field component by component (all x, all y, all z).
+ * The caller picks the field its output type shows (fDensity for .den, fPot
+ * for .pot, a for .acc ...); values in store order. */
+int gg_out_array(GGPKD pkd, const char *path, int field);
+
 /* Select FP32 or FP64 interaction arithmetic (default GG_PREC_FP32). */
 int gg_set_precision(GGPKD pkd, int prec);
 

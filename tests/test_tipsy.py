@@ -117,3 +117,25 @@ def test_write_groups_types_and_checks_bits(gpu, tmp_path):
         pk.put("iActive", np.full(n, 7, np.uint32))      # no type bit
         with pytest.raises(gg.GGError, match="type bit"):
             pk.write_tipsy(tmp_path / "x.std", 1.0)
+
+
+@pytest.mark.gpu
+def test_ascii_output_arrays(gpu, tmp_path):
+    # the count, then "%.14g" values; a vector component by component
+    rng = np.random.default_rng(9)
+    n = 1000
+    a = rng.normal(size=(n, 3)) * 10.0 ** rng.integers(-20, 20, (n, 3))
+    rho = rng.uniform(size=n)
+    with gpu.Pkd(n) as pk:
+        pk.put("a", a)
+        pk.put("fDensity", rho)
+        pk.out_array(tmp_path / "x.acc", "a")
+        pk.out_array(tmp_path / "x.den", "fDensity")
+    lines = (tmp_path / "x.acc").read_text().split("\n")
+    assert lines[0] == str(n) and lines[-1] == ""
+    assert lines[1:-1] == ["%.14g" % x for x in a.T.reshape(-1)]
+    lines = (tmp_path / "x.den").read_text().split("\n")
+    assert lines[0] == str(n) and lines[1:-1] == ["%.14g" % x for x in rho]
+    with pytest.raises(gg.GGError, match="not an FP64 field"):
+        with gpu.Pkd(n) as pk:
+            pk.out_array(tmp_path / "x.iord", "iActive")
