@@ -274,6 +274,23 @@ __global__ void k_dt_to_rung(int n, int iRung, double dDelta, int iMaxRung, int 
     }
 }
 
+// pkdActiveMaskRung (pkd.c:5228-5252): the mask bits set on the particles
+// of rung iRung (and above with bGreater), cleared on the others.
+__global__ void k_active_rung(int n, uint32_t mask, int iRung, int bGreater,
+                              const int *__restrict__ rung, uint32_t *__restrict__ bits,
+                              int *__restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool on = false;
+    if (i < n) {
+        const int r = rung[i];
+        on = r == iRung || (bGreater && r > iRung);
+        bits[i] = on ? (bits[i] | mask) : (bits[i] & ~mask);
+    }
+    const int c = __popcll(__ballot(on));
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
 __global__ void k_rung_count(int n, const int *__restrict__ rung, int *__restrict__ red)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -546,5 +563,30 @@ extern "C" int gg_dt_to_rung(GGPKD pkd, int iRung, double dDelta, int iMaxRung, 
     *piMaxRungOut = red[1];
     *pnMaxRung = red[3];
     *piMaxRungIdeal = red[2];
+    return GG_OK;
+}
+
+extern "C" int gg_active_mask_rung(GGPKD pkd, unsigned iSetMask, int iRung, int bGreater,
+                                   int *pnActive)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    const int n = pkd->nLocal;
+    if (pnActive) *pnActive = 0;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_IRUNG, GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(sizeof(int)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, sizeof(int), pkd->stream));
+    hipLaunchKernelGGL(k_active_rung, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
+                       (uint32_t)iSetMask, iRung, bGreater, pkd->f[GG_IRUNG].as<int>(),
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), pkd->red.as<int>());
+    GG_HIP(hipGetLastError());
+    int c = 0;
+    GG_HIP(hipMemcpyAsync(&c, pkd->red.p, sizeof(int), hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    // the type bits changed: trees built on them are stale
+    if (iSetMask & (GG_TYPE_ACTIVE | GG_TYPE_TREEACTIVE)) pkd->grav.built = 0;
+    pkd->sph.gasMaskValid = 0;
+    if (iSetMask & (GG_TYPE_TREEACTIVE | GG_TYPE_SMOOTHACTIVE)) pkd->sph.built = 0;
+    if (pnActive) *pnActive = c;
     return GG_OK;
 }

@@ -81,3 +81,70 @@ def start(pk, d, soft, bits=None):
     if bits is not None:
         pk.put("iActive", bits)
     return pk
+
+
+class RungKDK:
+    """msrTopStepKDK (master.c:8242-8400) for collisionless gravity: block
+    time steps on rungs 0..iMaxRung-1, particle rungs from the timestep
+    criteria (gg_init_dt, gg_grav_step, gg_accel_step, gg_dt_to_rung) at each
+    adjust, an open and a close kick of half the rung's step for the
+    particles on that rung, recursion into the next rung while any particle
+    is deeper, and one drift of every particle plus a force evaluation for
+    the rungs from iKickRung down at the bottom.  Kicks are those of
+    msrKickKDKOpen/Close without comoving factors (dvFacOne 1, dvFacTwo the
+    half step).  bAdjust = False keeps the rungs the store holds."""
+
+    def __init__(self, pk, iMaxRung=8, dEta=0.2, theta=0.7, iOrder=2, nBucket=16,
+                 bGravStep=True, bAccelStep=False, bAdjust=True):
+        self.pk, self.iMaxRung, self.dEta = pk, iMaxRung, dEta
+        self.theta, self.iOrder, self.nBucket = theta, iOrder, nBucket
+        self.bGravStep, self.bAccelStep, self.bAdjust = bGravStep, bAccelStep, bAdjust
+        self.iCurrMaxRung = 0
+        self.force_calls = []        # (iKickRung, particles active) of each force evaluation
+
+    def forces(self, iKickRung):
+        """Accelerations of the particles on rungs >= iKickRung (the rest
+        keep theirs); all particles are sources."""
+        pk = self.pk
+        n = pk.active_rung(iKickRung, True)
+        self.force_calls.append((iKickRung, n))
+        if n:
+            pk.init_accel()
+            pk.build_binary(self.nBucket, 1, self.theta, self.iOrder)
+            pk.grav_all(self.iOrder)
+
+    def adjust(self, iRung, dDelta):
+        pk = self.pk
+        pk.active_rung(iRung, True)
+        pk.init_dt(dDelta)
+        if self.bGravStep:
+            pk.grav_step(self.dEta)
+        if self.bAccelStep:
+            pk.accel_step(self.dEta)
+        top, _, _ = pk.dt_to_rung(iRung, dDelta, self.iMaxRung, bAll=True)
+        self.iCurrMaxRung = top
+
+    def start(self):
+        """Forces of every particle before the first step (msrInitAccel +
+        msrGravity of the start-up); the rungs come from the first adjust."""
+        self.forces(0)
+        if not self.bAdjust:
+            r = self.pk.get("iRung")
+            self.iCurrMaxRung = int(r.max()) if len(r) else 0
+
+    def top_step(self, dDelta, iRung=0, iKickRung=0, iAdjust=True):
+        pk = self.pk
+        if self.bAdjust and iAdjust and iRung < self.iMaxRung - 1:
+            self.adjust(iRung, dDelta)
+        pk.active_rung(iRung, False)
+        h = 0.5 * dDelta
+        pk.kick(1.0, h, 1.0, h, h, h)                  # open kick of this rung
+        if self.iCurrMaxRung > iRung:
+            self.top_step(h, iRung + 1, iRung + 1, False)
+            pk.active_rung(iRung, False)
+            self.top_step(h, iRung + 1, iKickRung, True)
+        else:
+            pk.drift(dDelta)
+            self.forces(iKickRung)
+        pk.active_rung(iRung, False)
+        pk.kick(1.0, h, 1.0, h, h, h)                  # close kick of this rung

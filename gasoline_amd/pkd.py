@@ -58,7 +58,7 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
        "gg_kick_vpred", "gg_init_dt", "gg_grav_step", "gg_accel_step", "gg_density_step",
-       "gg_sph_step", "gg_dt_to_rung",
+       "gg_sph_step", "gg_dt_to_rung", "gg_active_mask_rung",
        "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
        "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
 
@@ -163,6 +163,7 @@ def lib():
     L.gg_sph_step.argtypes = [_vp] + [C.c_double] * 7 + [C.c_int, _dp]
     L.gg_dt_to_rung.argtypes = [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _ip, _ip,
                                 _ip]
+    L.gg_active_mask_rung.argtypes = [_vp, C.c_uint, C.c_int, C.c_int, _ip]
     _fp = C.POINTER(C.c_float)
     L.gg_hmlp_init.argtypes = [_vp, C.c_uint]
     L.gg_hmlp_record.argtypes = [_vp, C.c_double]
@@ -452,6 +453,13 @@ class Pkd:
                                  dEtaDiffusion, dResolveJeans, int(bViscosityLimitdt),
                                  C.byref(m)))
         return m.value
+
+    def active_rung(self, iRung, bGreater, mask=TYPE_ACTIVE):
+        """gg_active_mask_rung; returns the number of particles set."""
+        n = C.c_int()
+        _check(lib().gg_active_mask_rung(self.h, int(mask), int(iRung), int(bool(bGreater)),
+                                         C.byref(n)))
+        return n.value
 
     def dt_to_rung(self, iRung, dDelta, iMaxRung, bAll=True, bDiagExceed=False):
         """Returns (iMaxRungOut, nMaxRung, iMaxRungIdeal)."""

@@ -373,6 +373,13 @@ int gg_drift(GGPKD pkd, double dDelta, const double *fCenter, int bPeriodic,
 int gg_kick_vpred(GGPKD pkd, double dvFacOne, double dvFacTwo, double duDelta, int bIsothermal);
 
 /* ------------------------------------------- timesteps and rungs */
+
+/* pkdActiveMaskRung (pkd.c:5228; pkdActiveRung pkd.c:4569 is iSetMask =
+ * TYPE_ACTIVE): the bits of iSetMask set on the particles of rung iRung
+ * (and above when bGreater) and cleared on the others; *pnActive (may be
+ * null) receives their number.  A gravity tree has to be built again
+ * afterwards, as gg_put of iActive requires. */
+int gg_active_mask_rung(GGPKD pkd, unsigned iSetMask, int iRung, int bGreater, int *pnActive);
 /* These update dt (and iRung, dtOld) of the store like their reference
  * functions, bit for bit: no contraction into FMA. */
 
