@@ -194,6 +194,8 @@ extern "C" int gg_read_tipsy(GGPKD pkd, const char *path, int bStandard, double 
     fclose(fp);
     if (rc != GG_OK) return rc;
     GG_TRY(gg_set_local(pkd, n));
+    if (pdTime) *pdTime = h.time;
+    if (n == 0) return GG_OK;
     GG_TRY(gg_put(pkd, GG_R, S.r.data(), 0));
     GG_TRY(gg_put(pkd, GG_V, S.v.data(), 0));
     GG_TRY(gg_put(pkd, GG_VPRED, S.vPred.data(), 0));
@@ -218,8 +220,9 @@ extern "C" int gg_write_tipsy(GGPKD pkd, const char *path, int bStandard, double
     GG_CHECK(pkd && path, GG_EINVAL, "null argument");
     const bool xdr = bStandard != 0;
     const int n = pkd->nLocal;
-    std::vector<double> r(3 * (size_t)n), v(3 * (size_t)n), m(n), soft(n), rho(n), u(n), pot(n);
-    std::vector<uint32_t> bits(n);
+    const size_t nn = (size_t)std::max(n, 1);   // non-empty buffers for gg_get
+    std::vector<double> r(3 * nn), v(3 * nn), m(nn), soft(nn), rho(nn), u(nn), pot(nn);
+    std::vector<uint32_t> bits(nn);
     GG_TRY(gg_get(pkd, GG_R, r.data(), 0));
     GG_TRY(gg_get(pkd, GG_V, v.data(), 0));
     GG_TRY(gg_get(pkd, GG_MASS, m.data(), 0));

@@ -139,3 +139,13 @@ def test_ascii_output_arrays(gpu, tmp_path):
     with pytest.raises(gg.GGError, match="not an FP64 field"):
         with gpu.Pkd(n) as pk:
             pk.out_array(tmp_path / "x.iord", "iActive")
+
+
+@pytest.mark.gpu
+def test_empty_file(gpu, tmp_path):
+    a, b = tmp_path / "e.std", tmp_path / "f.std"
+    write_file(a, True, 0, 0, 0, time=0.5)
+    with gpu.Pkd(10) as pk:
+        assert pk.read_tipsy(a) == 0.5 and pk.n == 0
+        pk.write_tipsy(b, 0.5)
+    assert a.read_bytes() == b.read_bytes()
