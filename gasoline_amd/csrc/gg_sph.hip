@@ -847,21 +847,43 @@ __device__ uint64_t wave_select(const uint64_t (&v)[KE], const bool (&ok)[KE], i
 // Returns false (nothing written) if the selection does not come to K.
 template <int KE>
 __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, const int *bs,
-                           int lane)
+                           double T, int *hist, int lane)
 {
     const int K = A.K;
     uint64_t v[KE];
     bool ok[KE];
     int s[KE];
+    // 64 bins of d2 / T (every d2 of the ball is <= T): floor(64 d2 / T) is
+    // monotone in d2, so each bin is a contiguous range of keys and the K-th
+    // smallest lies in the first bin where the running count reaches K
+    const double inv = T > 0 ? 64.0 / T : 0.0;
+    hist[lane] = 0;
+    wave_sync();
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
         const int e = lane + 64 * k;
         ok[k] = e < cnt;
-        v[k] = ok[k] ? (uint64_t)__double_as_longlong(bd[e]) : ~0ull;
+        const double d = ok[k] ? bd[e] : 0.0;
+        v[k] = ok[k] ? (uint64_t)__double_as_longlong(d) : ~0ull;
         s[k] = ok[k] ? bs[e] : -1;
+        if (ok[k]) atomicAdd(&hist[min(63, (int)(d * inv))], 1);
     }
+    wave_sync();
+    const int h = hist[lane];
+    int incl = h;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    const int b = __ffsll((long long)__ballot(incl >= K)) - 1;   // cnt >= K: exists
+    const int below = __shfl(incl - h, b);       // keys in the bins under b
+    bool okb[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k)   // the bin again, from the key (fewer registers)
+        okb[k] = ok[k] && min(63, (int)(__longlong_as_double((long long)v[k]) * inv)) == b;
     int nlt, neq;
-    const uint64_t pK = wave_select<KE>(v, ok, K - 1, &nlt, &neq);
+    const uint64_t pK = wave_select<KE>(v, okb, K - 1 - below, &nlt, &neq);
+    nlt += below;            // equal keys share a bin, so neq is whole
     bool take[KE];
     if (nlt + neq == K) {
 #pragma unroll
@@ -925,7 +947,7 @@ struct KLGroup {
 template <bool IMG>
 __device__ int knn_settle(const QArgs &A, const KLGroup &G, const int2 *sel, int nsel,
                           double x, double y, double z, double Tsel, double *bd, int *bs,
-                          int lane, int *cnt_out, unsigned &ncand)
+                          int lane, int *cnt_out, double *T_out, unsigned &ncand)
 {
     const int K = A.K;
     const int ns = (nsel + 1) >> 1;    // slots in use (wave-uniform)
@@ -989,6 +1011,7 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const int2 *sel, int
         }
     }
     *cnt_out = cnt;
+    *T_out = T;
     return 0;
 }
 
@@ -999,7 +1022,7 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const int2 *sel, int
 template <bool IMG>
 __device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, double x,
                           double y, double z, double Tsel, double RL2, double *bd, int *bs,
-                          int lane, unsigned &npass, unsigned &ncand)
+                          int *hist, int lane, unsigned &npass, unsigned &ncand)
 {
     const int K = A.K;
     for (int pass = 0; pass < 8; ++pass) {
@@ -1026,7 +1049,9 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, do
             continue;
         }
         int cnt = 0;
-        const int st = knn_settle<IMG>(A, G, sel, nsel, x, y, z, Tsel, bd, bs, lane, &cnt, ncand);
+        double Tball = 0;
+        const int st = knn_settle<IMG>(A, G, sel, nsel, x, y, z, Tsel, bd, bs, lane, &cnt, &Tball,
+                                       ncand);
         if (st == 1 && Tsel < RL2) {   // widen the ball towards the group radius
             const double f = cnt > 0 ? cbrt((1.5 * K / cnt) * (1.5 * K / cnt)) : 4.0;
             Tsel = fmin(RL2, Tsel * fmin(fmax(f, 1.2), 16.0));
@@ -1038,8 +1063,8 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, do
             return false;
         }
         wave_sync();
-        const bool done = cnt <= 128 ? knn_select<2>(A, q, cnt, bd, bs, lane)
-                                     : knn_select<4>(A, q, cnt, bd, bs, lane);
+        const bool done = cnt <= 128 ? knn_select<2>(A, q, cnt, bd, bs, Tball, hist, lane)
+                                     : knn_select<4>(A, q, cnt, bd, bs, Tball, hist, lane);
         if (!done && lane == 0) atomicAdd(&A.err[4], 1u);
         return done;
     }
@@ -1056,7 +1081,7 @@ __device__ __forceinline__ double readlane_d(double v, int l)
 }
 
 // One wavefront per bucket.
-__global__ __launch_bounds__(64) void k_knn_bucket(KLArgs KA)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_knn_bucket(KLArgs KA)
 {
     const QArgs &A = KA.Q;
     __shared__ float lbx[KL_MB][6];
@@ -1064,6 +1089,7 @@ __global__ __launch_bounds__(64) void k_knn_bucket(KLArgs KA)
     __shared__ int2 sel[KL_SB];                 // per query: first slot, count of its buckets
     __shared__ double bd[CAP];
     __shared__ int bs[CAP];
+    __shared__ int hist[64];                    // the selection's bins
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
@@ -1132,8 +1158,8 @@ __global__ __launch_bounds__(64) void k_knn_bucket(KLArgs KA)
         // first ball: slack * fBall2 from the hint, else 0.8 RL^2 (~1.6K expected)
         const double Tsel = gq > 0 ? fmin(gq * A.slack, RL2) : 0.8 * RL2;
         const bool done =
-            img ? knn_query<true>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand)
-                : knn_query<false>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand);
+            img ? knn_query<true>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, hist, lane, npass, ncand)
+                : knn_query<false>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, hist, lane, npass, ncand);
         if (!done && lane == 0) {
             const unsigned p = atomicAdd(KA.nretry, 1u);
             KA.retry[p] = q;
