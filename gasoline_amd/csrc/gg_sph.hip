@@ -1217,13 +1217,13 @@ struct DArgs {
     unsigned *err;                           // err[7]: an extras list overflowed
 };
 
-// DenDVDX (smoothfcn.c:3390-3627): 16 lanes per particle over its list,
+// DenDVDX (smoothfcn.c:3390-3627): 8 lanes per particle over its list,
 // pair arithmetic in T, sums in FP64.
 template <typename T>
 __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
 {
-    const int g = (blockIdx.x * 256 + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
+    const int g = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int sub = threadIdx.x & 7;
     if (g >= A.n) return;
     const int i = g;
     if (!A.qflag[i]) return;
@@ -1248,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
     const int64_t o = A.off[i], e1 = A.off[i + 1];
-    for (int64_t e = o + sub; e < e1; e += 16) {
+    for (int64_t e = o + sub; e < e1; e += 8) {
         const int q = A.idx[e];
         const double dxd = img_sep(x, A.sx[q], A.per, A.L[0]);
         const double dyd = img_sep(y, A.sy[q], A.per, A.L[1]);
@@ -1303,7 +1303,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
             gx += (double)(uq * dx * rs1); gy += (double)(uq * dy * rs1); gz += (double)(uq * dz * rs1);
         }
     }
-    for (int w = 1; w < 16; w <<= 1) {
+    for (int w = 1; w < 8; w <<= 1) {
         rho += __shfl_xor(rho, w);
         qact |= __shfl_xor(qact, w);
         if (A.op == GG_SMX_DENDVDX) {
@@ -1370,12 +1370,12 @@ struct DSArgs {
 // list with h_i and m_q, then over the owners q whose lists hold i (sorted by
 // slot, so the order is fixed) the half-normalised kernel with h_q and m_q,
 // on top of 0 (smooth-active i, initDensity smoothfcn.c:229-232) or of the
-// density i held.  16 lanes per particle; pair arithmetic in T, sums in FP64.
+// density i held.  8 lanes per particle; pair arithmetic in T, sums in FP64.
 template <typename T>
 __global__ __launch_bounds__(256) void k_density(DSArgs A)
 {
-    const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
+    const int i = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int sub = threadIdx.x & 7;
     if (i >= A.n) return;
     const bool own = A.qflag[i] != 0;
     if (!A.sym && !own) return;
@@ -1383,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
     double so = 0, sr = 0, ih2 = 0;
     if (own) {
         ih2 = 4.0 / A.fb2[i];
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 8) {
             const int q = A.idx[e];
             const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
                                       img_sep(y, A.sy[q], A.per, A.L[1]),
@@ -1392,7 +1392,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
         }
     }
     if (A.sym) {
-        for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 16) {
+        for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 8) {
             const int q = A.ridx[e];
             const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
                                       img_sep(y, A.sy[q], A.per, A.L[1]),
@@ -1402,7 +1402,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
             sr += fq * (double)(m4_w<T>((T)(d2 * ih2q)) * (T)A.m[q]);
         }
     }
-    for (int w = 1; w < 16; w <<= 1) {
+    for (int w = 1; w < 8; w <<= 1) {
         so += __shfl_xor(so, w);
         sr += __shfl_xor(sr, w);
     }
@@ -1711,15 +1711,15 @@ struct H2Args {
     double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
 };
 
-// Gather form of the symmetric force pass, 16 lanes per particle i.  Over
+// Gather form of the symmetric force pass, 8 lanes per particle i.  Over
 // i's own list: the pair as p = i, and, when i lies in the neighbour's ball,
 // the pair as q = i.  Over i's extras (particles whose balls hold i but which
 // are not in i's list): the pair as q = i.
 template <typename T>
 __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restrict__ rec)
 {
-    const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
+    const int i = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int sub = threadIdx.x & 7;
     if (i >= A.n) return;
     HPar<T> hp;
     hp.a = (T)A.a;
@@ -1734,7 +1734,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     const double xi = A.sx[i], yi = A.sy[i], zi = A.sz[i];
     HAcc acc;
     if (A.qflag[i]) {
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 8) {
             const int q = A.idx[e];
             const HRec<T> Q = rec[q];
             const double xq = A.sx[q], yq = A.sy[q], zq = A.sz[q];
@@ -1754,7 +1754,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     }
     const int64_t e0 = A.eoff ? A.eoff[i] : (int64_t)i * A.ecap;
     const int64_t e1 = A.eoff ? A.eoff[i + 1] : e0 + A.ecnt[i];
-    for (int64_t e = e0 + sub; e < e1; e += 16) {
+    for (int64_t e = e0 + sub; e < e1; e += 8) {
         const int p = A.eidx[e];
         const HRec<T> P = rec[p];
         const double ex = img_sep(A.sx[p], xi, A.per, A.L[0]);
@@ -1762,7 +1762,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
         const double ez = img_sep(A.sz[p], zi, A.per, A.L[2]);
         hpair<T>(P, I, (T)ex, (T)ey, (T)ez, (T)sq_dist(ex, ey, ez), 1, hp, acc);
     }
-    for (int w = 1; w < 16; w <<= 1) {
+    for (int w = 1; w < 8; w <<= 1) {
         acc.ax += __shfl_xor(acc.ax, w);
         acc.ay += __shfl_xor(acc.ay, w);
         acc.az += __shfl_xor(acc.az, w);
@@ -2300,7 +2300,7 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
         D.rho0 = S.rho.as<double>();
     }
     timer_mark(pkd, 4);
-    const int grid = nblk(n * 16, 256);
+    const int grid = nblk(n * 8, 256);
     if (pkd->prec == GG_PREC_FP32)
         hipLaunchKernelGGL(k_density<float>, dim3(grid), dim3(256), 0, st, D);
     else
@@ -2396,7 +2396,7 @@ static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, cons
     D.eidx = S.eidx.as<int>();
     D.ecap = ECAP;
     D.err = S.err.as<unsigned>();
-    const int grid = nblk(n * 16, 256);
+    const int grid = nblk(n * 8, 256);
     if (pkd->prec == GG_PREC_FP32)
         hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
     else
@@ -2678,7 +2678,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.kDiff = smf->dThermalDiffusionCoeff;
     H.bCannonical = smf->bCannonical;
     H.out = S.res.as<double>();
-    const int grid = nblk(n * 16, 256);
+    const int grid = nblk(n * 8, 256);
     timer_mark(pkd, 3);
     if (f32)
         hipLaunchKernelGGL(k_hydro2<float>, dim3(grid), dim3(256), 0, st, H,
