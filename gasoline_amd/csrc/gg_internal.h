@@ -178,6 +178,18 @@ struct Timers {
     double ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+
+// Workgroups are dealt to the eight XCDs in turn (workgroup i to XCD i mod 8),
+// each XCD with its own L2.  This maps the dispatch index to a logical block
+// so that every XCD serves one contiguous range of logical blocks: with the
+// work in Morton order the data an XCD reads stays in its own L2.  A
+// bijection of [0, nblocks) whatever nblocks is.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks)
+{
+    const int q = nblocks >> 3, r = nblocks & 7, x = bid & 7;
+    return x * q + (x < r ? x : r) + (bid >> 3);
+}
+
 }  // namespace gg
 
 struct gg_pkd_s {

@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ int bs[CAP];
     __shared__ int hist[64];                    // the selection's bins
     const int lane = threadIdx.x;
-    const int g = blockIdx.x;
+    const int g = xcd_block(blockIdx.x, gridDim.x);
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
     const int nb = KA.gnb[g];
     if (nb < 0) {          // group too large: every query retries
@@ -1222,7 +1222,7 @@ struct DArgs {
 template <typename T>
 __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
 {
-    const int g = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int g = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
     const int sub = threadIdx.x & 7;
     if (g >= A.n) return;
     const int i = g;
@@ -1374,7 +1374,7 @@ struct DSArgs {
 template <typename T>
 __global__ __launch_bounds__(256) void k_density(DSArgs A)
 {
-    const int i = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
     const int sub = threadIdx.x & 7;
     if (i >= A.n) return;
     const bool own = A.qflag[i] != 0;
@@ -1718,7 +1718,7 @@ struct H2Args {
 template <typename T>
 __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restrict__ rec)
 {
-    const int i = (blockIdx.x * 256 + threadIdx.x) >> 3;
+    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
     const int sub = threadIdx.x & 7;
     if (i >= A.n) return;
     HPar<T> hp;
