@@ -789,7 +789,7 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
     __shared__ int qs[WI_Q], qn[WI_Q], qpl[WI_Q], qpn[WI_Q];
     __shared__ int pidx[64];
     const int lane = threadIdx.x;
-    const int b = blockIdx.x;
+    const int b = xcd_chunk(blockIdx.x, gridDim.x, 128);
     const int ib = A.bucket[b];
     const WalkNode bn = A.wn[ib];
     Tgt<T> g;

@@ -190,6 +190,19 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
     return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
+// The same dealing in chunks of c logical blocks: XCD x serves chunks x,
+// x + 8, x + 16, ... (blocks past the last whole round of 8 chunks keep their
+// index).  Neighbouring blocks still share an L2 while uneven work (a dense
+// core in Morton order) spreads over all eight XCDs.  A bijection of
+// [0, nblocks).
+__device__ __forceinline__ int xcd_chunk(int bid, int nblocks, int c)
+{
+    const int whole = nblocks / (8 * c) * (8 * c);
+    if (bid >= whole) return bid;
+    const int x = bid & 7, j = bid >> 3;
+    return ((j / c) * 8 + x) * c + j % c;
+}
+
 }  // namespace gg
 
 struct gg_pkd_s {
