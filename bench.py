@@ -316,8 +316,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "copies": 5}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the force path runs on one GPU's share "
+              "and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
     import gasoline_amd as gg
