@@ -26,11 +26,10 @@
 
 namespace gg {
 
-constexpr int FCAP = 1024;        // BFS frontier capacity (nodes)
-         // leaves per query capacity
-constexpr int CAP = 256;
-constexpr int ECAP = 48;           // force-pass extras kept per particle by the density pass          // candidate buffer (4 per lane)
-constexpr int MAXTRY = 48;        // radius adjustments per query
+constexpr int FCAP = 1024;         // BFS frontier capacity (nodes)
+constexpr int CAP = 256;           // candidate buffer of a single-query search (4 per lane)
+constexpr int ECAP = 48;           // force-pass extras kept per particle by the density pass
+constexpr int MAXTRY = 48;         // radius adjustments per query
 constexpr double GG_FLT_MAX = (double)3.402823466E+38F;   // floattype.h:9
 
 // ------------------------------------------------------------ geometry
