@@ -123,7 +123,7 @@ struct SphDev {
     DevBuf rcnt, roff, rcur, ridx;             // extras lists
     DevBuf ecnt, eidx;                         // extras found by the density pass
     int exValid = 0;
-    DevBuf retry, nretry, hrec;                // kNN retries, force records
+    DevBuf retry, hrec;                        // kNN retries, force records
     DevBuf qf, act, nba, err, guess, diffcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
     DevBuf jv, ju;                             // FP32 density record (vPred, fMass; uPred)
@@ -137,7 +137,7 @@ struct SphDev {
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
-                         &err, &guess, &diffcnt, &retry, &nretry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
+                         &err, &guess, &diffcnt, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &jv, &ju, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();

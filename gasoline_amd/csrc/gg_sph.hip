@@ -394,8 +394,9 @@ struct QArgs {
     int *nbr;                          // kNN: K per slot; fill: CSR entries
     int *cnt;                          // gather count out
     const int64_t *off;                // gather fill offsets
-    unsigned int *err;                 // [0] failures, [1] boundary ties
+    unsigned int *err;                 // [0] failures, [1] boundary ties, [10] kNN retries
     const int *qlist;                  // optional: block b serves slot qlist[b]
+    const unsigned *nq;                // optional, with qlist: its length, on the device
 };
 
 // Squared radius of a ball expected to hold 2K particles at number density
@@ -560,17 +561,21 @@ __device__ int wave_bfs(const QArgs &A, Test test, int *fr, int FC, int *bf, int
 
 constexpr int QB = 512;   // buckets per single-query search
 
-// One 64-lane workgroup per query slot.  mode 0: exact kNN by (d2, caller
-// index) with a radius grown or shrunk until the ball holds between K and
-// CAP particles; mode 1/2: ball gather at fBall2 (d2 <= fBall2).
-__global__ __launch_bounds__(64) void k_query(QArgs A)
+// LDS of a single-query search.
+struct QLds {
+    int fr[2 * FCAP];
+    int bfst[QB], bcnt[QB];
+    double cd[CAP];
+    int cc[CAP], cs[CAP];
+};
+
+// One query slot q, by one 64-lane wavefront.  mode 0: exact kNN by (d2,
+// caller index) with a radius grown or shrunk until the ball holds between
+// K and CAP particles; mode 1/2: ball gather at fBall2 (d2 <= fBall2).
+__device__ void query_one(const QArgs &A, int q, QLds &S, int lane)
 {
-    __shared__ int fr[2 * FCAP];
-    __shared__ int bfst[QB], bcnt[QB];
-    __shared__ double cd[CAP];
-    __shared__ int cc[CAP], cs[CAP];
-    const int lane = threadIdx.x;
-    const int q = A.qlist ? A.qlist[blockIdx.x] : (int)blockIdx.x;
+    int *fr = S.fr, *bfst = S.bfst, *bcnt = S.bcnt, *cc = S.cc, *cs = S.cs;
+    double *cd = S.cd;
     if (!A.qflag[q]) {
         if (A.mode == 1 && lane == 0) A.cnt[q] = 0;
         return;
@@ -702,6 +707,24 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
         if (A.mode == 0) A.fb2[q] = -1.0;
         if (A.mode == 1) A.cnt[q] = 0;
     }
+}
+
+// One 64-lane workgroup per query slot (block b serves slot b, or qlist[b]);
+// with nq set, the first *nq entries of qlist shared out over the grid, so
+// a retry list whose length only the device knows needs no host round trip.
+__global__ __launch_bounds__(64) void k_query(QArgs A)
+{
+    __shared__ QLds S;
+    const int lane = threadIdx.x;
+    if (A.nq) {
+        const int nq = (int)*A.nq;
+        for (int b = blockIdx.x; b < nq; b += gridDim.x) {
+            query_one(A, A.qlist[b], S, lane);
+            wave_sync();
+        }
+        return;
+    }
+    query_one(A, A.qlist ? A.qlist[blockIdx.x] : (int)blockIdx.x, S, lane);
 }
 
 // ---------------------------------------------- kNN by bucket groups
@@ -2066,6 +2089,7 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.off = nullptr;
     Q.err = S.err.as<unsigned>();
     Q.qlist = nullptr;
+    Q.nq = nullptr;
     return Q;
 }
 
@@ -2225,6 +2249,7 @@ static int knn_finish(GGPKD pkd, const unsigned *e, int K, int *pnSmoothed)
     const int n = S.n;
     S.fails = e[0];
     S.ties = e[1];
+    S.retried = e[10];
     for (int k = 0; k < 4; ++k) S.diag[k] = e[2 + k];
     S.passes = e[6];
     uint64_t cand;
@@ -2447,8 +2472,6 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     const bool hint = S.seedSlack > 0.0;
     if (hint) GG_TRY(gather_field(pkd, GG_FBALL2, 1, 0, S.guess));
     GG_TRY(S.retry.reserve(sizeof(int) * n));
-    GG_TRY(S.nretry.reserve(sizeof(unsigned)));
-    GG_HIP(hipMemsetAsync(S.nretry.p, 0, sizeof(unsigned), st));
     KLArgs KL;
     KL.Q = query_args(pkd, 0, K);
     KL.Q.guess = hint ? S.guess.as<double>() : nullptr;
@@ -2465,23 +2488,20 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     KL.gnb = S.grpn.as<int>();
     KL.grl2 = S.grpr.as<double>();
     KL.retry = S.retry.as<int>();
-    KL.nretry = S.nretry.as<unsigned>();
+    KL.nretry = S.err.as<unsigned>() + 10;   // err[10]: retries (zeroed above)
     timer_mark(pkd, 0);
     hipLaunchKernelGGL(k_group_lists, dim3((S.nb + 3) / 4), dim3(256), 0, st, KL);
     timer_mark(pkd, 1);
     hipLaunchKernelGGL(k_knn_bucket, dim3(S.nb), dim3(64), 0, st, KL);
     timer_mark(pkd, 2);
     GG_HIP(hipGetLastError());
-    unsigned nretry = 0;
-    GG_HIP(hipMemcpyAsync(&nretry, S.nretry.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-    S.retried = nretry;
-    if (nretry > 0) {   // queries the leaf groups could not settle
+    {   // queries the leaf groups could not settle; their number stays on the device
         QArgs Q = query_args(pkd, 0, K);
         Q.guess = hint ? S.guess.as<double>() : nullptr;
         Q.nbr = S.nbr.as<int>();
         Q.qlist = S.retry.as<int>();
-        hipLaunchKernelGGL(k_query, dim3(nretry), dim3(64), 0, st, Q);
+        Q.nq = KL.nretry;
+        hipLaunchKernelGGL(k_query, dim3(std::min(n, 4096)), dim3(64), 0, st, Q);
         GG_HIP(hipGetLastError());
     }
     timer_mark(pkd, 3);
