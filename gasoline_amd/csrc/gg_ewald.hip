@@ -178,6 +178,7 @@ struct EArgs {
     const double *ewt;          // nEwh x (hx, hy, hz, hCfac, hSfac)
     double *acc;                // per slot: ax ay az pot dtg (added to)
     unsigned long long *nLoop;
+    int *slotLoops;             // per slot: real-space terms evaluated (fWeight)
 };
 
 // pkdBucketEwald (ewald.c:15-178) for the active particle of slot i.
@@ -248,6 +249,7 @@ __global__ __launch_bounds__(256) void k_ewald(EArgs A)
             ay += e[1] * (e[3] * s - e[4] * c);
             az += e[2] * (e[3] * s - e[4] * c);
         }
+        A.slotLoops[i] = (int)loops;
         double *o = A.acc + 5 * (size_t)i;
         o[0] += ax;
         o[1] += ay;
@@ -417,6 +419,8 @@ int ewald_apply(GGPKD pkd, int nReps, int iOrder, double L, double fEwCut, doubl
     A.ewt = G.ewh.as<double>();
     A.acc = G.acc.as<double>();
     A.nLoop = nl;
+    GG_TRY(G.ewl.reserve(sizeof(int) * (size_t)std::max(n, 1)));
+    A.slotLoops = G.ewl.as<int>();
     const dim3 grid(nblk(n, 256)), block(256);
     switch (iOrder) {
     case 1: hipLaunchKernelGGL(k_ewald<1>, grid, block, 0, st, A); break;

@@ -274,6 +274,38 @@ __global__ void k_grav_stats(int nb, const int *__restrict__ bucket,
         for (int k = 0; k < 5; ++k) atomicAdd(&out[k], red[k][0]);
 }
 
+// pkdBucketWeight (pkd.c:2851-2861, called at pkd.c:2998): every active
+// particle of a bucket gets the bucket's flops, pkdBucketInteract's count
+// (grav.c:246-247) plus, with Ewald, pkdBucketEwald's (ewald.c:175-176: per
+// real-space term 104 + MEVAL_FLOP, per active particle and k-space term 58).
+// One thread per active bucket.
+__global__ void k_grav_weight(int nb, const int *__restrict__ bucket,
+                              const double *__restrict__ bbox,
+                              const WalkNode *__restrict__ wn, const int *__restrict__ cnt,
+                              int iOrder, const int *__restrict__ perm,
+                              const uint32_t *__restrict__ pact,
+                              const int *__restrict__ ewLoops, int ewTermFlop, int nEwhFlop,
+                              double *__restrict__ fWeight)
+{
+    const int qf[5] = {10, 10, 41, 120, 277};
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const WalkNode c = wn[bucket[b]];
+    const long long na = (long long)bbox[8 * (size_t)b + 7];
+    const long long n = c.pUpper - c.pLower + 1;
+    const long long nP = cnt[3 * (size_t)b], nS = cnt[3 * (size_t)b + 1],
+                    nN = cnt[3 * (size_t)b + 2];
+    long long flop = na * ((nP + n) * 38 + nS * 82 + nN * (35 + qf[iOrder]));
+    if (ewLoops) {
+        long long loops = 0;
+        for (int s = c.pLower; s <= c.pUpper; ++s)
+            if (pact[s] & 1u) loops += ewLoops[s];
+        flop += loops * ewTermFlop + na * nEwhFlop;
+    }
+    for (int s = c.pLower; s <= c.pUpper; ++s)
+        if (pact[s] & 1u) fWeight[perm[s]] = (double)flop;
+}
+
 // --------------------------------------------------------------- interact
 template <typename T> __device__ __forceinline__ T rsq(T x);
 template <> __device__ __forceinline__ float rsq<float>(float x)
@@ -317,7 +349,7 @@ __device__ __forceinline__ void spline_ab(T r2, T dir, T twoh, T &a, T &b)
     if (r2 < twoh * twoh) {
         const T dih = T(2) / twoh;
         T u;
-        if constexpr (sizeof(T) == 4) u = dih * (r2 * dir);
+        if constexpr (sizeof(T) == 4) u = r2 > T(0) ? dih * (r2 * dir) : T(0);   // r = 0: coincident
         else u = dih / dir;
         if (u < T(1)) {
             a = dih * (T(7.0 / 5.0) - T(2.0 / 3.0) * u * u + T(3.0 / 10.0) * u * u * u * u
@@ -342,7 +374,7 @@ __device__ __forceinline__ void spline_abcd(T r2, T dir, T twoh, T &a, T &b, T &
     if (r2 < twoh * twoh) {
         const T dih = T(2) / twoh;
         T u;
-        if constexpr (sizeof(T) == 4) u = dih * (r2 * dir);
+        if constexpr (sizeof(T) == 4) u = r2 > T(0) ? dih * (r2 * dir) : T(0);   // r = 0: coincident
         else u = dih / dir;
         const T dih2 = dih * dih, dih3 = dih2 * dih, dih5 = dih3 * dih2;
         if (u < T(1)) {
@@ -669,19 +701,13 @@ __device__ __forceinline__ void bucket_store(const IArgs &A, const Tgt<T> &g, co
     }
 }
 
-// One 64-lane workgroup per active bucket over its three lists, in chunks
-// of 64 entries staged in LDS as cell-relative T.
+// The three CSR lists of bucket b, in chunks of 64 entries staged in LDS as
+// cell-relative T.
 template <typename T, int ORDER>
-__global__ __launch_bounds__(64) void k_interact(IArgs A)
+__device__ __forceinline__ void interact_lists(const IArgs &A, int b, const WalkNode &bn,
+                                               Stage<T, ORDER> &L, const Tgt<T> &g, Acc<T> &R)
 {
-    __shared__ Stage<T, ORDER> L;
     const int lane = threadIdx.x;
-    const int b = blockIdx.x;
-    const WalkNode bn = A.wn[A.bucket[b]];
-    Tgt<T> g;
-    Acc<T> R;
-    int it, TPW;
-    bucket_targets(A, bn, L, g, R, it, TPW);
     const size_t b3 = 3 * (size_t)b;
     for (int kind = 0; kind < 3; ++kind) {
         const int n = A.cnt[b3 + kind];
@@ -701,7 +727,53 @@ __global__ __launch_bounds__(64) void k_interact(IArgs A)
             __syncthreads();
         }
     }
-    bucket_store(A, g, R, it, TPW);
+}
+
+// One 64-lane workgroup per active bucket over its three lists.  A bucket
+// of more than 64 particles (a zero-extent cell of coincident particles,
+// which pkd.c:2489 makes a bucket whatever its size) takes its targets 64 at
+// a time, one per lane, each group over its own bucket in chunks of 64
+// (every pair i != j) and then over the lists.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void k_interact(IArgs A)
+{
+    __shared__ Stage<T, ORDER> L;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const WalkNode bn = A.wn[A.bucket[b]];
+    const int pL = bn.pLower, nT = bn.pUpper - bn.pLower + 1;
+    if (nT <= 64) {
+        Tgt<T> g;
+        Acc<T> R;
+        int it, TPW;
+        bucket_targets(A, bn, L, g, R, it, TPW);
+        interact_lists(A, b, bn, L, g, R);
+        bucket_store(A, g, R, it, TPW);
+        return;
+    }
+    for (int g0 = 0; g0 < nT; g0 += 64) {
+        Tgt<T> g;
+        Acc<T> R;
+        g.S = 1;
+        g.s = 0;
+        const bool valid = g0 + lane < nT;
+        const int it = pL + (valid ? g0 + lane : 0);
+        g.act = valid && (A.pact[it] & 1u);
+        g.xi = (T)(A.px[it] - bn.cx);
+        g.yi = (T)(A.py[it] - bn.cy);
+        g.zi = (T)(A.pz[it] - bn.cz);
+        g.hi = (T)A.ph[it];
+        g.mi = (T)A.pm[it];
+        for (int c0 = 0; c0 < nT; c0 += 64) {
+            const int m = min(64, nT - c0);
+            if (lane < m) stage_pp(L, lane, pL + c0 + lane, A, bn.cx, bn.cy, bn.cz);
+            __syncthreads();
+            eval_pp(L, m, g, R, g0 + lane - c0);     // skips the target itself
+            __syncthreads();
+        }
+        interact_lists(A, b, bn, L, g, R);
+        bucket_store(A, g, R, it, 64);
+    }
 }
 
 // The walk and the interactions of one bucket in one 64-lane workgroup,
@@ -778,11 +850,25 @@ __device__ __forceinline__ void flush_pp(Stage<T, ORDER> &L, int *qpl, int *qpn,
     np -= m;
 }
 
-template <typename T, int ORDER, bool PER>
+// splitmix64's finaliser: the per-entry term of the walk checksums.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// CHK: also count and sum, per list kind k, the members the walk hands to
+// the evaluation and their mix64 (order-independent checksums of the lists
+// this kernel evaluates, for parity checks): chk[9b + 3k ..] = count, sum,
+// sum of mixes.
+template <typename T, int ORDER, bool PER, bool CHK = false>
 __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
                                                       const int *__restrict__ broot,
                                                       int *__restrict__ cnt,
-                                                      int *__restrict__ err)
+                                                      int *__restrict__ err,
+                                                      unsigned long long *__restrict__ chk = nullptr)
 {
     __shared__ Stage<T, ORDER> L;
     __shared__ int fr[2][WI_FC];
@@ -802,6 +888,7 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
     const unsigned long long below = (1ull << lane) - 1;
     int nS = 0, nN = 0, nR = 0, nRp = 0;          // pending (wave-uniform)
     int totP = 0, totS = 0, totN = 0;
+    unsigned long long cs[6] = {0, 0, 0, 0, 0, 0};  // CHK: per lane, sum and mix sum per kind
     const int rx = PER && A.L[0] > 0 ? A.nReps : 0, ry = PER && A.L[1] > 0 ? A.nReps : 0,
               rz = PER && A.L[2] > 0 ? A.nReps : 0;
     // replicas x outermost (walk.c:325-338); each walks the whole tree with
@@ -846,6 +933,15 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
             const unsigned long long mp = __ballot(dc.pp);
             if (dc.soft) qs[nS + __popcll(ms & below)] = c;
             if (dc.newt) qn[nN + __popcll(mn & below)] = c;
+            if constexpr (CHK) {
+                if (dc.soft) { cs[2] += (unsigned)c; cs[3] += mix64((unsigned)c); }
+                if (dc.newt) { cs[4] += (unsigned)c; cs[5] += mix64((unsigned)c); }
+                if (dc.pp)
+                    for (int j = 0; j < dc.np; ++j) {
+                        cs[0] += (unsigned)(dc.pl + j);
+                        cs[1] += mix64((unsigned)(dc.pl + j));
+                    }
+            }
             if (dc.pp) {
                 const int r = nR + __popcll(mp & below);
                 qpl[r] = dc.pl;
@@ -881,6 +977,16 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
         cnt[3 * (size_t)b] = totP;
         cnt[3 * (size_t)b + 1] = totS;
         cnt[3 * (size_t)b + 2] = totN;
+    }
+    if constexpr (CHK) {
+        const int tot[3] = {totP, totS, totN};
+        for (int k = 0; k < 6; ++k) {
+            unsigned long long v = cs[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) chk[9 * (size_t)b + 3 * (k >> 1) + 1 + (k & 1)] = v;
+        }
+        if (lane == 0)
+            for (int k = 0; k < 3; ++k) chk[9 * (size_t)b + 3 * k] = (unsigned)tot[k];
     }
     bucket_store(A, g, R, it, TPW);
 }
@@ -1081,6 +1187,7 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
     GG_TRY(ensure_field(pkd, GG_A));
     GG_TRY(ensure_field(pkd, GG_POT));
     GG_TRY(ensure_field(pkd, GG_DTGRAV));
+    GG_TRY(ensure_field(pkd, GG_FWEIGHT));
     const int nb = G.nb;
     unsigned long long hs[5] = {0, 0, 0, 0, 0};
     if (nb > 0) {
@@ -1093,13 +1200,21 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
             for (int d = 0; d < 3; ++d) A.L[d] = L[d];
         }
         GG_TRY(G.werr.reserve(sizeof(int)));
-        GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
         G.listsCurrent = 0;
-        if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_fused<float>(iOrder, A, G, st));
-        else GG_TRY(launch_fused<double>(iOrder, A, G, st));
-        int werr = 0;
-        GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
+        // buckets of more than 64 coincident particles do not fit the fused
+        // kernel's wavefront: the listed walk and k_interact take them
+        const bool big = G.maxBucket > 64;
+        GG_CHECK(!(big && per), GG_EUNSUP,
+                 "periodic gravity with a bucket of %d coincident particles (> 64)",
+                 G.maxBucket);
+        int werr = 1;
+        if (!big) {
+            GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
+            if (pkd->prec == GG_PREC_FP32) GG_TRY(launch_fused<float>(iOrder, A, G, st));
+            else GG_TRY(launch_fused<double>(iOrder, A, G, st));
+            GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            GG_HIP(hipStreamSynchronize(st));
+        }
         G.fusedWalk = !werr;
         GG_CHECK(!(werr && per), GG_EUNSUP,
                  "a breadth-first frontier of the periodic walk outgrew its buffer");
@@ -1118,6 +1233,17 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
                            nb, G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
                            G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
         GG_HIP(hipGetLastError());
+        {
+            static const int meval[5] = {10, 10, 48, 151, 343};   // MEVAL_FLOP, meval.h:84
+            const bool ew = ewald && iEwOrder;
+            hipLaunchKernelGGL(k_grav_weight, dim3((nb + 255) / 256), dim3(256), 0, st, nb,
+                               G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                               G.cnt.as<int>(), iOrder, G.perm.as<int>(), G.pact.as<uint32_t>(),
+                               ew ? G.ewl.as<int>() : (const int *)nullptr,
+                               ew ? 104 + meval[iEwOrder > 4 ? 4 : iEwOrder] : 0,
+                               ew ? G.nEwh * 58 : 0, pkd->f[GG_FWEIGHT].as<double>());
+            GG_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_grav_scatter, dim3((G.n + 255) / 256), dim3(256), 0, st, G.n,
                            G.perm.as<int>(), G.pact.as<uint32_t>(), G.acc.as<double>(),
                            pkd->f[GG_A].as<double>(), pkd->f[GG_POT].as<double>(),
@@ -1138,6 +1264,56 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
     if (pdCellSum) *pdCellSum = (double)hs[2];
     if (pdSoftSum) *pdSoftSum = (double)hs[3];
     if (pdFlop) *pdFlop = (double)hs[4];
+    return GG_OK;
+}
+
+extern "C" int gg_grav_walk_checksums(GGPKD pkd, int iOrder, uint64_t *sums)
+{
+    GG_CHECK(pkd && sums, GG_EINVAL, "null argument");
+    GravDev &G = pkd->grav;
+    GG_CHECK(G.built, GG_ESTATE, "no gravity tree");
+    GG_CHECK(G.n == pkd->nLocal, GG_ESTATE, "particle count changed since the tree was built");
+    GG_CHECK(iOrder >= 1 && iOrder <= G.iOrder, GG_EINVAL,
+             "iOrder %d exceeds the tree's moments (%d)", iOrder, G.iOrder);
+    GG_CHECK(G.maxBucket <= 64, GG_EUNSUP, "a bucket of %d coincident particles (> 64) is "
+             "walked by the listed walk only", G.maxBucket);
+    const int nb = G.nb;
+    if (nb == 0) return GG_OK;
+    hipStream_t st = pkd->stream;
+    DevBuf out;
+    GG_TRY(out.reserve(sizeof(unsigned long long) * 9 * (size_t)nb));
+    GG_TRY(G.werr.reserve(sizeof(int)));
+    GG_HIP(hipMemsetAsync(G.werr.p, 0, sizeof(int), st));
+    const IArgs A = make_iargs(pkd, iOrder);
+    const dim3 grid(nb), block(64);
+    const double *bb = G.bbox.as<double>();
+    const int *br = G.broot.as<int>();
+    int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
+    unsigned long long *o = out.as<unsigned long long>();
+#define GG_CHKW(O)                                                                              \
+    if (pkd->prec == GG_PREC_FP32)                                                              \
+        hipLaunchKernelGGL((k_walk_interact<float, O, false, true>), grid, block, 0, st, A, bb, \
+                           br, cnt, err, o);                                                    \
+    else                                                                                        \
+        hipLaunchKernelGGL((k_walk_interact<double, O, false, true>), grid, block, 0, st, A,    \
+                           bb, br, cnt, err, o)
+    switch (iOrder) {
+    case 1: GG_CHKW(1); break;
+    case 2: GG_CHKW(2); break;
+    case 3: GG_CHKW(3); break;
+    default: GG_CHKW(4); break;
+    }
+#undef GG_CHKW
+    GG_HIP(hipGetLastError());
+    int werr = 0;
+    GG_HIP(hipMemcpyAsync(&werr, G.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipMemcpyAsync(sums, o, sizeof(unsigned long long) * 9 * (size_t)nb,
+                          hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    out.release();
+    G.listsCurrent = 0;
+    GG_CHECK(!werr, GG_EUNSUP, "a frontier of the fused walk outgrew its buffer: this tree is "
+             "walked by the listed walk");
     return GG_OK;
 }
 
@@ -1314,6 +1490,10 @@ static int grav_enqueue(GGPKD pkd, int iOrder)
     hipLaunchKernelGGL(k_grav_stats, dim3(std::min((nb + 255) / 256, 1024)), dim3(256), 0, st, nb,
                        G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
                        G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
+    hipLaunchKernelGGL(k_grav_weight, dim3((nb + 255) / 256), dim3(256), 0, st, nb,
+                       G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                       G.cnt.as<int>(), iOrder, G.perm.as<int>(), G.pact.as<uint32_t>(),
+                       (const int *)nullptr, 0, 0, pkd->f[GG_FWEIGHT].as<double>());
     hipLaunchKernelGGL(k_grav_scatter, dim3((G.n + 255) / 256), dim3(256), 0, st, G.n,
                        G.perm.as<int>(), G.pact.as<uint32_t>(), G.acc.as<double>(),
                        pkd->f[GG_A].as<double>(), pkd->f[GG_POT].as<double>(),
@@ -1340,7 +1520,13 @@ extern "C" int gg_grav_graph(GGPKD pkd, int iOrder)
     GG_CHECK(iOrder >= 1 && iOrder <= G.iOrder, GG_EINVAL,
              "iOrder %d exceeds the tree's moments (%d)", iOrder, G.iOrder);
     GG_CHECK(G.nb > 0, GG_EINVAL, "no active bucket");
-    const int f[] = {GG_A, GG_POT, GG_DTGRAV};
+    GG_CHECK(G.maxBucket <= 64, GG_EUNSUP,
+             "a bucket of %d coincident particles needs the listed walk: use gg_grav_all",
+             G.maxBucket);
+    GG_CHECK(!(pkd->fPeriod[0] > 0 || pkd->fPeriod[1] > 0 || pkd->fPeriod[2] > 0), GG_EUNSUP,
+             "the captured step is the isolated one: a handle with a periodic box uses "
+             "gg_grav_all");
+    const int f[] = {GG_A, GG_POT, GG_DTGRAV, GG_FWEIGHT};
     for (int k : f) GG_TRY(ensure_field(pkd, k));
     GG_TRY(G.werr.reserve(sizeof(int) * 11));
     GG_TRY(G.stats.reserve(sizeof(unsigned long long) * 5));
@@ -1405,25 +1591,28 @@ extern "C" int gg_tune_bucket(GGPKD pkd, int nSys, const int *sysOff, int iOpenT
 {
     GG_CHECK(pkd && cand && nCand >= 1, GG_EINVAL, "null argument or no candidate");
     const int n = pkd->nLocal;
-    const int f[] = {GG_A, GG_POT, GG_DTGRAV};
+    const int f[] = {GG_A, GG_POT, GG_DTGRAV, GG_FWEIGHT};
     for (int k : f) GG_TRY(ensure_field(pkd, k));
     hipStream_t st = pkd->stream;
     DevBuf save;
-    GG_TRY(save.reserve(sizeof(double) * 5 * (size_t)std::max(n, 1)));
+    GG_TRY(save.reserve(sizeof(double) * 6 * (size_t)std::max(n, 1)));
     double *sv = save.as<double>();
     int rc = GG_OK;
     auto copy_fields = [&](bool out) -> int {
         const size_t w3 = sizeof(double) * 3 * (size_t)n, w1 = sizeof(double) * (size_t)n;
         void *fa = pkd->f[GG_A].p, *fp = pkd->f[GG_POT].p, *fd = pkd->f[GG_DTGRAV].p;
+        void *fw = pkd->f[GG_FWEIGHT].p;
         char *b = reinterpret_cast<char *>(sv);
         if (out) {
             GG_HIP(hipMemcpyAsync(b, fa, w3, hipMemcpyDeviceToDevice, st));
             GG_HIP(hipMemcpyAsync(b + w3, fp, w1, hipMemcpyDeviceToDevice, st));
             GG_HIP(hipMemcpyAsync(b + w3 + w1, fd, w1, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(b + w3 + 2 * w1, fw, w1, hipMemcpyDeviceToDevice, st));
         } else {
             GG_HIP(hipMemcpyAsync(fa, b, w3, hipMemcpyDeviceToDevice, st));
             GG_HIP(hipMemcpyAsync(fp, b + w3, w1, hipMemcpyDeviceToDevice, st));
             GG_HIP(hipMemcpyAsync(fd, b + w3 + w1, w1, hipMemcpyDeviceToDevice, st));
+            GG_HIP(hipMemcpyAsync(fw, b + w3 + 2 * w1, w1, hipMemcpyDeviceToDevice, st));
         }
         GG_HIP(hipStreamSynchronize(st));
         return GG_OK;

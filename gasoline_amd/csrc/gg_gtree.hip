@@ -858,8 +858,8 @@ __global__ void k_gt_emit(int nn, int iOrder, int iOpenType, double dCrit,
 
 // Buckets with an active particle: flag by preorder id; bounds of the
 // active particles, largest softening, active count (pkd.c:2916-2937,
-// walk.c:319-324); the largest bucket (the interaction maps a bucket onto
-// one wavefront).
+// walk.c:319-324); the largest bucket, active or not (the fused walk maps a
+// bucket's targets, and an opened bucket's particles, onto one wavefront).
 __global__ void k_gt_bucket(int nn, const GNode *__restrict__ nd, const double *__restrict__ x,
                             const double *__restrict__ y, const double *__restrict__ z,
                             const double *__restrict__ ph, const uint32_t *__restrict__ pact,
@@ -888,7 +888,7 @@ __global__ void k_gt_bucket(int nn, const GNode *__restrict__ nd, const double *
         }
     }
     flag[c] = na > 0;
-    if (na > 0) atomicMax(maxb, g.hi - g.lo + 1);
+    atomicMax(maxb, g.hi - g.lo + 1);
     double *o = bb8 + 8 * (size_t)c;
     for (int d = 0; d < 3; ++d) {
         o[d] = lo[d];
@@ -1232,8 +1232,7 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
         GG_HIP(hipMemcpyAsync(&maxb, G.gerr.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost,
                               st));
         GG_HIP(hipStreamSynchronize(st));
-        GG_CHECK(maxb <= 64, GG_EUNSUP,
-                 "a bucket holds %d particles (> 64): coincident positions?", maxb);
+        G.maxBucket = maxb;     // > 64 only for cells of coincident particles (pkd.c:2489)
         GG_TRY(G.bucket.reserve(sizeof(int) * std::max(nb, 1)));
         GG_TRY(G.broot.reserve(sizeof(int) * std::max(nb, 1)));
         GG_TRY(G.bbox.reserve(sizeof(double) * 8 * std::max(nb, 1)));

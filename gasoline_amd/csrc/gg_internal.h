@@ -62,6 +62,7 @@ constexpr int OH_N = 25;  // octupole (10) + hexadecapole (15)
 struct GravDev {
     int built = 0, n = 0, nNodes = 0, iRoot = -1, iOrder = 2, nBucket = 0;
     int nb = 0;            // active buckets
+    int maxBucket = 0;     // particles in the largest bucket (> 64: coincident particles)
     int64_t nEnt = 0;      // list entries of the last walk
     DevBuf px, py, pz, pm, ph, pact, perm;    // storage-order particles
     DevBuf wnode;          // WalkNode per node
@@ -81,6 +82,7 @@ struct GravDev {
     DevBuf gpn, gF, gposR, gnode, gbox, gnch, gchoff, gerr, groot, gcm;
     DevBuf gnit, gioff, gicell, gpart, gbflag, gbb8, grootof, glroot, gdcnt, gdlist, glused, glbase, gscr;
     DevBuf ewt, emom, ewh;          // Ewald: moment partials, root moments, k-space table
+    DevBuf ewl;                     // Ewald: real-space terms per slot (for fWeight)
     int64_t ewLoops = 0;            // real-space terms of the last Ewald pass
     int nEwh = 0;                   // its k-space terms
     int nSys = 1;                   // systems (roots) of the tree
@@ -97,7 +99,7 @@ struct GravDev {
                          &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
                          &tdbl, &gpn, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
                          &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof, &glroot, &gdcnt, &gdlist, &glused, &glbase, &gscr,
-                         &ewt, &emom, &ewh};
+                         &ewt, &emom, &ewh, &ewl};
         for (DevBuf *b : all) b->release();
     }
 };

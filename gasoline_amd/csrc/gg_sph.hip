@@ -216,11 +216,14 @@ __global__ void k_gather_pos(int n, const int *__restrict__ order, const double 
 // Binary radix tree over the sorted Morton keys (Karras 2012): internal
 // node i in [0, n-2] covers the slot range rng[2i]..rng[2i+1]; a child code
 // c >= 0 is internal node c, c < 0 is the single particle ~c.  Equal keys are
-// ordered by slot.  Buckets are the maximal subtrees of at most BKT
-// particles: every bucket is one Morton-prefix cell, so its box is tight.
-// A node of more than BKT particles covers whole consecutive buckets, so its
-// box is the union of a bucket range, read from a sparse table of bucket
-// boxes (FP32, rounded outwards); no bottom-up pass is needed.
+// ordered by slot.  Buckets are the maximal subtrees of at most bkt
+// particles (the caller's nBucket, at most BKT): every bucket is one
+// Morton-prefix cell, so its box is tight.  A node of more than bkt
+// particles covers whole consecutive buckets, so its box is the union of a
+// bucket range, read from a sparse table of bucket boxes (FP32, rounded
+// outwards); no bottom-up pass is needed.  The kNN serves one bucket's
+// queries with one wavefront, two buckets' candidates per load, so a bucket
+// holds at most BKT = 32 particles.
 constexpr int BKT = 32;
 
 __device__ __forceinline__ int lcp_delta(const uint64_t *__restrict__ k, int n, int i, int j)
@@ -269,10 +272,10 @@ __device__ __forceinline__ void child_info(int c, const int *__restrict__ rng, i
     }
 }
 
-// Mark bucket starts: children of nodes larger than BKT that are not.  For
+// Mark bucket starts: children of nodes larger than bkt that are not.  For
 // each, also note the parent's particle count and the length of the key
 // prefix its particles share (the parent's Morton cell).
-__global__ void k_bucket_mark(int n, const uint64_t *__restrict__ key,
+__global__ void k_bucket_mark(int n, int bkt, const uint64_t *__restrict__ key,
                               const int *__restrict__ child, const int *__restrict__ rng,
                               int *__restrict__ flag, int *__restrict__ cnt,
                               int2 *__restrict__ par)
@@ -280,12 +283,12 @@ __global__ void k_bucket_mark(int n, const uint64_t *__restrict__ key,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int lo = rng[2 * i], hi = rng[2 * i + 1];
-    if (hi - lo + 1 <= BKT) return;
+    if (hi - lo + 1 <= bkt) return;
     const int pre = min(lcp_delta(key, n, lo, hi), 64) - 16;   // of the 48 key bits
     for (int k = 0; k < 2; ++k) {
         int f, m;
         child_info(child[2 * i + k], rng, f, m);
-        if (m <= BKT) {
+        if (m <= bkt) {
             flag[f] = 1;
             cnt[f] = m;
             par[f] = make_int2(hi - lo + 1, pre);
@@ -344,7 +347,7 @@ __global__ void k_sparse_level(int nb, int half, const float *__restrict__ prev,
 }
 
 // Child record of an internal node: the child's particle count, its bucket
-// index when it is a bucket (count <= BKT) or its internal-node index
+// index when it is a bucket (count <= bkt) or its internal-node index
 // otherwise, and its box in FP32 rounded outwards, so a box test on the
 // record never excludes what the FP64 box would include.
 struct __align__(16) CRec {
@@ -352,7 +355,7 @@ struct __align__(16) CRec {
     float lo[3], hi[3];
 };
 
-__global__ void k_child_rec(int n, int nb, const int *__restrict__ child,
+__global__ void k_child_rec(int n, int nb, int bkt, const int *__restrict__ child,
                             const int *__restrict__ rng, const int *__restrict__ bof,
                             const float *__restrict__ sparse, CRec *__restrict__ rec)
 {
@@ -366,7 +369,7 @@ __global__ void k_child_rec(int n, int nb, const int *__restrict__ child,
     const float *t0 = sparse + 6 * ((size_t)lv * nb + b0);
     const float *t1 = sparse + 6 * ((size_t)lv * nb + b1 - (1 << lv) + 1);
     CRec r;
-    r.code = m > BKT ? c : b0;
+    r.code = m > bkt ? c : b0;
     r.cnt = m;
     for (int d = 0; d < 3; ++d) {
         r.lo[d] = fminf(t0[d], t1[d]);
@@ -378,6 +381,7 @@ __global__ void k_child_rec(int n, int nb, const int *__restrict__ child,
 // -------------------------------------------------------- kNN / gather
 struct QArgs {
     int n, K, per, mode;               // mode 0 kNN, 1 gather count, 2 gather fill
+    int bkt;                           // bucket size of the structure
     double L[3];
     const double *sx, *sy, *sz;
     const CRec *rec;                   // radix tree child records
@@ -507,7 +511,7 @@ template <typename Test>
 __device__ int wave_bfs(const QArgs &A, Test test, int *fr, int FC, int *bf, int *bc, int BC,
                         int lane, int *tot)
 {
-    if (A.n <= BKT) {   // the root is the only bucket
+    if (A.n <= A.bkt) {   // the root is the only bucket
         if (lane == 0) {
             bf[0] = 0;
             bc[0] = A.n;
@@ -530,7 +534,7 @@ __device__ int wave_bfs(const QArgs &A, Test test, int *fr, int FC, int *bf, int
                 r = A.rec[2 * fr[cur * FC + (k >> 1)] + (k & 1)];
                 const double b[6] = {r.lo[0], r.lo[1], r.lo[2], r.hi[0], r.hi[1], r.hi[2]};
                 hit = test(b);
-                big = r.cnt > BKT;
+                big = r.cnt > A.bkt;
             }
             const unsigned long long mb = __ballot(hit && !big);
             const unsigned long long mi = __ballot(hit && big);
@@ -2065,6 +2069,7 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     SphDev &S = pkd->sph;
     QArgs Q;
     Q.n = S.n;
+    Q.bkt = S.nBucket;
     Q.K = K;
     Q.per = S.bPeriodic;
     Q.mode = mode;
@@ -2107,7 +2112,8 @@ using namespace gg;
 
 extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const double *fPeriod)
 {
-    (void)nBucket;
+    GG_CHECK(nBucket >= 1 && nBucket <= BKT, GG_EINVAL, "nBucket %d outside 1..%d", nBucket,
+             BKT);
     GG_CHECK(pkd, GG_EINVAL, "null handle");
     GG_CHECK(pkd->f[GG_R].p, GG_ESTATE, "positions must be put before gg_build_local");
     GG_CHECK(!bPeriodic || fPeriod, GG_EINVAL, "periodic build needs fPeriod");
@@ -2178,12 +2184,12 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     GG_TRY(S.bcode.reserve(sizeof(int) * n));
     GG_TRY(S.bof.reserve(sizeof(int) * n));
     int nb = 1;
-    if (n > BKT) {
+    if (n > nBucket) {
         GG_HIP(hipMemsetAsync(S.bflag.p, 0, sizeof(int) * n, st));
         hipLaunchKernelGGL(k_lbvh, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
                            S.key.as<uint64_t>(), S.child.as<int>(), S.rng.as<int>());
         GG_TRY(S.bpar.reserve(sizeof(int2) * n));
-        hipLaunchKernelGGL(k_bucket_mark, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n,
+        hipLaunchKernelGGL(k_bucket_mark, dim3(nblk(n - 1, 256)), dim3(256), 0, st, n, nBucket,
                            S.key.as<uint64_t>(), S.child.as<int>(), S.rng.as<int>(),
                            S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>());
         int64_t tb = 0;
@@ -2208,7 +2214,7 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
                                sp + 6 * (size_t)lv * nb);
         GG_TRY(S.crec.reserve(sizeof(CRec) * 2 * (size_t)(n - 1)));
         hipLaunchKernelGGL(k_child_rec, dim3(nblk(2 * (n - 1), 256)), dim3(256), 0, st, n, nb,
-                           S.child.as<int>(), S.rng.as<int>(), S.bof.as<int>(), sp,
+                           nBucket, S.child.as<int>(), S.rng.as<int>(), S.bof.as<int>(), sp,
                            S.crec.as<CRec>());
     } else {   // a single bucket: the whole set; the searches need no tree
         GG_TRY(S.bfirst.reserve(sizeof(int)));
@@ -2229,7 +2235,7 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     S.nb = nb;
     S.bPeriodic = bPeriodic;
     for (int d = 0; d < 3; ++d) S.L[d] = bPeriodic ? fPeriod[d] : 0.0;
-    S.nBucket = BKT;
+    S.nBucket = nBucket;
     S.built = 1;
     S.listValid = 0;
     timer_mark(pkd, 1);

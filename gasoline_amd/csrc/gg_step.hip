@@ -37,19 +37,21 @@ struct KickArgs {
     double *v, *vpred, *u, *upred;
 };
 
-// Active particles: vPred = v dvPredFacOne + a dvPredFacTwo,
-// v = v dvFacOne + a dvFacTwo; gas also uPred = u + uDot duPredDelta,
+// Active particles: v = v dvFacOne + a dvFacTwo; gas also
+// vPred = v dvPredFacOne + a dvPredFacTwo, uPred = u + uDot duPredDelta and
 // u = u + uDot duDelta with uDot = uDotPdV + uDotAV + uDotDiff (pkd.h:65).
+// A GASOLINE build writes vPred of gas only (pkd.c:3805-3812, 3953-3957).
 __global__ void k_kick(KickArgs K)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K.n) return;
     const uint32_t b = K.bits[i];
     if (!(b & GG_TYPE_ACTIVE)) return;
+    const bool gas = (b & GG_TYPE_GAS) != 0;
     for (int j = 0; j < 3; ++j) {
         const size_t e = 3 * (size_t)i + j;
         const double v = K.v[e], a = K.a[e];
-        K.vpred[e] = v * K.vpf1 + a * K.vpf2;
+        if (gas) K.vpred[e] = v * K.vpf1 + a * K.vpf2;
         K.v[e] = v * K.vf1 + a * K.vf2;
     }
     if (K.bGasU && (b & GG_TYPE_GAS)) {

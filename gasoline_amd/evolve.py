@@ -69,8 +69,14 @@ class KDK:
         pk.kick(1.0, 0.5 * dt, 1.0, 0.5 * dt, 0.5 * dt, 0.5 * dt)
 
 
+ALL_GAS = P.TYPE_ACTIVE | P.TYPE_TREEACTIVE | P.TYPE_SMOOTHACTIVE | P.TYPE_GAS
+
+
 def start(pk, d, soft, bits=None):
-    """Put initial conditions: r, v, fMass, fSoft, u (vPred = v, uPred = u)."""
+    """Put initial conditions: r, v, fMass, fSoft, u (vPred = v, uPred = u)
+    and the type bits (default: every particle active gas, the all-gas
+    workloads; the kick predicts vPred and advances u of gas only, as
+    pkdKick does, pkd.c:3805-3812)."""
     pk.put("r", d["r"])
     pk.put("v", d["v"])
     pk.put("vPred", d["v"])
@@ -78,8 +84,9 @@ def start(pk, d, soft, bits=None):
     pk.put("fSoft", np.broadcast_to(np.asarray(soft, float), d["m"].shape).copy())
     pk.put("u", d["u"])
     pk.put("uPred", d["u"])
-    if bits is not None:
-        pk.put("iActive", bits)
+    if bits is None:
+        bits = np.full(d["m"].shape, ALL_GAS, np.uint32)
+    pk.put("iActive", bits)
     return pk
 
 

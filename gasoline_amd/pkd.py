@@ -19,7 +19,7 @@ LIBPATH = os.path.join(_HERE, "libgasoline_amd.so")
 GG_R, GG_V, GG_VPRED, GG_A, GG_MASS, GG_SOFT, GG_POT, GG_DTGRAV, GG_U, GG_UPRED, \
     GG_FBALL2, GG_DENSITY, GG_DIVV, GG_CURLV, GG_DVDS, GG_FDIVV_T, GG_BALSARA, GG_DIFF, \
     GG_POVERRHO2, GG_C, GG_DT, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW, \
-    GG_IACTIVE, GG_IRUNG, GG_DTOLD = range(29)
+    GG_IACTIVE, GG_IRUNG, GG_DTOLD, GG_FWEIGHT = range(30)
 
 FIELDS = {
     "r": (GG_R, 3), "v": (GG_V, 3), "vPred": (GG_VPRED, 3), "a": (GG_A, 3),
@@ -32,6 +32,7 @@ FIELDS = {
     "uDotPdV": (GG_UDOTPDV, 1), "uDotAV": (GG_UDOTAV, 1), "uDotDiff": (GG_UDOTDIFF, 1),
     "mumax": (GG_MUMAX, 1), "dtNew": (GG_DTNEW, 1), "iActive": (GG_IACTIVE, 1),
     "iRung": (GG_IRUNG, 1), "dtOld": (GG_DTOLD, 1),
+    "fWeight": (GG_FWEIGHT, 1),
 }
 _INT_FIELDS = {"iActive": np.uint32, "iRung": np.int32}
 
@@ -53,7 +54,7 @@ def tipsy_header(path, standard=True):
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local", "gg_set_period",
        "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy", "gg_out_array",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
-       "gg_grav_lists", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
+       "gg_grav_lists", "gg_grav_walk_checksums", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
@@ -135,6 +136,7 @@ def lib():
                               C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, _dp,
                               _ip, _dp, _dp, _dp, _dp]
     L.gg_grav_lists.argtypes = [_vp, _ip, _i64p, _ip, _i64p, _ip, _ip]
+    L.gg_grav_walk_checksums.argtypes = [_vp, C.c_int, _vp]
     L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
     L.gg_direct_grav.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
     L.gg_reorder.argtypes = [_vp, _ip]
@@ -343,6 +345,18 @@ class Pkd:
                                    _p(off, _i64p), _p(cnt, _ip), _p(ent, _ip)))
         n = nb.value
         return bk[:n], off[:3 * n].reshape(n, 3), cnt[:3 * n].reshape(n, 3), ent[:ne.value]
+
+    def walk_checksums(self, iOrder=2):
+        """Per active bucket and list kind (PP, softened, Newtonian): the
+        number and the sum of the members the fused walk evaluates and the
+        sum of their splitmix64 mixes (gg_grav_walk_checksums), shape
+        (nb, 3, 3), uint64."""
+        nb = C.c_int()
+        ne = C.c_int64()
+        _check(lib().gg_grav_lists(self.h, C.byref(nb), C.byref(ne), None, None, None, None))
+        out = np.zeros((max(nb.value, 1), 3, 3), np.uint64)
+        _check(lib().gg_grav_walk_checksums(self.h, int(iOrder), out.ctypes.data_as(_vp)))
+        return out[:nb.value]
 
     def direct_grav(self, targets):
         """FP64 direct-sum (a, pot) of the given caller indices (G = 1, spline

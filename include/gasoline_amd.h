@@ -75,6 +75,8 @@ enum gg_field {
     GG_IACTIVE,      /* iActive type bits (uint32 per particle, pkd.h:288-329) */
     GG_IRUNG,        /* iRung (int32 per particle) */
     GG_DTOLD,        /* dtOld */
+    GG_FWEIGHT,      /* fWeight: flops of the particle's bucket in the last gravity
+                        pass (pkdBucketWeight, pkd.c:2851-2861), for load balance */
     GG_NFIELDS
 };
 
@@ -197,7 +199,10 @@ int gg_build_binary(GGPKD pkd, int nBucket, int iOpenType, double dCrit,
  * ewald.c:180, pkdBucketEwald ewald.c:15; real-space images within fEwCut
  * boxes, k-space |h| <= fEwhCut; cubic box).  A periodic tree must be a
  * single system.  bComove and bDoSun must be 0.  Counters follow
- * pkd.c:2946-2949 and the flop models of grav.c:246-247 and ewald.c:175. */
+ * pkd.c:2946-2949 and the flop models of grav.c:246-247 and ewald.c:175.
+ * fWeight of every active particle is set to its bucket's flops (the
+ * interaction and Ewald counts of that bucket), as pkdBucketWeight does
+ * (pkd.c:2851-2861, 2998); inactive particles keep theirs. */
 int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
                 int iEwOrder, double fEwCut, double fEwhCut, int bComove,
                 double dRhoFac, int bDoSun, double dSunSoft, double *aSun,
@@ -212,6 +217,20 @@ int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int bEwald,
  * Call with ent == NULL to get the sizes (*nBuckets, *nEnt). */
 int gg_grav_lists(GGPKD pkd, int *nBuckets, int64_t *nEnt, int *bucket_node,
                   int64_t *off, int *cnt, int *ent);
+
+/* Checksums of the lists that the fused walk + interaction kernel (the walk
+ * gg_grav_all runs) evaluates, for parity checks of that kernel's own walk:
+ * for each active bucket b (the order of gg_grav_lists) and list kind k
+ * (0 PP, 1 softened cell, 2 Newtonian cell), sums[9b+3k] = the number of
+ * members, sums[9b+3k+1] = their sum (particle slots or node ids, as
+ * gg_grav_lists numbers them) and sums[9b+3k+2] = the sum of mix(member),
+ * the sums modulo 2^64, where mix is
+ * splitmix64's finaliser (z += 0x9E3779B97F4A7C15; z = (z ^ z>>30) *
+ * 0xBF58476D1CE4E5B9; z = (z ^ z>>27) * 0x94D049BB133111EB; z ^ z>>31).
+ * The isolated walk (nReps 0); a, fPot and dtGrav are not touched.
+ * GG_EUNSUP if this tree is walked by the listed walk.  No counterpart in
+ * the reference. */
+int gg_grav_walk_checksums(GGPKD pkd, int iOrder, uint64_t *sums);
 
 /* Tree shape of the last gg_build_binary, for parity checks: perm[k] is
  * the caller index of storage slot k; per node ints[6*i..] = iDim, pLower,
@@ -335,18 +354,21 @@ int gg_smooth_stats(GGPKD pkd, int64_t *s, int n);
  * counterpart in the reference, which runs such systems as separate jobs. */
 int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenType,
                     double dCrit, int iOrder);
-/* Capture the device work of one gg_grav_all for the current tree in a HIP
- * graph (kernel launches only), and replay it: gg_grav_replay adds to a,
- * fPot and dtGrav exactly as gg_grav_all does and returns the same counters.
- * A rebuild invalidates the graph.  GG_EUNSUP if the tree needs the listed
- * walk (gg_grav_all handles it). */
+/* Capture the device work of one isolated gg_grav_all (nReps 0, no Ewald)
+ * for the current tree in a HIP graph (kernel launches only), and replay it:
+ * gg_grav_replay adds to a, fPot and dtGrav and sets fWeight exactly as
+ * gg_grav_all does and returns the same counters.  A rebuild invalidates
+ * the graph.  GG_EUNSUP if the tree needs the listed walk (gg_grav_all
+ * handles it) or if the handle has a periodic box (gg_set_period): the
+ * periodic step is gg_grav_all's. */
 int gg_grav_graph(GGPKD pkd, int iOrder);
 int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double *pdCellSum,
                    double *pdSoftSum, double *pdFlop);
 /* Bucket-size autotuner: builds the tree (sysOff = NULL) or the forest with
  * each of the nCand candidate bucket sizes, times the device work of
  * gg_grav_all, and leaves the handle built with the fastest (*pBest);
- * ms[k] receives candidate k's time.  a, fPot and dtGrav are preserved. */
+ * ms[k] receives candidate k's time.  a, fPot, dtGrav and fWeight are
+ * preserved. */
 int gg_tune_bucket(GGPKD pkd, int nSys, const int *sysOff, int iOpenType, double dCrit,
                    int iOrder, const int *cand, int nCand, double *ms, int *pBest);
 

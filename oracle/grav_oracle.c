@@ -1191,10 +1191,13 @@ static double bucket_ewald(orc_tree *t, int ib, int nReps, double L, double fEwC
 /* pkdGravAll (pkd.c:2869-3060): every bucket with active particles walks
  * the replicas -nReps..nReps (periodic dimensions L[d] > 0; L = NULL for an
  * isolated system) and, when bEwald, adds the Ewald correction of the
- * root's multipoles (pkdEwaldInit + pkdBucketEwald, cubic box L[0]). */
-void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
+ * root's multipoles (pkdEwaldInit + pkdBucketEwald, cubic box L[0]).
+ * fweight (may be NULL): the active particles of each bucket receive the
+ * bucket's interaction + Ewald flops (pkdBucketWeight, pkd.c:2851-2861,
+ * 2998); the others keep the value the array holds. */
+void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
                            int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
-                           double *dtgrav, double *stats)
+                           double *dtgrav, double *stats, double *fweight)
 {
     int n = t->n, ic, i, nEwh = 0;
     double *sa = (double *)calloc((size_t)(n > 0 ? n : 1) * 3, sizeof(double));
@@ -1230,10 +1233,17 @@ void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, 
             dPart += na * counts[0] + na * (2 * (c->pUpper - c->pLower) - na + 1) / 2;
             dCell += na * counts[2];
             dSoft += na * counts[1];
-            dFlop += bucket_interact(t, ic, iOrder, &s, counts[0], counts[1], counts[2],
-                                     sa, sp, sd);
-            if (ewald)
-                dFlop += bucket_ewald(t, ic, nReps, L[0], fEwCut, iEwOrder, &mom, ewt, nEwh, sa, sp);
+            {
+                double fI = bucket_interact(t, ic, iOrder, &s, counts[0], counts[1], counts[2],
+                                            sa, sp, sd), fE = 0;
+                if (ewald)
+                    fE = bucket_ewald(t, ic, nReps, L[0], fEwCut, iEwOrder, &mom, ewt, nEwh,
+                                      sa, sp);
+                dFlop += fI + fE;
+                if (fweight)
+                    for (i = c->pLower; i <= c->pUpper; ++i)
+                        if (t->p[i].active) fweight[t->p[i].src] = fI + fE;
+            }
         }
         ic = c->iUpper;
     }
@@ -1251,10 +1261,18 @@ void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, 
     free(s.po); free(s.so); free(s.no);
 }
 
+void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
+                           int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
+                           double *dtgrav, double *stats)
+{
+    orc_grav_all_weighted(t, iOrder, nReps, L, bEwald, iEwOrder, fEwCut, fEwhCut, a, pot,
+                          dtgrav, stats, NULL);
+}
+
 void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
                   double *dtgrav, double *stats)
 {
-    orc_grav_all_periodic(t, iOrder, 0, NULL, 0, 0, 0.0, 0.0, a, pot, dtgrav, stats);
+    orc_grav_all_weighted(t, iOrder, 0, NULL, 0, 0, 0.0, 0.0, a, pot, dtgrav, stats, NULL);
 }
 
 /* ======================================================================

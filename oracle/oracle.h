@@ -109,6 +109,12 @@ void orc_grav_all(orc_tree *t, int iOrder, double *a, double *pot,
 void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
                            int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
                            double *dtgrav, double *stats);
+/* The same, and fweight[i] (input order, may be NULL) of each active
+ * particle = its bucket's interaction + Ewald flops (pkdBucketWeight,
+ * pkd.c:2851-2861); inactive particles keep the value fweight holds. */
+void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,
+                           int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
+                           double *dtgrav, double *stats, double *fweight);
 
 /* FP64 O(N^2) direct sum with gasoline's SPLINE softening (grav.h:74-97),
  * twoh = h_i + h_j.  Targets are the nt indices in idx (or all if idx is

@@ -67,6 +67,8 @@ def lib():
         L.orc_grav_all.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_grav_all_periodic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
                                             C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_grav_all_weighted.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
+                                            C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_direct_sum.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp, _dp]
         L.orc_spline.argtypes = [C.c_double, C.c_double, _dp, _dp]
         L.orc_splineq.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
@@ -210,18 +212,21 @@ class Tree:
             cap = int(cnt.max()) * 2
 
     def grav_all(self, nReps=0, period=None, bEwald=False, iEwOrder=4, fEwCut=2.6,
-                 fEwhCut=2.8):
+                 fEwhCut=2.8, fweight=None):
+        """a, fPot, dtGrav (input order) and the counters; with fweight (an
+        array of n doubles) its active entries receive their bucket's flops."""
         a = np.zeros((self.n, 3))
         pot = np.zeros(self.n)
         dtg = np.zeros(self.n)
         st = np.zeros(5)
-        if period is None:
-            lib().orc_grav_all(self.h, self.iOrder, P(a), P(pot), P(dtg), P(st))
-        else:
-            L = _d(period)
-            lib().orc_grav_all_periodic(self.h, self.iOrder, int(nReps), P(L), int(bEwald),
-                                        int(iEwOrder), fEwCut, fEwhCut, P(a), P(pot), P(dtg),
-                                        P(st))
+        L = None if period is None else _d(period)
+        fw = None
+        if fweight is not None:
+            assert fweight.dtype == np.float64 and fweight.shape == (self.n,)
+            fw = P(fweight)
+        lib().orc_grav_all_weighted(self.h, self.iOrder, int(nReps) if L is not None else 0,
+                                    None if L is None else P(L), int(bEwald), int(iEwOrder),
+                                    fEwCut, fEwhCut, P(a), P(pot), P(dtg), P(st), fw)
         return a, pot, dtg, dict(nActive=st[0], dPartSum=st[1], dCellSum=st[2],
                                  dSoftSum=st[3], dFlop=st[4])
 

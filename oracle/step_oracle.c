@@ -28,9 +28,10 @@ void orc_init_accel(int n, const unsigned *bits, double *a, double *pot, double 
     }
 }
 
-/* pkdKick (pkd.c:3780-3967), NOCOOLING: active gas advances vPred, v and,
- * unless isothermal, uPred and u by UDOT_HYDRO (pkd.h:65); other active
- * particles vPred and v. */
+/* pkdKick (pkd.c:3780-3967), NOCOOLING GASOLINE build: active gas advances
+ * vPred, v and, unless isothermal, uPred and u by UDOT_HYDRO (pkd.h:65);
+ * other active particles only v (vPred is written for them only in builds
+ * without GASOLINE, pkd.c:3953-3957). */
 void orc_kick(int n, const unsigned *bits, double dvFacOne, double dvFacTwo,
               double dvPredFacOne, double dvPredFacTwo, double duDelta, double duPredDelta,
               int bIsothermal, const double *a, const double *uDotPdV, const double *uDotAV,
@@ -40,7 +41,8 @@ void orc_kick(int n, const unsigned *bits, double dvFacOne, double dvFacTwo,
     for (i = 0; i < n; ++i) {
         if (!(bits[i] & ACT)) continue;
         for (j = 0; j < 3; ++j) {
-            vPred[3 * i + j] = v[3 * i + j] * dvPredFacOne + a[3 * i + j] * dvPredFacTwo;
+            if (bits[i] & GAS)
+                vPred[3 * i + j] = v[3 * i + j] * dvPredFacOne + a[3 * i + j] * dvPredFacTwo;
             v[3 * i + j] = v[3 * i + j] * dvFacOne + a[3 * i + j] * dvFacTwo;
         }
         if ((bits[i] & GAS) && !bIsothermal) {

@@ -132,6 +132,47 @@ def test_interaction_lists_bitexact(gpu, plummer32k):
         assert st[key] == sto[key], key
 
 
+def mix64(x):
+    """splitmix64's finaliser, as gg_grav_walk_checksums sums it."""
+    z = np.asarray(x, np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+@pytest.mark.parametrize("case", ["plummer32k", "softened", "hexadecapole"])
+def test_fused_walk_lists_checksums(gpu, plummer32k, case):
+    # the walk inside the fused kernel that gg_grav_all runs evaluates, per
+    # bucket and kind, exactly the oracle's members: their count, sum and
+    # sum of mixes (order-independent) agree bucket by bucket
+    if case == "plummer32k":
+        r, m, soft, T, _ = plummer32k
+        nb, theta, order, act = 16, 0.7, 2, None
+    else:
+        r, v, m = ics.plummer(16384, seed=41)
+        soft = np.full(len(m), 0.15 if case == "softened" else 0.01)
+        nb, theta, order = (16, 0.7, 2) if case == "softened" else (8, 0.5, 4)
+        act = (np.random.default_rng(4).uniform(size=len(m)) < 0.5).astype(np.uint32) | 2
+        T = O.Tree(r, m, soft, iactive=act, nBucket=nb, dTheta=theta, iOrder=order)
+    for prec in (gpu.PREC_FP32, gpu.PREC_FP64):
+        with setup_pkd(gpu, r, m, soft, prec, iactive=act) as pk:
+            pk.build_binary(nb, 1, theta, order)
+            bk = pk.grav_lists()[0]
+            chk = pk.walk_checksums(order)
+            assert pk.grav_all(order) is not None
+        assert np.array_equal(bk, T.buckets())
+        nsoft = 0
+        for b, ib in enumerate(bk):
+            for k, ref in enumerate(T.walk(ib)):
+                ref = ref.astype(np.uint64)
+                assert chk[b, k, 0] == len(ref), (ib, k)
+                assert chk[b, k, 1] == ref.sum(dtype=np.uint64), (ib, k)
+                assert chk[b, k, 2] == mix64(ref).sum(dtype=np.uint64), (ib, k)
+                nsoft += len(ref) if k == 1 else 0
+        if case == "softened":
+            assert nsoft > 0
+
+
 @pytest.mark.parametrize("prec,rms,mx", [("FP64", 1e-8, 1e-7), ("FP32", 1e-5, 1e-4)])
 def test_forces_vs_oracle(gpu, plummer32k, prec, rms, mx):
     r, m, soft, T, (ao, po, dto, sto) = plummer32k
@@ -229,11 +270,17 @@ def test_small_and_degenerate_inputs(gpu):
         a = pk.get("a")
     assert st["dPartSum"] == 10 and st["dCellSum"] == 0
     assert np.allclose(a, ad, rtol=1e-13)
-    # 100 coincident particles make a bucket too large for one wavefront
+    # 100 coincident particles: one zero-extent bucket of 100 (pkd.c:2489),
+    # every pair at r = 0 inside the softening
     r = np.zeros((100, 3))
-    with setup_pkd(gpu, r, np.ones(100), np.full(100, 0.1), gpu.PREC_FP64) as pk:
-        with pytest.raises(gpu.GGError):
+    ad, pd = O.direct_sum(r, np.ones(100), np.full(100, 0.1))
+    for prec in (gpu.PREC_FP64, gpu.PREC_FP32):
+        with setup_pkd(gpu, r, np.ones(100), np.full(100, 0.1), prec) as pk:
             pk.build_binary(16, 1, 0.7, 2)
+            st = pk.grav_all(2)
+            a, pot = pk.get("a"), pk.get("fPot")
+        assert st["dPartSum"] == 100 * 99 / 2
+        assert (a == 0).all() and np.allclose(pot, pd, rtol=1e-6)
 
 
 def test_direct_sum_vs_oracle(gpu):

@@ -49,6 +49,11 @@ def test_kick_touches_only_active_and_energy_only_of_gas():
     gas = act & ((f["iActive"] & 256) != 0)
     assert np.array_equal(k["v"][~act], f["v"][~act])
     assert np.array_equal(k["u"][~gas], f["u"][~gas])
+    # vPred is predicted for active gas only: a GASOLINE build leaves dark
+    # and star vPred alone (pkd.c:3953-3957)
+    assert np.array_equal(k["vPred"][~gas], f["vPred"][~gas])
+    assert np.allclose(k["vPred"][gas], f["v"][gas] + 0.02 * f["a"][gas], rtol=1e-14)
+    assert not np.array_equal(k["v"][act & ~gas], f["v"][act & ~gas])
     ud = f["uDotPdV"] + f["uDotAV"] + f["uDotDiff"]
     assert np.allclose(k["u"][gas], f["u"][gas] + 0.01 * ud[gas], rtol=1e-15, atol=0)
     assert np.allclose(k["uPred"][gas], f["u"][gas] + 0.02 * ud[gas], rtol=1e-15, atol=0)
