@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cstring>
 #include <initializer_list>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -233,6 +234,33 @@ __device__ __forceinline__ int one_dt_to_rung(int iRung, double dDelta, double d
     return r;
 }
 
+// pkdEmergencyAdjust (pkd.c:3971-4030, GASOLINE, NOCOOLING): a gas particle on a rung
+// below iRung whose Courant step dtNew fell under dDeltaThresh moves to the
+// rung of dt = dtNew (capped at iMaxRung - 1) and is un-kicked to its
+// predicted velocity.  red[0] = particles moved, red[1] = largest rung
+// they took, red[2] = ideal largest rung, red[3 + r] = moved particles on
+// rung r.
+__global__ void k_emergency(int n, int iRung, int iMaxRung, double dDelta, double dDeltaThresh,
+                            const uint32_t *__restrict__ bits, const double *__restrict__ dtNew,
+                            double *__restrict__ dt, int *__restrict__ rung,
+                            const double *__restrict__ vpred, double *__restrict__ v,
+                            int *__restrict__ red)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(bits[i] & GG_TYPE_GAS)) return;
+    if (!(rung[i] < iRung && dtNew[i] < dDeltaThresh)) return;
+    const double d = dtNew[i];
+    dt[i] = d;
+    int t = one_dt_to_rung(iRung, dDelta, d);
+    atomicMax(&red[2], t + 1);
+    if (t >= iMaxRung) t = iMaxRung - 1;
+    rung[i] = t;
+    atomicAdd(&red[0], 1);
+    atomicMax(&red[1], t);
+    atomicAdd(&red[3 + t], 1);
+    for (int j = 0; j < 3; ++j) v[3 * (size_t)i + j] = vpred[3 * (size_t)i + j];
+}
+
 // pkdDtToRung (pkd.c:4729-4809) without the diagnostic print.  red[0]
 // counts broken assertions, red[1] = largest rung, red[2] = ideal largest
 // rung (k_rung_count adds red[3] = particles on red[1]).
@@ -276,21 +304,48 @@ __global__ void k_dt_to_rung(int n, int iRung, double dDelta, int iMaxRung, int 
     }
 }
 
-// pkdActiveMaskRung (pkd.c:5228-5252): the mask bits set on the particles
-// of rung iRung (and above with bGreater), cleared on the others.
-__global__ void k_active_rung(int n, uint32_t mask, int iRung, int bGreater,
-                              const int *__restrict__ rung, uint32_t *__restrict__ bits,
-                              int *__restrict__ count)
+// The type-bit updates of pkd.c:5114-5281 in one kernel.  A particle
+// matches when its bits pass the test of the operation (and, with a rung
+// test, it sits on rung iRung or above it with bGreater); then
+//   TB_ACTIVE  (pkdActiveType, ActiveTypeRung, ActiveMaskRung, ActiveExactType):
+//              iSetMask set on the matching particles, cleared on the others;
+//   TB_SET     (pkdSetType):   iSetMask set on the matching ones;
+//   TB_RESET   (pkdResetType): iSetMask cleared on the matching ones.
+// Tests: TT_ANY bits & iTestMask nonzero (TYPETest), TT_EXACT (bits &
+// iFilterMask) == iTestMask (TYPEFilter), TT_ALL every particle.
+// count[0] = matching particles, count[1] = particles whose TYPE_ACTIVE bit
+// changed, count[2] = particles whose TYPE_GAS bit changed.
+enum { TB_ACTIVE = 0, TB_SET = 1, TB_RESET = 2 };
+enum { TT_ANY = 0, TT_EXACT = 1, TT_ALL = 2 };
+__global__ void k_type_bits(int n, int op, int test, uint32_t iFilterMask, uint32_t iTestMask,
+                            uint32_t iSetMask, int bRung, int iRung, int bGreater,
+                            const int *__restrict__ rung, uint32_t *__restrict__ bits,
+                            int *__restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool on = false;
+    bool on = false, dAct = false, dGas = false;
     if (i < n) {
-        const int r = rung[i];
-        on = r == iRung || (bGreater && r > iRung);
-        bits[i] = on ? (bits[i] | mask) : (bits[i] & ~mask);
+        const uint32_t b = bits[i];
+        on = test == TT_ALL ? true
+             : test == TT_EXACT ? (b & iFilterMask) == iTestMask : (b & iTestMask) != 0;
+        if (bRung) {
+            const int r = rung[i];
+            on = on && (r == iRung || (bGreater && r > iRung));
+        }
+        uint32_t nb = b;
+        if (op == TB_ACTIVE) nb = on ? (b | iSetMask) : (b & ~iSetMask);
+        else if (on) nb = op == TB_SET ? (b | iSetMask) : (b & ~iSetMask);
+        dAct = ((nb ^ b) & GG_TYPE_ACTIVE) != 0;
+        dGas = ((nb ^ b) & GG_TYPE_GAS) != 0;
+        if (nb != b) bits[i] = nb;
     }
-    const int c = __popcll(__ballot(on));
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+    const int c0 = __popcll(__ballot(on)), c1 = __popcll(__ballot(dAct)),
+              c2 = __popcll(__ballot(dGas));
+    if ((threadIdx.x & 63) == 0) {
+        if (c0) atomicAdd(count, c0);
+        if (c1) atomicAdd(count + 1, c1);
+        if (c2) atomicAdd(count + 2, c2);
+    }
 }
 
 __global__ void k_rung_count(int n, const int *__restrict__ rung, int *__restrict__ red)
@@ -568,27 +623,112 @@ extern "C" int gg_dt_to_rung(GGPKD pkd, int iRung, double dDelta, int iMaxRung, 
     return GG_OK;
 }
 
-extern "C" int gg_active_mask_rung(GGPKD pkd, unsigned iSetMask, int iRung, int bGreater,
-                                   int *pnActive)
+// Run k_type_bits and keep the trees that do not depend on what changed:
+// the gravity tree holds the active buckets and their active boxes (TYPE_ACTIVE),
+// the SPH structure its gas particles (TYPE_GAS); the other bits are read by
+// the calls that use them.
+static int type_bits(GGPKD pkd, int op, int test, unsigned iFilterMask, unsigned iTestMask,
+                     unsigned iSetMask, int bRung, int iRung, int bGreater, int *pn)
 {
     GG_CHECK(pkd, GG_EINVAL, "null handle");
     const int n = pkd->nLocal;
-    if (pnActive) *pnActive = 0;
+    if (pn) *pn = 0;
     if (n == 0) return GG_OK;
-    GG_TRY(need_fields(pkd, {GG_IRUNG, GG_IACTIVE}));
-    GG_TRY(pkd->red.reserve(sizeof(int)));
-    GG_HIP(hipMemsetAsync(pkd->red.p, 0, sizeof(int), pkd->stream));
-    hipLaunchKernelGGL(k_active_rung, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
-                       (uint32_t)iSetMask, iRung, bGreater, pkd->f[GG_IRUNG].as<int>(),
+    if (bRung) GG_TRY(need_fields(pkd, {GG_IRUNG, GG_IACTIVE}));
+    else GG_TRY(need_fields(pkd, {GG_IACTIVE}));
+    GG_TRY(pkd->red.reserve(3 * sizeof(int)));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, 3 * sizeof(int), pkd->stream));
+    hipLaunchKernelGGL(k_type_bits, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, op, test,
+                       (uint32_t)iFilterMask, (uint32_t)iTestMask, (uint32_t)iSetMask, bRung,
+                       iRung, bGreater, bRung ? pkd->f[GG_IRUNG].as<int>() : (const int *)nullptr,
                        pkd->f[GG_IACTIVE].as<uint32_t>(), pkd->red.as<int>());
     GG_HIP(hipGetLastError());
-    int c = 0;
-    GG_HIP(hipMemcpyAsync(&c, pkd->red.p, sizeof(int), hipMemcpyDeviceToHost, pkd->stream));
+    int c[3] = {0, 0, 0};
+    GG_HIP(hipMemcpyAsync(c, pkd->red.p, sizeof(c), hipMemcpyDeviceToHost, pkd->stream));
     GG_HIP(hipStreamSynchronize(pkd->stream));
-    // the type bits changed: trees built on them are stale
-    if (iSetMask & (GG_TYPE_ACTIVE | GG_TYPE_TREEACTIVE)) pkd->grav.built = 0;
-    pkd->sph.gasMaskValid = 0;
-    if (iSetMask & (GG_TYPE_TREEACTIVE | GG_TYPE_SMOOTHACTIVE)) pkd->sph.built = 0;
-    if (pnActive) *pnActive = c;
+    if (c[1]) pkd->grav.built = 0;
+    if (c[2]) {
+        pkd->sph.gasMaskValid = 0;
+        pkd->sph.built = 0;
+    }
+    if (pn) *pn = c[0];
+    return GG_OK;
+}
+
+extern "C" int gg_emergency_adjust(GGPKD pkd, int iRung, int iMaxRung, double dDelta,
+                                   double dDeltaThresh, int *pnUn, int *piMaxRungIdeal,
+                                   int *pnMaxRung, int *piMaxRungOut)
+{
+    GG_CHECK(pkd && pnUn && piMaxRungIdeal && pnMaxRung && piMaxRungOut, GG_EINVAL,
+             "null argument");
+    GG_CHECK(dDeltaThresh < dDelta, GG_EINVAL, "dDeltaThresh must be below dDelta");
+    GG_CHECK(iMaxRung >= 1 && iMaxRung <= 60, GG_EINVAL, "iMaxRung %d outside 1..60", iMaxRung);
+    const int n = pkd->nLocal;
+    *pnUn = *piMaxRungIdeal = *pnMaxRung = *piMaxRungOut = 0;
+    if (n == 0) return GG_OK;
+    GG_TRY(need_fields(pkd, {GG_IRUNG, GG_DT, GG_DTNEW, GG_V, GG_VPRED, GG_IACTIVE}));
+    const int nred = 3 + iMaxRung;
+    GG_TRY(pkd->red.reserve(sizeof(int) * nred));
+    GG_HIP(hipMemsetAsync(pkd->red.p, 0, sizeof(int) * nred, pkd->stream));
+    hipLaunchKernelGGL(k_emergency, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n, iRung,
+                       iMaxRung, dDelta, dDeltaThresh, pkd->f[GG_IACTIVE].as<uint32_t>(),
+                       pkd->f[GG_DTNEW].as<double>(), pkd->f[GG_DT].as<double>(),
+                       pkd->f[GG_IRUNG].as<int>(), pkd->f[GG_VPRED].as<double>(),
+                       pkd->f[GG_V].as<double>(), pkd->red.as<int>());
+    GG_HIP(hipGetLastError());
+    std::vector<int> red(nred);
+    GG_HIP(hipMemcpyAsync(red.data(), pkd->red.p, sizeof(int) * nred, hipMemcpyDeviceToHost,
+                          pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    *pnUn = red[0];
+    *piMaxRungOut = red[1];
+    *piMaxRungIdeal = red[2];
+    *pnMaxRung = red[0] ? red[3 + red[1]] : 0;
+    return GG_OK;
+}
+
+extern "C" int gg_active_mask_rung(GGPKD pkd, unsigned iSetMask, int iRung, int bGreater,
+                                   int *pnActive)
+{
+    return type_bits(pkd, TB_ACTIVE, TT_ALL, 0, 0, iSetMask, 1, iRung, bGreater, pnActive);
+}
+
+extern "C" int gg_active_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pnActive)
+{
+    return type_bits(pkd, TB_ACTIVE, TT_ANY, 0, iTestMask, iSetMask, 0, 0, 0, pnActive);
+}
+
+extern "C" int gg_active_type_rung(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int iRung,
+                                   int bGreater, int *pnActive)
+{
+    return type_bits(pkd, TB_ACTIVE, TT_ANY, 0, iTestMask, iSetMask, 1, iRung, bGreater,
+                     pnActive);
+}
+
+extern "C" int gg_active_exact_type(GGPKD pkd, unsigned iFilterMask, unsigned iTestMask,
+                                    unsigned iSetMask, int *pnActive)
+{
+    return type_bits(pkd, TB_ACTIVE, TT_EXACT, iFilterMask, iTestMask, iSetMask, 0, 0, 0,
+                     pnActive);
+}
+
+extern "C" int gg_set_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pn)
+{
+    return type_bits(pkd, TB_SET, TT_ANY, 0, iTestMask, iSetMask, 0, 0, 0, pn);
+}
+
+extern "C" int gg_reset_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pn)
+{
+    return type_bits(pkd, TB_RESET, TT_ANY, 0, iTestMask, iSetMask, 0, 0, 0, pn);
+}
+
+extern "C" int gg_sph_curr_rung(GGPKD pkd, int iRung, int bGreater, int *piCurrent)
+{
+    // pkdSphCurrRung (pkd.c:5876-5890): is a gas particle on the rung(s)?
+    // (set bits of nothing: only the count of TYPE_GAS particles on them)
+    GG_CHECK(piCurrent, GG_EINVAL, "null argument");
+    int c = 0;
+    GG_TRY(type_bits(pkd, TB_SET, TT_ANY, 0, GG_TYPE_GAS, 0, 1, iRung, bGreater, &c));
+    *piCurrent = c > 0;
     return GG_OK;
 }

@@ -3,9 +3,10 @@
  *
  * These are the per-process parts of gasoline's master-level force steps,
  * written against include/gasoline_amd.h the way gasoline's PST leaves
- * would call it: msrBuildTree + msrGravity (master.c:4249, 5869) and the
- * SPH block of msrSph (master.c:9258-9362) for a step in which every gas
- * particle is active.
+ * would call it: msrBuildTree + msrGravity (master.c:4249, 5869), the SPH
+ * block of msrSph (master.c:9258-9362) and the block-timestep recursion
+ * msrTopStepKDK (master.c:8242-8450) with its kicks, drifts, timestep
+ * criteria and rungs, gravity and SPH.
  */
 #ifndef GG_MSR_H
 #define GG_MSR_H
@@ -33,6 +34,55 @@ int ggMsrGravity(GGPKD pkd, int nBucket, double dTheta, int iOrder, int bVerbose
  * not-FastGas branch).  Returns 0 or a GG_E* code. */
 int ggMsrSph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
              const double *fPeriod, int bVerbose);
+
+/* The msr->param fields that msrTopStepKDK and the calls it makes read
+ * (non-comoving, NOCOOLING; param defaults of master.c in brackets). */
+typedef struct ggStepParam {
+    double dDelta;            /* the top step (msrInitDt uses it) */
+    int iMaxRung;             /* rungs 0 .. iMaxRung-1 */
+    double dEta;              /* gravity / acceleration criteria */
+    double dEtaCourant;       /* [0.4] */
+    double dEtauDot;          /* [0.25] */
+    double duMinDt;           /* [0] */
+    int bGravStep, bAccelStep, bEpsAccStep, bSqrtPhiStep, bSphStep, bViscosityLimitdt;
+    int bDoGravity, bDoGas;
+    int nBucket, iOrder;      /* gravity tree */
+    double dTheta;
+    int nSmooth, nBucketSph;
+    gg_smf smf;               /* the operators' SMF: dtFacCourant = dEtaCourant 2/1.6,
+                                 dtFacDiffusion = 2 dEtaDiffusion (master.c:5614-5615) */
+    int bPeriodic;
+    double fPeriod[3], fCenter[3];
+} ggStepParam;
+
+/* What the recursion keeps between calls, and a log of its force
+ * evaluations (iKickRung and active count of each, up to nLogCap). */
+typedef struct ggStepState {
+    int iCurrMaxRung;         /* msr->iCurrMaxRung */
+    int nForce;               /* force evaluations so far */
+    int nLogCap;
+    int *logKickRung, *logActive;
+    int nEmergency;           /* particles moved by emergency adjusts */
+    double dtMinGas;
+} ggStepState;
+
+/* msrTopStepKDK (master.c:8242-8450) for one process: adjust the rungs of
+ * the particles on iRung and deeper (iAdjust; the emergency adjust and the
+ * gravity, acceleration and SPH timestep criteria, master.c:8261-8314), open
+ * kick of half the rung's step, then either recurse twice into rung iRung+1
+ * or drift everyone by two half steps (the gas's predicted velocity and
+ * energy advanced with each, msrDrift's KickVpred) and evaluate the forces
+ * of the particles on rungs >= iKickRung (gravity, and msrSph when a gas
+ * particle is on them), then the closing kick.  Particles need their type
+ * bits (TYPE_GAS / TYPE_DARK / TYPE_STAR).  Returns 0 or a GG_E* code. */
+int ggMsrTopStepKDK(GGPKD pkd, const ggStepParam *p, ggStepState *s, double dStep, double dTime,
+                    double dDelta, int iRung, int iKickRung, int iAdjust);
+
+/* The force block of msrTopStepKDK (master.c:8375-8421) for the particles
+ * on rungs >= iKickRung, and msrSph (master.c:9258-9362, not FastGas) on
+ * its own. */
+int ggMsrForces(GGPKD pkd, const ggStepParam *p, ggStepState *s, int iKickRung);
+int ggMsrSphRung(GGPKD pkd, const ggStepParam *p, int iKickRung);
 
 #ifdef __cplusplus
 }

@@ -59,7 +59,9 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
        "gg_kick_vpred", "gg_init_dt", "gg_grav_step", "gg_accel_step", "gg_density_step",
-       "gg_sph_step", "gg_dt_to_rung", "gg_active_mask_rung",
+       "gg_sph_step", "gg_dt_to_rung", "gg_active_mask_rung", "gg_active_type",
+       "gg_active_type_rung", "gg_active_exact_type", "gg_set_type", "gg_reset_type",
+       "gg_sph_curr_rung", "gg_emergency_adjust",
        "gg_hmlp_init", "gg_hmlp_record", "gg_hmlp_train", "gg_hmlp_predict",
        "gg_hmlp_params", "gg_hmlp_eval", "gg_hmlp_grad"]
 
@@ -166,6 +168,14 @@ def lib():
     L.gg_dt_to_rung.argtypes = [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _ip, _ip,
                                 _ip]
     L.gg_active_mask_rung.argtypes = [_vp, C.c_uint, C.c_int, C.c_int, _ip]
+    L.gg_active_type.argtypes = [_vp, C.c_uint, C.c_uint, _ip]
+    L.gg_active_type_rung.argtypes = [_vp, C.c_uint, C.c_uint, C.c_int, C.c_int, _ip]
+    L.gg_active_exact_type.argtypes = [_vp, C.c_uint, C.c_uint, C.c_uint, _ip]
+    L.gg_set_type.argtypes = [_vp, C.c_uint, C.c_uint, _ip]
+    L.gg_reset_type.argtypes = [_vp, C.c_uint, C.c_uint, _ip]
+    L.gg_sph_curr_rung.argtypes = [_vp, C.c_int, C.c_int, _ip]
+    L.gg_emergency_adjust.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_double, _ip, _ip,
+                                      _ip, _ip]
     _fp = C.POINTER(C.c_float)
     L.gg_hmlp_init.argtypes = [_vp, C.c_uint]
     L.gg_hmlp_record.argtypes = [_vp, C.c_double]
@@ -475,6 +485,39 @@ class Pkd:
                                          C.byref(n)))
         return n.value
 
+    def active_type(self, iTestMask, iSetMask, iRung=None, bGreater=True, iFilterMask=None):
+        """gg_active_type (pkdActiveType); with iRung gg_active_type_rung,
+        with iFilterMask gg_active_exact_type.  Returns the matching count."""
+        n = C.c_int()
+        if iFilterMask is not None:
+            _check(lib().gg_active_exact_type(self.h, int(iFilterMask), int(iTestMask),
+                                              int(iSetMask), C.byref(n)))
+        elif iRung is not None:
+            _check(lib().gg_active_type_rung(self.h, int(iTestMask), int(iSetMask), int(iRung),
+                                             int(bool(bGreater)), C.byref(n)))
+        else:
+            _check(lib().gg_active_type(self.h, int(iTestMask), int(iSetMask), C.byref(n)))
+        return n.value
+
+    def set_type(self, iTestMask, iSetMask, reset=False):
+        """gg_set_type / gg_reset_type (pkdSetType / pkdResetType)."""
+        n = C.c_int()
+        fn = lib().gg_reset_type if reset else lib().gg_set_type
+        _check(fn(self.h, int(iTestMask), int(iSetMask), C.byref(n)))
+        return n.value
+
+    def sph_curr_rung(self, iRung, bGreater=True):
+        c = C.c_int()
+        _check(lib().gg_sph_curr_rung(self.h, int(iRung), int(bool(bGreater)), C.byref(c)))
+        return bool(c.value)
+
+    def emergency_adjust(self, iRung, iMaxRung, dDelta, dDeltaThresh):
+        """gg_emergency_adjust; returns (nUn, iMaxRungOut, nMaxRung, iMaxRungIdeal)."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().gg_emergency_adjust(self.h, int(iRung), int(iMaxRung), float(dDelta),
+                                         float(dDeltaThresh), *[C.byref(x) for x in v]))
+        return v[0].value, v[3].value, v[2].value, v[1].value
+
     def dt_to_rung(self, iRung, dDelta, iMaxRung, bAll=True, bDiagExceed=False):
         """Returns (iMaxRungOut, nMaxRung, iMaxRungIdeal)."""
         nm, ideal, top = C.c_int(), C.c_int(), C.c_int()
@@ -533,3 +576,86 @@ class Pkd:
         return dict(zip(["walk", "interact", "sph_tree", "knn", "density", "lists",
                          "revlists", "hydro", "other", "knn_lists", "knn_retry"],
                         (float(x) for x in t)))
+
+
+# ---------------------------------------------------- C host sequences
+class StepParam(C.Structure):
+    """ggStepParam (gasoline_amd/host/gg_msr.h): the msr->param fields the
+    block-timestep recursion reads."""
+    _fields_ = [("dDelta", C.c_double), ("iMaxRung", C.c_int), ("dEta", C.c_double),
+                ("dEtaCourant", C.c_double), ("dEtauDot", C.c_double), ("duMinDt", C.c_double),
+                ("bGravStep", C.c_int), ("bAccelStep", C.c_int), ("bEpsAccStep", C.c_int),
+                ("bSqrtPhiStep", C.c_int), ("bSphStep", C.c_int), ("bViscosityLimitdt", C.c_int),
+                ("bDoGravity", C.c_int), ("bDoGas", C.c_int), ("nBucket", C.c_int),
+                ("iOrder", C.c_int), ("dTheta", C.c_double), ("nSmooth", C.c_int),
+                ("nBucketSph", C.c_int), ("smf", Smf), ("bPeriodic", C.c_int),
+                ("fPeriod", C.c_double * 3), ("fCenter", C.c_double * 3)]
+
+
+class StepState(C.Structure):
+    _fields_ = [("iCurrMaxRung", C.c_int), ("nForce", C.c_int), ("nLogCap", C.c_int),
+                ("logKickRung", _ip), ("logActive", _ip), ("nEmergency", C.c_int),
+                ("dtMinGas", C.c_double)]
+
+
+_MSR = None
+
+
+def msr_lib():
+    """gasoline_amd/host/libggmsr.so, the C host sequences (built with the
+    library)."""
+    global _MSR
+    if _MSR is None:
+        path = os.path.join(_HERE, "host", "libggmsr.so")
+        if not os.path.exists(path):
+            raise RuntimeError("gasoline_amd/host/libggmsr.so is not built (run make in csrc)")
+        lib()
+        L = C.CDLL(path)
+        L.ggMsrTopStepKDK.argtypes = [_vp, C.POINTER(StepParam), C.POINTER(StepState),
+                                      C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int]
+        L.ggMsrForces.argtypes = [_vp, C.POINTER(StepParam), C.POINTER(StepState), C.c_int]
+        _MSR = L
+    return _MSR
+
+
+class BlockSteps:
+    """Drives ggMsrTopStepKDK (msrTopStepKDK, master.c:8242) of the C host
+    over a Pkd: p holds the parameters, the state keeps iCurrMaxRung and a
+    log of the force evaluations."""
+
+    def __init__(self, pk, dDelta, iMaxRung=8, dEta=0.2, dEtaCourant=0.4, smf=None,
+                 bDoGravity=True, bDoGas=True, bGravStep=True, bSphStep=True, dTheta=0.7,
+                 iOrder=2, nBucket=16, nSmooth=32, nBucketSph=32, period=None, log_cap=4096):
+        smf = smf if smf is not None else Smf.default()
+        smf.dtFacCourant = dEtaCourant * 2 / 1.6
+        self.pk = pk
+        self.p = StepParam(dDelta=dDelta, iMaxRung=iMaxRung, dEta=dEta, dEtaCourant=dEtaCourant,
+                           dEtauDot=0.25, duMinDt=0.0, bGravStep=int(bGravStep), bAccelStep=0,
+                           bEpsAccStep=0, bSqrtPhiStep=0, bSphStep=int(bSphStep),
+                           bViscosityLimitdt=0, bDoGravity=int(bDoGravity), bDoGas=int(bDoGas),
+                           nBucket=nBucket, iOrder=iOrder, dTheta=dTheta, nSmooth=nSmooth,
+                           nBucketSph=nBucketSph, smf=smf, bPeriodic=int(period is not None))
+        if period is not None:
+            for k in range(3):
+                self.p.fPeriod[k] = period[k]
+        self._rung = np.zeros(log_cap, np.int32)
+        self._act = np.zeros(log_cap, np.int32)
+        self.s = StepState(iCurrMaxRung=0, nForce=0, nLogCap=log_cap,
+                           logKickRung=_p(self._rung, _ip), logActive=_p(self._act, _ip),
+                           nEmergency=0, dtMinGas=0.0)
+        self.dTime = 0.0
+
+    def forces(self, iKickRung=0):
+        rc = msr_lib().ggMsrForces(self.pk.h, C.byref(self.p), C.byref(self.s), int(iKickRung))
+        _check(rc)
+
+    def top_step(self, iAdjust=True):
+        rc = msr_lib().ggMsrTopStepKDK(self.pk.h, C.byref(self.p), C.byref(self.s), 0.0,
+                                      self.dTime, self.p.dDelta, 0, 0, int(iAdjust))
+        _check(rc)
+        self.dTime += self.p.dDelta
+
+    def force_log(self):
+        n = min(self.s.nForce, self.s.nLogCap)
+        return list(zip(self._rung[:n].tolist(), self._act[:n].tolist()))

@@ -400,8 +400,39 @@ int gg_kick_vpred(GGPKD pkd, double dvFacOne, double dvFacTwo, double duDelta, i
  * TYPE_ACTIVE): the bits of iSetMask set on the particles of rung iRung
  * (and above when bGreater) and cleared on the others; *pnActive (may be
  * null) receives their number.  A gravity tree has to be built again
- * afterwards, as gg_put of iActive requires. */
+ * afterwards if a TYPE_ACTIVE bit changed. */
 int gg_active_mask_rung(GGPKD pkd, unsigned iSetMask, int iRung, int bGreater, int *pnActive);
+/* Type bits, as the reference's pkd functions set them (TYPETest: any bit
+ * of iTestMask; TYPEFilter: (bits & iFilterMask) == iTestMask); *pn (may
+ * be null) receives the number of particles that matched.  A gravity tree
+ * is kept unless some TYPE_ACTIVE bit changed, the SPH structure unless
+ * some TYPE_GAS bit did.
+ *   gg_active_type       pkdActiveType (pkd.c:5203): iSetMask set where the
+ *                        bits test iTestMask, cleared elsewhere;
+ *   gg_active_type_rung  pkdActiveTypeRung (pkd.c:5255): the same, restricted
+ *                        to rung iRung (and above with bGreater);
+ *   gg_active_exact_type pkdActiveExactType (pkd.c:5114): set where the bits
+ *                        filter to iTestMask, cleared elsewhere;
+ *   gg_set_type          pkdSetType (pkd.c:5138): set where the bits test;
+ *   gg_reset_type        pkdResetType (pkd.c:5161): cleared where they test;
+ *   gg_sph_curr_rung     pkdSphCurrRung (pkd.c:5876): *piCurrent = 1 if a
+ *                        TYPE_GAS particle sits on the rung(s). */
+int gg_active_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pn);
+/* pkdEmergencyAdjust (pkd.c:3971, GASOLINE with NOCOOLING): every gas particle on
+ * a rung below iRung whose dtNew fell under dDeltaThresh (< dDelta) takes
+ * dt = dtNew, the rung of that dt (capped at iMaxRung - 1) and v = vPred.
+ * *pnUn particles moved, the largest rung they took (*piMaxRungOut) with
+ * *pnMaxRung of them on it, and the ideal largest rung. */
+int gg_emergency_adjust(GGPKD pkd, int iRung, int iMaxRung, double dDelta, double dDeltaThresh,
+                        int *pnUn, int *piMaxRungIdeal, int *pnMaxRung, int *piMaxRungOut);
+int gg_active_type_rung(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int iRung,
+                        int bGreater, int *pn);
+int gg_active_exact_type(GGPKD pkd, unsigned iFilterMask, unsigned iTestMask, unsigned iSetMask,
+                         int *pn);
+int gg_set_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pn);
+int gg_reset_type(GGPKD pkd, unsigned iTestMask, unsigned iSetMask, int *pn);
+int gg_sph_curr_rung(GGPKD pkd, int iRung, int bGreater, int *piCurrent);
+
 /* These update dt (and iRung, dtOld) of the store like their reference
  * functions, bit for bit: no contraction into FMA. */
 

@@ -68,7 +68,10 @@ static void vs_setup(void)
 void orc_vsqrt1(int n, const double *r2, double *out)
 {
     int i;
-    if (!vs_ready) vs_setup();
+    if (!vs_ready) {
+#pragma omp critical(orc_vs_setup)
+        if (!vs_ready) vs_setup();
+    }
     for (i = 0; i < n; ++i) {
         unsigned long long bits;
         int e;
@@ -1199,31 +1202,49 @@ void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, 
                            int iEwOrder, double fEwCut, double fEwhCut, double *a, double *pot,
                            double *dtgrav, double *stats, double *fweight)
 {
-    int n = t->n, ic, i, nEwh = 0;
+    int n = t->n, ic, i, nEwh = 0, nbk = 0, k;
     double *sa = (double *)calloc((size_t)(n > 0 ? n : 1) * 3, sizeof(double));
     double *sp = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     double *sd = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
-    OScratch s = {0};
+    int *bk = (int *)malloc((size_t)(t->nNodes > 0 ? t->nNodes : 1) * sizeof(int));
     OMom mom;
     OEwt *ewt = NULL;
     double nActive = 0, dPart = 0, dCell = 0, dSoft = 0, dFlop = 0;
     const int ewald = L && bEwald;
-    scratch_fit(&s, 1024);
     if (ewald) {
         calc_root(t, &mom);
         nEwh = ewald_init(&mom, L[0], fEwhCut, iEwOrder, &ewt);
     }
+    {   /* the 1/sqrt tables, before any thread reads them */
+        double one = 1.0, r;
+        orc_vsqrt1(1, &one, &r);
+    }
+    /* the buckets in the threaded walk order; each one's walk and
+     * interactions touch only its own particles' results, so the buckets
+     * are shared out over OpenMP threads (every counter is a sum of
+     * integers, exact in any order) */
     ic = t->iRoot;
     while (ic != -1) {
         const ONode *c = &t->c[ic];
-        OBnd b;
-        double fsm;
-        int na, counts[3];
         if (c->iLower != -1) { ic = c->iLower; continue; }
-        na = bucket_setup(t, ic, &b, &fsm);
-        if (na > 0) {
+        bk[nbk++] = ic;
+        ic = c->iUpper;
+    }
+#pragma omp parallel reduction(+ : nActive, dPart, dCell, dSoft, dFlop)
+    {
+        OScratch s = {0};
+        scratch_fit(&s, 1024);
+#pragma omp for schedule(dynamic, 16)
+        for (k = 0; k < nbk; ++k) {
+            const int ib = bk[k];
+            const ONode *c = &t->c[ib];
+            OBnd b;
+            double fsm;
+            int na, counts[3], j;
+            na = bucket_setup(t, ib, &b, &fsm);
+            if (na <= 0) continue;
             for (;;) {
-                walk_all_reps(t, ic, &b, fsm, L ? nReps : 0, L, &s, counts);
+                walk_all_reps(t, ib, &b, fsm, L ? nReps : 0, L, &s, counts);
                 if (counts[0] <= s.cap && counts[1] <= s.cap && counts[2] <= s.cap) break;
                 scratch_fit(&s, counts[0] > counts[1] ? (counts[0] > counts[2] ? counts[0] : counts[2])
                                                       : (counts[1] > counts[2] ? counts[1] : counts[2]));
@@ -1234,19 +1255,21 @@ void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, 
             dCell += na * counts[2];
             dSoft += na * counts[1];
             {
-                double fI = bucket_interact(t, ic, iOrder, &s, counts[0], counts[1], counts[2],
+                double fI = bucket_interact(t, ib, iOrder, &s, counts[0], counts[1], counts[2],
                                             sa, sp, sd), fE = 0;
                 if (ewald)
-                    fE = bucket_ewald(t, ic, nReps, L[0], fEwCut, iEwOrder, &mom, ewt, nEwh,
+                    fE = bucket_ewald(t, ib, nReps, L[0], fEwCut, iEwOrder, &mom, ewt, nEwh,
                                       sa, sp);
                 dFlop += fI + fE;
                 if (fweight)
-                    for (i = c->pLower; i <= c->pUpper; ++i)
-                        if (t->p[i].active) fweight[t->p[i].src] = fI + fE;
+                    for (j = c->pLower; j <= c->pUpper; ++j)
+                        if (t->p[j].active) fweight[t->p[j].src] = fI + fE;
             }
         }
-        ic = c->iUpper;
+        free(s.part); free(s.soft); free(s.newt); free(s.d2a); free(s.sq);
+        free(s.po); free(s.so); free(s.no);
     }
+    free(bk);
     for (i = 0; i < n; ++i) {
         int src = t->p[i].src;
         a[3 * src] = sa[3 * i];
@@ -1257,8 +1280,6 @@ void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, 
     }
     stats[0] = nActive; stats[1] = dPart; stats[2] = dCell; stats[3] = dSoft; stats[4] = dFlop;
     free(sa); free(sp); free(sd); free(ewt);
-    free(s.part); free(s.soft); free(s.newt); free(s.d2a); free(s.sq);
-    free(s.po); free(s.so); free(s.no);
 }
 
 void orc_grav_all_periodic(orc_tree *t, int iOrder, int nReps, const double *L, int bEwald,

@@ -199,6 +199,19 @@ void orc_sph_pressure_sym(int n, const double *m, const double *vpred,
                           const orc_smf *smf, double *acc, double *uDotPdV,
                           double *uDotAV, double *uDotDiff, double *mumax,
                           double *dtNew);
+/* The same in the serial scatter form (every pair updates both ends as it
+ * is met); the threaded gather form above sums the same terms in another
+ * order. */
+void orc_sph_pressure_sym_serial(int n, const double *m, const double *vpred,
+                          const double *upred, const double *rho,
+                          const double *PoverRho2, const double *c,
+                          const double *balsara, const double *diff,
+                          const double *dt, const unsigned char *active,
+                          const double *fBall2, const long long *off,
+                          const int *idx, const double *dxyz, const double *d2,
+                          const orc_smf *smf, double *acc, double *uDotPdV,
+                          double *uDotAV, double *uDotDiff, double *mumax,
+                          double *dtNew);
 
 /* ------------------------------------------- updates either side (step_oracle.c) */
 void orc_init_accel(int n, const unsigned *bits, double *a, double *pot, double *dtg);

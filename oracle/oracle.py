@@ -86,6 +86,7 @@ def lib():
                                            _dp, _dp, _dp, _up, _dp, _lp, _ip, _dp,
                                            _dp, C.POINTER(Smf), _dp, _dp, _dp, _dp,
                                            _dp, _dp]
+        L.orc_sph_pressure_sym_serial.argtypes = L.orc_sph_pressure_sym.argtypes
         _u = _uip
         L.orc_init_accel.argtypes = [C.c_int, _u, _dp, _dp, _dp]
         L.orc_kick.argtypes = [C.c_int, _u] + [C.c_double] * 6 + [C.c_int] + [_dp] * 8
@@ -339,7 +340,9 @@ def gas_pressure(upred, rho, fBall2, smf, dt=None, udot=None):
 
 
 def sph_pressure_sym(m, vpred, upred, rho, PoverRho2, c, balsara, diff, dt, fBall2,
-                     lists, smf, active=None, acc=None):
+                     lists, smf, active=None, acc=None, serial=False):
+    """SphPressureTermsSym over CSR lists: the threaded gather form, or with
+    serial=True the serial scatter form."""
     off, idx, dxyz, d2 = lists
     n = len(m)
     act = np.ones(n, np.uint8) if active is None else np.ascontiguousarray(active, np.uint8)
@@ -352,11 +355,11 @@ def sph_pressure_sym(m, vpred, upred, rho, PoverRho2, c, balsara, diff, dt, fBal
     off = np.ascontiguousarray(off, np.int64)
     dxyz = _d(dxyz)
     d2 = _d(d2)
-    lib().orc_sph_pressure_sym(n, P(arrs[0]), P(arrs[1]), P(arrs[2]), P(arrs[3]), P(arrs[4]),
-                               P(arrs[5]), P(arrs[6]), P(arrs[7]), P(arrs[8]), P(act, _up),
-                               P(arrs[9]), P(off, _lp), P(idx, _ip), P(dxyz), P(d2),
-                               C.byref(smf), P(acc), P(outs["uDotPdV"]), P(outs["uDotAV"]),
-                               P(outs["uDotDiff"]), P(outs["mumax"]), P(dtNew))
+    fn = lib().orc_sph_pressure_sym_serial if serial else lib().orc_sph_pressure_sym
+    fn(n, P(arrs[0]), P(arrs[1]), P(arrs[2]), P(arrs[3]), P(arrs[4]), P(arrs[5]), P(arrs[6]),
+       P(arrs[7]), P(arrs[8]), P(act, _up), P(arrs[9]), P(off, _lp), P(idx, _ip), P(dxyz),
+       P(d2), C.byref(smf), P(acc), P(outs["uDotPdV"]), P(outs["uDotAV"]),
+       P(outs["uDotDiff"]), P(outs["mumax"]), P(dtNew))
     outs["acc"] = acc
     outs["dtNew"] = dtNew
     return outs

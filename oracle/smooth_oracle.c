@@ -412,16 +412,107 @@ void orc_gas_pressure(int n, const double *upred, const double *rho,
     }
 }
 
-void orc_sph_pressure_sym(int n, const double *m, const double *vpred,
-                          const double *upred, const double *rho,
-                          const double *PoverRho2, const double *c,
-                          const double *balsara, const double *diff,
-                          const double *dt, const unsigned char *active,
-                          const double *fBall2, const long long *off,
-                          const int *idx, const double *dxyz, const double *d2,
-                          const orc_smf *smf, double *acc, double *uDotPdV,
-                          double *uDotAV, double *uDotDiff, double *mumax,
-                          double *dtNew)
+/* The terms of one pair of SphPressureTermsSym (smoothfcn.c:2276-2385,
+ * SphPressureTerms.h:97-152): entry k of p's list, q = idx[k], evaluated
+ * with p's kernel.  Each end's increments come out separately: tp for p,
+ * tq for q (acceleration 3, uDotPdV, uDotAV, uDotDiff), and absmu >= 0
+ * (0 for a receding pair) and dtv for mumax and dtNew of both.  Returns 0
+ * when neither end is active (the pair is skipped). */
+typedef struct {
+    const double *m, *vpred, *upred, *rho, *PoverRho2, *c, *balsara, *diff, *dt, *fBall2;
+    const unsigned char *active;
+    const int *idx;
+    const double *dxyz, *d2;
+    const orc_smf *smf;
+} OHydro;
+
+static int hydro_pair(const OHydro *H, int p, long long k, double tp[6], double tq[6],
+                      double *absmu_out, double *dtv_out)
+{
+    const orc_smf *smf = H->smf;
+    const int q = H->idx[k];
+    const double *m = H->m, *vpred = H->vpred, *rho = H->rho, *c = H->c;
+    double ph, ih2, fNorm, fNorm1, aFac, vFac, pPoverRho2, pPoverRho2f;
+    double pc = c[p], pMass = m[p], pDensity = rho[p];
+    double r2, rs1, rp, rq, dx, dy, dz, dvx, dvy, dvz, dvdotdr;
+    double qPoverRho2, qPoverRho2f, Accp = 0, Accq = 0, dtv, visc, absmu = 0;
+    int pa = H->active[p] != 0, qa = H->active[q] != 0, j;
+    for (j = 0; j < 6; ++j) tp[j] = tq[j] = 0.0;
+    if (!pa && !qa) return 0;
+    pPoverRho2 = H->PoverRho2[p];
+    {
+        double pd2 = rho[p] * rho[p];
+        pPoverRho2f = (pPoverRho2 * pd2 + 0) / pd2;
+    }
+    ph = sqrt(0.25 * H->fBall2[p]);
+    ih2 = 4.0 / H->fBall2[p];
+    fNorm = 0.5 * M_1_PI * ih2 / ph;
+    fNorm1 = fNorm * ih2;
+    aFac = smf->a;
+    vFac = (smf->bCannonical ? 1. / (smf->a * smf->a) : 1.0);
+    r2 = H->d2[k] * ih2;
+    rs1 = m4_dw(r2);
+    rs1 *= fNorm1;
+    rs1 *= 1.0;                 /* fDivv_Corrector (non-GDFORCE) */
+    rp = rs1 * pMass;
+    rq = rs1 * m[q];
+    dx = H->dxyz[3 * k]; dy = H->dxyz[3 * k + 1]; dz = H->dxyz[3 * k + 2];
+    dvx = vpred[3 * p] - vpred[3 * q];
+    dvy = vpred[3 * p + 1] - vpred[3 * q + 1];
+    dvz = vpred[3 * p + 2] - vpred[3 * q + 2];
+    dvdotdr = vFac * (dvx * dx + dvy * dy + dvz * dz) + H->d2[k] * smf->H;
+    qPoverRho2 = H->PoverRho2[q];
+    {
+        double qd2 = rho[q] * rho[q];
+        qPoverRho2f = (qPoverRho2 * qd2 + 0) / qd2;
+    }
+    /* body of SphPressureTerms.h:112-152 */
+    if (pa) tp[3] = rq * pPoverRho2 * dvdotdr;
+    if (qa) tq[3] = rp * qPoverRho2 * dvdotdr;
+    if (pa) Accp = (pPoverRho2f + qPoverRho2f);
+    if (qa) Accq = (qPoverRho2f + pPoverRho2f);
+    if (dvdotdr >= 0.0) {
+        dtv = smf->dtFacCourant * ph / (2 * (pc > c[q] ? pc : c[q]));
+    } else {
+        absmu = -dvdotdr * smf->a / sqrt(H->d2[k]);
+        visc = (smf->alpha * (pc + c[q]) + smf->beta * 1.5 * absmu);
+        dtv = smf->dtFacCourant * ph / (0.625 * (pc + c[q]) + 0.375 * visc);
+        visc = (0.5 * (H->balsara[p] + H->balsara[q])) * visc * absmu / (pDensity + rho[q]);
+        if (pa) tp[4] = rq * (0.5 * visc) * dvdotdr;
+        if (qa) tq[4] = rp * (0.5 * visc) * dvdotdr;
+        if (pa) Accp += visc;
+        if (qa) Accq += visc;
+    }
+    if (pa) Accp *= rq * aFac;
+    if (qa) Accq *= rp * aFac;
+    if (pa) {
+        tp[0] = -(Accp * dx);
+        tp[1] = -(Accp * dy);
+        tp[2] = -(Accp * dz);
+    }
+    if (qa) {
+        tq[0] = Accq * dx;
+        tq[1] = Accq * dy;
+        tq[2] = Accq * dz;
+    }
+    {   /* thermal diffusion (SphPressureTerms.h:32-70) */
+        double diffSum = H->diff[p] + H->diff[q];
+        double diffBase = (diffSum == 0 ? 0 : diffSum);
+        double diffTh = (2 * smf->dThermalDiffusionCoeff * diffBase / (pDensity + rho[q]));
+        double dt_diff, diffu;
+        if (diffTh > 0 && (dt_diff = smf->dtFacDiffusion * ph * ph / (diffTh * pDensity)) < dtv)
+            dtv = dt_diff;
+        diffu = (diffTh + 0 + 0) * (H->upred[p] - H->upred[q]);
+        if (pa) tp[5] = diffu * rq * 1;
+        if (qa) tq[5] = -(diffu * rp * 1);
+    }
+    *absmu_out = absmu;
+    *dtv_out = dtv;
+    return 1;
+}
+
+static void hydro_init(int n, const unsigned char *active, double *mumax, double *dtNew,
+                       double *uDotPdV, double *uDotAV, double *uDotDiff)
 {
     int p;
     /* initSphPressureTermsParticle (smoothfcn.c:2204-2221): a is NOT zeroed */
@@ -433,92 +524,109 @@ void orc_sph_pressure_sym(int n, const double *m, const double *vpred,
         uDotPdV[p] = 0.0;
         uDotAV[p] = 0.0;
     }
+}
+
+/* one end's share of a pair: its increments, then mumax and dtNew
+ * (SETDTNEW_PQ, SphPressureTerms.h:97-100: the pair's dt and 4 dt of the
+ * other end) */
+static void hydro_apply(int x, int other, const double t[6], double absmu, double dtv,
+                        const double *dt, double *acc, double *uDotPdV, double *uDotAV,
+                        double *uDotDiff, double *mumax, double *dtNew)
+{
+    acc[3 * x] += t[0];
+    acc[3 * x + 1] += t[1];
+    acc[3 * x + 2] += t[2];
+    uDotPdV[x] += t[3];
+    uDotAV[x] += t[4];
+    uDotDiff[x] += t[5];
+    if (absmu > mumax[x]) mumax[x] = absmu;
+    if (dtv < dtNew[x]) dtNew[x] = dtv;
+    if (4 * dt[other] < dtNew[x]) dtNew[x] = 4 * dt[other];
+}
+
+/* The serial scatter form: the reference's order, every pair updating both
+ * ends as it is met. */
+void orc_sph_pressure_sym_serial(int n, const double *m, const double *vpred,
+                                 const double *upred, const double *rho,
+                                 const double *PoverRho2, const double *c,
+                                 const double *balsara, const double *diff,
+                                 const double *dt, const unsigned char *active,
+                                 const double *fBall2, const long long *off,
+                                 const int *idx, const double *dxyz, const double *d2,
+                                 const orc_smf *smf, double *acc, double *uDotPdV,
+                                 double *uDotAV, double *uDotDiff, double *mumax,
+                                 double *dtNew)
+{
+    OHydro H = {m, vpred, upred, rho, PoverRho2, c, balsara, diff, dt, fBall2, active, idx,
+                dxyz, d2, smf};
+    int p;
+    hydro_init(n, active, mumax, dtNew, uDotPdV, uDotAV, uDotDiff);
     for (p = 0; p < n; ++p) {
-        double ph, ih2, fNorm, fNorm1, aFac, vFac, pPoverRho2, pPoverRho2f;
-        double pc = c[p], pMass = m[p], pDensity = rho[p];
         long long k;
-        /* smReSmooth visits every smooth-active particle; callers pass
-         * lists only for those (others have empty lists). */
-        pPoverRho2 = PoverRho2[p];
-        {
-            double pd2 = rho[p] * rho[p];
-            pPoverRho2f = (pPoverRho2 * pd2 + 0) / pd2;
-        }
-        ph = sqrt(0.25 * fBall2[p]);
-        ih2 = 4.0 / fBall2[p];
-        fNorm = 0.5 * M_1_PI * ih2 / ph;
-        fNorm1 = fNorm * ih2;
-        aFac = smf->a;
-        vFac = (smf->bCannonical ? 1. / (smf->a * smf->a) : 1.0);
         for (k = off[p]; k < off[p + 1]; ++k) {
-            int q = idx[k];
-            double r2, rs1, rp, rq, dx, dy, dz, dvx, dvy, dvz, dvdotdr;
-            double qPoverRho2, qPoverRho2f, Accp = 0, Accq = 0, dtv, visc, absmu;
-            int pa = active[p] != 0, qa = active[q] != 0;
-            if (!pa && !qa) continue;
-            r2 = d2[k] * ih2;
-            rs1 = m4_dw(r2);
-            rs1 *= fNorm1;
-            rs1 *= 1.0;                 /* fDivv_Corrector (non-GDFORCE) */
-            rp = rs1 * pMass;
-            rq = rs1 * m[q];
-            dx = dxyz[3 * k]; dy = dxyz[3 * k + 1]; dz = dxyz[3 * k + 2];
-            dvx = vpred[3 * p] - vpred[3 * q];
-            dvy = vpred[3 * p + 1] - vpred[3 * q + 1];
-            dvz = vpred[3 * p + 2] - vpred[3 * q + 2];
-            dvdotdr = vFac * (dvx * dx + dvy * dy + dvz * dz) + d2[k] * smf->H;
-            qPoverRho2 = PoverRho2[q];
-            {
-                double qd2 = rho[q] * rho[q];
-                qPoverRho2f = (qPoverRho2 * qd2 + 0) / qd2;
-            }
-            /* body of SphPressureTerms.h:112-152 */
-            if (pa) uDotPdV[p] += rq * pPoverRho2 * dvdotdr;
-            if (qa) uDotPdV[q] += rp * qPoverRho2 * dvdotdr;
-            if (pa) Accp = (pPoverRho2f + qPoverRho2f);
-            if (qa) Accq = (qPoverRho2f + pPoverRho2f);
-            if (dvdotdr >= 0.0) {
-                dtv = smf->dtFacCourant * ph / (2 * (pc > c[q] ? pc : c[q]));
-            } else {
-                absmu = -dvdotdr * smf->a / sqrt(d2[k]);
-                if (absmu > mumax[p]) mumax[p] = absmu;
-                if (absmu > mumax[q]) mumax[q] = absmu;
-                visc = (smf->alpha * (pc + c[q]) + smf->beta * 1.5 * absmu);
-                dtv = smf->dtFacCourant * ph / (0.625 * (pc + c[q]) + 0.375 * visc);
-                visc = (0.5 * (balsara[p] + balsara[q])) * visc * absmu / (pDensity + rho[q]);
-                if (pa) uDotAV[p] += rq * (0.5 * visc) * dvdotdr;
-                if (qa) uDotAV[q] += rp * (0.5 * visc) * dvdotdr;
-                if (pa) Accp += visc;
-                if (qa) Accq += visc;
-            }
-            if (pa) Accp *= rq * aFac;
-            if (qa) Accq *= rp * aFac;
-            if (pa) {
-                acc[3 * p] -= Accp * dx;
-                acc[3 * p + 1] -= Accp * dy;
-                acc[3 * p + 2] -= Accp * dz;
-            }
-            if (qa) {
-                acc[3 * q] += Accq * dx;
-                acc[3 * q + 1] += Accq * dy;
-                acc[3 * q + 2] += Accq * dz;
-            }
-            {   /* thermal diffusion (SphPressureTerms.h:32-70) */
-                double diffSum = diff[p] + diff[q];
-                double diffBase = (diffSum == 0 ? 0 : diffSum);
-                double diffTh = (2 * smf->dThermalDiffusionCoeff * diffBase / (pDensity + rho[q]));
-                double dt_diff, diffu;
-                if (diffTh > 0 && (dt_diff = smf->dtFacDiffusion * ph * ph / (diffTh * pDensity)) < dtv)
-                    dtv = dt_diff;
-                diffu = (diffTh + 0 + 0) * (upred[p] - upred[q]);
-                if (pa) uDotDiff[p] += diffu * rq * 1;
-                if (qa) uDotDiff[q] -= diffu * rp * 1;
-            }
-            /* SETDTNEW_PQ (SphPressureTerms.h:97-100) */
-            if (dtv < dtNew[p]) dtNew[p] = dtv;
-            if (dtv < dtNew[q]) dtNew[q] = dtv;
-            if (4 * dt[q] < dtNew[p]) dtNew[p] = 4 * dt[q];
-            if (4 * dt[p] < dtNew[q]) dtNew[q] = 4 * dt[p];
+            double tp[6], tq[6], absmu, dtv;
+            const int q = idx[k];
+            if (!hydro_pair(&H, p, k, tp, tq, &absmu, &dtv)) continue;
+            hydro_apply(p, q, tp, absmu, dtv, dt, acc, uDotPdV, uDotAV, uDotDiff, mumax, dtNew);
+            hydro_apply(q, p, tq, absmu, dtv, dt, acc, uDotPdV, uDotAV, uDotDiff, mumax, dtNew);
         }
     }
+}
+
+/* The gather form on OpenMP threads: each particle x takes its share of
+ * the pairs of its own list (x as p) and of the lists holding it (x as q,
+ * from the reverse lists, in list order), the same terms as the serial
+ * form summed in another order. */
+void orc_sph_pressure_sym(int n, const double *m, const double *vpred,
+                          const double *upred, const double *rho,
+                          const double *PoverRho2, const double *c,
+                          const double *balsara, const double *diff,
+                          const double *dt, const unsigned char *active,
+                          const double *fBall2, const long long *off,
+                          const int *idx, const double *dxyz, const double *d2,
+                          const orc_smf *smf, double *acc, double *uDotPdV,
+                          double *uDotAV, double *uDotDiff, double *mumax,
+                          double *dtNew)
+{
+    OHydro H = {m, vpred, upred, rho, PoverRho2, c, balsara, diff, dt, fBall2, active, idx,
+                dxyz, d2, smf};
+    const long long ne = off[n];
+    long long *roff = (long long *)calloc((size_t)n + 1, sizeof(long long));
+    long long *rk = (long long *)malloc((size_t)(ne > 0 ? ne : 1) * sizeof(long long));
+    long long *rpos = (long long *)malloc(((size_t)n + 1) * sizeof(long long));
+    int *rp = (int *)malloc((size_t)(ne > 0 ? ne : 1) * sizeof(int));
+    int p;
+    long long k;
+    hydro_init(n, active, mumax, dtNew, uDotPdV, uDotAV, uDotDiff);
+    /* reverse lists: for each q the entries k (and their owner p) that
+     * hold it, in increasing k */
+    for (k = 0; k < ne; ++k) roff[idx[k] + 1]++;
+    for (p = 0; p < n; ++p) roff[p + 1] += roff[p];
+    memcpy(rpos, roff, ((size_t)n + 1) * sizeof(long long));
+    for (p = 0; p < n; ++p)
+        for (k = off[p]; k < off[p + 1]; ++k) {
+            const long long o = rpos[idx[k]]++;
+            rk[o] = k;
+            rp[o] = p;
+        }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (p = 0; p < n; ++p) {
+        long long e;
+        for (e = off[p]; e < off[p + 1]; ++e) {
+            double tp[6], tq[6], absmu, dtv;
+            if (!hydro_pair(&H, p, e, tp, tq, &absmu, &dtv)) continue;
+            hydro_apply(p, idx[e], tp, absmu, dtv, dt, acc, uDotPdV, uDotAV, uDotDiff, mumax,
+                        dtNew);
+        }
+        for (e = roff[p]; e < roff[p + 1]; ++e) {
+            double tp[6], tq[6], absmu, dtv;
+            const int o = rp[e];
+            if (!hydro_pair(&H, o, rk[e], tp, tq, &absmu, &dtv)) continue;
+            hydro_apply(p, o, tq, absmu, dtv, dt, acc, uDotPdV, uDotAV, uDotDiff, mumax, dtNew);
+        }
+    }
+    free(roff);
+    free(rk);
+    free(rpos);
+    free(rp);
 }

@@ -38,3 +38,11 @@ def test_last_error_callable_without_device():
     lib = ctypes.CDLL(LIB)
     lib.gg_last_error.restype = ctypes.c_char_p
     assert isinstance(lib.gg_last_error(), bytes)
+
+
+def test_host_sequences_library_loads():
+    # the C host sequences (gasoline_amd/host/libggmsr.so) over the ABI
+    from gasoline_amd import pkd
+    L = pkd.msr_lib()
+    for name in ("ggMsrTopStepKDK", "ggMsrForces", "ggMsrSphRung", "ggMsrGravity", "ggMsrSph"):
+        assert hasattr(L, name), name

@@ -198,3 +198,25 @@ def test_density_sym_equals_gather_on_symmetric_lists():
     out = O.dendvdx(m, np.zeros((n, 3)), np.ones(n), fb2, lists, O.default_smf())
     sym = O.density_sym(m, fb2, lists)
     assert np.allclose(sym, out["rho"], rtol=1e-14, atol=0)
+
+
+def test_threaded_hydro_equals_serial_scatter(box):
+    # the threaded gather form sums the serial scatter form's pair terms in
+    # another order: the same to rounding, mumax and dtNew (max and min)
+    # bit for bit; with a third of the particles active
+    smf = O.default_smf(iViscosityLimiter=1, dThermalDiffusionCoeff=0.03)
+    act = (np.random.default_rng(9).uniform(size=len(box["m"])) < 0.33).astype(np.uint8)
+    v = box["v"] + 0.3 * (box["r"] - 0.5)
+    u = box["u"] * (1 + 0.5 * np.sin(2 * np.pi * box["r"][:, 0]))
+    d = O.dendvdx(box["m"], v, u, box["fb2"], box["lists"], smf)
+    pr2, c, dt = O.gas_pressure(u, d["rho"], box["fb2"], smf)
+    args = (box["m"], v, u, d["rho"], pr2, c, d["balsara"], d["diff"], dt, box["fb2"],
+            box["lists"], smf)
+    a = O.sph_pressure_sym(*args, active=act)
+    b = O.sph_pressure_sym(*args, active=act, serial=True)
+    for k in ("mumax", "dtNew"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("acc", "uDotPdV", "uDotAV", "uDotDiff"):
+        scale = np.abs(b[k]).max()
+        assert scale > 0 and np.abs(a[k] - b[k]).max() < 1e-13 * scale, k
+    assert (a["acc"][act == 0] == 0).all()
