@@ -863,19 +863,28 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
 // the evaluation and their mix64 (order-independent checksums of the lists
 // this kernel evaluates, for parity checks): chk[9b + 3k ..] = count, sum,
 // sum of mixes.
-template <typename T, int ORDER, bool PER, bool CHK = false>
-__global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
-                                                      const int *__restrict__ broot,
-                                                      int *__restrict__ cnt,
-                                                      int *__restrict__ err,
-                                                      unsigned long long *__restrict__ chk = nullptr)
+// The LDS of one fused walk + interaction workgroup.
+template <typename T, int ORDER> struct WIShared {
+    Stage<T, ORDER> L;
+    int fr[2][WI_FC];
+    int qs[WI_Q], qn[WI_Q], qpl[WI_Q], qpn[WI_Q];
+    int pidx[64];
+};
+
+// The walk and interactions of active bucket b.  Returns false (with *err
+// set, the bucket unevaluated) if a frontier outgrew WI_FC.
+template <typename T, int ORDER, bool PER, bool CHK>
+__device__ __forceinline__ bool walk_interact_one(const IArgs &A, int b,
+                                                  const double *__restrict__ bbox,
+                                                  const int *__restrict__ broot,
+                                                  int *__restrict__ cnt, int *__restrict__ err,
+                                                  unsigned long long *__restrict__ chk,
+                                                  WIShared<T, ORDER> &S)
 {
-    __shared__ Stage<T, ORDER> L;
-    __shared__ int fr[2][WI_FC];
-    __shared__ int qs[WI_Q], qn[WI_Q], qpl[WI_Q], qpn[WI_Q];
-    __shared__ int pidx[64];
+    Stage<T, ORDER> &L = S.L;
+    auto &fr = S.fr;
+    int *qs = S.qs, *qn = S.qn, *qpl = S.qpl, *qpn = S.qpn, *pidx = S.pidx;
     const int lane = threadIdx.x;
-    const int b = xcd_chunk(blockIdx.x, gridDim.x, 128);
     const int ib = A.bucket[b];
     const WalkNode bn = A.wn[ib];
     Tgt<T> g;
@@ -964,7 +973,7 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
         __syncthreads();
         if (nn > WI_FC) {
             if (lane == 0) atomicOr(err, 1);
-            return;
+            return false;
         }
         nf = nn;
         cur ^= 1;
@@ -989,6 +998,97 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
             for (int k = 0; k < 3; ++k) chk[9 * (size_t)b + 3 * k] = (unsigned)tot[k];
     }
     bucket_store(A, g, R, it, TPW);
+    return true;
+}
+
+// One workgroup per active bucket, dealt to the XCDs in chunks of 128.
+template <typename T, int ORDER, bool PER, bool CHK = false>
+__global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__restrict__ bbox,
+                                                      const int *__restrict__ broot,
+                                                      int *__restrict__ cnt,
+                                                      int *__restrict__ err,
+                                                      unsigned long long *__restrict__ chk = nullptr)
+{
+    __shared__ WIShared<T, ORDER> S;
+    walk_interact_one<T, ORDER, PER, CHK>(A, xcd_chunk(blockIdx.x, gridDim.x, 128), bbox, broot,
+                                          cnt, err, chk, S);
+}
+
+// Persistent form: a grid that fills the chip once; each workgroup takes
+// the next bucket of queue (a permutation of the active buckets, heaviest
+// first when the previous evaluation's costs are known) until none is left.
+// *next must be 0 at launch.  Every workgroup leaves the loop once the
+// queue is exhausted.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void k_walk_interact_queue(IArgs A,
+                                                            const double *__restrict__ bbox,
+                                                            const int *__restrict__ broot,
+                                                            int *__restrict__ cnt,
+                                                            int *__restrict__ err,
+                                                            const int *__restrict__ queue,
+                                                            unsigned *__restrict__ next)
+{
+    __shared__ WIShared<T, ORDER> S;
+    for (;;) {
+        unsigned slot = 0;
+        if (threadIdx.x == 0) slot = atomicAdd(next, 1u);
+        slot = __shfl(slot, 0);
+        if (slot >= (unsigned)A.nb) break;
+        walk_interact_one<T, ORDER, false, false>(A, queue[slot], bbox, broot, cnt, err, nullptr,
+                                                  S);
+        __syncthreads();
+    }
+}
+
+// The persistent walk's queue, heaviest buckets first: each active bucket's
+// cost by gasoline's flop model (grav.c:246-247) from the list sizes of the
+// last walk, binned by its binary exponent (bin 0 the heaviest), the bins
+// counted, scanned and scattered.  Buckets within a bin come in any order;
+// each bucket's results do not depend on when it runs.
+constexpr int QBINS = 64;
+__global__ void k_cost_bin(int nb, const int *__restrict__ bucket,
+                           const double *__restrict__ bbox, const WalkNode *__restrict__ wn,
+                           const int *__restrict__ cnt, int *__restrict__ bin,
+                           unsigned *__restrict__ hist)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const WalkNode c = wn[bucket[b]];
+    const double na = bbox[8 * (size_t)b + 7];
+    const double n = c.pUpper - c.pLower + 1;
+    const double f = na * ((cnt[3 * (size_t)b] + n) * 38.0 + cnt[3 * (size_t)b + 1] * 82.0
+                           + cnt[3 * (size_t)b + 2] * 76.0);
+    const int e = f >= 1.0 ? ilogb(f) : 0;
+    const int k = QBINS - 1 - min(e, QBINS - 1);
+    bin[b] = k;
+    atomicAdd(&hist[k], 1u);
+}
+
+// hist -> exclusive offsets, in place (one wavefront).
+__global__ void k_bin_scan(unsigned *__restrict__ hist)
+{
+    const int l = threadIdx.x;
+    const unsigned h = hist[l];
+    unsigned incl = h;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned u = __shfl_up(incl, o);
+        if (l >= o) incl += u;
+    }
+    hist[l] = incl - h;
+}
+
+__global__ void k_bin_scatter(int nb, const int *__restrict__ bin, unsigned *__restrict__ pos,
+                              int *__restrict__ queue)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    queue[atomicAdd(&pos[bin[b]], 1u)] = b;
+}
+
+__global__ void k_queue_iota(int n, int *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 
 // Add the per-slot results to the caller-order fields of active particles.
@@ -1093,6 +1193,48 @@ static IArgs make_iargs(GGPKD pkd, int iOrder)
     return A;
 }
 
+// Workgroups of the persistent grid: as many as fit on the chip at once.
+template <typename T, int O>
+static int persistent_grid(GravDev &G)
+{
+    const int key = (int)sizeof(T) * 10 + O;
+    if (G.qGrid > 0 && G.qGridKey == key) return GG_OK;
+    G.qGridKey = key;
+    int dev = 0, ncu = 0, per = 0;
+    GG_HIP(hipGetDevice(&dev));
+    GG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk_interact_queue<T, O>, 64, 0));
+    G.qGrid = std::max(1, ncu * std::max(per, 1));
+    return GG_OK;
+}
+
+// The persistent queue for the current tree: heaviest buckets first when a
+// fused walk of this tree has left its list sizes in G.cnt, else the
+// buckets in order.  Kernels only (it may sit in a captured graph).
+static int queue_order(GravDev &G, hipStream_t st, bool fromCosts)
+{
+    const int nb = G.nb;
+    GG_TRY(G.qorder.reserve(sizeof(int) * std::max(nb, 1)));
+    if (!fromCosts) {
+        hipLaunchKernelGGL(k_queue_iota, dim3((nb + 255) / 256), dim3(256), 0, st, nb,
+                           G.qorder.as<int>());
+        G.qTree = -1;
+    } else {
+        GG_TRY(G.qbin.reserve(sizeof(int) * std::max(nb, 1)));
+        GG_TRY(G.qhist.reserve(sizeof(unsigned) * QBINS));
+        hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, QBINS, G.qhist.as<uint32_t>());
+        hipLaunchKernelGGL(k_cost_bin, dim3((nb + 255) / 256), dim3(256), 0, st, nb,
+                           G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
+                           G.cnt.as<int>(), G.qbin.as<int>(), G.qhist.as<unsigned>());
+        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, st, G.qhist.as<unsigned>());
+        hipLaunchKernelGGL(k_bin_scatter, dim3((nb + 255) / 256), dim3(256), 0, st, nb,
+                           G.qbin.as<int>(), G.qhist.as<unsigned>(), G.qorder.as<int>());
+        G.qTree = G.treeId;
+    }
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
 template <typename T>
 static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
 {
@@ -1100,6 +1242,27 @@ static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
     const double *bb = G.bbox.as<double>();
     int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
     const int *br = G.broot.as<int>();
+    if (G.sched == 1 && A.nReps == 0) {   // persistent: workgroups take buckets from the queue
+        if (G.qTree != G.treeId) GG_TRY(queue_order(G, st, false));
+        GG_TRY(G.qnext.reserve(sizeof(unsigned)));
+        hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, 1, G.qnext.as<uint32_t>());
+        const int *q = G.qorder.as<int>();
+        unsigned *nx = G.qnext.as<unsigned>();
+#define GG_QUEUE(O)                                                                              \
+    GG_TRY((persistent_grid<T, O>(G)));                                                           \
+    hipLaunchKernelGGL((k_walk_interact_queue<T, O>), dim3(G.qGrid), block, 0, st, A, bb, br,   \
+                       cnt, err, q, nx)
+        switch (order) {
+        case 1: GG_QUEUE(1); break;
+        case 2: GG_QUEUE(2); break;
+        case 3: GG_QUEUE(3); break;
+        case 4: GG_QUEUE(4); break;
+        default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
+        }
+#undef GG_QUEUE
+        GG_HIP(hipGetLastError());
+        return GG_OK;
+    }
     // replicas only when there are some: the isolated walk keeps its code
 #define GG_FUSED(O)                                                                              \
     if (A.nReps > 0)                                                                             \
@@ -1233,6 +1396,8 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
                            nb, G.bucket.as<int>(), G.bbox.as<double>(), G.wnode.as<WalkNode>(),
                            G.cnt.as<int>(), iOrder, G.stats.as<unsigned long long>());
         GG_HIP(hipGetLastError());
+        if (G.sched == 1 && G.fusedWalk && !per)   // the next call's queue: heaviest first
+            GG_TRY(queue_order(G, st, true));
         {
             static const int meval[5] = {10, 10, 48, 151, 343};   // MEVAL_FLOP, meval.h:84
             const bool ew = ewald && iEwOrder;
@@ -1355,7 +1520,8 @@ __global__ __launch_bounds__(256) void k_direct(int n, int nt, const int *__rest
 {
     __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256];
     const int k = blockIdx.x * 256 + threadIdx.x;
-    const int i = k < nt ? tgt[k] : -1;
+    int i = k < nt ? tgt[k] : -1;
+    if (i >= n) i = -1;                  // a target outside the store computes nothing
     const double xi = i >= 0 ? r[3 * (size_t)i] : 0.0, yi = i >= 0 ? r[3 * (size_t)i + 1] : 0.0;
     const double zi = i >= 0 ? r[3 * (size_t)i + 2] : 0.0, hi = i >= 0 ? h[i] : 0.0;
     double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
@@ -1579,6 +1745,15 @@ extern "C" int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double
     if (pdCellSum) *pdCellSum = (double)hs[2];
     if (pdSoftSum) *pdSoftSum = (double)hs[3];
     if (pdFlop) *pdFlop = (double)hs[4];
+    return GG_OK;
+}
+
+extern "C" int gg_set_grav_schedule(GGPKD pkd, int mode)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GG_CHECK(mode == 0 || mode == 1, GG_EINVAL, "schedule %d: 0 or 1", mode);
+    pkd->grav.sched = mode;
+    pkd->grav.qTree = -1;
     return GG_OK;
 }
 

@@ -93,13 +93,17 @@ struct GravDev {
     int serialWalk = 0;    // the last listed walk fell back to one thread per bucket
     int listsCurrent = 0;  // cnt/off/ent hold the lists of the current tree
     int fusedWalk = 0;     // the last gg_grav_all walked inside the interaction kernel
+    int sched = 0;         // fused walk: 0 one workgroup per bucket, 1 persistent queue
+    DevBuf qorder, qbin, qhist, qnext;   // persistent queue (heaviest first), its scratch
+    long qTree = -1;       // tree whose costs ordered qorder (-1: identity order)
+    int qGrid = 0, qGridKey = 0;   // workgroups of the persistent grid (and its kernel)
     void release_all()
     {
         DevBuf *all[] = {&px, &py, &pz, &pm, &ph, &pact, &perm, &wnode, &cell, &momOH, &bucket,
                          &bbox, &cnt, &off, &ent, &acc, &stats, &scan, &werr, &broot, &tint,
                          &tdbl, &gpn, &gF, &gposR, &gnode, &gbox, &gnch, &gchoff, &gerr,
                          &groot, &gcm, &gnit, &gioff, &gicell, &gpart, &gbflag, &gbb8, &grootof, &glroot, &gdcnt, &gdlist, &glused, &glbase, &gscr,
-                         &ewt, &emom, &ewh, &ewl};
+                         &ewt, &emom, &ewh, &ewl, &qorder, &qbin, &qhist, &qnext};
         for (DevBuf *b : all) b->release();
     }
 };

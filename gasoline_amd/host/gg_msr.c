@@ -127,6 +127,7 @@ int ggMsrTopStepKDK(GGPKD pkd, const ggStepParam *p, ggStepState *s, double dSte
                     double dDelta, int iRung, int iKickRung, int iAdjust)
 {
     const double h = 0.5 * dDelta;
+    iAdjust = iAdjust && p->bAdjust;
     if (iAdjust && p->bDoGas) {      /* msrEmergencyAdjust (master.c:6742) */
         int nUn, ideal, nMax, top;
         GG_RET(gg_active_mask_rung(pkd, GG_TYPE_ACTIVE, iRung, 1, NULL));

@@ -40,6 +40,8 @@ int ggMsrSph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
 typedef struct ggStepParam {
     double dDelta;            /* the top step (msrInitDt uses it) */
     int iMaxRung;             /* rungs 0 .. iMaxRung-1 */
+    int bAdjust;              /* 1: the recursion assigns rungs (the reference);
+                                 0: rungs stay as the caller stored them */
     double dEta;              /* gravity / acceleration criteria */
     double dEtaCourant;       /* [0.4] */
     double dEtauDot;          /* [0.25] */

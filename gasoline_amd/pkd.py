@@ -54,9 +54,11 @@ def tipsy_header(path, standard=True):
 ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local", "gg_set_period",
        "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy", "gg_out_array",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
-       "gg_grav_lists", "gg_grav_walk_checksums", "gg_tree_info", "gg_direct_grav", "gg_build_local", "gg_smooth", "gg_resmooth",
+       "gg_grav_lists", "gg_grav_walk_checksums", "gg_tree_info", "gg_direct_grav",
+       "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
+       "gg_set_grav_schedule",
        "gg_set_seed_slack", "gg_reorder", "gg_init_accel", "gg_kick", "gg_drift",
        "gg_kick_vpred", "gg_init_dt", "gg_grav_step", "gg_accel_step", "gg_density_step",
        "gg_sph_step", "gg_dt_to_rung", "gg_active_mask_rung", "gg_active_type",
@@ -141,6 +143,7 @@ def lib():
     L.gg_grav_walk_checksums.argtypes = [_vp, C.c_int, _vp]
     L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
     L.gg_direct_grav.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
+    L.gg_set_grav_schedule.argtypes = [_vp, C.c_int]
     L.gg_reorder.argtypes = [_vp, _ip]
     L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
     L.gg_smooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int, _ip]
@@ -316,6 +319,11 @@ class Pkd:
         off = np.ascontiguousarray(sysOff, np.int32)
         _check(lib().gg_build_forest(self.h, len(off) - 1, _p(off, _ip), int(nBucket),
                                      int(iOpenType), float(dCrit), int(iOrder)))
+
+    def set_grav_schedule(self, mode):
+        """gg_set_grav_schedule: 0 one workgroup per bucket, 1 persistent
+        queue (heaviest buckets first)."""
+        _check(lib().gg_set_grav_schedule(self.h, int(mode)))
 
     def grav_graph(self, iOrder=2):
         _check(lib().gg_grav_graph(self.h, int(iOrder)))
@@ -582,7 +590,8 @@ class Pkd:
 class StepParam(C.Structure):
     """ggStepParam (gasoline_amd/host/gg_msr.h): the msr->param fields the
     block-timestep recursion reads."""
-    _fields_ = [("dDelta", C.c_double), ("iMaxRung", C.c_int), ("dEta", C.c_double),
+    _fields_ = [("dDelta", C.c_double), ("iMaxRung", C.c_int), ("bAdjust", C.c_int),
+                ("dEta", C.c_double),
                 ("dEtaCourant", C.c_double), ("dEtauDot", C.c_double), ("duMinDt", C.c_double),
                 ("bGravStep", C.c_int), ("bAccelStep", C.c_int), ("bEpsAccStep", C.c_int),
                 ("bSqrtPhiStep", C.c_int), ("bSphStep", C.c_int), ("bViscosityLimitdt", C.c_int),
@@ -626,11 +635,13 @@ class BlockSteps:
 
     def __init__(self, pk, dDelta, iMaxRung=8, dEta=0.2, dEtaCourant=0.4, smf=None,
                  bDoGravity=True, bDoGas=True, bGravStep=True, bSphStep=True, dTheta=0.7,
-                 iOrder=2, nBucket=16, nSmooth=32, nBucketSph=32, period=None, log_cap=4096):
+                 iOrder=2, nBucket=16, nSmooth=32, nBucketSph=32, period=None, log_cap=4096,
+                 bAdjust=True):
         smf = smf if smf is not None else Smf.default()
         smf.dtFacCourant = dEtaCourant * 2 / 1.6
         self.pk = pk
-        self.p = StepParam(dDelta=dDelta, iMaxRung=iMaxRung, dEta=dEta, dEtaCourant=dEtaCourant,
+        self.p = StepParam(dDelta=dDelta, iMaxRung=iMaxRung, bAdjust=int(bAdjust), dEta=dEta,
+                           dEtaCourant=dEtaCourant,
                            dEtauDot=0.25, duMinDt=0.0, bGravStep=int(bGravStep), bAccelStep=0,
                            bEpsAccStep=0, bSqrtPhiStep=0, bSphStep=int(bSphStep),
                            bViscosityLimitdt=0, bDoGravity=int(bDoGravity), bDoGas=int(bDoGas),
@@ -650,9 +661,9 @@ class BlockSteps:
         rc = msr_lib().ggMsrForces(self.pk.h, C.byref(self.p), C.byref(self.s), int(iKickRung))
         _check(rc)
 
-    def top_step(self, iAdjust=True):
+    def top_step(self):
         rc = msr_lib().ggMsrTopStepKDK(self.pk.h, C.byref(self.p), C.byref(self.s), 0.0,
-                                      self.dTime, self.p.dDelta, 0, 0, int(iAdjust))
+                                      self.dTime, self.p.dDelta, 0, 0, 1)
         _check(rc)
         self.dTime += self.p.dDelta
 

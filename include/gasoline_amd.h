@@ -247,6 +247,7 @@ int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
  * pot[k] are host arrays. */
 int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
 
+
 /* ----------------------------------------------------------------- SPH */
 
 /* Smoothing parameters: the subset of gasoline's SMF (smoothfcn.h:22-130)
@@ -364,6 +365,16 @@ int gg_build_forest(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOp
 int gg_grav_graph(GGPKD pkd, int iOrder);
 int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double *pdCellSum,
                    double *pdSoftSum, double *pdFlop);
+/* How the fused walk + interaction kernel of gg_grav_all (and of a graph
+ * captured afterwards) is scheduled: 0 (default) one workgroup per active
+ * bucket, dealt by the hardware in tree order; 1 a persistent grid that
+ * fills the chip once, whose workgroups take buckets from a queue: after a
+ * first evaluation of a tree the queue holds its buckets heaviest first (by
+ * gasoline's flop model on that evaluation's list sizes, grouped by binary
+ * exponent), so the longest buckets start first across systems of uneven
+ * N.  Results do not depend on the schedule. */
+int gg_set_grav_schedule(GGPKD pkd, int mode);
+
 /* Bucket-size autotuner: builds the tree (sysOff = NULL) or the forest with
  * each of the nCand candidate bucket sizes, times the device work of
  * gg_grav_all, and leaves the handle built with the fastest (*pBest);

@@ -151,3 +151,37 @@ def test_autotuner(gpu, ens):
         pk.build_forest(ens["off"], nBucket=best, dCrit=0.6)
         pk.grav_all(2)
         assert np.array_equal(pk.get("a"), a)
+
+
+@pytest.mark.parametrize("prec", ["FP32", "FP64"])
+def test_persistent_schedule_equals_per_bucket(gpu, ens, prec):
+    # the persistent grid (workgroups taking buckets from a queue, heaviest
+    # first once a first evaluation has measured them) gives the per-bucket
+    # launch's results bit for bit: each bucket's sums do not depend on when
+    # it runs; also replayed from a graph captured with the queue
+    p = gpu.PREC_FP32 if prec == "FP32" else gpu.PREC_FP64
+    n = len(ens["m"])
+    out = []
+    for mode in (0, 1):
+        with gpu.Pkd(n, prec=p) as pk:
+            load(pk, ens, 0.005)
+            pk.set_grav_schedule(mode)
+            pk.build_forest(ens["off"], nBucket=16, dCrit=0.6)
+            res = []
+            for call in range(3):            # identity queue, then heaviest first
+                pk.init_accel()
+                st = pk.grav_all(2)
+                res.append((pk.get("a"), pk.get("fPot"), pk.get("dtGrav"), st))
+            pk.grav_graph(2)
+            for _ in range(2):
+                pk.init_accel()
+                st = pk.grav_replay()
+                res.append((pk.get("a"), pk.get("fPot"), pk.get("dtGrav"), st))
+            out.append(res)
+    ref = out[0][0]
+    for res in out:
+        for a, pot, dtg, st in res:
+            assert np.array_equal(a, ref[0]) and np.array_equal(pot, ref[1])
+            assert np.array_equal(dtg, ref[2])
+            for k in ("nActive", "dPartSum", "dCellSum", "dSoftSum", "dFlop"):
+                assert st[k] == ref[3][k], k
