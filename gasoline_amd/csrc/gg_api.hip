@@ -153,6 +153,7 @@ extern "C" int gg_initialize(GGPKD *ppkd, int device, int nStore)
     GG_HIP(hipGetDeviceCount(&ndev));
     GG_CHECK(device >= 0 && device < ndev, GG_EINVAL, "device %d not in 0..%d", device, ndev - 1);
     GG_HIP(hipSetDevice(device));
+    GG_TRY(direct_scratch_setup(device));   // the no-alloc direct sum's array on this GPU
     gg_pkd_s *p = new gg_pkd_s();
     p->device = device;
     p->nStore = nStore;

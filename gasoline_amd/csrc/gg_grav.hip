@@ -6,6 +6,8 @@
 // (walk.c:306-439) -> pkdBucketInteract (grav.c:23-249).
 #include "gg_internal.h"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cstring>
 
@@ -1568,7 +1570,79 @@ __global__ void k_direct_sum(int nt, const double *__restrict__ part, double *__
         out[4 * (size_t)k + q] = s;
     }
 }
+// The shares' sums of gg_direct_grav_noalloc, added in order into the
+// caller's device arrays; a target outside the store gives NaN.
+__global__ void k_direct_out(int nt, int n, const int *__restrict__ tgt,
+                             const double *__restrict__ part, double *__restrict__ a,
+                             double *__restrict__ pot)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    const int i = tgt[k];
+    double s[4];
+    for (int q = 0; q < 4; ++q) {
+        s[q] = 0.0;
+        for (int p = 0; p < DSPLIT; ++p) s[q] += part[((size_t)p * nt + k) * 4 + q];
+        if (i < 0 || i >= n) s[q] = __builtin_nan("");
+    }
+    for (int d = 0; d < 3; ++d) a[3 * (size_t)k + d] = s[d];
+    pot[k] = s[3];
+}
+
+// Scratch of gg_direct_grav_noalloc: the DSPLIT shares' partial sums (4 per
+// target) of at most GG_DIRECT_NOALLOC_MAX targets, 8 MiB, a fixed device
+// array that every GPU the library runs on holds until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host the name
+// is only a stand-in): it uses the device address hipGetSymbolAddress gives,
+// looked up once per device by direct_scratch_setup.
+constexpr int NOALLOC_MAX = GG_DIRECT_NOALLOC_MAX;
+constexpr int NOALLOC_DEV = 64;
+static __device__ __align__(256) double scratch[(size_t)DSPLIT * 4 * NOALLOC_MAX];
+static std::mutex scratch_mu;
+static double *scratch_ptr[NOALLOC_DEV];
+
+int direct_scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    GG_CHECK(device >= 0 && device < NOALLOC_DEV, GG_EINVAL, "device %d outside 0..%d", device,
+             NOALLOC_DEV - 1);
+    if (scratch_ptr[device]) return GG_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    GG_CHECK(e == hipSuccess && p, GG_EHIP, "hipGetSymbolAddress of the direct-sum scratch: %s",
+             hipGetErrorString(e));
+    scratch_ptr[device] = static_cast<double *>(p);
+    return GG_OK;
+}
+
+static double *direct_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return device >= 0 && device < NOALLOC_DEV ? scratch_ptr[device] : nullptr;
+}
 }  // namespace gg
+
+extern "C" int gg_direct_grav_noalloc(GGPKD pkd, int nTarget, const int *target, double *a,
+                                      double *pot)
+{
+    GG_CHECK(pkd && (nTarget == 0 || (target && a && pot)), GG_EINVAL, "null argument");
+    GG_CHECK(nTarget >= 0 && nTarget <= NOALLOC_MAX, GG_EINVAL,
+             "nTarget %d outside 0..%d (GG_DIRECT_NOALLOC_MAX)", nTarget, NOALLOC_MAX);
+    GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
+             "positions, masses and softenings must be put before gg_direct_grav_noalloc");
+    double *part = direct_scratch(pkd->device);
+    GG_CHECK(part, GG_ESTATE, "no direct-sum scratch on device %d", pkd->device);
+    if (nTarget == 0) return GG_OK;
+    hipStream_t st = pkd->stream;
+    const int n = pkd->nLocal;
+    hipLaunchKernelGGL(k_direct, dim3((nTarget + 255) / 256, DSPLIT), dim3(256), 0, st, n,
+                       nTarget, target, pkd->f[GG_R].as<double>(), pkd->f[GG_MASS].as<double>(),
+                       pkd->f[GG_SOFT].as<double>(), part);
+    hipLaunchKernelGGL(k_direct_out, dim3((nTarget + 255) / 256), dim3(256), 0, st, nTarget, n,
+                       target, part, a, pot);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
 
 extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot)
 {

@@ -245,5 +245,8 @@ void sph_release(GGPKD pkd);
 int ewald_apply(GGPKD pkd, int nReps, int iOrder, double L, double fEwCut, double fEwhCut,
                 double *flop);
 void timer_mark(GGPKD pkd, int slot);
+// The device address of gg_direct_grav_noalloc's fixed scratch array on
+// `device` (the current device), looked up once per device (gg_initialize).
+int direct_scratch_setup(int device);
 void timer_collect(GGPKD pkd, int first_slot, int nphases, const int *phase);
 }  // namespace gg

@@ -247,6 +247,23 @@ int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
  * pot[k] are host arrays. */
 int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
 
+/* The largest nTarget of gg_direct_grav_noalloc. */
+#define GG_DIRECT_NOALLOC_MAX 4096
+
+/* gg_direct_grav without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  target
+ * (nTarget caller indices), a (3 per target) and pot (1 per target) are
+ * DEVICE pointers; the call only enqueues its kernels on the handle's
+ * stream and returns (order later work after it, or synchronize, before
+ * reading a and pot).  The same kernels as gg_direct_grav, so the results
+ * are the same bit for bit.  The partial sums live in a fixed device
+ * array of 8 MiB (64 source shares x 4 sums x GG_DIRECT_NOALLOC_MAX
+ * targets) that each GPU the library runs on holds until the process ends;
+ * calls that overlap on one GPU (other streams, threads or handles) race
+ * on it.  A target outside 0..nLocal-1 gives NaN.  GG_EINVAL if nTarget
+ * exceeds GG_DIRECT_NOALLOC_MAX. */
+int gg_direct_grav_noalloc(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
 
 /* ----------------------------------------------------------------- SPH */
 
