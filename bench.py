@@ -16,7 +16,9 @@ direct sum is at most 1e-4 (hexadecapole cells, theta 0.55 or 0.5) and at the
 configurations' own theta / quadrupole for comparison, each with its rms and
 max force error against the FP64 direct sum on sampled targets: a Plummer
 sphere, the batched King ensemble, the Evrard collapse (the full gravity +
-SPH step, 100 steps), the disk galaxy and the periodic box.  The CPU
+SPH step, 100 steps), the disk galaxy and the periodic box (against the
+Ewald-summed periodic direct sum, at the reference's nReps 1 and at two
+replica shells).  The CPU
 restatement of the same path is timed on a bounded sample on the host cores.
 
 Prints one JSON line.
@@ -60,6 +62,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 # timed beside it.
 GATE = dict(theta=0.55, iOrder=4)
 GATE_ENSEMBLE = dict(theta=0.5, iOrder=4)
+# The periodic box: (theta, nReps), hexadecapole walk and Ewald.
+PERIODIC_GATE = (0.35, 2)
 
 
 def log(msg):
@@ -223,32 +227,70 @@ def periodic_box(n, seed=21):
     return ics._f32((np.concatenate([u, cl]) + 0.5) % 1.0 - 0.5), np.full(n, 1.0 / n)
 
 
-def time_periodic(gg, torch, n, prec, steps, warmup):
-    """Periodic gravity with the reference's defaults for it: nReps 1,
-    hexadecapole walk and Ewald (iOrder = iEwOrder = 4), dEwCut 2.6, dEwhCut
-    2.8, theta 0.7; the same step without Ewald gives the Ewald pass's time."""
-    r, m = periodic_box(n)
+def time_periodic(gg, torch, r, m, soft, prec, theta, nReps, steps, warmup, idx):
+    """Periodic gravity in the unit box: hexadecapole walk and Ewald (iOrder =
+    iEwOrder = 4), dEwCut 2.6, dEwhCut 2.8, nReps replica shells; the same
+    call without Ewald gives the Ewald pass's time.  Returns the timings and
+    the accelerations of the sampled particles idx."""
+    n = len(m)
     pk = gg.Pkd(n, prec=prec)
     pk.put("r", r)
     pk.put("fMass", m)
-    pk.put("fSoft", np.full(n, 0.2 / n ** (1 / 3)))
+    pk.put("fSoft", soft)
     pk.set_period([1.0, 1.0, 1.0])
-    pk.build_binary(16, 1, 0.7, 4)
+    pk.build_binary(16, 1, theta, 4)
     out = {}
     for ew in (True, False):
+        pk.init_accel()
+        st = pk.grav_all(4, nReps=nReps, periodic=True, bEwald=ew)
+        if ew:
+            acc = pk.get("a")[idx]
         for _ in range(warmup):
-            st = pk.grav_all(4, nReps=1, periodic=True, bEwald=ew)
+            st = pk.grav_all(4, nReps=nReps, periodic=True, bEwald=ew)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            st = pk.grav_all(4, nReps=1, periodic=True, bEwald=ew)
+            st = pk.grav_all(4, nReps=nReps, periodic=True, bEwald=ew)
         torch.cuda.synchronize()
         out[ew] = ((time.perf_counter() - t0) / steps, st)
     pk.close()
     dt, st = out[True]
     nint = st["dPartSum"] + st["dCellSum"] + st["dSoftSum"]
     return {"ms": dt * 1e3, "ms_without_ewald": out[False][0] * 1e3,
-            "interactions": nint, "interactions_per_s": nint / dt, "flop": st["dFlop"]}
+            "interactions": nint, "interactions_per_s": nint / dt, "flop": st["dFlop"]}, acc
+
+
+def periodic_leg(gg, torch, n, steps, warmup, nsample=2000):
+    """The periodic box at the reference's defaults (theta 0.7, nReps 1) and
+    at the point whose rms error against the periodic direct sum is under
+    1e-4 (two replica shells, theta 0.35: with one shell the hexadecapole
+    expansion of the root in the Ewald correction leaves ~5e-4 at any
+    theta), FP32 and FP64, each with its force error against the FP64
+    Ewald-summed direct sum (gg_direct_ewald) on sampled targets."""
+    r, m = periodic_box(n)
+    soft = np.full(n, 0.2 / n ** (1 / 3))
+    idx = sample(n, nsample)
+    t0 = time.perf_counter()
+    with gg.Pkd(n, prec=gg.PREC_FP64) as pk:
+        pk.put("r", r)
+        pk.put("fMass", m)
+        pk.put("fSoft", soft)
+        pk.set_period([1.0, 1.0, 1.0])
+        ad = pk.direct_ewald(idx)[0]
+    t_direct = time.perf_counter() - t0
+    out = {"direct_ewald_s": t_direct}
+    for name, (theta, nReps) in {"reference_config": (0.7, 1), "gate": PERIODIC_GATE}.items():
+        p32, a32 = time_periodic(gg, torch, r, m, soft, gg.PREC_FP32, theta, nReps, steps,
+                                 warmup, idx)
+        p64, a64 = time_periodic(gg, torch, r, m, soft, gg.PREC_FP64, theta, nReps,
+                                 max(2, steps // 2), 1, idx)
+        out[name] = {"theta": theta, "nReps": nReps, "iOrder": 4, "fp32": p32, "fp64": p64,
+                     "speedup_vs_fp64": p64["ms"] / p32["ms"],
+                     "force_error": {"targets": len(idx),
+                                     "fp32_vs_direct_ewald": rel_errors(a32, ad),
+                                     "fp64_vs_direct_ewald": rel_errors(a64, ad),
+                                     "fp32_vs_fp64": rel_errors(a32, a64)}}
+    return out
 
 
 def time_ensemble(gg, torch, e, steps, prec, theta, order):
@@ -471,13 +513,11 @@ def main():
 
     per = None
     if args.periodic > 0:
-        p32 = time_periodic(gg, torch, args.periodic, gg.PREC_FP32, args.steps, args.warmup)
-        p64 = time_periodic(gg, torch, args.periodic, gg.PREC_FP64, max(2, args.steps // 2), 1)
         per = {"workload": "periodic unit box N=%d (half uniform, half in three clumps of "
-                           "sigma 0.03), eps 0.2 n^-1/3, theta 0.7, hexadecapole walk, nReps 1, "
-                           "Ewald iEwOrder 4, dEwCut 2.6, dEwhCut 2.8; gg_grav_all (tree build "
-                           "excluded)" % args.periodic,
-               "fp32": p32, "fp64": p64, "speedup_vs_fp64": p64["ms"] / p32["ms"]}
+                           "sigma 0.03), eps 0.2 n^-1/3, hexadecapole walk, Ewald iEwOrder 4, "
+                           "dEwCut 2.6, dEwhCut 2.8; gg_grav_all (tree build excluded)"
+                           % args.periodic}
+        per.update(periodic_leg(gg, torch, args.periodic, args.steps, args.warmup))
         log("periodic gravity done")
     ens = None
     if args.nsys > 0:

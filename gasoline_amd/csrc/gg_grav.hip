@@ -1021,8 +1021,17 @@ __global__ __launch_bounds__(64) void k_walk_interact(IArgs A, const double *__r
 // first when the previous evaluation's costs are known) until none is left.
 // *next must be 0 at launch.  Every workgroup leaves the loop once the
 // queue is exhausted.
+// The persistent kernels' loop around walk_interact_one costs the compiler
+// a few registers, which at its own choice would drop a wave per SIMD below
+// the per-bucket kernel's occupancy; they are held to that occupancy.
+template <typename T, int ORDER> constexpr int persistent_waves()
+{
+    return sizeof(T) == 4 ? (ORDER >= 4 ? 3 : 4) : (ORDER >= 4 ? 2 : 3);
+}
+
 template <typename T, int ORDER>
-__global__ __launch_bounds__(64) void k_walk_interact_queue(IArgs A,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(persistent_waves<T, ORDER>())))
+void k_walk_interact_queue(IArgs A,
                                                             const double *__restrict__ bbox,
                                                             const int *__restrict__ broot,
                                                             int *__restrict__ cnt,
@@ -1039,6 +1048,46 @@ __global__ __launch_bounds__(64) void k_walk_interact_queue(IArgs A,
         walk_interact_one<T, ORDER, false, false>(A, queue[slot], bbox, broot, cnt, err, nullptr,
                                                   S);
         __syncthreads();
+    }
+}
+
+// Persistent form that keeps each XCD's L2 on its own part of the tree:
+// the active buckets, in tree order, are dealt to the eight XCDs in chunks
+// of QC as xcd_chunk deals them, and next[x] counts the buckets XCD x's
+// workgroups have taken from its share.  A workgroup (on XCD blockIdx.x & 7,
+// the order in which the hardware deals workgroups) takes from its own
+// XCD's share and, once that is exhausted, from the others' in turn, so the
+// tail of a clumpy tree is shared out.  next[0..7] must be 0 at launch.
+// Every workgroup visits each of the eight shares once and stops at the
+// first slot past a share's end, so every workgroup leaves.
+constexpr int QC = 128;
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(persistent_waves<T, ORDER>())))
+void k_walk_interact_xcdq(IArgs A,
+                                                           const double *__restrict__ bbox,
+                                                           const int *__restrict__ broot,
+                                                           int *__restrict__ cnt,
+                                                           int *__restrict__ err,
+                                                           unsigned *__restrict__ next)
+{
+    __shared__ WIShared<T, ORDER> S;
+    const int x0 = blockIdx.x & 7;
+    const unsigned nb = (unsigned)A.nb;
+    for (int v = 0; v < 8; ++v) {
+        const int x = (x0 + v) & 7;
+        for (;;) {
+            unsigned slot = 0;
+            if (threadIdx.x == 0) slot = atomicAdd(&next[x], 1u);
+            slot = __shfl(slot, 0);
+            // slot s of XCD x: chunk (s / QC) * 8 + x, position s % QC in it;
+            // later slots of x lie further on, so the first one past nb ends x
+            const unsigned long long b =
+                ((unsigned long long)(slot / QC) * 8 + x) * QC + slot % QC;
+            if (b >= nb) break;
+            walk_interact_one<T, ORDER, false, false>(A, (int)b, bbox, broot, cnt, err, nullptr,
+                                                      S);
+            __syncthreads();
+        }
     }
 }
 
@@ -1244,6 +1293,25 @@ static int launch_fused(int order, const IArgs &A, GravDev &G, hipStream_t st)
     const double *bb = G.bbox.as<double>();
     int *cnt = G.cnt.as<int>(), *err = G.werr.as<int>();
     const int *br = G.broot.as<int>();
+    if (G.sched == 2 && A.nReps == 0) {   // persistent, a share of the tree per XCD
+        GG_TRY(G.qnext.reserve(sizeof(unsigned) * 8));
+        hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, 8, G.qnext.as<uint32_t>());
+        unsigned *nx = G.qnext.as<unsigned>();
+#define GG_XQ(O)                                                                                 \
+    GG_TRY((persistent_grid<T, O>(G)));                                                           \
+    hipLaunchKernelGGL((k_walk_interact_xcdq<T, O>), dim3(G.qGrid), block, 0, st, A, bb, br,    \
+                       cnt, err, nx)
+        switch (order) {
+        case 1: GG_XQ(1); break;
+        case 2: GG_XQ(2); break;
+        case 3: GG_XQ(3); break;
+        case 4: GG_XQ(4); break;
+        default: set_error("iOrder %d not in 1..4", order); return GG_EINVAL;
+        }
+#undef GG_XQ
+        GG_HIP(hipGetLastError());
+        return GG_OK;
+    }
     if (G.sched == 1 && A.nReps == 0) {   // persistent: workgroups take buckets from the queue
         if (G.qTree != G.treeId) GG_TRY(queue_order(G, st, false));
         GG_TRY(G.qnext.reserve(sizeof(unsigned)));
@@ -1693,6 +1761,265 @@ extern "C" int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double 
     return GG_OK;
 }
 
+namespace gg {
+// FP64 periodic direct sum for sampled targets, the reference force of the
+// periodic tree + Ewald forces: the Ewald-summed field (Ewald 1921) of every
+// other particle and of all images of every particle in the cube of side L,
+// with the uniform background that removes the k = 0 mode (the Green's
+// function pkdBucketEwald, ewald.c:15-178, expands for the root's moments),
+// and the spline softening (twoh = h_i + h_j, grav.c:81-108) of images
+// nearer than twoh.  Split alpha = 6 / L: the real-space sum is complete at
+// r = L (erfc(6) = 2e-17), so a source's 27 images about its nearest one
+// hold every term; the wave-vector sum runs to exp(-pi^2 h^2 / 36) < 1e-17
+// through structure factors S(k) = sum_j m_j exp(i k.x_j), one pass over
+// the sources per block of wave vectors.
+constexpr double EW_ALPHAL = 6.0;
+struct EwK {
+    double kx, ky, kz, c;   // k = 2 pi h / L (one of +h, -h), c = 2 e^(-pi^2 h^2/(aL)^2) / (pi L h^2)
+};
+constexpr int SF_T = 256, SF_KV = 4;
+
+__device__ __forceinline__ void kahan_add(double &s, double &c, double v)
+{
+    const double y = v - c, t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Structure factors of SF_KV wave vectors per workgroup: sf[2q] = sum_j m_j
+// cos(k_q.x_j), sf[2q+1] = sum_j m_j sin(k_q.x_j), each thread over a fixed
+// stride of sources, then a fixed-order tree over the workgroup.
+__global__ __launch_bounds__(SF_T) void k_ewald_sf(int n, int nk, const EwK *__restrict__ kt,
+                                                   const double *__restrict__ r,
+                                                   const double *__restrict__ m,
+                                                   double *__restrict__ sf)
+{
+    __shared__ double red[SF_T];
+    const int q0 = blockIdx.x * SF_KV;
+    double kx[SF_KV], ky[SF_KV], kz[SF_KV], s[2 * SF_KV], c[2 * SF_KV];
+    for (int q = 0; q < SF_KV; ++q) {
+        const bool ok = q0 + q < nk;
+        kx[q] = ok ? kt[q0 + q].kx : 0.0;
+        ky[q] = ok ? kt[q0 + q].ky : 0.0;
+        kz[q] = ok ? kt[q0 + q].kz : 0.0;
+        s[2 * q] = s[2 * q + 1] = c[2 * q] = c[2 * q + 1] = 0.0;
+    }
+    for (int j = threadIdx.x; j < n; j += SF_T) {
+        const double x = r[3 * (size_t)j], y = r[3 * (size_t)j + 1], z = r[3 * (size_t)j + 2];
+        const double mj = m[j];
+        for (int q = 0; q < SF_KV; ++q) {
+            double sn, cs;
+            sincos(kx[q] * x + ky[q] * y + kz[q] * z, &sn, &cs);
+            kahan_add(s[2 * q], c[2 * q], mj * cs);
+            kahan_add(s[2 * q + 1], c[2 * q + 1], mj * sn);
+        }
+    }
+    for (int v = 0; v < 2 * SF_KV; ++v) {
+        red[threadIdx.x] = s[v] - c[v];
+        __syncthreads();
+        for (int w = SF_T / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && q0 + v / 2 < nk) sf[2 * (q0 + v / 2) + (v & 1)] = red[0];
+        __syncthreads();
+    }
+}
+
+// Real-space share: one thread per target, DSPLIT workgroups per 256
+// targets, each over its share of the sources staged through LDS.
+__global__ __launch_bounds__(256) void k_direct_ewald(int n, int nt, const int *__restrict__ tgt,
+                                                      const double *__restrict__ r,
+                                                      const double *__restrict__ m,
+                                                      const double *__restrict__ h, double L,
+                                                      double *__restrict__ part)
+{
+    __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    int i = k < nt ? tgt[k] : -1;
+    if (i >= n) i = -1;
+    const double xi = i >= 0 ? r[3 * (size_t)i] : 0.0, yi = i >= 0 ? r[3 * (size_t)i + 1] : 0.0;
+    const double zi = i >= 0 ? r[3 * (size_t)i + 2] : 0.0, hi = i >= 0 ? h[i] : 0.0;
+    const double alpha = EW_ALPHAL / L, rc2 = L * L, ka = 2 * alpha / sqrt(M_PI);
+    double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+    const int per = (n + DSPLIT - 1) / DSPLIT;
+    const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
+    for (int t0 = j0; t0 < j1; t0 += 256) {
+        const int j = t0 + threadIdx.x;
+        if (j < j1) {
+            sx[threadIdx.x] = r[3 * (size_t)j];
+            sy[threadIdx.x] = r[3 * (size_t)j + 1];
+            sz[threadIdx.x] = r[3 * (size_t)j + 2];
+            sm[threadIdx.x] = m[j];
+            sh[threadIdx.x] = h[j];
+        }
+        __syncthreads();
+        const int nj = min(256, j1 - t0);
+        if (i >= 0)
+            for (int e = 0; e < nj; ++e) {
+                if (t0 + e == i) continue;
+                double dx = sx[e] - xi, dy = sy[e] - yi, dz = sz[e] - zi;
+                dx -= L * rint(dx / L);
+                dy -= L * rint(dy / L);
+                dz -= L * rint(dz / L);
+                const double twoh = hi + sh[e], mj = sm[e];
+                for (int nx = -1; nx <= 1; ++nx)
+                    for (int ny = -1; ny <= 1; ++ny)
+                        for (int nz = -1; nz <= 1; ++nz) {
+                            const double px = dx + nx * L, py = dy + ny * L, pz = dz + nz * L;
+                            const double s2 = px * px + py * py + pz * pz;
+                            if (s2 >= rc2) continue;
+                            if (s2 == 0.0) {   // coincident: spline(0) minus the erf part
+                                double aa, bb;
+                                spline_ab<double>(0.0, __builtin_inf(), twoh, aa, bb);
+                                kahan_add(s[3], c[3], -mj * (aa - ka));
+                                continue;
+                            }
+                            const double sr = sqrt(s2), dir = 1.0 / sr;
+                            double ee = erfc(alpha * sr) * dir;
+                            double g = (ee + ka * exp(-alpha * alpha * s2)) / s2;
+                            if (s2 < twoh * twoh) {   // softened: spline minus Newton
+                                double aa, bb;
+                                spline_ab<double>(s2, dir, twoh, aa, bb);
+                                ee += aa - dir;
+                                g += bb - dir / s2;
+                            }
+                            kahan_add(s[0], c[0], mj * g * px);
+                            kahan_add(s[1], c[1], mj * g * py);
+                            kahan_add(s[2], c[2], mj * g * pz);
+                            kahan_add(s[3], c[3], -mj * ee);
+                        }
+            }
+        __syncthreads();
+    }
+    if (i >= 0)
+        for (int q = 0; q < 4; ++q) part[((size_t)blockIdx.y * nt + k) * 4 + q] = s[q] - c[q];
+}
+
+// Each target: its real-space shares in order, the wave-vector sum through
+// the structure factors (its own term has no force), the background of
+// every other particle and xi m_i for its own images.  sf[2 nk] holds the
+// total mass (the k = 0 entry of the table).
+__global__ void k_direct_ewald_out(int nt, int nk, const int *__restrict__ tgt,
+                                   const double *__restrict__ r, const double *__restrict__ m,
+                                   const EwK *__restrict__ kt, const double *__restrict__ sf,
+                                   const double *__restrict__ part, double bg, double xi,
+                                   double *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    const int i = tgt[k];
+    const double x = r[3 * (size_t)i], y = r[3 * (size_t)i + 1], z = r[3 * (size_t)i + 2];
+    double s[4];
+    for (int q = 0; q < 4; ++q) {
+        s[q] = 0.0;
+        for (int p = 0; p < DSPLIT; ++p) s[q] += part[((size_t)p * nt + k) * 4 + q];
+    }
+    double ws[4] = {0, 0, 0, 0}, wc[4] = {0, 0, 0, 0};
+    for (int q = 0; q < nk; ++q) {
+        double sn, cs;
+        sincos(kt[q].kx * x + kt[q].ky * y + kt[q].kz * z, &sn, &cs);
+        const double C = sf[2 * q], S = sf[2 * q + 1];
+        // sum_j m_j sin(k.(x_i - x_j)) and cos(...)
+        const double f = kt[q].c * (sn * C - cs * S);
+        kahan_add(ws[0], wc[0], -f * kt[q].kx);
+        kahan_add(ws[1], wc[1], -f * kt[q].ky);
+        kahan_add(ws[2], wc[2], -f * kt[q].kz);
+        kahan_add(ws[3], wc[3], -kt[q].c * (cs * C + sn * S));
+    }
+    const double mi = m[i];
+    for (int q = 0; q < 3; ++q) out[4 * (size_t)k + q] = s[q] + (ws[q] - wc[q]);
+    // the k-space self term -sum c m_i is part of xi: take it back out of the sum
+    double cself = 0.0;
+    for (int q = 0; q < nk; ++q) cself += kt[q].c;
+    out[4 * (size_t)k + 3] = s[3] + (ws[3] - wc[3]) + cself * mi + (sf[2 * nk] - mi) * bg + xi * mi;
+}
+}  // namespace gg
+
+extern "C" int gg_direct_ewald(GGPKD pkd, int nTarget, const int *target, double *a, double *pot)
+{
+    GG_CHECK(pkd && (nTarget == 0 || (target && a && pot)), GG_EINVAL, "null argument");
+    GG_CHECK(nTarget >= 0, GG_EINVAL, "nTarget must be >= 0");
+    GG_CHECK(pkd->f[GG_R].p && pkd->f[GG_MASS].p && pkd->f[GG_SOFT].p, GG_ESTATE,
+             "positions, masses and softenings must be put before gg_direct_ewald");
+    const double L = pkd->fPeriod[0];
+    GG_CHECK(L > 0 && pkd->fPeriod[1] == L && pkd->fPeriod[2] == L, GG_EINVAL,
+             "gg_direct_ewald needs a cubic box (gg_set_period)");
+    const int n = pkd->nLocal;
+    for (int k = 0; k < nTarget; ++k)
+        GG_CHECK(target[k] >= 0 && target[k] < n, GG_EINVAL, "target %d outside 0..%d",
+                 target[k], n - 1);
+    if (nTarget == 0) return GG_OK;
+    // the wave-vector table (one of each +h, -h) and a k = 0 entry last
+    const double aL = EW_ALPHAL, alpha = aL / L, kc2 = 39.0 * aL * aL / (M_PI * M_PI);
+    const int hmax = (int)ceil(sqrt(kc2));
+    std::vector<EwK> kt;
+    for (int hx = 0; hx <= hmax; ++hx)
+        for (int hy = -hmax; hy <= hmax; ++hy)
+            for (int hz = -hmax; hz <= hmax; ++hz) {
+                const int h2 = hx * hx + hy * hy + hz * hz;
+                if (hx == 0 && (hy < 0 || (hy == 0 && hz <= 0))) continue;
+                if (h2 > kc2) continue;
+                kt.push_back({2 * M_PI * hx / L, 2 * M_PI * hy / L, 2 * M_PI * hz / L,
+                              2.0 * exp(-M_PI * M_PI * h2 / (aL * aL)) / (M_PI * L * h2)});
+            }
+    const int nk = (int)kt.size();
+    kt.push_back({0.0, 0.0, 0.0, 0.0});
+    // xi = lim_{d->0} (psi(d) + 1/|d|): own images, background, k-space
+    double xi = 2 * alpha / sqrt(M_PI) + M_PI / (alpha * alpha * L * L * L);
+    for (int nx = -1; nx <= 1; ++nx)
+        for (int ny = -1; ny <= 1; ++ny)
+            for (int nz = -1; nz <= 1; ++nz) {
+                if (!nx && !ny && !nz) continue;
+                const double rr = L * sqrt((double)(nx * nx + ny * ny + nz * nz));
+                if (rr < L) xi -= erfc(alpha * rr) / rr;
+            }
+    for (int q = 0; q < nk; ++q) xi -= kt[q].c;
+    const double bg = M_PI / (alpha * alpha * L * L * L);
+
+    hipStream_t st = pkd->stream;
+    DevBuf t, kd, sf, pt, o;
+    int rc = GG_OK;
+    if (t.reserve(sizeof(int) * nTarget) != GG_OK || kd.reserve(sizeof(EwK) * (nk + 1)) != GG_OK ||
+        sf.reserve(sizeof(double) * 2 * (nk + 1)) != GG_OK ||
+        pt.reserve(sizeof(double) * 4 * (size_t)DSPLIT * nTarget) != GG_OK ||
+        o.reserve(sizeof(double) * 4 * nTarget) != GG_OK)
+        rc = GG_EHIP;
+    std::vector<double> res(4 * (size_t)nTarget);
+    if (rc == GG_OK &&
+        (hipMemcpyAsync(t.p, target, sizeof(int) * nTarget, hipMemcpyHostToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(kd.p, kt.data(), sizeof(EwK) * (nk + 1), hipMemcpyHostToDevice, st) !=
+             hipSuccess))
+        rc = GG_EHIP;
+    if (rc == GG_OK) {
+        const double *r = pkd->f[GG_R].as<double>(), *m = pkd->f[GG_MASS].as<double>();
+        hipLaunchKernelGGL(k_ewald_sf, dim3((nk + 1 + SF_KV - 1) / SF_KV), dim3(SF_T), 0, st, n,
+                           nk + 1, kd.as<EwK>(), r, m, sf.as<double>());
+        hipLaunchKernelGGL(k_direct_ewald, dim3((nTarget + 255) / 256, DSPLIT), dim3(256), 0, st,
+                           n, nTarget, t.as<int>(), r, m, pkd->f[GG_SOFT].as<double>(), L,
+                           pt.as<double>());
+        hipLaunchKernelGGL(k_direct_ewald_out, dim3((nTarget + 255) / 256), dim3(256), 0, st,
+                           nTarget, nk, t.as<int>(), r, m, kd.as<EwK>(), sf.as<double>(),
+                           pt.as<double>(), bg, xi, o.as<double>());
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(res.data(), o.p, sizeof(double) * 4 * nTarget, hipMemcpyDeviceToHost,
+                           st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = GG_EHIP;
+    }
+    t.release();
+    kd.release();
+    sf.release();
+    pt.release();
+    o.release();
+    GG_CHECK(rc == GG_OK, GG_EHIP, "gg_direct_ewald: HIP error");
+    for (int k = 0; k < nTarget; ++k) {
+        for (int d = 0; d < 3; ++d) a[3 * (size_t)k + d] = res[4 * (size_t)k + d];
+        pot[k] = res[4 * (size_t)k + 3];
+    }
+    return GG_OK;
+}
+
 extern "C" int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
                             double *dbls)
 {
@@ -1825,7 +2152,7 @@ extern "C" int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double
 extern "C" int gg_set_grav_schedule(GGPKD pkd, int mode)
 {
     GG_CHECK(pkd, GG_EINVAL, "null handle");
-    GG_CHECK(mode == 0 || mode == 1, GG_EINVAL, "schedule %d: 0 or 1", mode);
+    GG_CHECK(mode >= 0 && mode <= 2, GG_EINVAL, "schedule %d: 0, 1 or 2", mode);
     pkd->grav.sched = mode;
     pkd->grav.qTree = -1;
     return GG_OK;

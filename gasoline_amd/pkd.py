@@ -55,7 +55,7 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy", "gg_out_array",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_grav_walk_checksums", "gg_tree_info", "gg_direct_grav",
-       "gg_direct_grav_noalloc", "gg_build_local", "gg_smooth", "gg_resmooth",
+       "gg_direct_grav_noalloc", "gg_direct_ewald", "gg_build_local", "gg_smooth", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_grav_schedule",
@@ -144,6 +144,7 @@ def lib():
     L.gg_tree_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
     L.gg_direct_grav.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
     L.gg_direct_grav_noalloc.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+    L.gg_direct_ewald.argtypes = [_vp, C.c_int, _ip, _dp, _dp]
     L.gg_set_grav_schedule.argtypes = [_vp, C.c_int]
     L.gg_reorder.argtypes = [_vp, _ip]
     L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
@@ -384,6 +385,16 @@ class Pkd:
         a = np.empty((len(t), 3))
         pot = np.empty(len(t))
         _check(lib().gg_direct_grav(self.h, len(t), _p(t, _ip), _p(a), _p(pot)))
+        return a, pot
+
+    def direct_ewald(self, targets):
+        """FP64 periodic direct sum (a, pot) of the given caller indices in the
+        cubic box set by set_period: Ewald-summed images with the uniform
+        background, spline softening inside h_i + h_j."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        a = np.empty((len(t), 3))
+        pot = np.empty(len(t))
+        _check(lib().gg_direct_ewald(self.h, len(t), _p(t, _ip), _p(a), _p(pot)))
         return a, pot
 
     def tree_info(self):

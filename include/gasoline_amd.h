@@ -247,6 +247,19 @@ int gg_tree_info(GGPKD pkd, int *nNodes, int *iRoot, int *perm, int *ints,
  * pot[k] are host arrays. */
 int gg_direct_grav(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
 
+/* The periodic counterpart of gg_direct_grav in the cubic box set by
+ * gg_set_period (side L): FP64 accelerations and potentials of target[k]
+ * from every other particle and all images of every particle, Ewald-summed
+ * (split alpha = 6 / L, both sums complete to 1e-16) with the uniform
+ * background that removes the k = 0 mode, and the spline softening of the
+ * images nearer than twoh = h_i + h_j.  This is the field whose expansion
+ * about the root's moments pkdBucketEwald (ewald.c:15-178) evaluates, so it
+ * is the reference force of the periodic tree + Ewald forces.  pot includes
+ * the target's own images, m_i xi (xi L = 2.8372974795, the simple cubic
+ * lattice's Madelung constant).  No counterpart in the reference; host
+ * arrays; GG_EINVAL without a cubic period. */
+int gg_direct_ewald(GGPKD pkd, int nTarget, const int *target, double *a, double *pot);
+
 /* The largest nTarget of gg_direct_grav_noalloc. */
 #define GG_DIRECT_NOALLOC_MAX 4096
 
@@ -389,7 +402,10 @@ int gg_grav_replay(GGPKD pkd, int *nActive, double *pdPartSum, double *pdCellSum
  * first evaluation of a tree the queue holds its buckets heaviest first (by
  * gasoline's flop model on that evaluation's list sizes, grouped by binary
  * exponent), so the longest buckets start first across systems of uneven
- * N.  Results do not depend on the schedule. */
+ * N; 2 a persistent grid in which each of the eight XCDs takes the buckets
+ * of its own share of the tree (in tree order, dealt in chunks of 128) and,
+ * once that is done, helps with the other shares.  Results do not depend on
+ * the schedule. */
 int gg_set_grav_schedule(GGPKD pkd, int mode);
 
 /* Bucket-size autotuner: builds the tree (sysOff = NULL) or the forest with

@@ -1324,3 +1324,148 @@ void orc_direct_sum(int n, const double *r, const double *m, const double *soft,
         pot[k] = ph;
     }
 }
+
+/* ======================================================================
+ * Periodic direct sum: the Ewald-summed field of every particle and all of
+ * its images in the cubic box of side L, with the uniform background that
+ * makes the k = 0 mode vanish (the Green's function pkdBucketEwald,
+ * ewald.c:15-178, expands for the root's moments), and the spline
+ * softening (grav.h:74-97) of the images nearer than twoh = h_i + h_j.
+ * Pair potential, G = 1, d = x_i - x_j:
+ *   psi(d) = - sum_n erfc(alpha |d + nL|) / |d + nL|
+ *            - sum_{h != 0} exp(-pi^2 h^2 / (alpha L)^2) / (pi L h^2)
+ *                           cos(2 pi h.d / L)
+ *            + pi / (alpha^2 L^3)
+ * (Ewald 1921; the two sums' split alpha cancels), plus, for each image
+ * within twoh, the spline's potential minus -1/r.  The particle's own
+ * images add m_i xi, xi = lim_{d->0} (psi(d) + 1/|d|), to its potential and
+ * nothing to its force.  Every pair sums its images and wave vectors
+ * itself (no structure factors), so this is a plain statement of the sum.
+ * alphaL = alpha L; the real-space sum runs to erfc(alpha r) < 3e-17
+ * (alpha r = 6), the wave-vector sum to exp(-pi^2 h^2/(alpha L)^2) < 1e-17.
+ * ====================================================================== */
+typedef struct { double kx, ky, kz, c; } OEwk;
+
+static int ewald_k_table(double L, double alphaL, OEwk **pk)
+{
+    const double kc2 = 39.0 * alphaL * alphaL / (M_PI * M_PI);
+    const int hmax = (int)ceil(sqrt(kc2));
+    int hx, hy, hz, nk = 0, cap = 256;
+    OEwk *k = (OEwk *)malloc(cap * sizeof(OEwk));
+    for (hx = 0; hx <= hmax; ++hx)
+        for (hy = -hmax; hy <= hmax; ++hy)
+            for (hz = -hmax; hz <= hmax; ++hz) {
+                int h2 = hx * hx + hy * hy + hz * hz;
+                /* one of each pair +h, -h: the cos terms are even in h */
+                if (hx == 0 && (hy < 0 || (hy == 0 && hz <= 0))) continue;
+                if (h2 > kc2) continue;
+                if (nk == cap) {
+                    cap *= 2;
+                    k = (OEwk *)realloc(k, cap * sizeof(OEwk));
+                }
+                k[nk].kx = 2 * M_PI * hx / L;
+                k[nk].ky = 2 * M_PI * hy / L;
+                k[nk].kz = 2 * M_PI * hz / L;
+                k[nk].c = 2.0 * exp(-M_PI * M_PI * h2 / (alphaL * alphaL)) / (M_PI * L * h2);
+                ++nk;
+            }
+    *pk = k;
+    return nk;
+}
+
+/* xi: the potential at a particle of unit mass from its own images and the
+ * background, lim_{d->0} (psi(d) + 1/|d|). */
+double orc_ewald_self(double L, double alphaL)
+{
+    const double alpha = alphaL / L, rc = 6.0 / alpha;
+    const int nr = (int)floor(rc / L + 0.5);
+    int nx, ny, nz, i, nk;
+    double xi = 2 * alpha / sqrt(M_PI) + M_PI / (alpha * alpha * L * L * L);
+    OEwk *k;
+    for (nx = -nr; nx <= nr; ++nx)
+        for (ny = -nr; ny <= nr; ++ny)
+            for (nz = -nr; nz <= nr; ++nz) {
+                double r;
+                if (!nx && !ny && !nz) continue;
+                r = L * sqrt((double)(nx * nx + ny * ny + nz * nz));
+                if (r < rc) xi -= erfc(alpha * r) / r;
+            }
+    nk = ewald_k_table(L, alphaL, &k);
+    for (i = 0; i < nk; ++i) xi -= k[i].c;
+    free(k);
+    return xi;
+}
+
+void orc_direct_ewald(int n, const double *r, const double *m, const double *soft, double L,
+                      double alphaL, int nt, const int *idx, double *a, double *pot)
+{
+    const double alpha = alphaL / L, rc = 6.0 / alpha, rc2 = rc * rc;
+    const double ka = 2 * alpha / sqrt(M_PI), bg = M_PI / (alpha * alpha * L * L * L);
+    const int nr = (int)floor(rc / L + 0.5);
+    const double xi = orc_ewald_self(L, alphaL);
+    OEwk *kt;
+    const int nk = ewald_k_table(L, alphaL, &kt);
+    int k;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (k = 0; k < nt; ++k) {
+        const int i = idx ? idx[k] : k;
+        double ax = 0, ay = 0, az = 0, ph = m[i] * xi, mt = 0;
+        int j, nx, ny, nz, q;
+        for (j = 0; j < n; ++j) {
+            double dx, dy, dz, twoh = soft[i] + soft[j];
+            mt += m[j];
+            if (j == i) continue;
+            /* real space: the source's images about the nearest one */
+            dx = r[3 * j] - r[3 * i];
+            dy = r[3 * j + 1] - r[3 * i + 1];
+            dz = r[3 * j + 2] - r[3 * i + 2];
+            dx -= L * rint(dx / L);
+            dy -= L * rint(dy / L);
+            dz -= L * rint(dz / L);
+            for (nx = -nr; nx <= nr; ++nx)
+                for (ny = -nr; ny <= nr; ++ny)
+                    for (nz = -nr; nz <= nr; ++nz) {
+                        double sx = dx + nx * L, sy = dy + ny * L, sz = dz + nz * L;
+                        double s2 = sx * sx + sy * sy + sz * sz, s, e, g;
+                        if (s2 >= rc2) continue;
+                        if (s2 == 0.0) {   /* coincident: spline(0) minus the erf part */
+                            double aa, bb;
+                            orc_spline(0.0, twoh, &aa, &bb);
+                            ph -= m[j] * (aa - ka);
+                            continue;
+                        }
+                        s = sqrt(s2);
+                        e = erfc(alpha * s) / s;
+                        g = (e + ka * exp(-alpha * alpha * s2)) / s2;
+                        if (s2 < twoh * twoh) {   /* softened: spline minus Newton */
+                            double aa, bb;
+                            orc_spline(s2, twoh, &aa, &bb);
+                            e += aa - 1 / s;
+                            g += bb - 1 / (s * s2);
+                        }
+                        ph -= m[j] * e;
+                        ax += m[j] * g * sx;
+                        ay += m[j] * g * sy;
+                        az += m[j] * g * sz;
+                    }
+            /* wave vectors, d = x_i - x_j */
+            dx = r[3 * i] - r[3 * j];
+            dy = r[3 * i + 1] - r[3 * j + 1];
+            dz = r[3 * i + 2] - r[3 * j + 2];
+            for (q = 0; q < nk; ++q) {
+                const double kd = kt[q].kx * dx + kt[q].ky * dy + kt[q].kz * dz;
+                const double cs = cos(kd), sn = sin(kd), f = m[j] * kt[q].c * sn;
+                ph -= m[j] * kt[q].c * cs;
+                ax -= f * kt[q].kx;
+                ay -= f * kt[q].ky;
+                az -= f * kt[q].kz;
+            }
+        }
+        ph += mt * bg - m[i] * bg;   /* the others' background (its own is in xi) */
+        a[3 * k] = ax;
+        a[3 * k + 1] = ay;
+        a[3 * k + 2] = az;
+        pot[k] = ph;
+    }
+    free(kt);
+}

@@ -27,6 +27,13 @@
  *                          closed forms of SPLINE at r=0 and r>=2h, single-
  *                          particle cells (Q=0) vs point masses.
  *   orc_direct_sum         Newton's law / Plummer analytic field.
+ *   orc_direct_ewald       the Madelung constants of the simple cubic lattice
+ *                          (own images and background) and of rock salt
+ *                          (signed masses); the same sums for three splits
+ *                          alpha; Newton plus the background's (4 pi/3) m r /
+ *                          L^3 at small r; the spline inside twoh; and the
+ *                          reference's own Ewald (orc_grav_all_periodic) for a
+ *                          point-mass root, which expands this Green's function.
  *   orc_grav_all_periodic  a test mass near a point mass in the periodic box:
  *                          zero force at the symmetry points of the lattice,
  *                          odd in the separation, Newton plus the uniform
@@ -121,6 +128,15 @@ void orc_grav_all_weighted(orc_tree *t, int iOrder, int nReps, const double *L, 
  * NULL, nt = n).  OpenMP over targets. */
 void orc_direct_sum(int n, const double *r, const double *m, const double *soft,
                     int nt, const int *idx, double *a, double *pot);
+
+/* The periodic counterpart of orc_direct_sum in a cubic box of side L:
+ * the Ewald sum (split alpha = alphaL / L) of every other particle and all
+ * images of every particle, with the uniform background, and the spline
+ * softening of images nearer than twoh.  pot includes the particle's own
+ * images, m_i * orc_ewald_self(L, alphaL).  OpenMP over targets. */
+void orc_direct_ewald(int n, const double *r, const double *m, const double *soft, double L,
+                      double alphaL, int nt, const int *idx, double *a, double *pot);
+double orc_ewald_self(double L, double alphaL);
 
 /* Spline kernels of grav.h, exported for closed-form tests. */
 void orc_spline(double r2, double twoh, double *a, double *b);

@@ -70,6 +70,10 @@ def lib():
         L.orc_grav_all_weighted.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
                                             C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_direct_sum.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp, _dp]
+        L.orc_direct_ewald.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                       _ip, _dp, _dp]
+        L.orc_ewald_self.restype = C.c_double
+        L.orc_ewald_self.argtypes = [C.c_double, C.c_double]
         L.orc_spline.argtypes = [C.c_double, C.c_double, _dp, _dp]
         L.orc_splineq.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_knn.restype = C.c_int
@@ -246,6 +250,27 @@ def direct_sum(r, m, soft, idx=None):
     pot = np.zeros(nt)
     lib().orc_direct_sum(n, P(r), P(m), P(soft), nt, ip, P(a), P(pot))
     return a, pot
+
+
+def direct_ewald(r, m, soft, L, idx=None, alphaL=6.0):
+    """Periodic direct sum (orc_direct_ewald) in the cube of side L."""
+    r = _d(r).reshape(-1, 3)
+    m = _d(m)
+    soft = _d(soft)
+    n = r.shape[0]
+    if idx is None:
+        nt, ip = n, None
+    else:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        nt, ip = idx.size, P(idx, _ip)
+    a = np.zeros((nt, 3))
+    pot = np.zeros(nt)
+    lib().orc_direct_ewald(n, P(r), P(m), P(soft), float(L), float(alphaL), nt, ip, P(a), P(pot))
+    return a, pot
+
+
+def ewald_self(L, alphaL=6.0):
+    return lib().orc_ewald_self(float(L), float(alphaL))
 
 
 def spline(r2, twoh):

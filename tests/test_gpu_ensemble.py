@@ -158,11 +158,13 @@ def test_persistent_schedule_equals_per_bucket(gpu, ens, prec):
     # the persistent grid (workgroups taking buckets from a queue, heaviest
     # first once a first evaluation has measured them) gives the per-bucket
     # launch's results bit for bit: each bucket's sums do not depend on when
-    # it runs; also replayed from a graph captured with the queue
+    # it runs; also replayed from a graph captured with the queue; and the
+    # persistent grid whose XCDs take the tree in shares and then help
+    # one another
     p = gpu.PREC_FP32 if prec == "FP32" else gpu.PREC_FP64
     n = len(ens["m"])
     out = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         with gpu.Pkd(n, prec=p) as pk:
             load(pk, ens, 0.005)
             pk.set_grav_schedule(mode)

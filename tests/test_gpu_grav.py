@@ -332,16 +332,16 @@ def periodic_box():
     return r, m, soft
 
 
-@pytest.mark.parametrize("order", [2, 4])
-def test_periodic_ewald_vs_oracle(gpu, periodic_box, order):
+@pytest.mark.parametrize("order,nReps,theta", [(2, 1, 0.7), (4, 1, 0.7), (4, 2, 0.35)])
+def test_periodic_ewald_vs_oracle(gpu, periodic_box, order, nReps, theta):
     r, m, soft = periodic_box
-    T = O.Tree(r, m, soft, nBucket=16, dTheta=0.7, iOrder=order)
-    ao, po, do, so = T.grav_all(nReps=1, period=[1.0] * 3, bEwald=True, iEwOrder=order)
+    T = O.Tree(r, m, soft, nBucket=16, dTheta=theta, iOrder=order)
+    ao, po, do, so = T.grav_all(nReps=nReps, period=[1.0] * 3, bEwald=True, iEwOrder=order)
     for prec, rms, mx in ((gpu.PREC_FP64, 1e-8, 1e-7), (gpu.PREC_FP32, 1e-5, 1e-4)):
         with setup_pkd(gpu, r, m, soft, prec) as pk:
             pk.set_period([1.0] * 3)
-            pk.build_binary(16, 1, 0.7, order)
-            st = pk.grav_all(order, nReps=1, periodic=True, bEwald=True, iEwOrder=order)
+            pk.build_binary(16, 1, theta, order)
+            st = pk.grav_all(order, nReps=nReps, periodic=True, bEwald=True, iEwOrder=order)
             a, pot = pk.get("a"), pk.get("fPot")
         for k in ("nActive", "dPartSum", "dCellSum", "dSoftSum", "dFlop"):
             assert st[k] == so[k], (prec, k)
@@ -368,3 +368,61 @@ def test_periodic_without_ewald_and_errors(gpu, periodic_box):
     assert st["dPartSum"] == so["dPartSum"] and st["dCellSum"] == so["dCellSum"]
     e = rel_err(a, ao)
     assert np.sqrt(np.mean(e * e)) < 1e-8 and e.max() < 1e-7
+
+
+# ------------------------------------------------ periodic direct sum
+def test_direct_ewald_vs_oracle(gpu, periodic_box):
+    # the periodic FP64 direct sum (wave vectors through structure factors,
+    # split alpha = 6 / L) against the oracle's pair-by-pair sum at another
+    # split (alpha = 3 / L): two softenings and a coincident pair included
+    r, m, soft = periodic_box
+    r = np.array(r, dtype=np.float64)
+    r[17] = r[5]
+    soft = np.where(np.arange(len(m)) % 2 == 0, 0.01, 0.02)
+    idx = np.random.default_rng(3).choice(len(m), 150, replace=False)
+    idx[0] = 5
+    ao, po = O.direct_ewald(r, m, soft, 1.0, idx=idx, alphaL=3.0)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64) as pk:
+        with pytest.raises(gpu.GGError, match="cubic"):
+            pk.direct_ewald(idx)
+        pk.set_period([1.0] * 3)
+        a, pot = pk.direct_ewald(idx)
+    assert rel_err(a, ao).max() < 1e-11
+    assert np.abs(pot - po).max() < 1e-12 * np.abs(po).max()
+
+
+def test_direct_ewald_rock_salt(gpu):
+    # signed masses on the rock-salt lattice: every ion at q M / a with the
+    # Madelung constant M, and no force
+    L = 1.3
+    pts = np.array([[x, y, z] for x in range(2) for y in range(2) for z in range(2)], float)
+    q = np.where(pts.sum(1) % 2 == 0, 1.0, -1.0)
+    r = pts * L / 2 - L / 4
+    with setup_pkd(gpu, r, q, np.full(8, 1e-9), gpu.PREC_FP64) as pk:
+        pk.set_period([L] * 3)
+        a, pot = pk.direct_ewald(np.arange(8))
+    assert np.allclose(pot * (L / 2), q * 1.747564594633182, rtol=1e-12, atol=0)
+    assert np.abs(a).max() < 1e-12
+
+
+def test_periodic_tree_error_vs_direct_ewald(gpu, periodic_box):
+    # the tree + Ewald forces against the periodic direct sum: at the
+    # reference's nReps 1 the hexadecapole expansion of the root in the Ewald
+    # correction leaves ~7e-4 rms whatever theta; two replica shells and
+    # theta 0.35 bring the rms under 1e-4
+    r, m, soft = periodic_box
+    idx = np.arange(0, len(m), 7)
+    with setup_pkd(gpu, r, m, soft, gpu.PREC_FP64) as pk:
+        pk.set_period([1.0] * 3)
+        ad, _ = pk.direct_ewald(idx)
+        pk.build_binary(16, 1, 0.35, 4)
+        pk.init_accel()
+        pk.grav_all(4, nReps=2, periodic=True, bEwald=True, iEwOrder=4)
+        a2 = pk.get("a")[idx]
+        pk.build_binary(16, 1, 0.2, 4)
+        pk.init_accel()                  # gg_grav_all adds onto a
+        pk.grav_all(4, nReps=1, periodic=True, bEwald=True, iEwOrder=4)
+        a1 = pk.get("a")[idx]
+    e2, e1 = rel_err(a2, ad), rel_err(a1, ad)
+    assert np.sqrt(np.mean(e2 * e2)) < 1e-4
+    assert 2e-4 < np.sqrt(np.mean(e1 * e1)) < 2e-3
