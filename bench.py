@@ -93,34 +93,42 @@ def sph_setup(gg, n, prec, seed=2):
     return pk
 
 
-def sph_step(pk, smf, zero_fb2, zero_a, K=64):
+def sph_step(pk, smf, zero_fb2, zero_a, K=64, nBucket=32):
     pk.put("fBall2", zero_fb2)           # device-to-device: cold kNN search
     pk.put("a", zero_a)                  # the force pass adds onto a
-    pk.build_local(period=[1.0, 1.0, 1.0])
+    pk.build_local(period=[1.0, 1.0, 1.0], nBucket=nBucket)
     pk.smooth(smf, K, True)
     pk.gas_pressure(smf)
     pk.resmooth(smf, K, True)
 
 
-def time_sph(gg, torch, n, prec, steps, warmup):
+def time_sph(gg, torch, n, prec, steps, warmup, tune=None, nBucket=32):
+    """The SPH step, its phases and outputs; with tune = candidate bucket
+    sizes, gg_tune_sph chooses the structure's bucket size first (from a
+    cold start, as every timed step searches)."""
     smf = gg.Smf.default()
     pk = sph_setup(gg, n, prec)
     zero = torch.zeros(n, dtype=torch.float64, device="cuda")
     zero3 = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    nb, tuned = nBucket, None
+    if tune:
+        pk.put("fBall2", zero)
+        nb, cand_ms = pk.tune_sph(smf, 64, [1.0, 1.0, 1.0], cand=tune)
+        tuned = {"nBucket": nb, "candidate_ms": cand_ms}
     for _ in range(warmup):
-        sph_step(pk, smf, zero, zero3)
+        sph_step(pk, smf, zero, zero3, nBucket=nb)
     torch.cuda.synchronize()
     phases = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        sph_step(pk, smf, zero, zero3)
+        sph_step(pk, smf, zero, zero3, nBucket=nb)
         phases.append(pk.timings())
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     out = {f: pk.get(f) for f in ("a", "fDensity", "uDotPdV", "uDotAV")}
     pk.close()
-    return dt, ph, out
+    return dt, ph, out, tuned
 
 
 # ------------------------------------------------------------ gravity
@@ -499,8 +507,10 @@ def main():
     n = args.n
     K = 64
     pairs = 2 * K * n                      # density pass + force pass
-    t64, ph64, o64 = time_sph(gg, torch, n, gg.PREC_FP64, args.steps, args.warmup)
-    t32, ph32, o32 = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup)
+    t64, ph64, o64, tune64 = time_sph(gg, torch, n, gg.PREC_FP64, args.steps, args.warmup,
+                                      tune=(32, 24, 16))
+    t32, ph32, o32, _ = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup,
+                                 nBucket=tune64["nBucket"])
     sph_prec = {f: rel_errors(o32[f], o64[f]) for f in o32}
     log("SPH box done")
 
@@ -630,6 +640,7 @@ def main():
         "fp32_value": pairs / t32,
         "fp32_ms_per_step": t32 * 1e3,
         "speedup_fp32_vs_fp64": t64 / t32,
+        "sph_tuner": tune64,
         "fp32_vs_fp64": sph_prec,
         "phases_ms_fp64": ph64,
         "phases_ms_fp32": ph32,

@@ -2553,6 +2553,62 @@ extern "C" int gg_gas_pressure(GGPKD pkd, const gg_smf *smf)
     return GG_OK;
 }
 
+// SPH bucket-size autotuner: for each candidate nBucket, build the search
+// structure and run the DenDVDX pass (kNN + density) twice from the
+// caller's fBall2, time the device work of the second run (structure build,
+// group lists, kNN, retries, DenDVDX and its stores, from the handle's
+// events), keep the fastest, and finish with a build and a pass at it, so
+// the structure and every output are those of gg_build_local(best) +
+// gg_smooth.
+extern "C" int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+                           const double *fPeriod, const int *cand, int nCand, double *ms,
+                           int *pBest)
+{
+    GG_CHECK(pkd && smf && cand && nCand >= 1, GG_EINVAL, "null argument or no candidate");
+    for (int k = 0; k < nCand; ++k)
+        GG_CHECK(cand[k] >= 1 && cand[k] <= BKT, GG_EINVAL, "candidate nBucket %d outside 1..%d",
+                 cand[k], BKT);
+    const int n = pkd->nLocal;
+    GG_TRY(ensure_field(pkd, GG_FBALL2));
+    hipStream_t st = pkd->stream;
+    DevBuf save;
+    GG_TRY(save.reserve(sizeof(double) * (size_t)std::max(n, 1)));
+    const size_t w1 = sizeof(double) * (size_t)n;
+    auto restore = [&]() -> int {
+        if (n > 0)
+            GG_HIP(hipMemcpyAsync(pkd->f[GG_FBALL2].p, save.p, w1, hipMemcpyDeviceToDevice, st));
+        return GG_OK;
+    };
+    auto pass = [&](int nb) -> int {
+        int ns = 0;
+        GG_TRY(restore());
+        GG_TRY(gg_build_local(pkd, nb, bPeriodic, fPeriod));
+        return gg_smooth(pkd, smf, nSmooth, bPeriodic, 0, GG_SMX_DENDVDX, &ns);
+    };
+    int rc = GG_OK;
+    if (n > 0 && hipMemcpyAsync(save.p, pkd->f[GG_FBALL2].p, w1, hipMemcpyDeviceToDevice, st) !=
+                     hipSuccess)
+        rc = GG_EHIP;
+    int best = -1;
+    double bestMs = 0;
+    for (int k = 0; k < nCand && rc == GG_OK; ++k) {
+        for (int rep = 0; rep < 2 && rc == GG_OK; ++rep) rc = pass(cand[k]);
+        if (rc != GG_OK) break;
+        const double *t = pkd->tm.ms;
+        const double tk = t[2] + t[3] + t[4] + t[8] + t[9] + t[10];
+        if (ms) ms[k] = tk;
+        if (best < 0 || tk < bestMs) {
+            best = k;
+            bestMs = tk;
+        }
+    }
+    if (rc == GG_OK) rc = pass(cand[best]);
+    save.release();
+    if (rc != GG_OK) return rc;
+    if (pBest) *pBest = cand[best];
+    return GG_OK;
+}
+
 extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
                            int bSymmetric, int iSmoothType)
 {

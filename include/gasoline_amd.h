@@ -321,6 +321,17 @@ int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const double *fPeriod)
 int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
               int bSymmetric, int iSmoothType, int *pnSmoothed);
 
+/* SPH bucket-size autotuner (no counterpart in the reference; the tiling
+ * the reference fixes with nBucket): for each candidate nBucket (1..32)
+ * build the structure (gg_build_local) and run the DenDVDX pass (gg_smooth,
+ * iSmoothType GG_SMX_DENDVDX, not symmetric) twice from the caller's fBall2,
+ * time the device work of the second run, keep the fastest, and finish with
+ * a build and a pass at it: the structure and every output are those of
+ * gg_build_local(best) + gg_smooth.  ms[k] (may be NULL) gets candidate k's
+ * milliseconds, *pBest the chosen nBucket. */
+int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic, const double *fPeriod,
+                const int *cand, int nCand, double *ms, int *pBest);
+
 /* Ball-gather re-smooth at each particle's stored fBall2 (d2 <= fBall2) and
  * the operator iSmoothType: GG_SMX_SPHPRESSURETERMS (bSymmetric=1,
  * SphPressureTermsSym smoothfcn.c:2276-2385), evaluated in gather form over

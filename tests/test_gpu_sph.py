@@ -408,3 +408,32 @@ def test_momentum_conservation_fp32(gpu, box64k):
         acc = pk.get("a")
     p = (d["m"][:, None] * acc).sum(0)
     assert np.abs(p).max() < 1e-6 * (d["m"][:, None] * np.abs(acc)).sum()
+
+
+def test_sph_bucket_tuner(gpu, box64k):
+    # gg_tune_sph times each candidate bucket size, keeps the fastest and
+    # leaves the structure and the DenDVDX outputs as a plain build at that
+    # size gives them; the neighbour sets do not depend on the bucket size
+    d, ref = box64k
+    smf = gpu.Smf.default()
+    n = len(d["m"])
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        best, ms = pk.tune_sph(smf, 64, [1.0] * 3, cand=(32, 16, 8))
+        assert set(ms) == {32, 16, 8} and all(v > 0 for v in ms.values())
+        assert ms[best] == min(ms.values())
+        tuned = {f: pk.get(f) for f in ("fBall2", "fDensity", "divv")}
+        lists = pk.smooth_lists(64)
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        pk.build_local(period=[1.0] * 3, nBucket=best)
+        pk.smooth(smf, 64, True)
+        for f, v in tuned.items():
+            assert np.array_equal(pk.get(f), v), f
+        assert np.array_equal(np.sort(lists, axis=1), np.sort(pk.smooth_lists(64), axis=1))
+    assert np.array_equal(tuned["fBall2"], ref["fb2"])
+    assert np.array_equal(np.sort(lists, axis=1), np.sort(ref["nbr"], axis=1))
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        with pytest.raises(gpu.GGError, match="outside 1..32"):
+            pk.tune_sph(smf, 64, [1.0] * 3, cand=(32, 40))
