@@ -406,8 +406,14 @@ __device__ __forceinline__ void spline_abcd(T r2, T dir, T twoh, T &a, T &b, T &
 // LDS staging of one chunk of 64 list entries, structure of arrays.
 // PP: x y z m h.  Softened cell: x y z m h Q(6, raw).  Newtonian cell:
 // x y z m Q(6, traceless) [O(10) H(15)].
+// A chunk of up to 64 staged interactions, field-major.  Newtonian cells
+// carry the traceless quadrupole (6), and for ORDER >= 3 only the
+// independent components of the traceless octupole (7: xxx xyy xxy yyy xxz
+// yyz xyz) and hexadecapole (9: xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz
+// xyyz); the others follow from the vanishing traces when evaluated, which
+// keeps the chunk small enough for one more workgroup per CU.
 template <typename T, int ORDER> struct Stage {
-    static constexpr int NF = ORDER >= 4 ? 35 : (ORDER == 3 ? 20 : 11);
+    static constexpr int NF = ORDER >= 4 ? 26 : (ORDER == 3 ? 17 : 11);
     T f[NF][64];
 };
 
@@ -478,9 +484,9 @@ __device__ __forceinline__ void stage_newt(Stage<T, ORDER> &L, int lane, int ic,
     for (int k = 0; k < 6; ++k) L.f[4 + k][lane] = (T)c[CR_QT + k];
     if constexpr (ORDER >= 3) {
         const double *oh = A.momOH + (size_t)OH_N * ic;
-        for (int k = 0; k < 10; ++k) L.f[10 + k][lane] = (T)oh[k];
+        for (int k = 0; k < 7; ++k) L.f[10 + k][lane] = (T)oh[k];
         if constexpr (ORDER >= 4)
-            for (int k = 0; k < 15; ++k) L.f[20 + k][lane] = (T)oh[10 + k];
+            for (int k = 0; k < 9; ++k) L.f[17 + k][lane] = (T)oh[10 + k];
     }
 }
 
@@ -573,29 +579,30 @@ __device__ __forceinline__ void eval_newt(const Stage<T, ORDER> &L, int m, const
         g[4] = T(7) * g[3] * dir2;
         g[5] = T(9) * g[4] * dir2;
         T Qx, Qy, Qz, Qm, Qta = 0;
+        // Orders 3 and 4 in the independent components of the traceless
+        // moments: with the traces' vanishing folded into the monomials, the
+        // sums QEVAL (qeval.h:21-64) forms over all components regroup into
+        // seven cubic (octupole: five quadratic) combinations shared by the
+        // three components of the field.
+        const T x2 = dx * dx, y2 = dy * dy, z2 = dz * dz;
         if constexpr (ORDER >= 4) {
-            const T *H = nullptr;
-            (void)H;
-            const T xxxx = L.f[20][e], xyyy = L.f[21][e], xxxy = L.f[22][e],
-                    yyyy = L.f[23][e], xxxz = L.f[24][e], yyyz = L.f[25][e],
-                    xxyy = L.f[26][e], xxyz = L.f[27][e], xyyz = L.f[28][e],
-                    xxzz = L.f[29][e], xyzz = L.f[30][e], xzzz = L.f[31][e],
-                    yyzz = L.f[32][e], yzzz = L.f[33][e], zzzz = L.f[34][e];
-            Qx = T(1.0 / 6.0) * (xzzz * dz * dz * dz + T(3) * xyzz * dy * dz * dz
-                 + T(3) * xyyz * dy * dy * dz + xyyy * dy * dy * dy
-                 + T(3) * xxzz * dx * dz * dz + T(6) * xxyz * dx * dy * dz
-                 + T(3) * xxyy * dx * dy * dy + T(3) * xxxz * dx * dx * dz
-                 + T(3) * xxxy * dx * dx * dy + xxxx * dx * dx * dx);
-            Qy = T(1.0 / 6.0) * (yzzz * dz * dz * dz + T(3) * xyzz * dx * dz * dz
-                 + T(3) * xxyz * dx * dx * dz + xxxy * dx * dx * dx
-                 + T(3) * yyzz * dy * dz * dz + T(6) * xyyz * dx * dy * dz
-                 + T(3) * xxyy * dx * dx * dy + T(3) * yyyz * dy * dy * dz
-                 + T(3) * xyyy * dx * dy * dy + yyyy * dy * dy * dy);
-            Qz = T(1.0 / 6.0) * (yyyz * dy * dy * dy + T(3) * xyyz * dx * dy * dy
-                 + T(3) * xxyz * dx * dx * dy + xxxz * dx * dx * dx
-                 + T(3) * yyzz * dy * dy * dz + T(6) * xyzz * dx * dy * dz
-                 + T(3) * xxzz * dx * dx * dz + T(3) * yzzz * dy * dz * dz
-                 + T(3) * xzzz * dx * dz * dz + zzzz * dz * dz * dz);
+            const T xxxx = L.f[17][e], xyyy = L.f[18][e], xxxy = L.f[19][e],
+                    yyyy = L.f[20][e], xxxz = L.f[21][e], yyyz = L.f[22][e],
+                    xxyy = L.f[23][e], xxyz = L.f[24][e], xyyz = L.f[25][e];
+            const T xz2 = dx * z2, yz2 = dy * z2, z3 = dz * z2;
+            const T cA = dx * x2 - T(3) * xz2;               // x^3 - 3 x z^2
+            const T cB = T(3) * (x2 * dy - yz2);             // 3 x^2 y - 3 y z^2
+            const T cC = T(3) * x2 * dz - z3;                // 3 x^2 z - z^3
+            const T cD = T(3) * (dx * y2 - xz2);             // 3 x y^2 - 3 x z^2
+            const T cE = T(6) * dx * dy * dz;                // 6 x y z
+            const T cF = dy * y2 - T(3) * yz2;               // y^3 - 3 y z^2
+            const T cG = T(3) * y2 * dz - z3;                // 3 y^2 z - z^3
+            Qx = T(1.0 / 6.0) * (xxxx * cA + xxxy * cB + xxxz * cC + xxyy * cD + xxyz * cE
+                                 + xyyy * cF + xyyz * cG);
+            Qy = T(1.0 / 6.0) * (xxxy * cA + xxyy * cB + xxyz * cC + xyyy * cD + xyyz * cE
+                                 + yyyy * cF + yyyz * cG);
+            Qz = T(1.0 / 6.0) * (xxxz * cA + xxyz * cB - xxxx * cC - xxyy * (cC + cG)
+                                 + xyyz * cD - (xxxy + xyyy) * cE + yyyz * cF - yyyy * cG);
             Qm = T(1.0 / 4.0) * (Qx * dx + Qy * dy + Qz * dz);
             P.pot -= g[4] * Qm;
             Qta += g[5] * Qm;
@@ -606,14 +613,13 @@ __device__ __forceinline__ void eval_newt(const Stage<T, ORDER> &L, int m, const
         if constexpr (ORDER >= 3) {
             const T xxx = L.f[10][e], xyy = L.f[11][e], xxy = L.f[12][e],
                     yyy = L.f[13][e], xxz = L.f[14][e], yyz = L.f[15][e],
-                    xyz = L.f[16][e], xzz = L.f[17][e], yzz = L.f[18][e],
-                    zzz = L.f[19][e];
-            Qx = T(0.5) * (xzz * dz * dz + T(2) * xyz * dy * dz + xyy * dy * dy
-                 + T(2) * xxz * dx * dz + T(2) * xxy * dx * dy + xxx * dx * dx);
-            Qy = T(0.5) * (yzz * dz * dz + T(2) * xyz * dx * dz + xxy * dx * dx
-                 + T(2) * yyz * dy * dz + T(2) * xyy * dx * dy + yyy * dy * dy);
-            Qz = T(0.5) * (yyz * dy * dy + T(2) * xyz * dx * dy + xxz * dx * dx
-                 + T(2) * yzz * dy * dz + T(2) * xzz * dx * dz + zzz * dz * dz);
+                    xyz = L.f[16][e];
+            const T cP = x2 - z2, cR = y2 - z2;
+            const T cxy = T(2) * dx * dy, cxz = T(2) * dx * dz, cyz = T(2) * dy * dz;
+            Qx = T(0.5) * (xxx * cP + xyy * cR + xxy * cxy + xxz * cxz + xyz * cyz);
+            Qy = T(0.5) * (xxy * cP + yyy * cR + xyy * cxy + xyz * cxz + yyz * cyz);
+            Qz = T(0.5) * (xxz * cP + yyz * cR + xyz * cxy - (xxx + xyy) * cxz
+                           - (xxy + yyy) * cyz);
             Qm = T(1.0 / 3.0) * (Qx * dx + Qy * dy + Qz * dz);
             P.pot -= g[3] * Qm;
             Qta += g[4] * Qm;
