@@ -314,10 +314,28 @@ template <> __device__ __forceinline__ float rsq<float>(float x)
 {
     return __builtin_amdgcn_rsqf(x);
 }
-template <> __device__ __forceinline__ double rsq<double>(double x)
+// FP64: the hardware estimate refined by two Newton steps, y <- y + y (1/2 -
+// x y^2 / 2), each of which roughly doubles its correct bits: within 2 ulp
+// of the correctly rounded 1/sqrt(x) for positive finite x (the reference's
+// own v_sqrt1 is good to 2e-10), in a third of the instructions of the
+// correctly rounded square root and division.  Subnormal x is scaled by
+// 2^104 first (the estimate does not take subnormals); 0 and +inf keep the
+// estimate, inf and 0, as 1/sqrt gives them.
+__device__ __forceinline__ double rsq_f64(double x)
 {
-    return 1.0 / sqrt(x);
+    const bool sub = x < 0x1p-1022;
+    const double xs = sub ? x * 0x1p104 : x;
+    const double y0 = __builtin_amdgcn_rsq(xs);
+    const double h = 0.5 * xs;
+    double y = y0;
+    double t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+    y = (xs > 0.0 && xs < __builtin_inf()) ? y : y0;
+    return sub ? y * 0x1p52 : y;
 }
+template <> __device__ __forceinline__ double rsq<double>(double x) { return rsq_f64(x); }
 
 // Running sum of per-chunk partials: Kahan-compensated in FP32, plain in FP64.
 template <typename T> struct Sum {

@@ -34,6 +34,7 @@ int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 constexpr int MOM_CH = 2048;     // particles per moment work item
 constexpr int NM = 32;           // per item: raw Q (6), O (10), H (15), Bmax
+constexpr int MOM_SMALL = 64;    // cells of at most this many particles: a thread each
 constexpr int BOX_CH = 1024;     // particles per bounds work item
 constexpr int LMAX = 512;        // cells of at most LMAX particles finish in one workgroup
 
@@ -667,6 +668,54 @@ __global__ void k_gt_itemcell(int nn, const int64_t *__restrict__ ioff, int *__r
     for (int64_t k = ioff[c]; k < ioff[c + 1]; ++k) icell[k] = c;
 }
 
+// One particle's terms of the moments about (cx, cy, cz): raw Q (6),
+// reduced O (10) and H (15), and its distance for Bmax.
+template <int ORDER>
+__device__ __forceinline__ void mom_accum(double mm, double dx, double dy, double dz, double *Q,
+                                          double *O, double *H, double &Bmax)
+{
+#pragma clang fp contract(off)
+    constexpr bool wantO = ORDER >= 3, wantH = ORDER >= 4;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double d1 = sqrt(d2);
+    if (d1 > Bmax) Bmax = d1;
+    if (wantH) {
+        H[0] += mm * (dx * dx * dx * dx - 6.0 / 7.0 * d2 * (dx * dx - 0.1 * d2));
+        H[1] += mm * (dx * dy * dy * dy - 3.0 / 7.0 * d2 * dx * dy);
+        H[2] += mm * (dx * dx * dx * dy - 3.0 / 7.0 * d2 * dx * dy);
+        H[3] += mm * (dy * dy * dy * dy - 6.0 / 7.0 * d2 * (dy * dy - 0.1 * d2));
+        H[4] += mm * (dx * dx * dx * dz - 3.0 / 7.0 * d2 * dx * dz);
+        H[5] += mm * (dy * dy * dy * dz - 3.0 / 7.0 * d2 * dy * dz);
+        H[6] += mm * (dx * dx * dy * dy - 1.0 / 7.0 * d2 * (dx * dx + dy * dy - 0.2 * d2));
+        H[7] += mm * (dx * dx * dy * dz - 1.0 / 7.0 * d2 * dy * dz);
+        H[8] += mm * (dx * dy * dy * dz - 1.0 / 7.0 * d2 * dx * dz);
+        H[9] += mm * (dx * dx * dz * dz - 1.0 / 7.0 * d2 * (dx * dx + dz * dz - 0.2 * d2));
+        H[10] += mm * (dx * dy * dz * dz - 1.0 / 7.0 * d2 * dx * dy);
+        H[11] += mm * (dx * dz * dz * dz - 3.0 / 7.0 * d2 * dx * dz);
+        H[12] += mm * (dy * dy * dz * dz - 1.0 / 7.0 * d2 * (dy * dy + dz * dz - 0.2 * d2));
+        H[13] += mm * (dy * dz * dz * dz - 3.0 / 7.0 * d2 * dy * dz);
+        H[14] += mm * (dz * dz * dz * dz - 6.0 / 7.0 * d2 * (dz * dz - 0.1 * d2));
+    }
+    if (wantO) {
+        O[0] += mm * (dx * dx * dx - 0.6 * d2 * dx);
+        O[1] += mm * (dx * dy * dy - 0.2 * d2 * dx);
+        O[2] += mm * (dx * dx * dy - 0.2 * d2 * dy);
+        O[3] += mm * (dy * dy * dy - 0.6 * d2 * dy);
+        O[4] += mm * (dx * dx * dz - 0.2 * d2 * dz);
+        O[5] += mm * (dy * dy * dz - 0.2 * d2 * dz);
+        O[6] += mm * dx * dy * dz;
+        O[7] += mm * (dx * dz * dz - 0.2 * d2 * dx);
+        O[8] += mm * (dy * dz * dz - 0.2 * d2 * dy);
+        O[9] += mm * (dz * dz * dz - 0.6 * d2 * dz);
+    }
+    Q[0] += mm * dx * dx;
+    Q[1] += mm * dy * dy;
+    Q[2] += mm * dz * dz;
+    Q[3] += mm * dx * dy;
+    Q[4] += mm * dx * dz;
+    Q[5] += mm * dy * dz;
+}
+
 // Partial moments of one item (one wavefront): raw Q (6), reduced O (10)
 // and H (15) about the cell's centre of mass, and the largest distance.
 // Per-lane sums, then a fixed butterfly: the result does not depend on
@@ -687,6 +736,7 @@ __global__ __launch_bounds__(256) void k_gt_moments(int nitem, const int *__rest
     if (it >= nitem) return;
     const int c = icell[it];
     const GNode &g = nd[c];
+    if (g.hi - g.lo + 1 <= MOM_SMALL) return;   // k_gt_moments_small's
     const int first = g.lo + (int)(it - ioff[c]) * MOM_CH;
     const int last = min(g.hi, first + MOM_CH - 1);
     const double cx = cm[5 * (size_t)c + 2], cy = cm[5 * (size_t)c + 3], cz = cm[5 * (size_t)c + 4];
@@ -697,44 +747,7 @@ __global__ __launch_bounds__(256) void k_gt_moments(int nitem, const int *__rest
     for (int i = first + lane; i <= last; i += 64) {
         const double mm = pm[i];
         const double dx = x[i] - cx, dy = y[i] - cy, dz = z[i] - cz;
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        const double d1 = sqrt(d2);
-        if (d1 > Bmax) Bmax = d1;
-        if (wantH) {
-            H[0] += mm * (dx * dx * dx * dx - 6.0 / 7.0 * d2 * (dx * dx - 0.1 * d2));
-            H[1] += mm * (dx * dy * dy * dy - 3.0 / 7.0 * d2 * dx * dy);
-            H[2] += mm * (dx * dx * dx * dy - 3.0 / 7.0 * d2 * dx * dy);
-            H[3] += mm * (dy * dy * dy * dy - 6.0 / 7.0 * d2 * (dy * dy - 0.1 * d2));
-            H[4] += mm * (dx * dx * dx * dz - 3.0 / 7.0 * d2 * dx * dz);
-            H[5] += mm * (dy * dy * dy * dz - 3.0 / 7.0 * d2 * dy * dz);
-            H[6] += mm * (dx * dx * dy * dy - 1.0 / 7.0 * d2 * (dx * dx + dy * dy - 0.2 * d2));
-            H[7] += mm * (dx * dx * dy * dz - 1.0 / 7.0 * d2 * dy * dz);
-            H[8] += mm * (dx * dy * dy * dz - 1.0 / 7.0 * d2 * dx * dz);
-            H[9] += mm * (dx * dx * dz * dz - 1.0 / 7.0 * d2 * (dx * dx + dz * dz - 0.2 * d2));
-            H[10] += mm * (dx * dy * dz * dz - 1.0 / 7.0 * d2 * dx * dy);
-            H[11] += mm * (dx * dz * dz * dz - 3.0 / 7.0 * d2 * dx * dz);
-            H[12] += mm * (dy * dy * dz * dz - 1.0 / 7.0 * d2 * (dy * dy + dz * dz - 0.2 * d2));
-            H[13] += mm * (dy * dz * dz * dz - 3.0 / 7.0 * d2 * dy * dz);
-            H[14] += mm * (dz * dz * dz * dz - 6.0 / 7.0 * d2 * (dz * dz - 0.1 * d2));
-        }
-        if (wantO) {
-            O[0] += mm * (dx * dx * dx - 0.6 * d2 * dx);
-            O[1] += mm * (dx * dy * dy - 0.2 * d2 * dx);
-            O[2] += mm * (dx * dx * dy - 0.2 * d2 * dy);
-            O[3] += mm * (dy * dy * dy - 0.6 * d2 * dy);
-            O[4] += mm * (dx * dx * dz - 0.2 * d2 * dz);
-            O[5] += mm * (dy * dy * dz - 0.2 * d2 * dz);
-            O[6] += mm * dx * dy * dz;
-            O[7] += mm * (dx * dz * dz - 0.2 * d2 * dx);
-            O[8] += mm * (dy * dz * dz - 0.2 * d2 * dy);
-            O[9] += mm * (dz * dz * dz - 0.6 * d2 * dz);
-        }
-        Q[0] += mm * dx * dx;
-        Q[1] += mm * dy * dy;
-        Q[2] += mm * dz * dz;
-        Q[3] += mm * dx * dy;
-        Q[4] += mm * dx * dz;
-        Q[5] += mm * dy * dz;
+        mom_accum<ORDER>(mm, dx, dy, dz, Q, O, H, Bmax);
     }
     for (int o = 32; o > 0; o >>= 1) {
         for (int k = 0; k < 6; ++k) Q[k] += __shfl_xor(Q[k], o);
@@ -751,6 +764,41 @@ __global__ __launch_bounds__(256) void k_gt_moments(int nitem, const int *__rest
         for (int k = 0; k < 15; ++k) p[16 + k] = H[k];
         p[31] = Bmax;
     }
+}
+
+// Cells of at most MOM_SMALL particles (most cells: the buckets and the
+// levels just above them): one thread each sums its cell's particles in
+// order, with no cross-lane reduction.  Each such cell is one item.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_gt_moments_small(int nitem, const int *__restrict__ icell,
+                                                          const GNode *__restrict__ nd,
+                                                          const double *__restrict__ cm,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ pm,
+                                                          double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nitem) return;
+    const int c = icell[it];
+    const GNode &g = nd[c];
+    if (g.hi - g.lo + 1 > MOM_SMALL) return;
+    const double cx = cm[5 * (size_t)c + 2], cy = cm[5 * (size_t)c + 3], cz = cm[5 * (size_t)c + 4];
+    double Q[6] = {0, 0, 0, 0, 0, 0}, O[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double H[15] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double Bmax = 0.0;
+    for (int i = g.lo; i <= g.hi; ++i) {
+        const double mm = pm[i];
+        const double dx = x[i] - cx, dy = y[i] - cy, dz = z[i] - cz;
+        mom_accum<ORDER>(mm, dx, dy, dz, Q, O, H, Bmax);
+    }
+    double *p = part + (size_t)NM * it;
+    for (int k = 0; k < 6; ++k) p[k] = Q[k];
+    for (int k = 0; k < 10; ++k) p[6 + k] = O[k];
+    for (int k = 0; k < 15; ++k) p[16 + k] = H[k];
+    p[31] = Bmax;
 }
 
 // Per cell in preorder: the parity record (iDim, pLower, pUpper, iLower,
@@ -1206,6 +1254,12 @@ int gtree_build(GGPKD pkd, int nSys, const int *sysOff, int nBucket, int iOpenTy
                                G.gicell.as<int>(), G.gioff.as<int64_t>(), nd, G.gcm.as<double>(),
                                G.px.as<double>(), G.py.as<double>(), G.pz.as<double>(),
                                G.pm.as<double>(), G.gpart.as<double>());
+            auto small = iOrder >= 4 ? k_gt_moments_small<4>
+                                     : (iOrder >= 3 ? k_gt_moments_small<3> : k_gt_moments_small<2>);
+            hipLaunchKernelGGL(small, dim3(nblk(nitem, 256)), dim3(256), 0, st, (int)nitem,
+                               G.gicell.as<int>(), nd, G.gcm.as<double>(), G.px.as<double>(),
+                               G.py.as<double>(), G.pz.as<double>(), G.pm.as<double>(),
+                               G.gpart.as<double>());
         }
         const int nv = iOrder >= 4 ? 31 : (iOrder >= 3 ? 16 : 6);
         hipLaunchKernelGGL(k_gt_combine, dim3(nblk(nn, 4)), dim3(256), 0, st, nn, nv,
