@@ -610,6 +610,10 @@ def main():
                 "unit": "GB/s", "frac": a / HBM_PEAK_GBS, "ms": ms, "algorithmic_bytes": by,
                 "traffic": (t["fetch_bytes"] + t["write_bytes"]) if t else None}
 
+    def gtraffic(name):
+        t = traffic.get(name)
+        return (t["fetch_bytes"] + t["write_bytes"]) if t else None
+
     dom = max(kern, key=lambda k: kern[k][0])
     knn_strict_gbs = n * (KNN_BYTES_PER_QUERY + 4 * K) / (ph64["knn"] * 1e-3) / 1e9
     gate = grav["gate"]
@@ -658,13 +662,14 @@ def main():
             "achieved": gate["fp32"]["tflops_interact"], "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": gate["fp32"]["tflops_interact"] / FP32_PEAK_TFLOPS,
             "frac_nonpacked": gate["fp32"]["tflops_interact"] / FP32_NONPACKED_TFLOPS,
-            "traffic": None,
+            "traffic": gtraffic("k_walk_interact<float,%d>" % GATE["iOrder"]),
         },
         "roofline_gravity_fp64": {
             "bound": "valu", "kernel": "k_walk_interact<double,%d> (walk included)"
                                        % GATE["iOrder"],
             "achieved": gate["fp64"]["tflops_interact"], "peak": 78.6, "unit": "TFLOP/s",
-            "frac": gate["fp64"]["tflops_interact"] / 78.6, "traffic": None,
+            "frac": gate["fp64"]["tflops_interact"] / 78.6,
+            "traffic": gtraffic("k_walk_interact<double,%d>" % GATE["iOrder"]),
         },
     }
     if ens:
