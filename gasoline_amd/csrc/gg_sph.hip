@@ -873,7 +873,7 @@ __device__ uint64_t wave_select(const uint64_t (&v)[KE], const bool (&ok)[KE], i
 // Returns false (nothing written) if the selection does not come to K.
 template <int KE>
 __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, const int *bs,
-                           double T, int *hist, int lane)
+                           double T, int lane)
 {
     const int K = A.K;
     uint64_t v[KE];
@@ -883,8 +883,7 @@ __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, con
     // monotone in d2, so each bin is a contiguous range of keys and the K-th
     // smallest lies in the first bin where the running count reaches K
     const double inv = T > 0 ? 64.0 / T : 0.0;
-    hist[lane] = 0;
-    wave_sync();
+    int bin[KE];
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
         const int e = lane + 64 * k;
@@ -892,21 +891,26 @@ __device__ bool knn_select(const QArgs &A, int q, int cnt, const double *bd, con
         const double d = ok[k] ? bd[e] : 0.0;
         v[k] = ok[k] ? (uint64_t)__double_as_longlong(d) : ~0ull;
         s[k] = ok[k] ? bs[e] : -1;
-        if (ok[k]) atomicAdd(&hist[min(63, (int)(d * inv))], 1);
+        bin[k] = ok[k] ? min(63, (int)(d * inv)) : 64;
     }
-    wave_sync();
-    const int h = hist[lane];
-    int incl = h;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(incl, o);
-        if (lane >= o) incl += u;
+    // the first bin b whose running count reaches K (bin 63 holds the rest,
+    // and cnt >= K), by bisection on ballot counts of bin <= mid
+    int blo = 0, bhi = 63;
+    while (blo < bhi) {
+        const int mid = (blo + bhi) >> 1;
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) c += __popcll(__ballot(bin[k] <= mid));
+        if (c >= K) bhi = mid;
+        else blo = mid + 1;
     }
-    const int b = __ffsll((long long)__ballot(incl >= K)) - 1;   // cnt >= K: exists
-    const int below = __shfl(incl - h, b);       // keys in the bins under b
+    const int b = blo;
+    int below = 0;                     // keys in the bins under b
+#pragma unroll
+    for (int k = 0; k < KE; ++k) below += __popcll(__ballot(bin[k] < b));
     bool okb[KE];
 #pragma unroll
-    for (int k = 0; k < KE; ++k)   // the bin again, from the key (fewer registers)
-        okb[k] = ok[k] && min(63, (int)(__longlong_as_double((long long)v[k]) * inv)) == b;
+    for (int k = 0; k < KE; ++k) okb[k] = bin[k] == b;
     int nlt, neq;
     const uint64_t pK = wave_select<KE>(v, okb, K - 1 - below, &nlt, &neq);
     nlt += below;            // equal keys share a bin, so neq is whole
@@ -1048,7 +1052,7 @@ __device__ int knn_settle(const QArgs &A, const KLGroup &G, const int2 *sel, int
 template <bool IMG>
 __device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, double x,
                           double y, double z, double Tsel, double RL2, double *bd, int *bs,
-                          int *hist, int lane, unsigned &npass, unsigned &ncand)
+                          int lane, unsigned &npass, unsigned &ncand)
 {
     const int K = A.K;
     for (int pass = 0; pass < 8; ++pass) {
@@ -1089,8 +1093,8 @@ __device__ bool knn_query(const QArgs &A, const KLGroup &G, int2 *sel, int q, do
             return false;
         }
         wave_sync();
-        const bool done = cnt <= 128 ? knn_select<2>(A, q, cnt, bd, bs, Tball, hist, lane)
-                                     : knn_select<4>(A, q, cnt, bd, bs, Tball, hist, lane);
+        const bool done = cnt <= 128 ? knn_select<2>(A, q, cnt, bd, bs, Tball, lane)
+                                     : knn_select<4>(A, q, cnt, bd, bs, Tball, lane);
         if (!done && lane == 0) atomicAdd(&A.err[4], 1u);
         return done;
     }
@@ -1115,7 +1119,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ int2 sel[KL_SB];                 // per query: first slot, count of its buckets
     __shared__ double bd[CAP];
     __shared__ int bs[CAP];
-    __shared__ int hist[64];                    // the selection's bins
     const int lane = threadIdx.x;
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
@@ -1184,8 +1187,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         // first ball: slack * fBall2 from the hint, else 0.8 RL^2 (~1.6K expected)
         const double Tsel = gq > 0 ? fmin(gq * A.slack, RL2) : 0.8 * RL2;
         const bool done =
-            img ? knn_query<true>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, hist, lane, npass, ncand)
-                : knn_query<false>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, hist, lane, npass, ncand);
+            img ? knn_query<true>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand)
+                : knn_query<false>(A, G, sel, q, x, y, z, Tsel, RL2, bd, bs, lane, npass, ncand);
         if (!done && lane == 0) {
             const unsigned p = atomicAdd(KA.nretry, 1u);
             KA.retry[p] = q;
