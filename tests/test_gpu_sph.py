@@ -6,7 +6,9 @@ broken by the smaller caller index in both (a list's order is not compared:
 the library writes it in ball order, the oracle by (d2, caller index), the
 reference in its priority-queue order); FP64 operator outputs within 1e-12
 relative; FP32 within 1e-5 relative to the field's scale (1e-4 for the
-second-derivative quantities dvds and for single values near zero)."""
+second-derivative quantities dvds and for single values near zero), and per
+particle: FP64 within 1e-10 of each particle's own value, FP32 within 1e-5
+rms and 1e-3 max (floored at 1% of the field's rms)."""
 import numpy as np
 import pytest
 
@@ -28,6 +30,17 @@ def put_gas(pk, d):
 
 def scaled(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def per_particle(got, ref):
+    """rms and max of |got - ref| / max(|ref|, 0.01 rms|ref|), per particle
+    (vectors by their norm)."""
+    g = np.asarray(got, float).reshape(len(got), -1)
+    r = np.asarray(ref, float).reshape(len(ref), -1)
+    mag = np.linalg.norm(r, axis=1)
+    floor = 0.01 * np.sqrt(np.mean(mag * mag))
+    e = np.linalg.norm(g - r, axis=1) / np.maximum(mag, max(floor, 1e-300))
+    return float(np.sqrt(np.mean(e * e))), float(e.max())
 
 
 def oracle_step(d, period, K, smf, fb2=None):
@@ -247,6 +260,17 @@ def test_density_and_hydro_vs_oracle(gpu, case, prec, request):
     for k in ("uDotPdV", "uDotAV", "mumax", "dtNew"):
         assert scaled(h[k], hy[k]) < tol, k
     assert (h["uDotDiff"] == 0).all()
+    # per particle as well (relative to each particle's own value, floored
+    # at 1% of the field's rms so that values crossing zero do not divide
+    # by nothing)
+    for name, g, r in (("rho", got["rho"], den["rho"]), ("a", h["a"], hy["acc"]),
+                       ("uDotPdV", h["uDotPdV"], hy["uDotPdV"]),
+                       ("uDotAV", h["uDotAV"], hy["uDotAV"])):
+        rms, mx = per_particle(g, r)
+        if prec == "FP64":
+            assert mx < 1e-10, (name, rms, mx)
+        else:
+            assert rms < 1e-5 and mx < 1e-3, (name, rms, mx)
 
 
 @pytest.mark.parametrize("prec", ["FP64", "FP32"])
