@@ -117,23 +117,62 @@ int exclusive_scan_i64(hipStream_t st, const int *in, int64_t *out, int n, DevBu
 // ----------------------------------------------------------- timing
 void timer_mark(GGPKD pkd, int slot)
 {
-    if (!pkd->tm.created) {
-        for (int k = 0; k < 16; ++k) (void)hipEventCreate(&pkd->tm.ev[k]);
-        pkd->tm.created = 1;
+    Timers &T = pkd->tm;
+    if (!T.created) {
+        for (int f = 0; f < Timers::NFR; ++f)
+            for (int k = 0; k < Timers::NEV; ++k) (void)hipEventCreate(&T.ev[f][k]);
+        T.created = 1;
     }
-    (void)hipEventRecord(pkd->tm.ev[slot], pkd->stream);
+    if (slot == 0) {   // a new call: the next frame, once its old marks are read
+        const int next = (T.frame + 1) % Timers::NFR;
+        for (int i = 0; i < T.npend; ++i)
+            if (T.pend[i].frame == next) {
+                timer_resolve(pkd);
+                break;
+            }
+        T.frame = next;
+    }
+    (void)hipEventRecord(T.ev[T.frame][slot], pkd->stream);
 }
 
-// Elapsed time between consecutive marks first..first+nphases, stored into
-// the phase slots (the stream must have been synchronised).
+// The phases between consecutive marks first..first+nphases of the current
+// call, read later (timer_resolve): each phase slot then holds this call's
+// time (phases listed twice add up).
 void timer_collect(GGPKD pkd, int first, int nphases, const int *phase)
 {
-    for (int k = 0; k < nphases; ++k) pkd->tm.ms[phase[k]] = 0;
-    for (int k = 0; k < nphases; ++k) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, pkd->tm.ev[first + k], pkd->tm.ev[first + k + 1]) == hipSuccess)
-            pkd->tm.ms[phase[k]] += ms;
+    Timers &T = pkd->tm;
+    if (T.npend == Timers::NPEND) timer_resolve(pkd);
+    Timers::Pend &p = T.pend[T.npend++];
+    p.frame = T.frame;
+    p.first = first;
+    p.n = nphases < 8 ? nphases : 8;
+    for (int k = 0; k < p.n; ++k) p.phase[k] = phase[k];
+}
+
+// The same, read at once (the stream has been synchronised), after every
+// earlier call's times.
+void timer_collect_now(GGPKD pkd, int first, int nphases, const int *phase)
+{
+    timer_collect(pkd, first, nphases, phase);
+    timer_resolve(pkd);
+}
+
+// Read every pending call's times, oldest first.
+void timer_resolve(GGPKD pkd)
+{
+    Timers &T = pkd->tm;
+    for (int i = 0; i < T.npend; ++i) {
+        const Timers::Pend &p = T.pend[i];
+        hipEvent_t *ev = T.ev[p.frame];
+        (void)hipEventSynchronize(ev[p.first + p.n]);
+        for (int k = 0; k < p.n; ++k) T.ms[p.phase[k]] = 0;
+        for (int k = 0; k < p.n; ++k) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev[p.first + k], ev[p.first + k + 1]) == hipSuccess)
+                T.ms[p.phase[k]] += ms;
+        }
     }
+    T.npend = 0;
 }
 
 void sph_release(GGPKD pkd) { pkd->sph.release_all(); }
@@ -184,7 +223,8 @@ extern "C" int gg_finish(GGPKD pkd)
     pkd->mlp.release_all();
     aos_release(pkd);
     if (pkd->tm.created)
-        for (int k = 0; k < 16; ++k) (void)hipEventDestroy(pkd->tm.ev[k]);
+        for (int f = 0; f < Timers::NFR; ++f)
+            for (int k = 0; k < Timers::NEV; ++k) (void)hipEventDestroy(pkd->tm.ev[f][k]);
     (void)hipStreamDestroy(pkd->stream);
     delete pkd;
     return GG_OK;
@@ -429,6 +469,7 @@ extern "C" int gg_get_aos(GGPKD pkd, void *base, size_t stride, const gg_aos_fie
 extern "C" int gg_timings(GGPKD pkd, double *t, int n)
 {
     GG_CHECK(pkd && t, GG_EINVAL, "null argument");
+    timer_resolve(pkd);
     for (int k = 0; k < n && k < 12; ++k) t[k] = pkd->tm.ms[k];
     return GG_OK;
 }

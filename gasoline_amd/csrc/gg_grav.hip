@@ -1512,7 +1512,7 @@ extern "C" int gg_grav_all(GGPKD pkd, int nReps, int bPeriodic, int iOrder, int 
         GG_HIP(hipMemcpyAsync(hs, G.stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
         GG_HIP(hipStreamSynchronize(st));
         const int phase[2] = {1, 8};
-        timer_collect(pkd, 0, 2, phase);
+        timer_collect_now(pkd, 0, 2, phase);
         pkd->tm.ms[0] = 0;     // the walk is inside the interaction kernel
         if (ewald && iEwOrder)   // pkdBucketEwald's count (ewald.c:175-176)
             ewFlop += (double)hs[0] * G.nEwh * 58;
@@ -2230,6 +2230,7 @@ extern "C" int gg_tune_bucket(GGPKD pkd, int nSys, const int *sysOff, int iOpenT
             rc = gg_grav_all(pkd, 0, 0, iOrder, 0, 0, 0, 0, 0, 0, 0, 0, nullptr, nullptr,
                              nullptr, nullptr, nullptr, nullptr);
         if (rc != GG_OK) break;
+        timer_resolve(pkd);
         const double t = pkd->tm.ms[1] + pkd->tm.ms[8];
         if (ms) ms[k] = t;
         if (best < 0 || t < bestMs) {

@@ -178,10 +178,22 @@ struct AosStage {
     int pending[2] = {0, 0};
 };
 
+// Phase timers from HIP events on the handle's stream.  Each API call marks
+// its phases in a frame of its own (a ring of NFR frames), and its elapsed
+// times are read only when someone asks for them (gg_timings, the tuners)
+// or when its frame comes round again, so no call waits for its own work
+// just to time it.  ms[] then holds what the last call of each phase took.
 struct Timers {
-    hipEvent_t ev[16];
-    int created = 0;
+    static constexpr int NFR = 8, NEV = 16, NPEND = 32;
+    hipEvent_t ev[NFR][NEV];
+    int created = 0, frame = 0;
     double ms[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    struct Pend {
+        int frame, first, n;
+        int phase[8];
+    };
+    Pend pend[NPEND];
+    int npend = 0;
 };
 
 
@@ -249,4 +261,6 @@ void timer_mark(GGPKD pkd, int slot);
 // `device` (the current device), looked up once per device (gg_initialize).
 int direct_scratch_setup(int device);
 void timer_collect(GGPKD pkd, int first_slot, int nphases, const int *phase);
+void timer_collect_now(GGPKD pkd, int first_slot, int nphases, const int *phase);
+void timer_resolve(GGPKD pkd);
 }  // namespace gg

@@ -2242,7 +2242,6 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     S.built = 1;
     S.listValid = 0;
     timer_mark(pkd, 1);
-    GG_HIP(hipStreamSynchronize(st));
     const int ph[1] = {2};
     timer_collect(pkd, 0, 1, ph);
     return GG_OK;
@@ -2352,7 +2351,6 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
                        pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 6);
-    GG_HIP(hipStreamSynchronize(st));
     const int ph[6] = {9, 3, 10, bSymmetric ? 6 : 8, 4, 8};
     timer_collect(pkd, 0, 6, ph);
     return GG_OK;
@@ -2529,7 +2527,6 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(dendvdx_store(pkd));
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 6);
-    GG_HIP(hipStreamSynchronize(st));
     const int ph[6] = {9, 3, 10, 8, 4, 8};
     timer_collect(pkd, 0, 6, ph);
     return GG_OK;
@@ -2552,7 +2549,6 @@ extern "C" int gg_gas_pressure(GGPKD pkd, const gg_smf *smf)
                        smf->dResolveJeans, pkd->f[GG_POVERRHO2].as<double>(),
                        pkd->f[GG_C].as<double>(), pkd->f[GG_DT].as<double>());
     GG_HIP(hipGetLastError());
-    GG_HIP(hipStreamSynchronize(pkd->stream));
     return GG_OK;
 }
 
@@ -2597,6 +2593,7 @@ extern "C" int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     for (int k = 0; k < nCand && rc == GG_OK; ++k) {
         for (int rep = 0; rep < 2 && rc == GG_OK; ++rep) rc = pass(cand[k]);
         if (rc != GG_OK) break;
+        timer_resolve(pkd);
         const double *t = pkd->tm.ms;
         const double tk = t[2] + t[3] + t[4] + t[8] + t[9] + t[10];
         if (ms) ms[k] = tk;
@@ -2684,7 +2681,6 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         S.exValid = reuse && e[7] == 0;
         GG_TRY(dendvdx_store(pkd));
         timer_mark(pkd, 3);
-        GG_HIP(hipStreamSynchronize(st));
         const int ph[3] = {5, 4, 8};
         timer_collect(pkd, 0, 3, ph);
         return GG_OK;
@@ -2785,7 +2781,6 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
                        pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 5);
-    GG_HIP(hipStreamSynchronize(st));
     const int ph[5] = {5, 6, 8, 7, 8};
     timer_collect(pkd, 0, 5, ph);
     return GG_OK;

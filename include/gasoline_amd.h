@@ -20,8 +20,15 @@
  * Conventions: plain pointers and sizes; every entry point returns 0 on
  * success and a negative GG_E* code on error, with a message available from
  * gg_last_error() (gasoline itself asserts; errors here are returned).
- * All calls on one GGPKD are issued on that handle's HIP stream and are
- * synchronous with respect to the host unless stated otherwise.
+ * All calls on one GGPKD are issued on that handle's HIP stream, in order.
+ * gg_build_local, gg_smooth, gg_gas_pressure and gg_resmooth may return
+ * before their last kernels have finished (each still waits for what its
+ * host side must read: counts, error words); every call that hands results
+ * to the host (gg_get, gg_grav_all's counters, gg_timings, ...) waits for
+ * the stream first.  A caller that reads the store's device memory on a
+ * stream of its own synchronises the handle's stream (or the device)
+ * before it does.  The other calls are synchronous with respect to the
+ * host unless stated otherwise.
  */
 #ifndef GASOLINE_AMD_H
 #define GASOLINE_AMD_H
