@@ -132,7 +132,7 @@ struct SphDev {
     DevBuf retry, hrec;                        // kNN retries, force records
     DevBuf qf, act, nba, err, guess, diffcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
-    DevBuf jv, ju;                             // FP32 density record (vPred, fMass; uPred)
+    DevBuf drec;                               // DenDVDX per-slot records
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
     std::vector<DevBuf> spares;                // gg_reorder's spare buffers, by capacity
     void release_all()
@@ -144,7 +144,7 @@ struct SphDev {
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
-                         &bal, &dif, &dt, &jv, &ju, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
+                         &bal, &dif, &dt, &drec, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();
     }

@@ -1225,18 +1225,59 @@ template <typename T> __device__ __forceinline__ T m4_dw(T r2)
     return T(-0.75) * t * t * trcp<T>(r);
 }
 
+// Per-slot record DenDVDX reads for each neighbour, so that one neighbour
+// costs a handful of 16-byte loads from one place instead of a load from each
+// of a dozen arrays: the position and fBall2 in FP64 (separations are formed
+// in FP64), the pair fields in T, and the flags.
+template <typename T> struct DRec;
+template <> struct alignas(16) DRec<double> {
+    double x, y, z, fb2;
+    double m, vx, vy, vz;            // fMass, vPred
+    double u;                        // uPred
+    uint32_t qf, act;                // smoothed (has a list), TYPE_ACTIVE
+};
+template <> struct alignas(16) DRec<float> {
+    double x, y, z, fb2;
+    float vx, vy, vz, m;
+    float u;
+    uint32_t qf, act, pad;
+};
+static_assert(sizeof(DRec<double>) == 80 && sizeof(DRec<float>) == 64, "DRec layout");
+
+template <typename T>
+__global__ void k_dpack(int n, const int *__restrict__ order, const double *__restrict__ sx,
+                        const double *__restrict__ sy, const double *__restrict__ sz,
+                        const double *__restrict__ fb2, const uint32_t *__restrict__ qf,
+                        const uint32_t *__restrict__ act, const double *__restrict__ m,
+                        const double *__restrict__ vp, const double *__restrict__ u,
+                        DRec<T> *__restrict__ rec)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = order[s];
+    DRec<T> r;
+    r.x = sx[s];
+    r.y = sy[s];
+    r.z = sz[s];
+    r.fb2 = fb2[s];
+    r.m = (T)m[i];
+    r.vx = (T)vp[3 * (size_t)i];
+    r.vy = (T)vp[3 * (size_t)i + 1];
+    r.vz = (T)vp[3 * (size_t)i + 2];
+    r.u = (T)u[i];
+    r.qf = qf[s];
+    r.act = act[s];
+    if constexpr (sizeof(T) == 4) r.pad = 0;
+    rec[s] = r;
+}
+
 struct DArgs {
     int n, op;                  // op: GG_SMX_DENSITY or GG_SMX_DENDVDX
     int per;
     double L[3];
-    const double *sx, *sy, *sz, *fb2;        // slot order
     const int64_t *off;                      // CSR (slot order)
     const int *idx;
-    const uint32_t *qflag;
-    const double *m, *vx, *vy, *vz, *u;      // slot order, FP64 (pair arithmetic in FP64)
-    const float4 *jv;                        // slot order, FP32 form: vPred xyz, fMass
-    const float *ju;                         //   and uPred
-    const uint32_t *act;                     // slot order: TYPE_ACTIVE
+    const void *rec;                         // DRec<T> per slot
     double H, a, gamma;
     int bCannonical, iViscosityLimiter, bConstantDiffusion;
     double *rho, *divv, *curl, *dvds, *divvt, *bal, *diff;   // slot order
@@ -1255,60 +1296,40 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     const int sub = threadIdx.x & 7;
     if (g >= A.n) return;
     const int i = g;
-    if (!A.qflag[i]) return;
-    const double fb2 = A.fb2[i];
+    const DRec<T> *__restrict__ rec = static_cast<const DRec<T> *>(A.rec);
+    const DRec<T> I = rec[i];
+    if (!I.qf) return;
+    const double fb2 = I.fb2;
     const double ih2d = 4.0 / fb2;
     const T vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
-    const double x = A.sx[i], y = A.sy[i], z = A.sz[i];
-    constexpr bool F32 = sizeof(T) == 4;
-    T vxi, vyi, vzi;
-    if constexpr (F32) {
-        const float4 r = A.jv[i];
-        vxi = r.x;
-        vyi = r.y;
-        vzi = r.z;
-    } else {
-        vxi = A.vx[i];
-        vyi = A.vy[i];
-        vzi = A.vz[i];
-    }
-    const bool pact = A.act[i] & 1u;
+    const double x = I.x, y = I.y, z = I.z;
+    const T vxi = I.vx, vyi = I.vy, vzi = I.vz;
+    const bool pact = I.act & 1u;
     double rho = 0, xx = 0, xy = 0, xz = 0, yx = 0, yy = 0, yz = 0, zx = 0, zy = 0, zz = 0;
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
     const int64_t o = A.off[i], e1 = A.off[i + 1];
     for (int64_t e = o + sub; e < e1; e += 8) {
         const int q = A.idx[e];
-        const double dxd = img_sep(x, A.sx[q], A.per, A.L[0]);
-        const double dyd = img_sep(y, A.sy[q], A.per, A.L[1]);
-        const double dzd = img_sep(z, A.sz[q], A.per, A.L[2]);
+        const DRec<T> Q = rec[q];
+        const double dxd = img_sep(x, Q.x, A.per, A.L[0]);
+        const double dyd = img_sep(y, Q.y, A.per, A.L[1]);
+        const double dzd = img_sep(z, Q.z, A.per, A.L[2]);
         const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
         const T r2 = (T)(sq_dist(dxd, dyd, dzd) * ih2d);
-        T mq, vxq, vyq, vzq;
-        if constexpr (F32) {
-            const float4 r = A.jv[q];
-            vxq = r.x;
-            vyq = r.y;
-            vzq = r.z;
-            mq = r.w;
-        } else {
-            mq = A.m[q];
-            vxq = A.vx[q];
-            vyq = A.vy[q];
-            vzq = A.vz[q];
-        }
+        const T mq = Q.m, vxq = Q.vx, vyq = Q.vy, vzq = Q.vz;
         if (pact) A.nbract[q] = 32u;   // every writer stores the same flag
-        qact |= A.act[q];
+        qact |= Q.act;
         {   // extras of the force pass: q's own list misses i when i lies
             // outside q's ball, seen from q (or q has no list)
-            bool extra = !A.qflag[q];
+            bool extra = !Q.qf;
             if (!extra) {
                 double d2q = sq_dist(dxd, dyd, dzd);
-                if (A.per && (fabs(x - A.sx[q]) > 0.5 * A.L[0] || fabs(y - A.sy[q]) > 0.5 * A.L[1] ||
-                              fabs(z - A.sz[q]) > 0.5 * A.L[2]))
-                    d2q = sq_dist(img_sep(A.sx[q], x, 1, A.L[0]), img_sep(A.sy[q], y, 1, A.L[1]),
-                                  img_sep(A.sz[q], z, 1, A.L[2]));
-                extra = d2q > A.fb2[q];
+                if (A.per && (fabs(x - Q.x) > 0.5 * A.L[0] || fabs(y - Q.y) > 0.5 * A.L[1] ||
+                              fabs(z - Q.z) > 0.5 * A.L[2]))
+                    d2q = sq_dist(img_sep(Q.x, x, 1, A.L[0]), img_sep(Q.y, y, 1, A.L[1]),
+                                  img_sep(Q.z, z, 1, A.L[2]));
+                extra = d2q > Q.fb2;
             }
             if (extra) {
                 const int k = atomicAdd(&A.ecnt[q], 1);
@@ -1316,15 +1337,13 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
                 else atomicOr(&A.err[7], 1u);
             }
         }
-        rho += (double)(m4_w<T>(r2) * mq);
+    rho += (double)(m4_w<T>(r2) * mq);
         if (A.op == GG_SMX_DENDVDX) {
             const T rs1 = m4_dw<T>(r2) * mq;
             const T dvx = (vxq - vxi) * vFac;
             const T dvy = (vyq - vyi) * vFac;
             const T dvz = (vzq - vzi) * vFac;
-            T uq;
-            if constexpr (F32) uq = A.ju[q];
-            else uq = A.u[q];
+            const T uq = Q.u;
             xx += (double)(dvx * dx * rs1); xy += (double)(dvx * dy * rs1); xz += (double)(dvx * dz * rs1);
             yx += (double)(dvy * dx * rs1); yy += (double)(dvy * dy * rs1); yz += (double)(dvy * dz * rs1);
             zx += (double)(dvz * dx * rs1); zy += (double)(dvz * dy * rs1); zz += (double)(dvz * dz * rs1);
@@ -1469,121 +1488,6 @@ __global__ void k_gas_pressure(int n, const uint32_t *__restrict__ bits, uint32_
     if (dti < dt[i]) dt[i] = dti;
 }
 
-struct HArgs {
-    int n, per;
-    double L[3];
-    const double *sx, *sy, *sz, *fb2;
-    const int64_t *off, *roff;
-    const int *idx, *ridx;
-    const uint32_t *qflag, *act;
-    const double *m, *vx, *vy, *vz, *u, *rho, *pr2, *c, *bal, *diff, *dt;
-    double a, H, alpha, beta, dtFacCourant, dtFacDiffusion, dThermalDiffusionCoeff;
-    int bCannonical;
-    double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
-};
-
-// One (p, q) pair of SphPressureTermsSym (SphPressureTerms.h:112-152) as it
-// affects particle "self": role 0 = self is p (own list), 1 = self is q
-// (reverse list, evaluated with p's smoothing length).
-template <typename T>
-__device__ __forceinline__ void hydro_pair(const HArgs &A, int self, int other, int role,
-                                           double &ax, double &ay, double &az, double &pdv,
-                                           double &av, double &dif, double &mu, double &dtn)
-{
-    const int p = role ? other : self, q = role ? self : other;
-    const bool pa = A.act[p] & 1u, qa = A.act[q] & 1u;
-    if (!pa && !qa) return;
-    const bool sa = role ? qa : pa;    // is self active
-    const double dxd = img_sep(A.sx[p], A.sx[q], A.per, A.L[0]);
-    const double dyd = img_sep(A.sy[p], A.sy[q], A.per, A.L[1]);
-    const double dzd = img_sep(A.sz[p], A.sz[q], A.per, A.L[2]);
-    const double d2d = sq_dist(dxd, dyd, dzd);
-    const double fbp = A.fb2[p];
-    const T ph = (T)sqrt(0.25 * fbp);
-    const T ih2 = (T)(4.0 / fbp);
-    const T fNorm1 = (T)(0.5 * M_1_PI * (4.0 / fbp) / sqrt(0.25 * fbp) * (4.0 / fbp));
-    const T dx = (T)dxd, dy = (T)dyd, dz = (T)dzd, d2 = (T)d2d;
-    const T r2 = (T)(d2d * (4.0 / fbp));
-    (void)ih2;
-    const T rs1 = m4_dw<T>(r2) * fNorm1;
-    const T rp = rs1 * (T)A.m[p], rq = rs1 * (T)A.m[q];
-    const T vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
-    const T dvx = (T)A.vx[p] - (T)A.vx[q];
-    const T dvy = (T)A.vy[p] - (T)A.vy[q];
-    const T dvz = (T)A.vz[p] - (T)A.vz[q];
-    const T dvdotdr = vFac * (dvx * dx + dvy * dy + dvz * dz) + d2 * (T)A.H;
-    const T pP = (T)A.pr2[p], qP = (T)A.pr2[q];
-    const T pc = (T)A.c[p], qc = (T)A.c[q];
-    const T prho = (T)A.rho[p], qrho = (T)A.rho[q];
-    // PRES_PDV(a,b) = a, PRES_ACC(a,b) = a+b (smoothfcn.c:20-23); without
-    // an external pressure P/rho^2 "f" equals P/rho^2.
-    const T r_self = role ? rp : rq;          // rq for p, rp for q
-    if (sa) pdv += (double)(r_self * (role ? qP : pP) * dvdotdr);
-    T Acc = pP + qP;
-    T dtv;
-    if (dvdotdr >= T(0)) {
-        dtv = (T)A.dtFacCourant * ph * trcp<T>(T(2) * (pc > qc ? pc : qc));
-    } else {
-        const T absmu = -dvdotdr * (T)A.a * trcp<T>(tsqrt<T>(d2));
-        mu = fmax(mu, (double)absmu);
-        T visc = (T)A.alpha * (pc + qc) + (T)A.beta * T(1.5) * absmu;
-        dtv = (T)A.dtFacCourant * ph * trcp<T>(T(0.625) * (pc + qc) + T(0.375) * visc);
-        visc = (T(0.5) * ((T)A.bal[p] + (T)A.bal[q])) * visc * absmu * trcp<T>(prho + qrho);
-        if (sa) av += (double)(r_self * (T(0.5) * visc) * dvdotdr);
-        Acc += visc;
-    }
-    if (sa) {
-        Acc *= r_self * (T)A.a;
-        const double s = role ? 1.0 : -1.0;      // a_p -= Acc dx, a_q += Acc dx
-        ax += s * (double)(Acc * dx);
-        ay += s * (double)(Acc * dy);
-        az += s * (double)(Acc * dz);
-    }
-    // thermal diffusion (SphPressureTerms.h:32-70, 142-145)
-    const double dsum = A.diff[p] + A.diff[q];
-    if (dsum != 0 && A.dThermalDiffusionCoeff > 0) {
-        const double diffTh = 2 * A.dThermalDiffusionCoeff * dsum / (A.rho[p] + A.rho[q]);
-        const double dtd = A.dtFacDiffusion * (double)ph * (double)ph / (diffTh * A.rho[p]);
-        if (diffTh > 0 && dtd < (double)dtv) dtv = (T)dtd;
-        const double du = diffTh * (A.u[p] - A.u[q]);
-        if (sa) dif += role ? -du * (double)rp : du * (double)rq;
-    }
-    // SETDTNEW_PQ (SphPressureTerms.h:97-100)
-    dtn = fmin(dtn, (double)dtv);
-    dtn = fmin(dtn, 4 * A.dt[other]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_hydro(HArgs A)
-{
-    const int g = (blockIdx.x * 256 + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
-    if (g >= A.n) return;
-    const int i = g;
-    double ax = 0, ay = 0, az = 0, pdv = 0, av = 0, dif = 0, mu = 0, dtn = GG_FLT_MAX;
-    if (A.qflag[i]) {
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 16)
-            hydro_pair<T>(A, i, A.idx[e], 0, ax, ay, az, pdv, av, dif, mu, dtn);
-    }
-    for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 16)
-        hydro_pair<T>(A, i, A.ridx[e], 1, ax, ay, az, pdv, av, dif, mu, dtn);
-    for (int w = 1; w < 16; w <<= 1) {
-        ax += __shfl_xor(ax, w);
-        ay += __shfl_xor(ay, w);
-        az += __shfl_xor(az, w);
-        pdv += __shfl_xor(pdv, w);
-        av += __shfl_xor(av, w);
-        dif += __shfl_xor(dif, w);
-        mu = fmax(mu, __shfl_xor(mu, w));
-        dtn = fmin(dtn, __shfl_xor(dtn, w));
-    }
-    if (sub == 0) {
-        double *o = A.out + 8 * (size_t)i;
-        o[0] = ax; o[1] = ay; o[2] = az; o[3] = pdv; o[4] = av; o[5] = dif;
-        o[6] = mu; o[7] = dtn;
-    }
-}
-
 // Apply the hydro results to the caller-order fields: for smooth-active
 // particles initSphPressureTermsParticle (smoothfcn.c:2204-2221) resets
 // mumax, dtNew and the uDot terms of ACTIVE ones; a is accumulated.
@@ -1616,19 +1520,24 @@ __global__ void k_hydro_scatter(int n, const int *__restrict__ order,
 }
 
 // ------------------------------------------- hydro, own list + extras
-// Per-particle record of the force pass, slot order, in the pair precision:
-// velocity, mass, density, P/rho^2, sound speed, Balsara switch, h = 2h/2,
-// the kernel normalisation 0.5/pi * ih2^2 / h (SphPressureTermsSym,
-// smoothfcn.c:2302-2305), 4/fBall2, Courant dt, uPred, diff, activity.
+// Per-particle record of the force pass, slot order: the position and
+// fBall2 in FP64, then in the pair precision velocity, mass, density,
+// P/rho^2, sound speed, Balsara switch, h = 2h/2, the kernel normalisation
+// 0.5/pi * ih2^2 / h (SphPressureTermsSym, smoothfcn.c:2302-2305), 4/fBall2,
+// Courant dt, uPred, diff, activity and whether it has a list.  One record
+// per neighbour: a few 16-byte loads from one place.
 template <typename T> struct alignas(16) HRec {
+    double x, y, z, fb2;
     T vx, vy, vz, m;
     T rho, pr2, c, bal;
     T ph, fn1, ih2, dt;
-    T u, diff, act, pad;
+    T u, diff, act, qf;
 };
 
 template <typename T>
-__global__ void k_hpack(int n, const int *__restrict__ order, const double *__restrict__ fb2,
+__global__ void k_hpack(int n, const int *__restrict__ order, const double *__restrict__ sx,
+                        const double *__restrict__ sy, const double *__restrict__ sz,
+                        const uint32_t *__restrict__ qflag, const double *__restrict__ fb2,
                         const double *__restrict__ m, const double *__restrict__ vp,
                         const double *__restrict__ u, const double *__restrict__ rho,
                         const double *__restrict__ pr2, const double *__restrict__ c,
@@ -1643,6 +1552,10 @@ __global__ void k_hpack(int n, const int *__restrict__ order, const double *__re
     const double ph = sqrt(0.25 * f), ih2 = 4.0 / f;
     const double fNorm = 0.5 * M_1_PI * ih2 / ph;
     HRec<T> r;
+    r.x = sx[s];
+    r.y = sy[s];
+    r.z = sz[s];
+    r.fb2 = f;
     r.vx = (T)vp[3 * (size_t)i];
     r.vy = (T)vp[3 * (size_t)i + 1];
     r.vz = (T)vp[3 * (size_t)i + 2];
@@ -1658,7 +1571,7 @@ __global__ void k_hpack(int n, const int *__restrict__ order, const double *__re
     r.u = (T)u[i];
     r.diff = (T)diff[i];
     r.act = (T)((bits[i] & 1u) ? 1 : 0);
-    r.pad = 0;
+    r.qf = (T)(qflag[s] ? 1 : 0);
     rec[s] = r;
 }
 
@@ -1729,12 +1642,10 @@ __device__ __forceinline__ void hpair(const HRec<T> &P, const HRec<T> &Q, T dx, 
 struct H2Args {
     int n, per;
     double L[3];
-    const double *sx, *sy, *sz, *fb2;
     const int64_t *off, *eoff;         // eoff: CSR extras, or nullptr for
     const int *idx, *eidx;             // ecap-strided lists of ecnt entries
     const int *ecnt;
     int ecap;
-    const uint32_t *qflag;
     double a, H, alpha, beta, dtFacCourant, dtFacDiffusion, kDiff;
     int bCannonical;
     double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
@@ -1760,23 +1671,23 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     hp.dtFacDiffusion = A.dtFacDiffusion;
     hp.kDiff = A.kDiff;
     const HRec<T> I = rec[i];
-    const double xi = A.sx[i], yi = A.sy[i], zi = A.sz[i];
+    const double xi = I.x, yi = I.y, zi = I.z;
     HAcc acc;
-    if (A.qflag[i]) {
+    if (I.qf > T(0)) {
         for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 8) {
             const int q = A.idx[e];
             const HRec<T> Q = rec[q];
-            const double xq = A.sx[q], yq = A.sy[q], zq = A.sz[q];
+            const double xq = Q.x, yq = Q.y, zq = Q.z;
             const double dx = img_sep(xi, xq, A.per, A.L[0]);
             const double dy = img_sep(yi, yq, A.per, A.L[1]);
             const double dz = img_sep(zi, zq, A.per, A.L[2]);
             hpair<T>(I, Q, (T)dx, (T)dy, (T)dz, (T)sq_dist(dx, dy, dz), 0, hp, acc);
-            if (A.qflag[q]) {
+            if (Q.qf > T(0)) {
                 const double ex = img_sep(xq, xi, A.per, A.L[0]);
                 const double ey = img_sep(yq, yi, A.per, A.L[1]);
                 const double ez = img_sep(zq, zi, A.per, A.L[2]);
                 const double d2q = sq_dist(ex, ey, ez);
-                if (d2q <= A.fb2[q])
+                if (d2q <= Q.fb2)
                     hpair<T>(Q, I, (T)ex, (T)ey, (T)ez, (T)d2q, 1, hp, acc);
             }
         }
@@ -1786,9 +1697,9 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     for (int64_t e = e0 + sub; e < e1; e += 8) {
         const int p = A.eidx[e];
         const HRec<T> P = rec[p];
-        const double ex = img_sep(A.sx[p], xi, A.per, A.L[0]);
-        const double ey = img_sep(A.sy[p], yi, A.per, A.L[1]);
-        const double ez = img_sep(A.sz[p], zi, A.per, A.L[2]);
+        const double ex = img_sep(P.x, xi, A.per, A.L[0]);
+        const double ey = img_sep(P.y, yi, A.per, A.L[1]);
+        const double ez = img_sep(P.z, zi, A.per, A.L[2]);
         hpair<T>(P, I, (T)ex, (T)ey, (T)ez, (T)sq_dist(ex, ey, ez), 1, hp, acc);
     }
     for (int w = 1; w < 8; w <<= 1) {
@@ -1987,18 +1898,6 @@ __global__ void k_flags_out(int n, const int *__restrict__ order, const uint32_t
 
 // The FP32 form's j-side record of the density pass, slot order, gathered
 // from the store: vPred, fMass (one 16-byte load per pair) and uPred.
-__global__ void k_dpack(int n, const int *__restrict__ order, const double *__restrict__ m,
-                        const double *__restrict__ vp, const double *__restrict__ u,
-                        float4 *__restrict__ jv, float *__restrict__ ju)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int i = order[s];
-    jv[s] = make_float4((float)vp[3 * (size_t)i], (float)vp[3 * (size_t)i + 1],
-                        (float)vp[3 * (size_t)i + 2], (float)m[i]);
-    ju[s] = (float)u[i];
-}
-
 // ------------------------------------------------------------ host side
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
@@ -2365,20 +2264,17 @@ static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, cons
     hipStream_t st = pkd->stream;
     const int n = S.n;
     const bool f32 = pkd->prec == GG_PREC_FP32;
-    if (f32) {
-        for (int f : {GG_MASS, GG_VPRED, GG_UPRED}) GG_TRY(ensure_field(pkd, f));
-        GG_TRY(S.jv.reserve(sizeof(float4) * (size_t)n));
-        GG_TRY(S.ju.reserve(sizeof(float) * (size_t)n));
-        hipLaunchKernelGGL(k_dpack, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
-                           pkd->f[GG_MASS].as<double>(), pkd->f[GG_VPRED].as<double>(),
-                           pkd->f[GG_UPRED].as<double>(), S.jv.as<float4>(), S.ju.as<float>());
-    } else {
-        GG_TRY(gather_field(pkd, GG_MASS, 1, 0, S.m));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 0, S.vx));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 1, S.vy));
-        GG_TRY(gather_field(pkd, GG_VPRED, 3, 2, S.vz));
-        GG_TRY(gather_field(pkd, GG_UPRED, 1, 0, S.u));
-    }
+    for (int f : {GG_MASS, GG_VPRED, GG_UPRED}) GG_TRY(ensure_field(pkd, f));
+    GG_TRY(S.drec.reserve((f32 ? sizeof(DRec<float>) : sizeof(DRec<double>)) * (size_t)n));
+#define GG_DPACK(T)                                                                             \
+    hipLaunchKernelGGL(k_dpack<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(), \
+                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),               \
+                       S.fb2s.as<double>(), S.qf.as<uint32_t>(), S.act.as<uint32_t>(),        \
+                       pkd->f[GG_MASS].as<double>(), pkd->f[GG_VPRED].as<double>(),           \
+                       pkd->f[GG_UPRED].as<double>(), S.drec.as<DRec<T>>())
+    if (f32) GG_DPACK(float);
+    else GG_DPACK(double);
+#undef GG_DPACK
     GG_TRY(S.o_rho.reserve(sizeof(double) * n));
     GG_TRY(S.o_divv.reserve(sizeof(double) * n));
     GG_TRY(S.o_curl.reserve(sizeof(double) * 3 * n));
@@ -2391,21 +2287,9 @@ static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, cons
     D.op = GG_SMX_DENDVDX;
     D.per = S.bPeriodic;
     for (int d = 0; d < 3; ++d) D.L[d] = S.L[d];
-    D.sx = S.sx.as<double>();
-    D.sy = S.sy.as<double>();
-    D.sz = S.sz.as<double>();
-    D.fb2 = S.fb2s.as<double>();
     D.off = off;
     D.idx = idx;
-    D.qflag = S.qf.as<uint32_t>();
-    D.m = S.m.as<double>();
-    D.vx = S.vx.as<double>();
-    D.vy = S.vy.as<double>();
-    D.vz = S.vz.as<double>();
-    D.u = S.u.as<double>();
-    D.jv = S.jv.as<float4>();
-    D.ju = S.ju.as<float>();
-    D.act = S.act.as<uint32_t>();
+    D.rec = S.drec.p;
     D.H = smf->H;
     D.a = smf->a;
     D.gamma = smf->gamma;
@@ -2725,7 +2609,8 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     GG_TRY(S.res.reserve(sizeof(double) * 8 * n));
 #define GG_HPACK(T)                                                                           \
     hipLaunchKernelGGL(k_hpack<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(), \
-                       S.fb2s.as<double>(), pkd->f[GG_MASS].as<double>(),                     \
+                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),               \
+                       S.qf.as<uint32_t>(), S.fb2s.as<double>(), pkd->f[GG_MASS].as<double>(), \
                        pkd->f[GG_VPRED].as<double>(), pkd->f[GG_UPRED].as<double>(),          \
                        pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_POVERRHO2].as<double>(),     \
                        pkd->f[GG_C].as<double>(), pkd->f[GG_BALSARA].as<double>(),             \
@@ -2738,17 +2623,12 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.n = n;
     H.per = S.bPeriodic;
     for (int d = 0; d < 3; ++d) H.L[d] = S.L[d];
-    H.sx = S.sx.as<double>();
-    H.sy = S.sy.as<double>();
-    H.sz = S.sz.as<double>();
-    H.fb2 = S.fb2s.as<double>();
     H.off = off;
     H.eoff = fixedExtras ? nullptr : S.roff.as<int64_t>();
     H.ecnt = fixedExtras ? S.ecnt.as<int>() : nullptr;
     H.ecap = ECAP;
     H.idx = idx;
     H.eidx = fixedExtras ? S.eidx.as<int>() : S.ridx.as<int>();
-    H.qflag = S.qf.as<uint32_t>();
     H.a = smf->a;
     H.H = smf->H;
     H.alpha = smf->alpha;
