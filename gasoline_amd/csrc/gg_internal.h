@@ -119,6 +119,7 @@ struct SphDev {
     int gasMaskValid = 0;                      // gasMask holds whether any particle is TYPE_GAS
     uint32_t gasMask = 0;
     int64_t nExtra = 0;
+    int nExtraDev = 0;                         // > 0: nExtra still in that many partials in nxd
     DevBuf key, order, k2, v2, sortTmp, sel, pos, gidx, part;
     DevBuf sx, sy, sz;                         // slot order positions
     DevBuf child, rng, sparse;                 // radix tree, bucket-box sparse table
@@ -130,7 +131,7 @@ struct SphDev {
     DevBuf ecnt, eidx;                         // extras found by the density pass
     int exValid = 0;
     DevBuf retry, hrec;                        // kNN retries, force records
-    DevBuf qf, act, nba, err, guess, diffcnt;
+    DevBuf qf, act, nba, err, guess, diffcnt, nxd;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
     DevBuf drec;                               // DenDVDX per-slot records
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
@@ -143,7 +144,7 @@ struct SphDev {
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
-                         &err, &guess, &diffcnt, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
+                         &err, &guess, &diffcnt, &nxd, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &drec, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();

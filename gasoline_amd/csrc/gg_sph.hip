@@ -1793,13 +1793,25 @@ __global__ void k_rev_fill(int n, const uint32_t *__restrict__ qflag,
 }
 
 // Sort each reverse list so the summation order is deterministic.
-__global__ void k_rev_sort_fixed(int n, int ecap, const int *__restrict__ ecnt,
-                                 int *__restrict__ eidx)
+// Each particle's extras list (ecap-strided, ecnt entries, in the order the
+// density pass's atomics left them) sorted by slot in place, so the force
+// pass sums in a fixed order; each block's total of entries goes to
+// part[block] (256 threads).
+__global__ __launch_bounds__(256) void k_rev_sort_fixed(int n, int ecap,
+                                                        const int *__restrict__ ecnt,
+                                                        int *__restrict__ eidx,
+                                                        int64_t *__restrict__ part)
 {
+    __shared__ int64_t ws[4];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = q < n ? ecnt[q] : 0;
+    int64_t t = m;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
     if (q >= n) return;
     int *a = eidx + (size_t)q * ecap;
-    const int m = ecnt[q];
     for (int i = 1; i < m; ++i) {
         const int v = a[i];
         int j = i - 1;
@@ -2571,12 +2583,10 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     }
     const bool fixedExtras = reuse && S.exValid;
     if (fixedExtras) {   // the density pass found them: sort each list
+        GG_TRY(S.nxd.reserve(sizeof(int64_t) * nblk(n, 256)));
         hipLaunchKernelGGL(k_rev_sort_fixed, dim3(nblk(n, 256)), dim3(256), 0, st, n, ECAP,
-                           S.ecnt.as<int>(), S.eidx.as<int>());
-        int64_t etot = 0;
-        GG_TRY(S.roff.reserve(sizeof(int64_t) * (n + 1)));
-        GG_TRY(exclusive_scan_i64(st, S.ecnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &etot));
-        S.nExtra = etot;
+                           S.ecnt.as<int>(), S.eidx.as<int>(), S.nxd.as<int64_t>());
+        S.nExtraDev = nblk(n, 256);   // partial totals, added up when the statistics are read
     } else {
     // extras: particles whose balls hold i but which are not in i's list
     GG_TRY(S.rcnt.reserve(sizeof(int) * (n + 1)));
@@ -2589,6 +2599,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     int64_t etot = 0;
     GG_TRY(exclusive_scan_i64(st, S.rcnt.as<int>(), S.roff.as<int64_t>(), n, S.sortTmp, &etot));
     S.nExtra = etot;
+    S.nExtraDev = 0;
     GG_TRY(S.ridx.reserve(sizeof(int) * std::max<int64_t>(etot, 1)));
     GG_HIP(hipMemsetAsync(S.rcnt.p, 0, sizeof(int) * (n + 1), st));
     hipLaunchKernelGGL(k_extra, dim3(nblk(n * 16, 256)), dim3(256), 0, st, n, 1, S.bPeriodic,
@@ -2818,7 +2829,16 @@ extern "C" int gg_set_seed_slack(GGPKD pkd, double slack)
 extern "C" int gg_smooth_stats(GGPKD pkd, int64_t *s, int n)
 {
     GG_CHECK(pkd && s, GG_EINVAL, "null argument");
-    const SphDev &S = pkd->sph;
+    SphDev &S = pkd->sph;
+    if (S.nExtraDev > 0) {
+        std::vector<int64_t> part(S.nExtraDev);
+        GG_HIP(hipMemcpyAsync(part.data(), S.nxd.p, sizeof(int64_t) * part.size(),
+                              hipMemcpyDeviceToHost, pkd->stream));
+        GG_HIP(hipStreamSynchronize(pkd->stream));
+        S.nExtra = 0;
+        for (int64_t v : part) S.nExtra += v;
+        S.nExtraDev = 0;
+    }
     const int64_t v[10] = {(int64_t)S.retried, (int64_t)S.ties, S.nExtra, (int64_t)S.listQueries,
                            (int64_t)S.diag[0], (int64_t)S.diag[1], (int64_t)S.diag[2],
                            (int64_t)S.diag[3], (int64_t)S.passes, S.cand};
