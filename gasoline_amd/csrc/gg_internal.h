@@ -131,7 +131,7 @@ struct SphDev {
     DevBuf ecnt, eidx;                         // extras found by the density pass
     int exValid = 0;
     DevBuf retry, hrec;                        // kNN retries, force records
-    DevBuf qf, act, nba, err, guess, diffcnt, nxd;
+    DevBuf qf, act, nba, err, guess, diffcnt, nxd, qcnt;
     DevBuf m, vx, vy, vz, u, rho, pr2, c, bal, dif, dt;   // slot-order inputs
     DevBuf drec;                               // DenDVDX per-slot records
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
@@ -144,7 +144,7 @@ struct SphDev {
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
-                         &err, &guess, &diffcnt, &nxd, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
+                         &err, &guess, &diffcnt, &nxd, &qcnt, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &drec, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,
                          &o_bal, &o_diff, &res};
         for (DevBuf *b : all) b->release();

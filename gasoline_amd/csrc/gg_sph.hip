@@ -1883,16 +1883,27 @@ __global__ void k_scatter(int n, const int *__restrict__ order, const uint32_t *
     for (int k = 0; k < width; ++k) dst[(size_t)width * c + k] = src[(size_t)width * s + k];
 }
 
-__global__ void k_qflag(int n, const int *__restrict__ order, const uint32_t *__restrict__ bits,
-                        uint32_t mask, uint32_t *__restrict__ qf, uint32_t *__restrict__ act,
-                        uint32_t *__restrict__ nba)
+// Slot-order flags (query, TYPE_ACTIVE) and each block's number of queries
+// in qcnt[block] (256 threads).
+__global__ __launch_bounds__(256) void k_qflag(int n, const int *__restrict__ order,
+                                               const uint32_t *__restrict__ bits, uint32_t mask,
+                                               uint32_t *__restrict__ qf, uint32_t *__restrict__ act,
+                                               uint32_t *__restrict__ nba, int *__restrict__ qcnt)
 {
+    __shared__ int wc[4];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t b = bits[order[s]];
-    qf[s] = (b & mask) == mask ? 1u : 0u;
-    act[s] = b & 1u;
-    if (nba) nba[s] = 0;
+    bool q = false;
+    if (s < n) {
+        const uint32_t b = bits[order[s]];
+        q = (b & mask) == mask;
+        qf[s] = q ? 1u : 0u;
+        act[s] = b & 1u;
+        if (nba) nba[s] = 0;
+    }
+    const int c = __popcll(__ballot(q));
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) qcnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
 __global__ void k_flags_out(int n, const int *__restrict__ order, const uint32_t *__restrict__ qf,
@@ -1971,9 +1982,11 @@ static int make_flags(GGPKD pkd, uint32_t qmask)
     GG_TRY(S.qf.reserve(sizeof(uint32_t) * n));
     GG_TRY(S.act.reserve(sizeof(uint32_t) * n));
     GG_TRY(S.nba.reserve(sizeof(uint32_t) * n));
+    GG_TRY(S.qcnt.reserve(sizeof(int) * nblk(n, 256)));
     hipLaunchKernelGGL(k_qflag, dim3(nblk(n, 256)), dim3(256), 0, pkd->stream, n,
                        S.order.as<int>(), pkd->f[GG_IACTIVE].as<uint32_t>(), qmask,
-                       S.qf.as<uint32_t>(), S.act.as<uint32_t>(), S.nba.as<uint32_t>());
+                       S.qf.as<uint32_t>(), S.act.as<uint32_t>(), S.nba.as<uint32_t>(),
+                       S.qcnt.as<int>());
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -2010,6 +2023,22 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.qlist = nullptr;
     Q.nq = nullptr;
     return Q;
+}
+
+// The twelve error words and the number of queries make_flags counted, in
+// one wait on the stream.
+static int read_err_queries(GGPKD pkd, unsigned *e, int64_t *nq)
+{
+    SphDev &S = pkd->sph;
+    std::vector<int> qc(nblk(S.n, 256));
+    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, pkd->stream));
+    GG_HIP(hipMemcpyAsync(qc.data(), S.qcnt.p, sizeof(int) * qc.size(), hipMemcpyDeviceToHost,
+                          pkd->stream));
+    GG_HIP(hipStreamSynchronize(pkd->stream));
+    int64_t t = 0;
+    for (int v : qc) t += v;
+    *nq = t;
+    return GG_OK;
 }
 
 static int read_err(GGPKD pkd, unsigned *e)
@@ -2161,7 +2190,7 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
 // The kNN lists' bookkeeping once the operator has run: the search radii
 // (-1 marks a failed search), the convergence check, what gg_resmooth may
 // reuse, and the number of particles smoothed.
-static int knn_finish(GGPKD pkd, const unsigned *e, int K, int *pnSmoothed)
+static int knn_finish(GGPKD pkd, const unsigned *e, int64_t nq, int K, int *pnSmoothed)
 {
     SphDev &S = pkd->sph;
     hipStream_t st = pkd->stream;
@@ -2183,9 +2212,6 @@ static int knn_finish(GGPKD pkd, const unsigned *e, int K, int *pnSmoothed)
     GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     S.listK = K;
     S.listValid = (S.ties == 0);
-    int64_t nq = 0;
-    GG_TRY(exclusive_scan_i64(st, (const int *)S.qf.as<uint32_t>(), S.pos.as<int64_t>(), n,
-                              S.sortTmp, &nq));
     S.listQueries = (int)nq;
     if (pnSmoothed) *pnSmoothed = (int)nq;
     return GG_OK;
@@ -2251,9 +2277,9 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 5);
     unsigned e[12];
-    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-    GG_TRY(knn_finish(pkd, e, K, pnSmoothed));
+    int64_t nq = 0;
+    GG_TRY(read_err_queries(pkd, e, &nq));
+    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
     // DensitySym writes every particle of the structure (the unsmoothed ones
     // receive their neighbours' halves); Density the smoothed ones
     GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, bSymmetric ? nullptr : S.qf.as<uint32_t>()));
@@ -2416,10 +2442,10 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     GG_TRY(dendvdx_launch(pkd, smf, S.off.as<int64_t>(), S.nbr.as<int>()));
     timer_mark(pkd, 5);
     unsigned e[12];
-    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
+    int64_t nq = 0;
+    GG_TRY(read_err_queries(pkd, e, &nq));
     S.exValid = (e[7] == 0);   // the extras lists are whole
-    GG_TRY(knn_finish(pkd, e, K, pnSmoothed));
+    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
     GG_TRY(dendvdx_store(pkd));
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 6);
