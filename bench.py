@@ -102,19 +102,26 @@ def sph_step(pk, smf, zero_fb2, zero_a, K=64, nBucket=32):
     pk.resmooth(smf, K, True)
 
 
-def time_sph(gg, torch, n, prec, steps, warmup, tune=None, nBucket=32):
+def time_sph(gg, torch, n, prec, steps, warmup, tune=None, nBucket=32, lanes=(4, 8, 16)):
     """The SPH step, its phases and outputs; with tune = candidate bucket
     sizes, gg_tune_sph chooses the structure's bucket size first (from a
-    cold start, as every timed step searches)."""
+    cold start, as every timed step searches), then gg_tune_sph_lanes the
+    lanes per particle of DenDVDX and the force pass."""
     smf = gg.Smf.default()
     pk = sph_setup(gg, n, prec)
     zero = torch.zeros(n, dtype=torch.float64, device="cuda")
     zero3 = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
-    nb, tuned = nBucket, None
+    nb, tuned = nBucket, {"nBucket": nBucket}
     if tune:
         pk.put("fBall2", zero)
         nb, cand_ms = pk.tune_sph(smf, 64, [1.0, 1.0, 1.0], cand=tune)
         tuned = {"nBucket": nb, "candidate_ms": cand_ms}
+    if lanes:
+        pk.put("fBall2", zero)
+        pk.put("a", zero3)
+        pk.build_local(period=[1.0, 1.0, 1.0], nBucket=nb)
+        lp, lane_ms = pk.tune_sph_lanes(smf, 64, True, cand=lanes)
+        tuned.update(lanes=lp, lanes_candidate_ms=lane_ms)
     for _ in range(warmup):
         sph_step(pk, smf, zero, zero3, nBucket=nb)
     torch.cuda.synchronize()
@@ -509,8 +516,10 @@ def main():
     pairs = 2 * K * n                      # density pass + force pass
     t64, ph64, o64, tune64 = time_sph(gg, torch, n, gg.PREC_FP64, args.steps, args.warmup,
                                       tune=(32, 24, 16))
-    t32, ph32, o32, _ = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup,
-                                 nBucket=tune64["nBucket"])
+    t32, ph32, o32, tune32 = time_sph(gg, torch, n, gg.PREC_FP32, args.steps, args.warmup,
+                                      nBucket=tune64["nBucket"])
+    tune64["fp32_lanes"] = tune32["lanes"]
+    tune64["fp32_lanes_candidate_ms"] = tune32["lanes_candidate_ms"]
     sph_prec = {f: rel_errors(o32[f], o64[f]) for f in o32}
     log("SPH box done")
 

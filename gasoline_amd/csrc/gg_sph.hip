@@ -1287,13 +1287,20 @@ struct DArgs {
     unsigned *err;                           // err[7]: an extras list overflowed
 };
 
-// DenDVDX (smoothfcn.c:3390-3627): 8 lanes per particle over its list,
+// Lanes per particle of the SPH pair kernels: 4, 8 (the default) or 16,
+// chosen per structure by gg_tune_sph_lanes.
+template <int L> struct Lanes {
+    static_assert(L == 4 || L == 8 || L == 16, "4, 8 or 16 lanes per particle");
+    static constexpr int shift = L == 4 ? 2 : (L == 8 ? 3 : 4);
+};
+
+// DenDVDX (smoothfcn.c:3390-3627): L lanes per particle over its list,
 // pair arithmetic in T, sums in FP64.
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
 {
-    const int g = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
-    const int sub = threadIdx.x & 7;
+    const int g = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> Lanes<L>::shift;
+    const int sub = threadIdx.x & (L - 1);
     if (g >= A.n) return;
     const int i = g;
     const DRec<T> *__restrict__ rec = static_cast<const DRec<T> *>(A.rec);
@@ -1309,7 +1316,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
     const int64_t o = A.off[i], e1 = A.off[i + 1];
-    for (int64_t e = o + sub; e < e1; e += 8) {
+    for (int64_t e = o + sub; e < e1; e += L) {
         const int q = A.idx[e];
         const DRec<T> Q = rec[q];
         const double dxd = img_sep(x, Q.x, A.per, A.L[0]);
@@ -1351,7 +1358,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
             gx += (double)(uq * dx * rs1); gy += (double)(uq * dy * rs1); gz += (double)(uq * dz * rs1);
         }
     }
-    for (int w = 1; w < 8; w <<= 1) {
+    for (int w = 1; w < L; w <<= 1) {
         rho += __shfl_xor(rho, w);
         qact |= __shfl_xor(qact, w);
         if (A.op == GG_SMX_DENDVDX) {
@@ -1651,15 +1658,15 @@ struct H2Args {
     double *out;     // per slot: ax ay az uDotPdV uDotAV uDotDiff mumax dtNew
 };
 
-// Gather form of the symmetric force pass, 8 lanes per particle i.  Over
+// Gather form of the symmetric force pass, L lanes per particle i.  Over
 // i's own list: the pair as p = i, and, when i lies in the neighbour's ball,
 // the pair as q = i.  Over i's extras (particles whose balls hold i but which
 // are not in i's list): the pair as q = i.
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restrict__ rec)
 {
-    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
-    const int sub = threadIdx.x & 7;
+    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> Lanes<L>::shift;
+    const int sub = threadIdx.x & (L - 1);
     if (i >= A.n) return;
     HPar<T> hp;
     hp.a = (T)A.a;
@@ -1674,7 +1681,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     const double xi = I.x, yi = I.y, zi = I.z;
     HAcc acc;
     if (I.qf > T(0)) {
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 8) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += L) {
             const int q = A.idx[e];
             const HRec<T> Q = rec[q];
             const double xq = Q.x, yq = Q.y, zq = Q.z;
@@ -1694,7 +1701,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     }
     const int64_t e0 = A.eoff ? A.eoff[i] : (int64_t)i * A.ecap;
     const int64_t e1 = A.eoff ? A.eoff[i + 1] : e0 + A.ecnt[i];
-    for (int64_t e = e0 + sub; e < e1; e += 8) {
+    for (int64_t e = e0 + sub; e < e1; e += L) {
         const int p = A.eidx[e];
         const HRec<T> P = rec[p];
         const double ex = img_sep(P.x, xi, A.per, A.L[0]);
@@ -1702,7 +1709,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
         const double ez = img_sep(P.z, zi, A.per, A.L[2]);
         hpair<T>(P, I, (T)ex, (T)ey, (T)ez, (T)sq_dist(ex, ey, ez), 1, hp, acc);
     }
-    for (int w = 1; w < 8; w <<= 1) {
+    for (int w = 1; w < L; w <<= 1) {
         acc.ax += __shfl_xor(acc.ax, w);
         acc.ay += __shfl_xor(acc.ay, w);
         acc.az += __shfl_xor(acc.az, w);
@@ -2349,11 +2356,13 @@ static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, cons
     D.eidx = S.eidx.as<int>();
     D.ecap = ECAP;
     D.err = S.err.as<unsigned>();
-    const int grid = nblk(n * 8, 256);
-    if (pkd->prec == GG_PREC_FP32)
-        hipLaunchKernelGGL(k_dendvdx<float>, dim3(grid), dim3(256), 0, st, D);
-    else
-        hipLaunchKernelGGL(k_dendvdx<double>, dim3(grid), dim3(256), 0, st, D);
+    const int L = S.lpp, grid = (int)(((int64_t)n * L + 255) / 256);
+    const bool f32d = pkd->prec == GG_PREC_FP32;
+#define GG_DDX(T, LL) hipLaunchKernelGGL((k_dendvdx<T, LL>), dim3(grid), dim3(256), 0, st, D)
+    if (L == 4) { if (f32d) GG_DDX(float, 4); else GG_DDX(double, 4); }
+    else if (L == 16) { if (f32d) GG_DDX(float, 16); else GG_DDX(double, 16); }
+    else { if (f32d) GG_DDX(float, 8); else GG_DDX(double, 8); }
+#undef GG_DDX
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -2531,6 +2540,78 @@ extern "C" int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     return GG_OK;
 }
 
+extern "C" int gg_set_sph_lanes(GGPKD pkd, int lanes)
+{
+    GG_CHECK(pkd, GG_EINVAL, "null handle");
+    GG_CHECK(lanes == 4 || lanes == 8 || lanes == 16, GG_EINVAL,
+             "lanes per particle %d is not 4, 8 or 16", lanes);
+    pkd->sph.lpp = lanes;
+    return GG_OK;
+}
+
+extern "C" int gg_tune_sph_lanes(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
+                                 const int *cand, int nCand, double *ms, int *pBest)
+{
+    GG_CHECK(pkd && smf && cand && nCand >= 1, GG_EINVAL, "null argument or no candidate");
+    for (int k = 0; k < nCand; ++k)
+        GG_CHECK(cand[k] == 4 || cand[k] == 8 || cand[k] == 16, GG_EINVAL,
+                 "candidate lanes per particle %d is not 4, 8 or 16", cand[k]);
+    SphDev &S = pkd->sph;
+    GG_CHECK(S.built, GG_ESTATE, "gg_tune_sph_lanes before gg_build_local");
+    const int n = pkd->nLocal;
+    hipStream_t st = pkd->stream;
+    // what the passes accumulate into (or start from): saved once, put back
+    // before every pass
+    const int keep[] = {GG_FBALL2, GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX,
+                        GG_DTNEW, GG_DT};
+    constexpr int NK = sizeof(keep) / sizeof(keep[0]);
+    DevBuf save[NK];
+    size_t wb[NK];
+    int rc = GG_OK;
+    for (int k = 0; k < NK && rc == GG_OK; ++k) {
+        rc = ensure_field(pkd, keep[k]);
+        wb[k] = sizeof(double) * (size_t)field_width(keep[k]) * (size_t)n;
+        if (rc == GG_OK) rc = save[k].reserve(std::max<size_t>(wb[k], 8));
+        if (rc == GG_OK && n > 0 &&
+            hipMemcpyAsync(save[k].p, pkd->f[keep[k]].p, wb[k], hipMemcpyDeviceToDevice, st) !=
+                hipSuccess)
+            rc = GG_EHIP;
+    }
+    const int lpp0 = S.lpp;
+    auto pass = [&](int L) -> int {
+        for (int k = 0; k < NK; ++k)
+            if (n > 0)
+                GG_HIP(hipMemcpyAsync(pkd->f[keep[k]].p, save[k].p, wb[k], hipMemcpyDeviceToDevice,
+                                      st));
+        S.lpp = L;
+        int ns = 0;
+        GG_TRY(gg_smooth(pkd, smf, nSmooth, bPeriodic, 0, GG_SMX_DENDVDX, &ns));
+        GG_TRY(gg_gas_pressure(pkd, smf));
+        return gg_resmooth(pkd, smf, nSmooth, bPeriodic, 1, GG_SMX_SPHPRESSURETERMS);
+    };
+    int best = -1;
+    double bestMs = 0;
+    for (int k = 0; k < nCand && rc == GG_OK; ++k) {
+        for (int rep = 0; rep < 2 && rc == GG_OK; ++rep) rc = pass(cand[k]);
+        if (rc != GG_OK) break;
+        timer_resolve(pkd);
+        const double tk = pkd->tm.ms[4] + pkd->tm.ms[7];   // DenDVDX + force pass
+        if (ms) ms[k] = tk;
+        if (best < 0 || tk < bestMs) {
+            best = k;
+            bestMs = tk;
+        }
+    }
+    if (rc == GG_OK) rc = pass(cand[best]);
+    for (auto &b : save) b.release();
+    if (rc != GG_OK) {
+        S.lpp = lpp0;
+        return rc;
+    }
+    if (pBest) *pBest = cand[best];
+    return GG_OK;
+}
+
 extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
                            int bSymmetric, int iSmoothType)
 {
@@ -2675,14 +2756,15 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.kDiff = smf->dThermalDiffusionCoeff;
     H.bCannonical = smf->bCannonical;
     H.out = S.res.as<double>();
-    const int grid = nblk(n * 8, 256);
+    const int L = S.lpp, grid = (int)(((int64_t)n * L + 255) / 256);
     timer_mark(pkd, 3);
-    if (f32)
-        hipLaunchKernelGGL(k_hydro2<float>, dim3(grid), dim3(256), 0, st, H,
-                           (const HRec<float> *)S.hrec.as<HRec<float>>());
-    else
-        hipLaunchKernelGGL(k_hydro2<double>, dim3(grid), dim3(256), 0, st, H,
-                           (const HRec<double> *)S.hrec.as<HRec<double>>());
+#define GG_H2(T, LL)                                                                          \
+    hipLaunchKernelGGL((k_hydro2<T, LL>), dim3(grid), dim3(256), 0, st, H,                  \
+                       (const HRec<T> *)S.hrec.as<HRec<T>>())
+    if (L == 4) { if (f32) GG_H2(float, 4); else GG_H2(double, 4); }
+    else if (L == 16) { if (f32) GG_H2(float, 16); else GG_H2(double, 16); }
+    else { if (f32) GG_H2(float, 8); else GG_H2(double, 8); }
+#undef GG_H2
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 4);
     const int fl[] = {GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW};

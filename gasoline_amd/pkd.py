@@ -55,7 +55,7 @@ ABI = ["gg_initialize", "gg_finish", "gg_last_error", "gg_set_local", "gg_local"
        "gg_tipsy_header", "gg_read_tipsy", "gg_write_tipsy", "gg_out_array",
        "gg_put", "gg_get", "gg_put_aos", "gg_get_aos", "gg_set_precision", "gg_build_binary", "gg_grav_all",
        "gg_grav_lists", "gg_grav_walk_checksums", "gg_tree_info", "gg_direct_grav",
-       "gg_direct_grav_noalloc", "gg_direct_ewald", "gg_build_local", "gg_smooth", "gg_tune_sph", "gg_resmooth",
+       "gg_direct_grav_noalloc", "gg_direct_ewald", "gg_build_local", "gg_smooth", "gg_tune_sph", "gg_set_sph_lanes", "gg_tune_sph_lanes", "gg_resmooth",
        "gg_gas_pressure", "gg_smooth_lists", "gg_smooth_stats", "gg_timings",
        "gg_build_forest", "gg_grav_graph", "gg_grav_replay", "gg_tune_bucket",
        "gg_set_grav_schedule",
@@ -149,6 +149,8 @@ def lib():
     L.gg_reorder.argtypes = [_vp, _ip]
     L.gg_build_local.argtypes = [_vp, C.c_int, C.c_int, _dp]
     L.gg_tune_sph.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, _dp, _ip, C.c_int, _dp, _ip]
+    L.gg_set_sph_lanes.argtypes = [_vp, C.c_int]
+    L.gg_tune_sph_lanes.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, _ip, C.c_int, _dp, _ip]
     L.gg_smooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int, _ip]
     L.gg_resmooth.argtypes = [_vp, C.POINTER(Smf), C.c_int, C.c_int, C.c_int, C.c_int]
     L.gg_gas_pressure.argtypes = [_vp, C.POINTER(Smf)]
@@ -433,6 +435,20 @@ class Pkd:
         _check(lib().gg_tune_sph(self.h, C.byref(smf), int(nSmooth), int(period is not None),
                                  None if L is None else _p(L), _p(c, _ip), len(c), _p(ms),
                                  C.byref(best)))
+        return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
+    def set_sph_lanes(self, lanes):
+        """gg_set_sph_lanes: 4, 8 or 16 lanes per particle in DenDVDX and
+        the force pass."""
+        _check(lib().gg_set_sph_lanes(self.h, int(lanes)))
+
+    def tune_sph_lanes(self, smf, nSmooth=64, periodic=False, cand=(4, 8, 16)):
+        """gg_tune_sph_lanes: (chosen lanes per particle, {candidate: ms})."""
+        c = np.ascontiguousarray(cand, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int()
+        _check(lib().gg_tune_sph_lanes(self.h, C.byref(smf), int(nSmooth), int(bool(periodic)),
+                                       _p(c, _ip), len(c), _p(ms), C.byref(best)))
         return best.value, {int(k): float(v) for k, v in zip(c, ms)}
 
     def resmooth(self, smf, nSmooth=64, periodic=False, iSmoothType=SMX_SPHPRESSURETERMS):

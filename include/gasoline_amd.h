@@ -339,6 +339,28 @@ int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic,
 int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic, const double *fPeriod,
                 const int *cand, int nCand, double *ms, int *pBest);
 
+/* Lanes per particle of the SPH pair kernels, DenDVDX and the symmetric force
+ * pass (no counterpart in the reference, which runs one particle per thread):
+ * 4, 8 (the default) or 16 lanes share a particle's neighbour list and sum
+ * their parts in a fixed order.  The results agree to FP64 rounding between
+ * settings and are bit-identical between runs at one setting.  GG_EINVAL for
+ * other values. */
+int gg_set_sph_lanes(GGPKD pkd, int lanes);
+
+/* Autotuner of the lanes per particle on the structure already built (no
+ * counterpart in the reference).  For each candidate (4, 8 or 16) it runs
+ * twice, from the caller's fBall2, a, uDot terms, mumax, dtNew and dt each
+ * time:
+ *   gg_smooth(GG_SMX_DENDVDX, not symmetric) + gg_gas_pressure
+ *   + gg_resmooth(GG_SMX_SPHPRESSURETERMS).
+ * It times the DenDVDX and force-pass kernels of the second run and keeps
+ * the fastest.  It finishes with one such sequence at the winner, so every
+ * output is that of the sequence run once at gg_set_sph_lanes(best).
+ * ms[k] (may be NULL) gets candidate k's milliseconds; *pBest gets the
+ * choice. */
+int gg_tune_sph_lanes(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic, const int *cand,
+                      int nCand, double *ms, int *pBest);
+
 /* Ball-gather re-smooth at each particle's stored fBall2 (d2 <= fBall2) and
  * the operator iSmoothType: GG_SMX_SPHPRESSURETERMS (bSymmetric=1,
  * SphPressureTermsSym smoothfcn.c:2276-2385), evaluated in gather form over

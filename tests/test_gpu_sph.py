@@ -461,3 +461,78 @@ def test_sph_bucket_tuner(gpu, box64k):
         put_gas(pk, d)
         with pytest.raises(gpu.GGError, match="outside 1..32"):
             pk.tune_sph(smf, 64, [1.0] * 3, cand=(32, 40))
+
+
+def _sph_step(pk, smf, lanes=None):
+    if lanes is not None:
+        pk.set_sph_lanes(lanes)
+    pk.smooth(smf, 64, True)
+    pk.gas_pressure(smf)
+    pk.resmooth(smf, 64, True)
+
+
+SPH_OUT = ("fBall2", "fDensity", "divv", "a", "uDotPdV", "uDotAV", "mumax", "dtNew", "dt")
+
+
+def test_sph_lanes_tuner(gpu, box64k):
+    # gg_tune_sph_lanes times DenDVDX + the force pass at 4, 8 and 16 lanes
+    # per particle, each run from the caller's state (a is accumulated by the
+    # force pass, dt lowered by the EOS), and leaves every output as one
+    # plain step at the chosen setting
+    d, _ = box64k
+    smf = gpu.Smf.default()
+    n = len(d["m"])
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        pk.put("a", np.full((n, 3), 0.25))
+        pk.put("dt", np.full(n, 1e-3))
+        pk.build_local(period=[1.0] * 3)
+        best, ms = pk.tune_sph_lanes(smf, 64, True, cand=(4, 8, 16))
+        assert set(ms) == {4, 8, 16} and all(v > 0 for v in ms.values())
+        assert ms[best] == min(ms.values())
+        tuned = {f: pk.get(f) for f in SPH_OUT}
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        pk.put("a", np.full((n, 3), 0.25))
+        pk.put("dt", np.full(n, 1e-3))
+        pk.build_local(period=[1.0] * 3)
+        _sph_step(pk, smf, best)
+        for f in SPH_OUT:
+            assert np.array_equal(pk.get(f), tuned[f]), f
+    with gpu.Pkd(n, prec=gpu.PREC_FP64) as pk:
+        put_gas(pk, d)
+        with pytest.raises(gpu.GGError, match="before gg_build_local"):
+            pk.tune_sph_lanes(smf, 64, True)
+        pk.build_local(period=[1.0] * 3)
+        with pytest.raises(gpu.GGError, match="not 4, 8 or 16"):
+            pk.tune_sph_lanes(smf, 64, True, cand=(8, 6))
+        with pytest.raises(gpu.GGError, match="not 4, 8 or 16"):
+            pk.set_sph_lanes(32)
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32"])
+def test_sph_lanes_agree(gpu, box64k, prec):
+    # the lanes per particle change only the order of each particle's sums:
+    # the same neighbour sets, and the outputs at 4 and 16 lanes equal those
+    # at 8 to the rounding of the precision's sums (per particle, against the
+    # field's own scale)
+    d, _ = box64k
+    smf = gpu.Smf.default()
+    n = len(d["m"])
+    p = gpu.PREC_FP64 if prec == "FP64" else gpu.PREC_FP32
+    out = {}
+    for lanes in (8, 4, 16):
+        with gpu.Pkd(n, prec=p) as pk:
+            put_gas(pk, d)
+            pk.build_local(period=[1.0] * 3)
+            _sph_step(pk, smf, lanes)
+            out[lanes] = {f: pk.get(f) for f in SPH_OUT}
+            out[lanes]["lists"] = np.sort(pk.smooth_lists(64), axis=1)
+    tol = 1e-12 if prec == "FP64" else 1e-5
+    for lanes in (4, 16):
+        assert np.array_equal(out[lanes]["lists"], out[8]["lists"])
+        assert np.array_equal(out[lanes]["fBall2"], out[8]["fBall2"])
+        for f in ("fDensity", "divv", "a", "uDotPdV", "uDotAV"):
+            got, ref = out[lanes][f], out[8][f]
+            scale = np.abs(ref).max()
+            assert np.abs(got - ref).max() <= tol * scale, (lanes, f)
