@@ -111,7 +111,7 @@ struct GravDev {
 // SPH state on the device (search structure, lists and per-call scratch).
 struct SphDev {
     int built = 0, n = 0, bPeriodic = 0, nb = 0, nBucket = 0;
-    int lpp = 8;                               // lanes per particle of DenDVDX and the force pass
+    int lpp = 4;                               // lanes per particle of DenDVDX and the force pass
     double L[3] = {0, 0, 0};
     int listK = 0, listValid = 0, listQueries = 0;
     unsigned ties = 0, fails = 0, retried = 0, diag[4] = {0, 0, 0, 0}, passes = 0;

@@ -1287,7 +1287,7 @@ struct DArgs {
     unsigned *err;                           // err[7]: an extras list overflowed
 };
 
-// Lanes per particle of the SPH pair kernels: 4, 8 (the default) or 16,
+// Lanes per particle of the SPH pair kernels: 4 (the default), 8 or 16,
 // chosen per structure by gg_tune_sph_lanes.
 template <int L> struct Lanes {
     static_assert(L == 4 || L == 8 || L == 16, "4, 8 or 16 lanes per particle");
@@ -1425,12 +1425,13 @@ struct DSArgs {
 // list with h_i and m_q, then over the owners q whose lists hold i (sorted by
 // slot, so the order is fixed) the half-normalised kernel with h_q and m_q,
 // on top of 0 (smooth-active i, initDensity smoothfcn.c:229-232) or of the
-// density i held.  8 lanes per particle; pair arithmetic in T, sums in FP64.
-template <typename T>
+// density i held.  L lanes per particle, as DenDVDX; pair arithmetic in T,
+// sums in FP64.
+template <typename T, int L>
 __global__ __launch_bounds__(256) void k_density(DSArgs A)
 {
-    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> 3;
-    const int sub = threadIdx.x & 7;
+    const int i = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) >> Lanes<L>::shift;
+    const int sub = threadIdx.x & (L - 1);
     if (i >= A.n) return;
     const bool own = A.qflag[i] != 0;
     if (!A.sym && !own) return;
@@ -1438,7 +1439,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
     double so = 0, sr = 0, ih2 = 0;
     if (own) {
         ih2 = 4.0 / A.fb2[i];
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += 8) {
+        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += L) {
             const int q = A.idx[e];
             const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
                                       img_sep(y, A.sy[q], A.per, A.L[1]),
@@ -1447,7 +1448,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
         }
     }
     if (A.sym) {
-        for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += 8) {
+        for (int64_t e = A.roff[i] + sub; e < A.roff[i + 1]; e += L) {
             const int q = A.ridx[e];
             const double d2 = sq_dist(img_sep(x, A.sx[q], A.per, A.L[0]),
                                       img_sep(y, A.sy[q], A.per, A.L[1]),
@@ -1457,7 +1458,7 @@ __global__ __launch_bounds__(256) void k_density(DSArgs A)
             sr += fq * (double)(m4_w<T>((T)(d2 * ih2q)) * (T)A.m[q]);
         }
     }
-    for (int w = 1; w < 8; w <<= 1) {
+    for (int w = 1; w < L; w <<= 1) {
         so += __shfl_xor(so, w);
         sr += __shfl_xor(sr, w);
     }
@@ -2276,11 +2277,13 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
         D.rho0 = S.rho.as<double>();
     }
     timer_mark(pkd, 4);
-    const int grid = nblk(n * 8, 256);
-    if (pkd->prec == GG_PREC_FP32)
-        hipLaunchKernelGGL(k_density<float>, dim3(grid), dim3(256), 0, st, D);
-    else
-        hipLaunchKernelGGL(k_density<double>, dim3(grid), dim3(256), 0, st, D);
+    const int L = S.lpp, grid = (int)(((int64_t)n * L + 255) / 256);
+    const bool f32d = pkd->prec == GG_PREC_FP32;
+#define GG_DEN(T, LL) hipLaunchKernelGGL((k_density<T, LL>), dim3(grid), dim3(256), 0, st, D)
+    if (L == 4) { if (f32d) GG_DEN(float, 4); else GG_DEN(double, 4); }
+    else if (L == 16) { if (f32d) GG_DEN(float, 16); else GG_DEN(double, 16); }
+    else { if (f32d) GG_DEN(float, 8); else GG_DEN(double, 8); }
+#undef GG_DEN
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 5);
     unsigned e[12];

@@ -341,7 +341,7 @@ int gg_tune_sph(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodic, const 
 
 /* Lanes per particle of the SPH pair kernels, DenDVDX and the symmetric force
  * pass (no counterpart in the reference, which runs one particle per thread):
- * 4, 8 (the default) or 16 lanes share a particle's neighbour list and sum
+ * 4 (the default), 8 or 16 lanes share a particle's neighbour list and sum
  * their parts in a fixed order.  The results agree to FP64 rounding between
  * settings and are bit-identical between runs at one setting.  GG_EINVAL for
  * other values. */
