@@ -141,9 +141,35 @@ __global__ void k_bbox(int n, const int *__restrict__ gidx, const double *__rest
         for (int d = 0; d < 6; ++d) part[6 * (size_t)blockIdx.x + d] = s[d][0];
 }
 
+// The Morton grid from k_bbox's nbb partial boxes (one wavefront): out[0..2]
+// the lower corner, out[3] the scale 65535 / the largest extent (1 for a
+// point), so the keys' quantisation never leaves the device.
+__global__ void k_bbox_final(int nbb, const double *__restrict__ part, double *__restrict__ out)
+{
+    const int l = threadIdx.x;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = l; b < nbb; b += 64)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], part[6 * (size_t)b + d]);
+            hi[d] = fmax(hi[d], part[6 * (size_t)b + 3 + d]);
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+        }
+    if (l == 0) {
+        double ext = 0;
+        for (int d = 0; d < 3; ++d) ext = fmax(ext, hi[d] - lo[d]);
+        if (!(ext > 0)) ext = 1.0;
+        for (int d = 0; d < 3; ++d) out[d] = lo[d];
+        out[3] = 65535.0 / ext;
+    }
+}
+
 // The one bucket of slots [0, n), n <= 64 (one wavefront): its box, the box
 // in FP32 rounded outwards, and its density over the quantisation cube.
-__global__ void k_bbox_slots(int n, double sc, const double *__restrict__ sx,
+__global__ void k_bbox_slots(int n, const double *__restrict__ grid, const double *__restrict__ sx,
                              const double *__restrict__ sy, const double *__restrict__ sz,
                              double *__restrict__ box, float *__restrict__ fbox,
                              double *__restrict__ den)
@@ -167,6 +193,7 @@ __global__ void k_bbox_slots(int n, double sc, const double *__restrict__ sx,
             fbox[d] = f_down(v[d]);
             fbox[3 + d] = f_up(u[d]);
         }
+        const double sc = grid[3];
         den[0] = ldexp((double)n * sc * sc * sc, -48);
     }
 }
@@ -183,13 +210,13 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v)
 }
 
 __global__ void k_morton(int n, const int *__restrict__ gidx, const double *__restrict__ r,
-                         double lx, double ly, double lz, double sx, double sy, double sz,
-                         uint64_t *__restrict__ key, int *__restrict__ val)
+                         const double *__restrict__ grid, uint64_t *__restrict__ key,
+                         int *__restrict__ val)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int i = gidx[k];
-    const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
+    const double lo[3] = {grid[0], grid[1], grid[2]}, sc[3] = {grid[3], grid[3], grid[3]};
     uint64_t q[3];
     for (int d = 0; d < 3; ++d) {
         double t = (r[3 * (size_t)i + d] - lo[d]) * sc[d];
@@ -301,7 +328,7 @@ __global__ void k_bucket_mark(int n, int bkt, const uint64_t *__restrict__ key,
 // parent's particle count over the volume of its Morton cell (a key prefix
 // of p bits spans 2^(48-p) quantisation cells of volume 1/sc^3).
 __global__ void k_bucket_write(int n, const int *__restrict__ flag, const int *__restrict__ cnt,
-                               const int2 *__restrict__ par, double sc,
+                               const int2 *__restrict__ par, const double *__restrict__ grid,
                                const int64_t *__restrict__ pos, const double *__restrict__ sx,
                                const double *__restrict__ sy, const double *__restrict__ sz,
                                int *__restrict__ bfirst, int *__restrict__ bcnt,
@@ -312,6 +339,7 @@ __global__ void k_bucket_write(int n, const int *__restrict__ flag, const int *_
     if (i >= n || !flag[i]) return;
     const int b = (int)pos[i], m = cnt[i];
     const int2 pp = par[i];
+    const double sc = grid[3];
     bden[b] = ldexp((double)pp.x * sc * sc * sc, pp.y - 48);
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int k = i; k < i + m; ++k) {
@@ -2091,29 +2119,19 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     hipLaunchKernelGGL(k_compact, dim3(nblk(N, 256)), dim3(256), 0, st, N, S.sel.as<int>(),
                        S.pos.as<int64_t>(), S.gidx.as<int>());
     const int nbb = std::min(nblk(n, 256), 1024);
-    GG_TRY(S.part.reserve(sizeof(double) * 6 * nbb));
+    GG_TRY(S.part.reserve(sizeof(double) * (6 * (size_t)nbb + 4)));
     hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, st, n, S.gidx.as<int>(),
                        pkd->f[GG_R].as<double>(), S.part.as<double>());
-    std::vector<double> part(6 * (size_t)nbb);
-    GG_HIP(hipMemcpyAsync(part.data(), S.part.p, sizeof(double) * 6 * nbb, hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int b = 0; b < nbb; ++b)
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = std::min(lo[d], part[6 * (size_t)b + d]);
-            hi[d] = std::max(hi[d], part[6 * (size_t)b + 3 + d]);
-        }
-    double ext = 0;
-    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
-    if (!(ext > 0)) ext = 1.0;
-    const double sc = 65535.0 / ext;
+    // the Morton grid (lower corner, scale) stays on the device, after the partials
+    double *grid = S.part.as<double>() + 6 * (size_t)nbb;
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, nbb, S.part.as<double>(), grid);
     GG_TRY(S.key.reserve(sizeof(uint64_t) * n));
     GG_TRY(S.order.reserve(sizeof(int) * n));
     GG_TRY(S.k2.reserve(sizeof(uint64_t) * n));
     GG_TRY(S.v2.reserve(sizeof(int) * n));
     hipLaunchKernelGGL(k_morton, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.gidx.as<int>(),
-                       pkd->f[GG_R].as<double>(), lo[0], lo[1], lo[2], sc, sc, sc,
-                       S.k2.as<uint64_t>(), S.v2.as<int>());
+                       pkd->f[GG_R].as<double>(), (const double *)grid, S.k2.as<uint64_t>(),
+                       S.v2.as<int>());
     size_t tmpb = 0;
     GG_HIP(rocprim::radix_sort_pairs(nullptr, tmpb, S.k2.as<uint64_t>(), S.key.as<uint64_t>(),
                                      S.v2.as<int>(), S.order.as<int>(), n, 0, 48, st));
@@ -2155,7 +2173,8 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         float *sp = S.sparse.as<float>();
         GG_TRY(S.bden.reserve(sizeof(double) * nb));
         hipLaunchKernelGGL(k_bucket_write, dim3(nblk(n, 256)), dim3(256), 0, st, n,
-                           S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>(), sc,
+                           S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>(),
+                           (const double *)grid,
                            S.pos.as<int64_t>(), S.sx.as<double>(), S.sy.as<double>(),
                            S.sz.as<double>(), S.bfirst.as<int>(), S.bcnt.as<int>(),
                            S.bbox.as<double>(), sp, S.bden.as<double>(), S.bof.as<int>());
@@ -2177,7 +2196,8 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         const int one[2] = {0, n};
         GG_HIP(hipMemcpyAsync(S.bfirst.p, &one[0], sizeof(int), hipMemcpyHostToDevice, st));
         GG_HIP(hipMemcpyAsync(S.bcnt.p, &one[1], sizeof(int), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_bbox_slots, dim3(1), dim3(64), 0, st, n, sc, S.sx.as<double>(),
+        hipLaunchKernelGGL(k_bbox_slots, dim3(1), dim3(64), 0, st, n, (const double *)grid,
+                           S.sx.as<double>(),
                            S.sy.as<double>(), S.sz.as<double>(), S.bbox.as<double>(),
                            S.sparse.as<float>(), S.bden.as<double>());
     }
