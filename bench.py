@@ -597,14 +597,15 @@ def main():
     # roofline of each SPH kernel: algorithmic bytes per launch over its
     # HIP-event duration on the handle's stream (the headline FP64 step; the
     # kNN is the same kernel in both precisions); the dominant one is reported
+    L64, L32 = tune64.get("lanes", 4), tune64.get("fp32_lanes", 4)   # the tuners' lanes
     kern = {
         "k_knn_bucket": (ph64["knn"], n * (KNN_BYTES_PER_QUERY + KNN_BYTES_PER_NEIGHBOUR * K)),
-        "k_dendvdx<double>": (ph64["density"], n * K * DENS_BYTES_PER_PAIR),
-        "k_hydro2<double>": (ph64["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
+        "k_dendvdx<double,%d>" % L64: (ph64["density"], n * K * DENS_BYTES_PER_PAIR),
+        "k_hydro2<double,%d>" % L64: (ph64["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
     }
     kern32 = {
-        "k_dendvdx<float>": (ph32["density"], n * K * DENS_BYTES_PER_PAIR),
-        "k_hydro2<float>": (ph32["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
+        "k_dendvdx<float,%d>" % L32: (ph32["density"], n * K * DENS_BYTES_PER_PAIR),
+        "k_hydro2<float,%d>" % L32: (ph32["hydro"], n * K * HYDRO_BYTES_PER_PAIR),
     }
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
