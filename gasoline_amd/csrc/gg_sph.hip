@@ -1344,8 +1344,12 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
     const int64_t o = A.off[i], e1 = A.off[i + 1];
+    // the next neighbour's index is loaded with this neighbour's record, so
+    // each step waits for one load instead of two in a row
+    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
     for (int64_t e = o + sub; e < e1; e += L) {
-        const int q = A.idx[e];
+        const int q = qn;
+        if (e + L < e1) qn = A.idx[e + L];
         const DRec<T> Q = rec[q];
         const double dxd = img_sep(x, Q.x, A.per, A.L[0]);
         const double dyd = img_sep(y, Q.y, A.per, A.L[1]);
@@ -1710,8 +1714,11 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     const double xi = I.x, yi = I.y, zi = I.z;
     HAcc acc;
     if (I.qf > T(0)) {
-        for (int64_t e = A.off[i] + sub; e < A.off[i + 1]; e += L) {
-            const int q = A.idx[e];
+        const int64_t o = A.off[i], e1 = A.off[i + 1];
+        int qn = o + sub < e1 ? A.idx[o + sub] : 0;   // next index in flight, as in DenDVDX
+        for (int64_t e = o + sub; e < e1; e += L) {
+            const int q = qn;
+            if (e + L < e1) qn = A.idx[e + L];
             const HRec<T> Q = rec[q];
             const double xq = Q.x, yq = Q.y, zq = Q.z;
             const double dx = img_sep(xi, xq, A.per, A.L[0]);
