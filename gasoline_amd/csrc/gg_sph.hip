@@ -1344,13 +1344,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
     const int64_t o = A.off[i], e1 = A.off[i + 1];
-    // the next neighbour's index is loaded with this neighbour's record, so
-    // each step waits for one load instead of two in a row
-    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
-    for (int64_t e = o + sub; e < e1; e += L) {
-        const int q = qn;
-        if (e + L < e1) qn = A.idx[e + L];
-        const DRec<T> Q = rec[q];
+    auto pair = [&](int q, const DRec<T> &Q) {
         const double dxd = img_sep(x, Q.x, A.per, A.L[0]);
         const double dyd = img_sep(y, Q.y, A.per, A.L[1]);
         const double dzd = img_sep(z, Q.z, A.per, A.L[2]);
@@ -1376,7 +1370,7 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
                 else atomicOr(&A.err[7], 1u);
             }
         }
-    rho += (double)(m4_w<T>(r2) * mq);
+        rho += (double)(m4_w<T>(r2) * mq);
         if (A.op == GG_SMX_DENDVDX) {
             const T rs1 = m4_dw<T>(r2) * mq;
             const T dvx = (vxq - vxi) * vFac;
@@ -1388,6 +1382,31 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
             zx += (double)(dvz * dx * rs1); zy += (double)(dvz * dy * rs1); zz += (double)(dvz * dz * rs1);
             dn += (double)((dx * dx + dy * dy + dz * dz) * rs1);
             gx += (double)(uq * dx * rs1); gy += (double)(uq * dy * rs1); gz += (double)(uq * dz * rs1);
+        }
+    };
+    if constexpr (sizeof(T) == 8) {
+        // FP64: the next neighbour's record is loaded while this one is used,
+        // and the index after it with it (registers allow it at 4 waves)
+        int qn = o + sub < e1 ? A.idx[o + sub] : 0;
+        DRec<T> Qn;
+        if (o + sub < e1) Qn = rec[qn];
+        int qnn = o + sub + L < e1 ? A.idx[o + sub + L] : 0;
+        for (int64_t e = o + sub; e < e1; e += L) {
+            const int q = qn;
+            const DRec<T> Q = Qn;
+            qn = qnn;
+            if (e + L < e1) Qn = rec[qn];
+            if (e + 2 * L < e1) qnn = A.idx[e + 2 * L];
+            pair(q, Q);
+        }
+    } else {
+        // the next neighbour's index is loaded with this neighbour's record,
+        // so each step waits for one load instead of two in a row
+        int qn = o + sub < e1 ? A.idx[o + sub] : 0;
+        for (int64_t e = o + sub; e < e1; e += L) {
+            const int q = qn;
+            if (e + L < e1) qn = A.idx[e + L];
+            pair(q, rec[q]);
         }
     }
     for (int w = 1; w < L; w <<= 1) {
