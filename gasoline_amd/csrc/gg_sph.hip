@@ -327,50 +327,84 @@ __global__ void k_bucket_mark(int n, int bkt, const uint64_t *__restrict__ key,
 // level 0), the bucket of each of its slots, and a number density: the
 // parent's particle count over the volume of its Morton cell (a key prefix
 // of p bits spans 2^(48-p) quantisation cells of volume 1/sc^3).
-__global__ void k_bucket_write(int n, const int *__restrict__ flag, const int *__restrict__ cnt,
+__global__ void k_bucket_start(int n, const int *__restrict__ flag, const int *__restrict__ cnt,
                                const int2 *__restrict__ par, const double *__restrict__ grid,
-                               const int64_t *__restrict__ pos, const double *__restrict__ sx,
-                               const double *__restrict__ sy, const double *__restrict__ sz,
-                               int *__restrict__ bfirst, int *__restrict__ bcnt,
-                               double *__restrict__ bbox, float *__restrict__ fbox,
-                               double *__restrict__ bden, int *__restrict__ bof)
+                               const int64_t *__restrict__ pos, int *__restrict__ bfirst,
+                               int *__restrict__ bcnt, double *__restrict__ bden,
+                               int *__restrict__ bof)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    const int b = (int)pos[i], m = cnt[i];
-    const int2 pp = par[i];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int f = flag[k];
+    const int b = (int)pos[k] + f - 1;   // the buckets starting at or before k, less one
+    bof[k] = b;
+    if (!f) return;
+    const int2 pp = par[k];
     const double sc = grid[3];
     bden[b] = ldexp((double)pp.x * sc * sc * sc, pp.y - 48);
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int k = i; k < i + m; ++k) {
-        const double q[3] = {sx[k], sy[k], sz[k]};
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], q[d]);
-            hi[d] = fmax(hi[d], q[d]);
-        }
-        bof[k] = b;
-    }
-    bfirst[b] = i;
-    bcnt[b] = m;
-    for (int d = 0; d < 3; ++d) {
-        bbox[6 * (size_t)b + d] = lo[d];
-        bbox[6 * (size_t)b + 3 + d] = hi[d];
-        fbox[6 * (size_t)b + d] = f_down(lo[d]);
-        fbox[6 * (size_t)b + 3 + d] = f_up(hi[d]);
-    }
+    bfirst[b] = k;
+    bcnt[b] = cnt[k];
 }
 
-// Sparse table level k: union of the two level k-1 boxes it spans.
-__global__ void k_sparse_level(int nb, int half, const float *__restrict__ prev,
-                               float *__restrict__ cur)
+// Each bucket's box (FP64, and FP32 rounded outwards), 32 lanes per bucket
+// (a bucket holds at most 32 slots), the slots' positions read coalesced.
+__global__ __launch_bounds__(256) void k_bucket_box(int nb, const int *__restrict__ bfirst,
+                                                    const int *__restrict__ bcnt,
+                                                    const double *__restrict__ sx,
+                                                    const double *__restrict__ sy,
+                                                    const double *__restrict__ sz,
+                                                    double *__restrict__ bbox,
+                                                    float *__restrict__ fbox)
+{
+    const int b = (blockIdx.x * blockDim.x + threadIdx.x) >> 5, l = threadIdx.x & 31;
+    if (b >= nb) return;
+    const int f = bfirst[b], m = bcnt[b];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    if (l < m) {
+        lo[0] = hi[0] = sx[f + l];
+        lo[1] = hi[1] = sy[f + l];
+        lo[2] = hi[2] = sz[f + l];
+    }
+    for (int o = 16; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+        }
+    if (l == 0)
+        for (int d = 0; d < 3; ++d) {
+            bbox[6 * (size_t)b + d] = lo[d];
+            bbox[6 * (size_t)b + 3 + d] = hi[d];
+            fbox[6 * (size_t)b + d] = f_down(lo[d]);
+            fbox[6 * (size_t)b + 3 + d] = f_up(hi[d]);
+        }
+}
+
+// Sparse table levels lv0 + 1 .. lv0 + nl (nl <= 4) from level lv0: level
+// lv0 + j at b is the union of the 2^j level-lv0 boxes at b + m 2^lv0 (the
+// same unions the pairwise construction forms, min and max being exact),
+// stored where b + 2^(lv0 + j) <= nb.
+__global__ void k_sparse_levels(int nb, int lv0, int nl, float *__restrict__ sp)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b + 2 * half > nb) return;
-    const float *a = prev + 6 * (size_t)b, *c = prev + 6 * (size_t)(b + half);
-    float *o = cur + 6 * (size_t)b;
-    for (int d = 0; d < 3; ++d) {
-        o[d] = fminf(a[d], c[d]);
-        o[3 + d] = fmaxf(a[3 + d], c[3 + d]);
+    const int w = 1 << lv0;
+    if (b + 2 * w > nb) return;
+    const float *src = sp + 6 * (size_t)lv0 * nb;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int m = 0; m < (1 << nl); ++m) {
+        if (b + (m + 1) * w > nb) break;
+        const float *t = src + 6 * ((size_t)b + (size_t)m * w);
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fminf(lo[d], t[d]);
+            hi[d] = fmaxf(hi[d], t[3 + d]);
+        }
+        const int j = 31 - __clz(m + 1);   // m + 1 = 2^j: level lv0 + j is complete
+        if (m > 0 && (m + 1) == (1 << j) && b + (m + 1) * w <= nb) {
+            float *o = sp + 6 * ((size_t)(lv0 + j) * nb + b);
+            for (int d = 0; d < 3; ++d) {
+                o[d] = lo[d];
+                o[3 + d] = hi[d];
+            }
+        }
     }
 }
 
@@ -2198,16 +2232,16 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
         GG_TRY(S.sparse.reserve(sizeof(float) * 6 * (size_t)nb * nlev));
         float *sp = S.sparse.as<float>();
         GG_TRY(S.bden.reserve(sizeof(double) * nb));
-        hipLaunchKernelGGL(k_bucket_write, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+        hipLaunchKernelGGL(k_bucket_start, dim3(nblk(n, 256)), dim3(256), 0, st, n,
                            S.bflag.as<int>(), S.bcode.as<int>(), S.bpar.as<int2>(),
-                           (const double *)grid,
-                           S.pos.as<int64_t>(), S.sx.as<double>(), S.sy.as<double>(),
-                           S.sz.as<double>(), S.bfirst.as<int>(), S.bcnt.as<int>(),
-                           S.bbox.as<double>(), sp, S.bden.as<double>(), S.bof.as<int>());
-        for (int lv = 1; lv < nlev; ++lv)
-            hipLaunchKernelGGL(k_sparse_level, dim3(nblk(nb, 256)), dim3(256), 0, st, nb,
-                               1 << (lv - 1), sp + 6 * (size_t)(lv - 1) * nb,
-                               sp + 6 * (size_t)lv * nb);
+                           (const double *)grid, S.pos.as<int64_t>(), S.bfirst.as<int>(),
+                           S.bcnt.as<int>(), S.bden.as<double>(), S.bof.as<int>());
+        hipLaunchKernelGGL(k_bucket_box, dim3((int)(((int64_t)nb * 32 + 255) / 256)), dim3(256), 0, st, nb,
+                           S.bfirst.as<int>(), S.bcnt.as<int>(), S.sx.as<double>(),
+                           S.sy.as<double>(), S.sz.as<double>(), S.bbox.as<double>(), sp);
+        for (int lv = 0; lv + 1 < nlev; lv += 4)
+            hipLaunchKernelGGL(k_sparse_levels, dim3(nblk(nb, 256)), dim3(256), 0, st, nb, lv,
+                               std::min(4, nlev - 1 - lv), sp);
         GG_TRY(S.crec.reserve(sizeof(CRec) * 2 * (size_t)(n - 1)));
         hipLaunchKernelGGL(k_child_rec, dim3(nblk(2 * (n - 1), 256)), dim3(256), 0, st, n, nb,
                            nBucket, S.child.as<int>(), S.rng.as<int>(), S.bof.as<int>(), sp,
