@@ -118,6 +118,7 @@ struct SphDev {
     int64_t cand = 0;                          // kNN candidates examined
     double seedSlack = 1.15;                   // first trial ball / fBall2 hint
     int gasMaskValid = 0;                      // gasMask holds whether any particle is TYPE_GAS
+    int allSelected = 0;                       // with it: every particle goes into the structure
     uint32_t gasMask = 0;
     int64_t nExtra = 0;
     int nExtraDev = 0;                         // > 0: nExtra still in that many partials in nxd

@@ -1850,18 +1850,28 @@ __global__ __launch_bounds__(256) void k_extra(int n, int mode, int per, double 
 }
 
 // Bitwise OR of the selection bits over all particles (is any gas flagged?).
+__global__ void k_iota(int n, int *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// out[0] |= the bits of mask some particle has, out[1] |= those some lacks.
 __global__ void k_any_bits(int n, const uint32_t *__restrict__ bits, uint32_t mask,
                            unsigned *__restrict__ out)
 {
-    __shared__ unsigned s;
-    if (threadIdx.x == 0) s = 0;
+    __shared__ unsigned s[2];
+    if (threadIdx.x < 2) s[threadIdx.x] = 0;
     __syncthreads();
-    unsigned v = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    unsigned v = 0, w = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v |= bits[i] & mask;
-    if (v) atomicOr(&s, v);
+        w |= ~bits[i] & mask;
+    }
+    if (v) atomicOr(&s[0], v);
+    if (w) atomicOr(&s[1], w);
     __syncthreads();
-    if (threadIdx.x == 0 && s) atomicOr(out, s);
+    if (threadIdx.x < 2 && s[threadIdx.x]) atomicOr(&out[threadIdx.x], s[threadIdx.x]);
 }
 
 // ------------------------------------------------------- list plumbing
@@ -2056,16 +2066,18 @@ static int gas_mask(GGPKD pkd, uint32_t *mask)
         *mask = S.gasMask;
         return GG_OK;
     }
-    GG_TRY(S.diffcnt.reserve(sizeof(unsigned)));
-    GG_HIP(hipMemsetAsync(S.diffcnt.p, 0, sizeof(unsigned), pkd->stream));
+    GG_TRY(S.diffcnt.reserve(2 * sizeof(unsigned)));
+    GG_HIP(hipMemsetAsync(S.diffcnt.p, 0, 2 * sizeof(unsigned), pkd->stream));
     hipLaunchKernelGGL(k_any_bits, dim3(std::max(1, std::min(nblk(n, 256), 1024))), dim3(256), 0,
                        pkd->stream, n, pkd->f[GG_IACTIVE].as<uint32_t>(), (uint32_t)GG_TYPE_GAS,
                        S.diffcnt.as<unsigned>());
-    unsigned any = 0;
-    GG_HIP(hipMemcpyAsync(&any, S.diffcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, pkd->stream));
+    unsigned any[2] = {0, 0};
+    GG_HIP(hipMemcpyAsync(any, S.diffcnt.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                          pkd->stream));
     GG_HIP(hipStreamSynchronize(pkd->stream));
-    *mask = any ? GG_TYPE_GAS : 0;
+    *mask = any[0] ? GG_TYPE_GAS : 0;
     S.gasMask = *mask;
+    S.allSelected = !any[0] || !any[1];   // no gas (all taken), or all gas
     S.gasMaskValid = 1;
     return GG_OK;
 }
@@ -2170,14 +2182,19 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
     GG_TRY(S.sel.reserve(sizeof(int) * N));
     GG_TRY(S.pos.reserve(sizeof(int64_t) * (N + 1)));
     GG_TRY(S.gidx.reserve(sizeof(int) * N));
-    hipLaunchKernelGGL(k_select, dim3(nblk(N, 256)), dim3(256), 0, st, N,
-                       pkd->f[GG_IACTIVE].as<uint32_t>(), mask, S.sel.as<int>());
-    int64_t ng = 0;
-    GG_TRY(exclusive_scan_i64(st, S.sel.as<int>(), S.pos.as<int64_t>(), N, S.sortTmp, &ng));
+    int64_t ng = N;
+    if (S.allSelected) {   // every particle: no selection to count
+        hipLaunchKernelGGL(k_iota, dim3(nblk(N, 256)), dim3(256), 0, st, N, S.gidx.as<int>());
+    } else {
+        hipLaunchKernelGGL(k_select, dim3(nblk(N, 256)), dim3(256), 0, st, N,
+                           pkd->f[GG_IACTIVE].as<uint32_t>(), mask, S.sel.as<int>());
+        GG_TRY(exclusive_scan_i64(st, S.sel.as<int>(), S.pos.as<int64_t>(), N, S.sortTmp, &ng));
+    }
     const int n = (int)ng;
     GG_CHECK(n > 0, GG_EINVAL, "no particles for the SPH structure");
-    hipLaunchKernelGGL(k_compact, dim3(nblk(N, 256)), dim3(256), 0, st, N, S.sel.as<int>(),
-                       S.pos.as<int64_t>(), S.gidx.as<int>());
+    if (!S.allSelected)
+        hipLaunchKernelGGL(k_compact, dim3(nblk(N, 256)), dim3(256), 0, st, N, S.sel.as<int>(),
+                           S.pos.as<int64_t>(), S.gidx.as<int>());
     const int nbb = std::min(nblk(n, 256), 1024);
     GG_TRY(S.part.reserve(sizeof(double) * (6 * (size_t)nbb + 4)));
     hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, st, n, S.gidx.as<int>(),
@@ -2925,11 +2942,6 @@ __global__ void k_permute(int n, int w, int np, const int *__restrict__ perm,
             dst[(size_t)p * w * n + (size_t)w * k + c] = src[(size_t)p * w * n + o + c];
 }
 
-__global__ void k_iota(int n, int *__restrict__ p)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = i;
-}
 
 // The permuted copy goes to a spare buffer of the same capacity, which then
 // takes the old one's place: spares are kept per capacity, so after the first
