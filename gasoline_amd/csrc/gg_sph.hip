@@ -1418,30 +1418,19 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
             gx += (double)(uq * dx * rs1); gy += (double)(uq * dy * rs1); gz += (double)(uq * dz * rs1);
         }
     };
-    if constexpr (sizeof(T) == 8) {
-        // FP64: the next neighbour's record is loaded while this one is used,
-        // and the index after it with it (registers allow it at 4 waves)
-        int qn = o + sub < e1 ? A.idx[o + sub] : 0;
-        DRec<T> Qn;
-        if (o + sub < e1) Qn = rec[qn];
-        int qnn = o + sub + L < e1 ? A.idx[o + sub + L] : 0;
-        for (int64_t e = o + sub; e < e1; e += L) {
-            const int q = qn;
-            const DRec<T> Q = Qn;
-            qn = qnn;
-            if (e + L < e1) Qn = rec[qn];
-            if (e + 2 * L < e1) qnn = A.idx[e + 2 * L];
-            pair(q, Q);
-        }
-    } else {
-        // the next neighbour's index is loaded with this neighbour's record,
-        // so each step waits for one load instead of two in a row
-        int qn = o + sub < e1 ? A.idx[o + sub] : 0;
-        for (int64_t e = o + sub; e < e1; e += L) {
-            const int q = qn;
-            if (e + L < e1) qn = A.idx[e + L];
-            pair(q, rec[q]);
-        }
+    // the next neighbour's record is loaded while this one is used, and the
+    // index after it with it, so no step waits on a load issued in it
+    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
+    DRec<T> Qn;
+    if (o + sub < e1) Qn = rec[qn];
+    int qnn = o + sub + L < e1 ? A.idx[o + sub + L] : 0;
+    for (int64_t e = o + sub; e < e1; e += L) {
+        const int q = qn;
+        const DRec<T> Q = Qn;
+        qn = qnn;
+        if (e + L < e1) Qn = rec[qn];
+        if (e + 2 * L < e1) qnn = A.idx[e + 2 * L];
+        pair(q, Q);
     }
     for (int w = 1; w < L; w <<= 1) {
         rho += __shfl_xor(rho, w);
