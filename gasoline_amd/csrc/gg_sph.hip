@@ -463,6 +463,8 @@ struct QArgs {
     unsigned int *err;                 // [0] failures, [1] boundary ties, [10] kNN retries
     const int *qlist;                  // optional: block b serves slot qlist[b]
     const unsigned *nq;                // optional, with qlist: its length, on the device
+    const double *grl2;                // optional, kNN retries: the bucket groups' radii^2
+    const int *gnb;                    //   and sizes (-1: overflowed)
 };
 
 // Squared radius of a ball expected to hold 2K particles at number density
@@ -655,6 +657,12 @@ __device__ void query_one(const QArgs &A, int q, QLds &S, int lane)
         R2 = A.guess[q] * A.slack;
     } else {
         R2 = cold_r2(A.bden[A.bof[q]], K);
+    }
+    if (A.mode == 0 && A.grl2) {
+        // a retry whose bucket group was whole: the group radius held too
+        // few (or no settling radius) for it, so start beyond it
+        const int b = A.bof[q];
+        if (A.gnb[b] >= 0) R2 = fmax(R2, 1.6 * A.grl2[b]);
     }
     // kNN: lo2 < R2 < hi2 brackets the radii that hold K..CAP particles (lo2:
     // the largest tried that held fewer than K; hi2: the smallest that held
@@ -2119,6 +2127,8 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     Q.err = S.err.as<unsigned>();
     Q.qlist = nullptr;
     Q.nq = nullptr;
+    Q.grl2 = nullptr;
+    Q.gnb = nullptr;
     return Q;
 }
 
@@ -2528,6 +2538,8 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
         Q.nbr = S.nbr.as<int>();
         Q.qlist = S.retry.as<int>();
         Q.nq = KL.nretry;
+        Q.grl2 = KL.grl2;
+        Q.gnb = KL.gnb;
         hipLaunchKernelGGL(k_query, dim3(std::min(n, 4096)), dim3(64), 0, st, Q);
         GG_HIP(hipGetLastError());
     }
