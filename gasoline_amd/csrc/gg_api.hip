@@ -282,7 +282,9 @@ extern "C" int gg_put(GGPKD pkd, int field, const void *src, int bDevice)
         GG_HIP(hipMemcpyAsync(pkd->f[field].p, src, nb,
                               bDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               pkd->stream));
-    GG_HIP(hipStreamSynchronize(pkd->stream));
+    // a host source is free again on return; a device copy stays queued on
+    // the handle's stream, ahead of everything the handle does next
+    if (!bDevice) GG_HIP(hipStreamSynchronize(pkd->stream));
     put_done(pkd, field);
     return GG_OK;
 }

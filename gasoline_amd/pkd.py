@@ -240,6 +240,10 @@ class Pkd:
         """Copy a field in, from a numpy array or a device tensor (torch)."""
         fid, w = FIELDS[name]
         if hasattr(arr, "data_ptr"):             # device tensor
+            import torch
+            # the handle's stream does not wait for torch's: the tensor's
+            # values must be written before the copy is queued
+            torch.cuda.current_stream(arr.device).synchronize()
             _check(lib().gg_put(self.h, fid, _vp(arr.data_ptr()), 1))
             return
         dt = _INT_FIELDS.get(name, np.float64)

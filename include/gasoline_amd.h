@@ -123,6 +123,12 @@ int gg_local(GGPKD pkd);
 
 /* Copy a field in (gg_put) or out (gg_get), in the caller's particle order.
  * src/dst are host pointers when bDevice == 0, device pointers otherwise.
+ * gg_put from a host source returns with the copy done.  From a device
+ * source, it queues the copy on the handle's stream, ahead of the handle's
+ * later work: the source must hold its values when gg_put is called (work
+ * on other streams that writes it has finished), and keep them, and stay
+ * allocated, until a call that reads back to the host (gg_get,
+ * gg_timings ...) has returned.
  * Stands in for the particle I/O the reference does through pStore
  * (pkdReadTipsy pkd.c:297, pkdOutVector outtype.c:806). */
 int gg_put(GGPKD pkd, int field, const void *src, int bDevice);
