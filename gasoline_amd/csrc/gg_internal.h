@@ -138,10 +138,18 @@ struct SphDev {
     DevBuf drec;                               // DenDVDX per-slot records
     DevBuf o_rho, o_divv, o_curl, o_dvds, o_divvt, o_bal, o_diff, res;
     std::vector<DevBuf> spares;                // gg_reorder's spare buffers, by capacity
+    uint32_t *rb = nullptr;                    // pinned read-back words (error words, counts)
+    size_t rbCap = 0;                          // words
+    hipEvent_t rbEv = nullptr;                 // recorded behind the read-back copies
     void release_all()
     {
         for (DevBuf &b : spares) b.release();
         spares.clear();
+        if (rb) (void)hipHostFree(rb);
+        rb = nullptr;
+        rbCap = 0;
+        if (rbEv) (void)hipEventDestroy(rbEv);
+        rbEv = nullptr;
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
                          &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,

@@ -1986,6 +1986,27 @@ __global__ void k_scatter(int n, const int *__restrict__ order, const uint32_t *
     for (int k = 0; k < width; ++k) dst[(size_t)width * c + k] = src[(size_t)width * s + k];
 }
 
+// Up to eight FP64 fields from slot order into the caller's order in one
+// launch (the smoothed slots only, when qflag is given).
+struct ScatterSet {
+    const double *src[8];
+    double *dst[8];
+    int width[8];
+    int nf;
+};
+
+__global__ void k_scatter_set(int n, const int *__restrict__ order,
+                              const uint32_t *__restrict__ qflag, ScatterSet F)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || (qflag && !qflag[s])) return;
+    const int c = order[s];
+    for (int f = 0; f < F.nf; ++f) {
+        const int w = F.width[f];
+        for (int k = 0; k < w; ++k) F.dst[f][(size_t)w * c + k] = F.src[f][(size_t)w * s + k];
+    }
+}
+
 // Slot-order flags (query, TYPE_ACTIVE) and each block's number of queries
 // in qcnt[block] (256 threads).
 __global__ __launch_bounds__(256) void k_qflag(int n, const int *__restrict__ order,
@@ -2132,18 +2153,50 @@ static QArgs query_args(GGPKD pkd, int mode, int K)
     return Q;
 }
 
-// The twelve error words and the number of queries make_flags counted, in
-// one wait on the stream.
-static int read_err_queries(GGPKD pkd, unsigned *e, int64_t *nq)
+// Read-back through pinned host memory: rb_queue copies device words behind
+// the work queued so far and records an event; rb_wait returns once they
+// have arrived, while the work queued after them still runs.
+static int rb_queue(GGPKD pkd, const void *src, size_t words, size_t at)
 {
     SphDev &S = pkd->sph;
-    std::vector<int> qc(nblk(S.n, 256));
-    GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, pkd->stream));
-    GG_HIP(hipMemcpyAsync(qc.data(), S.qcnt.p, sizeof(int) * qc.size(), hipMemcpyDeviceToHost,
+    if (at + words > S.rbCap) {
+        const size_t cap = std::max<size_t>(at + words, 1024);
+        if (S.rb) GG_HIP(hipHostFree(S.rb));
+        S.rb = nullptr;
+        S.rbCap = 0;
+        GG_HIP(hipHostMalloc((void **)&S.rb, sizeof(uint32_t) * cap, hipHostMallocDefault));
+        S.rbCap = cap;
+    }
+    if (!S.rbEv) GG_HIP(hipEventCreateWithFlags(&S.rbEv, hipEventDisableTiming));
+    GG_HIP(hipMemcpyAsync(S.rb + at, src, sizeof(uint32_t) * words, hipMemcpyDeviceToHost,
                           pkd->stream));
-    GG_HIP(hipStreamSynchronize(pkd->stream));
+    GG_HIP(hipEventRecord(S.rbEv, pkd->stream));
+    return GG_OK;
+}
+
+static int rb_wait(GGPKD pkd)
+{
+    GG_HIP(hipEventSynchronize(pkd->sph.rbEv));
+    return GG_OK;
+}
+
+// The twelve error words and the per-block query counts of make_flags,
+// queued (then the caller may queue more work) and read in one wait.
+static int queue_err_queries(GGPKD pkd)
+{
+    SphDev &S = pkd->sph;
+    GG_TRY(rb_queue(pkd, S.qcnt.p, nblk(S.n, 256), 12));
+    return rb_queue(pkd, S.err.p, 12, 0);
+}
+
+static int wait_err_queries(GGPKD pkd, unsigned *e, int64_t *nq)
+{
+    SphDev &S = pkd->sph;
+    GG_TRY(rb_wait(pkd));
+    for (int k = 0; k < 12; ++k) e[k] = S.rb[k];
     int64_t t = 0;
-    for (int v : qc) t += v;
+    const size_t nbq = nblk(S.n, 256);
+    for (size_t k = 0; k < nbq; ++k) t += (int)S.rb[12 + k];
     *nq = t;
     return GG_OK;
 }
@@ -2294,11 +2347,25 @@ extern "C" int gg_build_local(GGPKD pkd, int nBucket, int bPeriodic, const doubl
 // The kNN lists' bookkeeping once the operator has run: the search radii
 // (-1 marks a failed search), the convergence check, what gg_resmooth may
 // reuse, and the number of particles smoothed.
-static int knn_finish(GGPKD pkd, const unsigned *e, int64_t nq, int K, int *pnSmoothed)
+// The device part, queued before the host waits for the error words: the
+// radii and flags gg_resmooth compares with (fBall2 itself goes into the
+// store with the operator's outputs).  A failed search leaves no list valid.
+static int knn_store(GGPKD pkd)
 {
     SphDev &S = pkd->sph;
     hipStream_t st = pkd->stream;
     const int n = S.n;
+    S.listValid = 0;
+    GG_TRY(S.fb2knn.reserve(sizeof(double) * n));
+    GG_TRY(S.qfknn.reserve(sizeof(uint32_t) * n));
+    GG_HIP(hipMemcpyAsync(S.fb2knn.p, S.fb2s.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    return GG_OK;
+}
+
+static int knn_finish(GGPKD pkd, const unsigned *e, int64_t nq, int K, int *pnSmoothed)
+{
+    SphDev &S = pkd->sph;
     S.fails = e[0];
     S.ties = e[1];
     S.retried = e[10];
@@ -2307,13 +2374,8 @@ static int knn_finish(GGPKD pkd, const unsigned *e, int64_t nq, int K, int *pnSm
     uint64_t cand;
     std::memcpy(&cand, &e[8], sizeof(cand));
     S.cand = (int64_t)cand;
-    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, S.qf.as<uint32_t>()));
     GG_CHECK(e[0] == 0, GG_EUNSUP,
              "kNN search did not converge for %u particles (degenerate positions?)", e[0]);
-    GG_TRY(S.fb2knn.reserve(sizeof(double) * n));
-    GG_TRY(S.qfknn.reserve(sizeof(uint32_t) * n));
-    GG_HIP(hipMemcpyAsync(S.fb2knn.p, S.fb2s.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    GG_HIP(hipMemcpyAsync(S.qfknn.p, S.qf.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     S.listK = K;
     S.listValid = (S.ties == 0);
     S.listQueries = (int)nq;
@@ -2382,10 +2444,9 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
 #undef GG_DEN
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 5);
-    unsigned e[12];
-    int64_t nq = 0;
-    GG_TRY(read_err_queries(pkd, e, &nq));
-    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
+    GG_TRY(queue_err_queries(pkd));
+    GG_TRY(knn_store(pkd));
+    GG_TRY(scatter_field(pkd, S.fb2s, GG_FBALL2, 1, S.qf.as<uint32_t>()));
     // DensitySym writes every particle of the structure (the unsmoothed ones
     // receive their neighbours' halves); Density the smoothed ones
     GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, bSymmetric ? nullptr : S.qf.as<uint32_t>()));
@@ -2394,6 +2455,10 @@ static int density_pass(GGPKD pkd, int K, int bSymmetric, int *pnSmoothed)
                        pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 6);
+    unsigned e[12];
+    int64_t nq = 0;
+    GG_TRY(wait_err_queries(pkd, e, &nq));
+    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
     const int ph[6] = {9, 3, 10, bSymmetric ? 6 : 8, 4, 8};
     timer_collect(pkd, 0, 6, ph);
     return GG_OK;
@@ -2467,19 +2532,32 @@ static int dendvdx_launch(GGPKD pkd, const gg_smf *smf, const int64_t *off, cons
 }
 
 // DenDVDX's outputs into the store (smoothed particles only) and the flags.
-static int dendvdx_store(GGPKD pkd)
+static int dendvdx_store(GGPKD pkd, bool withFb2)
 {
     SphDev &S = pkd->sph;
     hipStream_t st = pkd->stream;
     const int n = S.n;
     const uint32_t *qf = S.qf.as<uint32_t>();
-    GG_TRY(scatter_field(pkd, S.o_rho, GG_DENSITY, 1, qf));
-    GG_TRY(scatter_field(pkd, S.o_divv, GG_DIVV, 1, qf));
-    GG_TRY(scatter_field(pkd, S.o_curl, GG_CURLV, 3, qf));
-    GG_TRY(scatter_field(pkd, S.o_dvds, GG_DVDS, 1, qf));
-    GG_TRY(scatter_field(pkd, S.o_divvt, GG_FDIVV_T, 1, qf));
-    GG_TRY(scatter_field(pkd, S.o_bal, GG_BALSARA, 1, qf));
-    GG_TRY(scatter_field(pkd, S.o_diff, GG_DIFF, 1, qf));
+    ScatterSet F;
+    F.nf = 0;
+    auto add = [&](const DevBuf &src, int field, int width) -> int {
+        GG_TRY(ensure_field(pkd, field));
+        F.src[F.nf] = src.as<double>();
+        F.dst[F.nf] = pkd->f[field].as<double>();
+        F.width[F.nf] = width;
+        ++F.nf;
+        return GG_OK;
+    };
+    GG_TRY(add(S.o_rho, GG_DENSITY, 1));
+    GG_TRY(add(S.o_divv, GG_DIVV, 1));
+    GG_TRY(add(S.o_curl, GG_CURLV, 3));
+    GG_TRY(add(S.o_dvds, GG_DVDS, 1));
+    GG_TRY(add(S.o_divvt, GG_FDIVV_T, 1));
+    GG_TRY(add(S.o_bal, GG_BALSARA, 1));
+    GG_TRY(add(S.o_diff, GG_DIFF, 1));
+    if (withFb2) GG_TRY(add(S.fb2s, GG_FBALL2, 1));
+    hipLaunchKernelGGL(k_scatter_set, dim3(nblk(n, 256)), dim3(256), 0, st, n,
+                       S.order.as<int>(), qf, F);
     hipLaunchKernelGGL(k_flags_out, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(),
                        S.qf.as<uint32_t>(), S.nba.as<uint32_t>(), pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
@@ -2551,14 +2629,17 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     timer_mark(pkd, 4);
     GG_TRY(dendvdx_launch(pkd, smf, S.off.as<int64_t>(), S.nbr.as<int>()));
     timer_mark(pkd, 5);
-    unsigned e[12];
-    int64_t nq = 0;
-    GG_TRY(read_err_queries(pkd, e, &nq));
-    S.exValid = (e[7] == 0);   // the extras lists are whole
-    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
-    GG_TRY(dendvdx_store(pkd));
+    GG_TRY(queue_err_queries(pkd));
+    GG_TRY(knn_store(pkd));
+    GG_TRY(dendvdx_store(pkd, true));
     GG_HIP(hipGetLastError());
     timer_mark(pkd, 6);
+    // the stores run while the host reads the error words
+    unsigned e[12];
+    int64_t nq = 0;
+    GG_TRY(wait_err_queries(pkd, e, &nq));
+    S.exValid = (e[7] == 0);   // the extras lists are whole
+    GG_TRY(knn_finish(pkd, e, nq, K, pnSmoothed));
     const int ph[6] = {9, 3, 10, 8, 4, 8};
     timer_collect(pkd, 0, 6, ph);
     return GG_OK;
@@ -2742,10 +2823,36 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         hipLaunchKernelGGL(k_count_diff, dim3(nblk(n, 256)), dim3(256), 0, st, n,
                            S.fb2s.as<double>(), S.fb2knn.as<double>(), S.qf.as<uint32_t>(),
                            S.qfknn.as<uint32_t>(), S.diffcnt.as<unsigned>());
-        unsigned nd = 1;
-        GG_HIP(hipMemcpyAsync(&nd, S.diffcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
-        reuse = (nd == 0);
+        GG_TRY(rb_queue(pkd, S.diffcnt.p, 1, 0));
+    }
+    timer_mark(pkd, 1);
+    // the force pass's per-particle records in slot order (they do not depend
+    // on the lists), packed while the host reads the count
+    const bool f32 = pkd->prec == GG_PREC_FP32;
+    if (iSmoothType == GG_SMX_SPHPRESSURETERMS) {
+        const int in[] = {GG_MASS, GG_VPRED, GG_UPRED, GG_DENSITY, GG_POVERRHO2, GG_C,
+                          GG_BALSARA, GG_DIFF, GG_DT};
+        for (int f : in) GG_TRY(ensure_field(pkd, f));
+        GG_TRY(S.hrec.reserve((f32 ? sizeof(HRec<float>) : sizeof(HRec<double>)) * (size_t)n));
+        GG_TRY(S.res.reserve(sizeof(double) * 8 * n));
+#define GG_HPACK(T)                                                                           \
+    hipLaunchKernelGGL(k_hpack<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(), \
+                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),               \
+                       S.qf.as<uint32_t>(), S.fb2s.as<double>(), pkd->f[GG_MASS].as<double>(), \
+                       pkd->f[GG_VPRED].as<double>(), pkd->f[GG_UPRED].as<double>(),          \
+                       pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_POVERRHO2].as<double>(),     \
+                       pkd->f[GG_C].as<double>(), pkd->f[GG_BALSARA].as<double>(),             \
+                       pkd->f[GG_DIFF].as<double>(), pkd->f[GG_DT].as<double>(),               \
+                       pkd->f[GG_IACTIVE].as<uint32_t>(), S.hrec.as<HRec<T>>())
+        if (f32) GG_HPACK(float);
+        else GG_HPACK(double);
+#undef GG_HPACK
+        GG_HIP(hipGetLastError());
+    }
+    timer_mark(pkd, 2);
+    if (S.listValid) {
+        GG_TRY(rb_wait(pkd));
+        reuse = (S.rb[0] == 0);
     }
     const int64_t *off;
     const int *idx;
@@ -2774,19 +2881,18 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
         off = S.goff.as<int64_t>();
         idx = S.gent.as<int>();
     }
-    timer_mark(pkd, 1);
+    timer_mark(pkd, 3);
     if (iSmoothType == GG_SMX_DENDVDX) {   // DenDVDX over the ball lists
         GG_TRY(dendvdx_launch(pkd, smf, off, idx));
-        timer_mark(pkd, 2);
-        unsigned e[12];
-        GG_HIP(hipMemcpyAsync(e, S.err.p, sizeof(unsigned) * 12, hipMemcpyDeviceToHost, st));
-        GG_HIP(hipStreamSynchronize(st));
+        timer_mark(pkd, 4);
+        GG_TRY(rb_queue(pkd, S.err.p, 12, 0));
+        GG_TRY(dendvdx_store(pkd, false));
+        timer_mark(pkd, 5);
+        GG_TRY(rb_wait(pkd));
         // the extras belong to the kNN lists only when those were the lists
-        S.exValid = reuse && e[7] == 0;
-        GG_TRY(dendvdx_store(pkd));
-        timer_mark(pkd, 3);
-        const int ph[3] = {5, 4, 8};
-        timer_collect(pkd, 0, 3, ph);
+        S.exValid = reuse && S.rb[7] == 0;
+        const int ph[5] = {5, 8, 5, 4, 8};
+        timer_collect(pkd, 0, 5, ph);
         return GG_OK;
     }
     const bool fixedExtras = reuse && S.exValid;
@@ -2818,26 +2924,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
                        S.ridx.as<int>());
     }
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 2);
-    // per-particle records in slot order
-    const int in[] = {GG_MASS, GG_VPRED, GG_UPRED, GG_DENSITY, GG_POVERRHO2, GG_C, GG_BALSARA,
-                      GG_DIFF, GG_DT};
-    for (int f : in) GG_TRY(ensure_field(pkd, f));
-    const bool f32 = pkd->prec == GG_PREC_FP32;
-    GG_TRY(S.hrec.reserve((f32 ? sizeof(HRec<float>) : sizeof(HRec<double>)) * (size_t)n));
-    GG_TRY(S.res.reserve(sizeof(double) * 8 * n));
-#define GG_HPACK(T)                                                                           \
-    hipLaunchKernelGGL(k_hpack<T>, dim3(nblk(n, 256)), dim3(256), 0, st, n, S.order.as<int>(), \
-                       S.sx.as<double>(), S.sy.as<double>(), S.sz.as<double>(),               \
-                       S.qf.as<uint32_t>(), S.fb2s.as<double>(), pkd->f[GG_MASS].as<double>(), \
-                       pkd->f[GG_VPRED].as<double>(), pkd->f[GG_UPRED].as<double>(),          \
-                       pkd->f[GG_DENSITY].as<double>(), pkd->f[GG_POVERRHO2].as<double>(),     \
-                       pkd->f[GG_C].as<double>(), pkd->f[GG_BALSARA].as<double>(),             \
-                       pkd->f[GG_DIFF].as<double>(), pkd->f[GG_DT].as<double>(),               \
-                       pkd->f[GG_IACTIVE].as<uint32_t>(), S.hrec.as<HRec<T>>())
-    if (f32) GG_HPACK(float);
-    else GG_HPACK(double);
-#undef GG_HPACK
+    timer_mark(pkd, 4);
     H2Args H;
     H.n = n;
     H.per = S.bPeriodic;
@@ -2858,7 +2945,6 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     H.bCannonical = smf->bCannonical;
     H.out = S.res.as<double>();
     const int L = S.lpp, grid = (int)(((int64_t)n * L + 255) / 256);
-    timer_mark(pkd, 3);
 #define GG_H2(T, LL)                                                                          \
     hipLaunchKernelGGL((k_hydro2<T, LL>), dim3(grid), dim3(256), 0, st, H,                  \
                        (const HRec<T> *)S.hrec.as<HRec<T>>())
@@ -2867,7 +2953,7 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
     else { if (f32) GG_H2(float, 8); else GG_H2(double, 8); }
 #undef GG_H2
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 4);
+    timer_mark(pkd, 5);
     const int fl[] = {GG_A, GG_UDOTPDV, GG_UDOTAV, GG_UDOTDIFF, GG_MUMAX, GG_DTNEW};
     for (int f : fl) GG_TRY(ensure_field(pkd, f));
     hipLaunchKernelGGL(k_hydro_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n,
@@ -2880,9 +2966,9 @@ extern "C" int gg_resmooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPerio
                        S.qf.as<uint32_t>(), (const uint32_t *)nullptr,
                        pkd->f[GG_IACTIVE].as<uint32_t>());
     GG_HIP(hipGetLastError());
-    timer_mark(pkd, 5);
-    const int ph[5] = {5, 6, 8, 7, 8};
-    timer_collect(pkd, 0, 5, ph);
+    timer_mark(pkd, 6);
+    const int ph[6] = {5, 8, 5, 6, 7, 8};
+    timer_collect(pkd, 0, 6, ph);
     return GG_OK;
 }
 
