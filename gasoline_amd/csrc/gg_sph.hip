@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cfloat>
+#include <climits>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -29,6 +30,7 @@ namespace gg {
 constexpr int FCAP = 1024;         // BFS frontier capacity (nodes)
 constexpr int CAP = 256;           // candidate buffer of a single-query search (4 per lane)
 constexpr int ECAP = 48;           // force-pass extras kept per particle by the density pass
+static_assert(ECAP % 4 == 0 && ECAP >= 16, "extras lists are sorted 16 bytes at a time");
 constexpr int MAXTRY = 48;         // radius adjustments per query
 constexpr double GG_FLT_MAX = (double)3.402823466E+38F;   // floattype.h:9
 
@@ -1913,8 +1915,40 @@ __global__ __launch_bounds__(256) void k_rev_sort_fixed(int n, int ecap,
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-    if (q >= n) return;
+    if (q >= n || m <= 1) return;
     int *a = eidx + (size_t)q * ecap;
+    if (m <= 16) {   // most lists: 16-byte loads, a bitonic network in registers
+        int v[16];
+        int4 *a4 = reinterpret_cast<int4 *>(a);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int4 t = 4 * c < m ? a4[c] : make_int4(0, 0, 0, 0);
+            v[4 * c] = 4 * c < m ? t.x : INT_MAX;
+            v[4 * c + 1] = 4 * c + 1 < m ? t.y : INT_MAX;
+            v[4 * c + 2] = 4 * c + 2 < m ? t.z : INT_MAX;
+            v[4 * c + 3] = 4 * c + 3 < m ? t.w : INT_MAX;
+        }
+#pragma unroll
+        for (int k = 2; k <= 16; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+                        const bool up = (i & k) == 0;
+                        v[i] = up ? lo : hi;
+                        v[l] = up ? hi : lo;
+                    }
+                }
+        // the padding sorts last: entries past m (within the list's
+        // capacity) take INT_MAX, which no reader looks at
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (4 * c < m) a4[c] = make_int4(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+        return;
+    }
     for (int i = 1; i < m; ++i) {
         const int v = a[i];
         int j = i - 1;
