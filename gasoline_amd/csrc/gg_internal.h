@@ -126,7 +126,7 @@ struct SphDev {
     DevBuf sx, sy, sz;                         // slot order positions
     DevBuf child, rng, sparse;                 // radix tree, bucket-box sparse table
     DevBuf bflag, bcode, bpar, bfirst, bcnt, bbox, bden, bof; // buckets
-    DevBuf crec, grpf, grpc, grpn, grpr;       // child records, kNN groups
+    DevBuf crec, grpn, grpr;                   // child records, kNN groups (size, radius^2)
     DevBuf nbr, fb2s, fb2knn, qfknn, off;      // kNN lists (slot ids), CSR
     DevBuf gcnt, goff, gent;                   // ball-gather CSR
     DevBuf rcnt, roff, rcur, ridx;             // extras lists
@@ -152,7 +152,7 @@ struct SphDev {
         rbEv = nullptr;
         DevBuf *all[] = {&key, &order, &k2, &v2, &sortTmp, &sel, &pos, &gidx, &part,
                          &sx, &sy, &sz, &child, &rng, &sparse, &bflag, &bcode, &bpar, &bden,
-                         &bfirst, &bcnt, &bbox, &bof, &crec, &grpf, &grpc, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
+                         &bfirst, &bcnt, &bbox, &bof, &crec, &grpn, &grpr, &nbr, &fb2s, &fb2knn, &qfknn, &off,
                          &gcnt, &goff, &gent, &rcnt, &roff, &rcur, &ridx, &ecnt, &eidx, &qf, &act, &nba,
                          &err, &guess, &diffcnt, &nxd, &qcnt, &retry, &hrec, &m, &vx, &vy, &vz, &u, &rho, &pr2, &c,
                          &bal, &dif, &dt, &drec, &o_rho, &o_divv, &o_curl, &o_dvds, &o_divvt,

@@ -805,10 +805,11 @@ __global__ __launch_bounds__(64) void k_query(QArgs A)
 
 // ---------------------------------------------- kNN by bucket groups
 // Group of a bucket: every bucket within a group radius RL of the bucket's
-// box.  k_group_lists finds the groups (one wavefront per bucket, a
-// breadth-first descent of the radix tree) and k_knn_bucket settles the
-// bucket's queries against its group, one wavefront per bucket holding the
-// group's bucket boxes and ranges in LDS.  For each query the wavefront
+// box.  k_knn_bucket, one wavefront per bucket, finds the group (a
+// breadth-first descent of the radix tree, whose waits on the child records
+// other wavefronts' selections fill) into LDS, then settles the bucket's
+// queries against it, holding the group's bucket boxes and ranges in LDS.
+// For each query the wavefront
 // takes the group's buckets within a trial radius of it and reads their
 // positions from the slot-ordered arrays (each bucket is 32 consecutive
 // slots at most, one load per half-wavefront); squared distances, computed
@@ -828,7 +829,6 @@ struct KLArgs {
     QArgs Q;
     int nb;                            // buckets
     const int *bfirst, *bcnt;
-    int *gfirst, *gcnt;                // group lists (bucket, count), KL_MB per bucket
     int *gnb;                          // buckets in the group, -1: too large
     double *grl2;                      // group radius^2
     int *retry;
@@ -854,42 +854,6 @@ __device__ __forceinline__ double boxbox_d2(const double *a, const double *b, in
 
 // One wavefront per bucket: group radius from the fBall2 guesses of the
 // bucket's particles (else from the bucket's density) and the group list.
-__global__ __launch_bounds__(256) void k_group_lists(KLArgs KA)
-{
-    const QArgs &A = KA.Q;
-    __shared__ int fr[4][2 * KL_FC];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = blockIdx.x * 4 + w;
-    if (g >= KA.nb) return;
-    const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
-    const int K = A.K;
-    const double *lb = A.bbox + 6 * (size_t)g;
-    double gmax = 0.0;
-    bool none = false;
-    if (lane < nq) {
-        const double v = A.guess ? A.guess[s0 + lane] : 0.0;
-        none = !(v > 0);
-        gmax = none ? 0.0 : v;
-    }
-    for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
-    const unsigned long long nh = __ballot(none);
-    double R2;
-    if (nh == 0) R2 = gmax * A.slack * 1.25;   // 1.44 fBall2 at the default slack
-    else R2 = cold_r2(A.bden[g], K);
-    const double bx[6] = {lb[0], lb[1], lb[2], lb[3], lb[4], lb[5]};
-    const double RL2b = R2 * (1.0 + 1e-12);
-    int tot = 0;
-    int nb = wave_bfs(
-        A, [&](const double *b) { return boxbox_d2(b, bx, A.per, A.L) <= RL2b; }, fr[w], KL_FC,
-        KA.gfirst + (size_t)g * KL_MB, KA.gcnt + (size_t)g * KL_MB, KL_MB, lane, &tot);
-    if (nb == 0 || tot > KL_CM) nb = -1;   // none only for a NaN radius
-    if (lane == 0) {
-        KA.gnb[g] = nb;
-        KA.grl2[g] = R2;
-        if (nb >= 0) atomicMax(&A.err[5], (unsigned)nb);
-    }
-}
-
 // r-th smallest (0-based) of the values v[k] held by the elements
 // lane + 64 k with ok[k] set, by a wavefront quickselect whose pivot is the
 // lowest-numbered element still in range; *nlt and *neq count the values
@@ -1188,13 +1152,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const QArgs &A = KA.Q;
     __shared__ float lbx[KL_MB][6];
     __shared__ int lfst[KL_MB], lcnt[KL_MB];
-    __shared__ int2 sel[KL_SB];                 // per query: first slot, count of its buckets
-    __shared__ double bd[CAP];
-    __shared__ int bs[CAP];
+    // the descent's frontier, then (once the group is found) each query's
+    // buckets (first slot, count) and ball
+    constexpr int QB_ = KL_SB * 8 + CAP * 12, FB_ = 2 * KL_FC * 4;
+    __shared__ __align__(16) unsigned char ubuf[QB_ > FB_ ? QB_ : FB_];
+    int *fr = reinterpret_cast<int *>(ubuf);
+    int2 *sel = reinterpret_cast<int2 *>(ubuf);
+    double *bd = reinterpret_cast<double *>(ubuf + KL_SB * 8);
+    int *bs = reinterpret_cast<int *>(ubuf + KL_SB * 8 + CAP * 8);
     const int lane = threadIdx.x;
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int s0 = KA.bfirst[g], nq = KA.bcnt[g];
-    const int nb = KA.gnb[g];
+    const int K = A.K;
+    // the group radius: from the queries' fBall2 hints, else the bucket's
+    // density estimate
+    double qg = 0, RL2;
+    int nb;
+    {
+        double gmax = 0.0;
+        bool none = false;
+        if (lane < nq) {
+            qg = A.guess ? A.guess[s0 + lane] : 0.0;
+            none = !(qg > 0);
+            gmax = none ? 0.0 : qg;
+        }
+        for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
+        const unsigned long long nh = __ballot(none);
+        const double R2 = nh == 0 ? gmax * A.slack * 1.25   // 1.44 fBall2 at the default slack
+                                  : cold_r2(A.bden[g], K);
+        const double *lb = A.bbox + 6 * (size_t)g;
+        const double bx[6] = {lb[0], lb[1], lb[2], lb[3], lb[4], lb[5]};
+        const double RL2b = R2 * (1.0 + 1e-12);
+        int tot = 0;
+        // the group's buckets (bucket index, count) into lfst, lcnt
+        int nbg = wave_bfs(
+            A, [&](const double *b) { return boxbox_d2(b, bx, A.per, A.L) <= RL2b; }, fr, KL_FC,
+            lfst, lcnt, KL_MB, lane, &tot);
+        if (nbg == 0 || tot > KL_CM) nbg = -1;   // none only for a NaN radius
+        if (lane == 0) {   // for the retries
+            KA.gnb[g] = nbg;
+            KA.grl2[g] = R2;
+            if (nbg >= 0) atomicMax(&A.err[5], (unsigned)nbg);
+        }
+        nb = nbg;
+        RL2 = R2;
+    }
     if (nb < 0) {          // group too large: every query retries
         if (lane == 0) atomicAdd(&A.err[2], 1u);
         if (lane < nq && A.qflag[s0 + lane]) {
@@ -1203,22 +1205,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }
         return;
     }
-    const double RL2 = KA.grl2[g];
     // the bucket's queries, one per lane, broadcast as each is served
-    double qx = 0, qy = 0, qz = 0, qg = 0;
+    double qx = 0, qy = 0, qz = 0;
     int qf = 0;
     if (lane < nq) {
         qx = A.sx[s0 + lane];
         qy = A.sy[s0 + lane];
         qz = A.sz[s0 + lane];
-        qg = A.guess ? A.guess[s0 + lane] : 0.0;
         qf = A.qflag[s0 + lane] != 0;
     }
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (int k = lane; k < nb; k += 64) {
-        const int b = KA.gfirst[(size_t)g * KL_MB + k];
+        const int b = lfst[k];          // bucket index, as the descent left it
         lfst[k] = A.bfirst[b];
-        lcnt[k] = KA.gcnt[(size_t)g * KL_MB + k];
         for (int d = 0; d < 3; ++d) {
             const float l = A.fbox[6 * (size_t)b + d], h = A.fbox[6 * (size_t)b + 3 + d];
             lbx[k][d] = l;
@@ -2628,19 +2627,14 @@ extern "C" int gg_smooth(GGPKD pkd, const gg_smf *smf, int nSmooth, int bPeriodi
     KL.nb = S.nb;
     KL.bfirst = S.bfirst.as<int>();
     KL.bcnt = S.bcnt.as<int>();
-    GG_TRY(S.grpf.reserve(sizeof(int) * KL_MB * (size_t)S.nb));
-    GG_TRY(S.grpc.reserve(sizeof(int) * KL_MB * (size_t)S.nb));
     GG_TRY(S.grpn.reserve(sizeof(int) * S.nb));
     GG_TRY(S.grpr.reserve(sizeof(double) * S.nb));
-    KL.gfirst = S.grpf.as<int>();
-    KL.gcnt = S.grpc.as<int>();
     KL.gnb = S.grpn.as<int>();
     KL.grl2 = S.grpr.as<double>();
     KL.retry = S.retry.as<int>();
     KL.nretry = S.err.as<unsigned>() + 10;   // err[10]: retries (zeroed above)
     timer_mark(pkd, 0);
-    hipLaunchKernelGGL(k_group_lists, dim3((S.nb + 3) / 4), dim3(256), 0, st, KL);
-    timer_mark(pkd, 1);
+    timer_mark(pkd, 1);   // the group lists are found inside k_knn_bucket
     hipLaunchKernelGGL(k_knn_bucket, dim3(S.nb), dim3(64), 0, st, KL);
     timer_mark(pkd, 2);
     GG_HIP(hipGetLastError());
