@@ -1763,6 +1763,10 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     hp.kDiff = A.kDiff;
     const HRec<T> I = rec[i];
     const double xi = I.x, yi = I.y, zi = I.z;
+    // the extras' range and first index, loaded ahead of the own list
+    const int64_t x0 = A.eoff ? A.eoff[i] : (int64_t)i * A.ecap;
+    const int64_t x1 = A.eoff ? A.eoff[i + 1] : x0 + A.ecnt[i];
+    int pn = x0 + sub < x1 ? A.eidx[x0 + sub] : 0;
     HAcc acc;
     if (I.qf > T(0)) {
         const int64_t o = A.off[i], e1 = A.off[i + 1];
@@ -1786,10 +1790,9 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
             }
         }
     }
-    const int64_t e0 = A.eoff ? A.eoff[i] : (int64_t)i * A.ecap;
-    const int64_t e1 = A.eoff ? A.eoff[i + 1] : e0 + A.ecnt[i];
-    for (int64_t e = e0 + sub; e < e1; e += L) {
-        const int p = A.eidx[e];
+    for (int64_t e = x0 + sub; e < x1; e += L) {
+        const int p = pn;
+        if (e + L < x1) pn = A.eidx[e + L];
         const HRec<T> P = rec[p];
         const double ex = img_sep(P.x, xi, A.per, A.L[0]);
         const double ey = img_sep(P.y, yi, A.per, A.L[1]);
