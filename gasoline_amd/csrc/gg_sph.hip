@@ -1375,6 +1375,10 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     if (g >= A.n) return;
     const int i = g;
     const DRec<T> *__restrict__ rec = static_cast<const DRec<T> *>(A.rec);
+    // the list's range and first neighbour issued with the particle's record
+    // (a particle that is not smoothed has an empty list, or a fixed-K one)
+    const int64_t o = A.off[i], e1 = A.off[i + 1];
+    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
     const DRec<T> I = rec[i];
     if (!I.qf) return;
     const double fb2 = I.fb2;
@@ -1386,7 +1390,6 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     double rho = 0, xx = 0, xy = 0, xz = 0, yx = 0, yy = 0, yz = 0, zx = 0, zy = 0, zz = 0;
     double dn = 0, gx = 0, gy = 0, gz = 0;
     unsigned qact = 0;
-    const int64_t o = A.off[i], e1 = A.off[i + 1];
     auto pair = [&](int q, const DRec<T> &Q) {
         const double dxd = img_sep(x, Q.x, A.per, A.L[0]);
         const double dyd = img_sep(y, Q.y, A.per, A.L[1]);
@@ -1429,7 +1432,6 @@ __global__ __launch_bounds__(256) void k_dendvdx(DArgs A)
     };
     // the next neighbour's record is loaded while this one is used, and the
     // index after it with it, so no step waits on a load issued in it
-    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
     DRec<T> Qn;
     if (o + sub < e1) Qn = rec[qn];
     int qnn = o + sub + L < e1 ? A.idx[o + sub + L] : 0;
@@ -1761,6 +1763,9 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     hp.vFac = (T)(A.bCannonical ? 1. / (A.a * A.a) : 1.0);
     hp.dtFacDiffusion = A.dtFacDiffusion;
     hp.kDiff = A.kDiff;
+    // the own list's range and first index issued with the particle's record
+    const int64_t o = A.off[i], e1 = A.off[i + 1];
+    int qn = o + sub < e1 ? A.idx[o + sub] : 0;
     const HRec<T> I = rec[i];
     const double xi = I.x, yi = I.y, zi = I.z;
     // the extras' range and first index, loaded ahead of the own list
@@ -1769,9 +1774,7 @@ __global__ __launch_bounds__(256) void k_hydro2(H2Args A, const HRec<T> *__restr
     int pn = x0 + sub < x1 ? A.eidx[x0 + sub] : 0;
     HAcc acc;
     if (I.qf > T(0)) {
-        const int64_t o = A.off[i], e1 = A.off[i + 1];
-        int qn = o + sub < e1 ? A.idx[o + sub] : 0;   // next index in flight, as in DenDVDX
-        for (int64_t e = o + sub; e < e1; e += L) {
+        for (int64_t e = o + sub; e < e1; e += L) {   // the next index in flight
             const int q = qn;
             if (e + L < e1) qn = A.idx[e + L];
             const HRec<T> Q = rec[q];
